@@ -1,0 +1,529 @@
+// alm.hip — actuator-line model (src/alm/*) on gfx950.
+//
+// k_alm_points: one thread per actuator point does what the reference does
+//   in three launches (kernel_UpdateTurbineAngles, kernel_UpdateAPX,
+//   kernel_CalcAPForce, src/alm/almDevCall.cu:43-253).
+// k_alm_project: the Gaussian projection (kernel_DistributeAPForce,
+//   :255-294) as a cell-major gather.  Each 64x4 tile first culls, in LDS,
+//   the points whose kernel value is exactly zero in double precision over
+//   the whole tile (exp underflows to 0 for r^2/eps^2 > 746), so the sum over
+//   the remaining points in ascending id order is bit-identical to the
+//   all-pairs sum; a finite cutoff (radius in units of eps) trades that for
+//   speed.
+// k_alm_totals: per-turbine torque/thrust in a fixed order (the reference
+//   uses thrust::reduce, src/alm/almDevCall.cu:362-370), kept on device.
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+namespace falm {
+
+constexpr double kPi = 3.14159265358979323846;  // Falm::Pi (src/falmath.h:10)
+
+struct TurbDev {
+    double base[3], hub[3], tip_rate, motion[3], surge[3], pitch_deg[8];
+    int angle_type;
+};
+
+struct AlmDev {
+    int nt, nb, napb, nap, nat;
+    double radius, hub_radius, eps;
+    int flags;
+    int nx, ny, nz, gc;
+    const TurbDev *tb;
+    const double *r, *chord, *twist, *attack, *cl, *cd, *twist_off;
+    const double *x, *y, *z;
+    double *ap_xyz, *ap_force, *ap_torque, *ap_thrust, *torque, *thrust;
+    int *ap_ijk, *ap_rank;
+};
+
+// one_angle_frame_rotation, src/falmath.h:55-81
+__device__ inline void rot(const double x[3], const double ang[3], int type, double out[3]) {
+    const double X = x[0], Y = x[1], Z = x[2];
+    if (type == 1) {
+        const double C = cos(ang[0]), S = sin(ang[0]);
+        out[0] = X; out[1] = C * Y + S * Z; out[2] = -S * Y + C * Z;
+    } else if (type == 2) {
+        const double C = cos(ang[1]), S = sin(ang[1]);
+        out[0] = C * X - S * Z; out[1] = Y; out[2] = S * X + C * Z;
+    } else if (type == 3) {
+        const double C = cos(ang[2]), S = sin(ang[2]);
+        out[0] = C * X + S * Y; out[1] = -S * X + C * Y; out[2] = Z;
+    } else {
+        out[0] = X; out[1] = Y; out[2] = Z;
+    }
+}
+
+// one_angle_frame_rotation_dt, src/falmath.h:83-131
+__device__ inline void rot_dt(const double x[3], const double v[3], const double ang[3], const double om[3], int type,
+                              double out[3]) {
+    const double X = x[0], Y = x[1], Z = x[2], U = v[0], V = v[1], W = v[2];
+    if (type == 1) {
+        const double C = cos(ang[0]), S = sin(ang[0]), O = om[0];
+        out[0] = 0. + U; out[1] = O * (-S * Y + C * Z) + (C * V + S * W); out[2] = O * (-C * Y - S * Z) + (-S * V + C * W);
+    } else if (type == 2) {
+        const double C = cos(ang[1]), S = sin(ang[1]), O = om[1];
+        out[0] = O * (-S * X - C * Z) + (C * U - S * W); out[1] = 0. + V; out[2] = O * (C * X - S * Z) + (S * U + C * W);
+    } else if (type == 3) {
+        const double C = cos(ang[2]), S = sin(ang[2]), O = om[2];
+        out[0] = O * (-S * X + C * Y) + (C * U + S * V); out[1] = O * (-C * X - S * Y) + (-S * U + C * V); out[2] = 0. + W;
+    } else {
+        out[0] = U; out[1] = V; out[2] = W;
+    }
+}
+
+// find_index, src/alm/almDevCall.cu:12-26
+__device__ inline int find_index(const double *seq, double v, int size) {
+    if (v < seq[0]) return -1;
+    if (v >= seq[size - 1]) return size - 1;
+    for (int i = 0; i < size - 1; i++)
+        if (seq[i] <= v && v < seq[i + 1]) return i;
+    return size - 1;
+}
+
+// trilinear_interpolation, src/alm/almDevCall.cu:28-41
+__device__ inline double trilinear(double xd, double yd, double zd, const double c[8]) {
+    const double c01 = c[0] * (1. - xd) + c[1] * xd;
+    const double c23 = c[2] * (1. - xd) + c[3] * xd;
+    const double c45 = c[4] * (1. - xd) + c[5] * xd;
+    const double c67 = c[6] * (1. - xd) + c[7] * xd;
+    const double c0123 = c01 * (1. - yd) + c23 * yd;
+    const double c4567 = c45 * (1. - yd) + c67 * yd;
+    return c0123 * (1. - zd) + c4567 * zd;
+}
+
+template <class T>
+__global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, double t) {
+    const int ap = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ap >= A.nap) return;
+    const int napt = A.napb * A.nb;
+    const int tid = ap / napt, bid = (ap % napt) / A.napb, sec = ap % A.napb;
+    const TurbDev &T0 = A.tb[tid];
+    // kernel_UpdateTurbineAngles (+ base surge, no reference counterpart)
+    double ang[3] = {0, 0, 0}, om[3] = {0, 0, 0};
+    if (T0.angle_type) {
+        ang[T0.angle_type - 1] = T0.motion[0] * sin(T0.motion[1] * t + T0.motion[2]);
+        om[T0.angle_type - 1] = T0.motion[0] * T0.motion[1] * cos(T0.motion[1] * t + T0.motion[2]);
+    }
+    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {0, 0, 0};
+    if (T0.surge[0] != 0.0) {
+        base[0] = T0.base[0] + T0.surge[0] * sin(T0.surge[1] * t + T0.surge[2]);
+        bvel[0] = T0.surge[0] * T0.surge[1] * cos(T0.surge[1] * t + T0.surge[2]);
+    }
+    const double nang[3] = {-ang[0], -ang[1], -ang[2]};
+    // kernel_UpdateAPX
+    const double tip = T0.tip_rate;
+    const double theta0 = (2 * kPi / A.nb) * bid;
+    const double theta = tip * t + theta0;
+    const double apr = A.r[ap];
+    double c1[3] = {T0.hub[0], T0.hub[1], T0.hub[2]};
+    c1[1] += apr * cos(theta);
+    c1[2] += apr * sin(theta);
+    double P[3];
+    rot(c1, nang, T0.angle_type, P);
+    for (int m = 0; m < 3; m++) P[m] += base[m];
+    for (int m = 0; m < 3; m++) A.ap_xyz[3 * ap + m] = P[m];
+    const int I = find_index(A.x, P[0], A.nx), J = find_index(A.y, P[1], A.ny), K = find_index(A.z, P[2], A.nz);
+    A.ap_ijk[3 * ap] = I;
+    A.ap_ijk[3 * ap + 1] = J;
+    A.ap_ijk[3 * ap + 2] = K;
+    const int gc = A.gc;
+    const bool in = I >= gc - 1 && I < A.nx - gc - 1 && J >= gc - 1 && J < A.ny - gc - 1 && K >= gc - 1 &&
+                    K < A.nz - gc - 1;
+    A.ap_rank[ap] = in ? 0 : -1;
+    // kernel_CalcAPForce
+    if (!in) {
+        for (int m = 0; m < 3; m++) A.ap_force[3 * ap + m] = 0.0;
+        A.ap_torque[ap] = 0.0;
+        A.ap_thrust[ap] = 0.0;
+        return;
+    }
+    const double dr = (A.radius - A.hub_radius) / A.napb;
+    const int i0 = I, i1 = I + 1, j0 = J, j1 = J + 1, k0 = K, k1 = K + 1;
+    const double x0 = A.x[i0], x1 = A.x[i1], y0 = A.y[j0], y1 = A.y[j1], z0 = A.z[k0], z1 = A.z[k1];
+    const double xd = fabs(P[0] - x0) / fabs(x1 - x0);
+    const double yd = fabs(P[1] - y0) / fabs(y1 - y0);
+    const double zd = fabs(P[2] - z0) / fabs(z1 - z0);
+    const int nx = A.nx, nxy = A.nx * A.ny;
+    const int id[8] = {i0 + nx * j0 + nxy * k0, i1 + nx * j0 + nxy * k0, i0 + nx * j1 + nxy * k0,
+                       i1 + nx * j1 + nxy * k0, i0 + nx * j0 + nxy * k1, i1 + nx * j0 + nxy * k1,
+                       i0 + nx * j1 + nxy * k1, i1 + nx * j1 + nxy * k1};
+    const T *uc[3] = {u0, u1, u2};
+    double uap[3];
+    for (int m = 0; m < 3; m++) {
+        double c[8];
+        for (int q = 0; q < 8; q++) c[q] = (double)uc[m][id[q]];
+        uap[m] = trilinear(xd, yd, zd, c);
+    }
+    double rel[3], vrel[3], utt[3];
+    for (int m = 0; m < 3; m++) {
+        rel[m] = P[m] - base[m];
+        vrel[m] = uap[m] - bvel[m];
+    }
+    rot_dt(rel, vrel, ang, om, T0.angle_type, utt);
+    const double ux = utt[0];
+    const double ut = tip * apr + utt[1] * sin(theta) - utt[2] * cos(theta);
+    const double urel2 = ux * ux + ut * ut;
+    const double phi = atan(ux / ut);
+    // APFrame::get_airfoil_params (src/alm/apHandler.h:121-141); blade pitch
+    // and per-section twist offset have no reference counterpart (0 default)
+    const double chord = A.chord[ap];
+    const double twist = A.twist[ap] + T0.pitch_deg[bid < 8 ? bid : 7] + A.twist_off[sec];
+    const double att = phi * 180. / kPi - twist;
+    double cl = 0.0, cd = 0.0;
+    const int nat = A.nat;
+    if (att < A.attack[0]) {
+        cl = A.cl[ap];
+        cd = A.cd[ap];
+    } else if (att >= A.attack[nat - 1]) {
+        cl = A.cl[ap + (nat - 1) * A.nap];
+        cd = A.cd[ap + (nat - 1) * A.nap];
+    } else {
+        for (int a = 0; a < nat - 1; a++) {
+            if (A.attack[a] <= att && A.attack[a + 1] > att) {
+                const double p = (att - A.attack[a]) / (A.attack[a + 1] - A.attack[a]);
+                cl = (1. - p) * A.cl[ap + a * A.nap] + p * A.cl[ap + (a + 1) * A.nap];
+                cd = (1. - p) * A.cd[ap + a * A.nap] + p * A.cd[ap + (a + 1) * A.nap];
+                break;
+            }
+        }
+    }
+    const double fl = .5 * cl * urel2 * chord * dr;
+    const double fd = .5 * cd * urel2 * chord * dr;
+    const double fx = fl * cos(phi) + fd * sin(phi);
+    double ft = (A.flags & FALM_ALM_FIX_DRAG_TANGENT) ? fl * sin(phi) - fd * cos(phi) : fl * sin(phi) - fd * sin(phi);
+    const int sg = (tip > 0) - (tip < 0);
+    ft *= sg;
+    const double ftt[3] = {-fx, ft * sin(theta), -ft * cos(theta)};
+    double F[3];
+    rot(ftt, nang, T0.angle_type, F);
+    for (int m = 0; m < 3; m++) A.ap_force[3 * ap + m] = F[m];
+    A.ap_torque[ap] = fabs(ft) * apr;
+    A.ap_thrust[ap] = fx;
+}
+
+constexpr int kCullChunk = 256;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0, T *ff1, T *ff2, double cut2) {
+    __shared__ double sx[kCullChunk], sy[kCullChunk], sz[kCullChunk], sf0[kCullChunk], sf1[kCullChunk],
+        sf2[kCullChunk];
+    __shared__ int nkeep;
+    __shared__ double box[6];
+    int i, j, k;
+    const bool valid = tile_cell(tl, i, j, k);
+    // tile bounding box in coordinates (cell centres of the tile)
+    if (threadIdx.x == 0) {
+        const int tile = xcd_tile(blockIdx.x, gridDim.x);
+        const int txy = tl.tx * tl.ty;
+        const int kk = tile / txy;
+        const int r = tile - kk * txy;
+        const int by = r / tl.tx, bx = r - by * tl.tx;
+        const int ilo = tl.box.lo[0] + bx * TX, ihi = min(ilo + TX, tl.box.lo[0] + tl.box.n[0]) - 1;
+        const int jlo = tl.box.lo[1] + by * TY, jhi = min(jlo + TY, tl.box.lo[1] + tl.box.n[1]) - 1;
+        const int kc = tl.box.lo[2] + kk;
+        box[0] = fmin(A.x[ilo], A.x[ihi]); box[1] = fmax(A.x[ilo], A.x[ihi]);
+        box[2] = fmin(A.y[jlo], A.y[jhi]); box[3] = fmax(A.y[jlo], A.y[jhi]);
+        box[4] = A.z[kc]; box[5] = A.z[kc];
+    }
+    __syncthreads();
+    const double eps = A.eps;
+    const double eps2 = eps * eps;
+    const double sp = eps * sqrt(kPi);
+    const double eta = 1. / (sp * sp * sp);
+    double cx = 0, cy = 0, cz = 0;
+    if (valid) {
+        cx = A.x[i];
+        cy = A.y[j];
+        cz = A.z[k];
+    }
+    __shared__ int wcount[256 / 64];
+    double fx = 0., fy = 0., fz = 0.;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int base = 0; base < A.nap; base += kCullChunk) {
+        // keep the point unless every cell of the tile is beyond the cutoff
+        const int ap = base + (int)threadIdx.x;
+        bool keep = false;
+        double px = 0, py = 0, pz = 0;
+        if (ap < A.nap) {
+            px = A.ap_xyz[3 * ap];
+            py = A.ap_xyz[3 * ap + 1];
+            pz = A.ap_xyz[3 * ap + 2];
+            const double dx = fmax(0.0, fmax(box[0] - px, px - box[1]));
+            const double dy = fmax(0.0, fmax(box[2] - py, py - box[3]));
+            const double dz = fmax(0.0, fmax(box[4] - pz, pz - box[5]));
+            const double d2 = (dx * dx + dy * dy + dz * dz) / eps2;
+            keep = !(d2 > cut2);
+        }
+        // order-preserving compaction: ballot prefix within the wavefront,
+        // wavefront totals across the block
+        const unsigned long long mask = __ballot(keep);
+        const int before = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[wid] = __popcll(mask);
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int w = 0; w < 256 / 64; w++) {
+            if (w < wid) off += wcount[w];
+            tot += wcount[w];
+        }
+        if (keep) {
+            const int slot = off + before;
+            sx[slot] = px;
+            sy[slot] = py;
+            sz[slot] = pz;
+            sf0[slot] = A.ap_force[3 * ap];
+            sf1[slot] = A.ap_force[3 * ap + 1];
+            sf2[slot] = A.ap_force[3 * ap + 2];
+        }
+        if (threadIdx.x == 0) nkeep = tot;
+        __syncthreads();
+        const int nk = nkeep;
+        if (valid) {
+            for (int q = 0; q < nk; q++) {
+                const double ddx = cx - sx[q], ddy = cy - sy[q], ddz = cz - sz[q];
+                const double rr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                const double e = exp(-rr2 / eps2);
+                fx += sf0[q] * eta * e;
+                fy += sf1[q] * eta * e;
+                fz += sf2[q] * eta * e;
+            }
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        const int cc = i + A.nx * j + A.nx * A.ny * k;
+        ff0[cc] = (T)fx;
+        ff1[cc] = (T)fy;
+        ff2[cc] = (T)fz;
+    }
+}
+
+__global__ void k_alm_totals(AlmDev A) {
+    const int tid = blockIdx.x;
+    if (tid >= A.nt || threadIdx.x != 0) return;
+    const int napt = A.napb * A.nb;
+    double tq = 0.0, th = 0.0;
+    for (int a = 0; a < napt; a++) {
+        tq += A.ap_torque[tid * napt + a];
+        th += A.ap_thrust[tid * napt + a];
+    }
+    A.torque[tid] = tq;
+    A.thrust[tid] = th;
+}
+
+}  // namespace falm
+
+using namespace falm;
+
+struct falm_alm {
+    falm_domain dom;
+    AlmDev d;
+    std::vector<void *> bufs;
+    double cut2;
+    int nt, nap, napb;
+};
+
+namespace falm {
+
+template <class V>
+static falm_status upload(falm_alm *A, const V *src, size_t n, V **dst) {
+    void *p = nullptr;
+    size_t bytes = (n ? n : 1) * sizeof(V);
+    FALM_HIP(hipMalloc(&p, bytes));
+    A->bufs.push_back(p);
+    if (src && n) FALM_HIP(hipMemcpy(p, src, n * sizeof(V), hipMemcpyHostToDevice));
+    else FALM_HIP(hipMemset(p, 0, bytes));
+    *dst = (V *)p;
+    return FALM_OK;
+}
+
+}  // namespace falm
+
+extern "C" {
+
+falm_status falm_alm_destroy(falm_alm *A) {
+    if (!A) return FALM_OK;
+    for (void *p : A->bufs) (void)hipFree(p);
+    delete A;
+    return FALM_OK;
+}
+
+falm_status falm_alm_create(const falm_domain *dom, const falm_alm_config *c, const double *x, const double *y,
+                            const double *z, falm_alm **out) {
+    FALM_TRY(check_domain(dom));
+    if (!c || !x || !y || !z || !out || !c->turbines || !c->ap_r || !c->ap_chord || !c->ap_twist || !c->attack ||
+        !c->cl || !c->cd) {
+        set_error("falm_alm_create: null argument");
+        return FALM_ERR_INVALID;
+    }
+    if (c->n_turbine < 1 || c->n_blade < 1 || c->n_blade > 8 || c->n_ap_per_blade < 1 || c->n_attack < 2 ||
+        !(c->eps > 0.0)) {
+        set_error("falm_alm_create: bad sizes");
+        return FALM_ERR_INVALID;
+    }
+    *out = nullptr;
+    falm_alm *A = new falm_alm();
+    A->dom = *dom;
+    AlmDev &d = A->d;
+    d.nt = c->n_turbine;
+    d.nb = c->n_blade;
+    d.napb = c->n_ap_per_blade;
+    d.nap = d.nt * d.nb * d.napb;
+    d.nat = c->n_attack;
+    d.radius = c->radius;
+    d.hub_radius = c->hub_radius;
+    d.eps = c->eps;
+    d.flags = c->flags;
+    d.nx = dom->shape[0];
+    d.ny = dom->shape[1];
+    d.nz = dom->shape[2];
+    d.gc = dom->gc;
+    A->nt = d.nt;
+    A->nap = d.nap;
+    A->napb = d.napb;
+    // exact mode: points whose exp(-r^2/eps^2) is 0 in double at every cell
+    // of a tile are skipped; 746 > -log(smallest subnormal double) = 744.44
+    A->cut2 = c->cutoff > 0.0 ? c->cutoff * c->cutoff : 746.0;
+    std::vector<TurbDev> tb(d.nt);
+    for (int t = 0; t < d.nt; t++) {
+        const falm_turbine &T = c->turbines[t];
+        for (int m = 0; m < 3; m++) {
+            tb[t].base[m] = T.base[m];
+            tb[t].hub[m] = T.hub[m];
+            tb[t].motion[m] = T.motion[m];
+            tb[t].surge[m] = T.surge[m];
+        }
+        for (int b = 0; b < 8; b++) tb[t].pitch_deg[b] = T.pitch_deg[b];
+        tb[t].tip_rate = T.tip_rate;
+        tb[t].angle_type = T.angle_type;
+        if (T.angle_type < 0 || T.angle_type > 3) {
+            falm_alm_destroy(A);
+            return FALM_ERR_INVALID;
+        }
+    }
+    std::vector<double> toff(d.napb, 0.0);
+    if (c->twist_offset) toff.assign(c->twist_offset, c->twist_offset + d.napb);
+    falm_status st = FALM_OK;
+    TurbDev *tbd = nullptr;
+    double *dr_ = nullptr, *dch = nullptr, *dtw = nullptr, *dat = nullptr, *dcl = nullptr, *dcd = nullptr,
+           *dto = nullptr, *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    if ((st = upload(A, tb.data(), tb.size(), &tbd)) || (st = upload(A, c->ap_r, d.nap, &dr_)) ||
+        (st = upload(A, c->ap_chord, d.nap, &dch)) || (st = upload(A, c->ap_twist, d.nap, &dtw)) ||
+        (st = upload(A, c->attack, d.nat, &dat)) || (st = upload(A, c->cl, (size_t)d.nat * d.nap, &dcl)) ||
+        (st = upload(A, c->cd, (size_t)d.nat * d.nap, &dcd)) || (st = upload(A, toff.data(), toff.size(), &dto)) ||
+        (st = upload(A, x, d.nx, &dx)) || (st = upload(A, y, d.ny, &dy)) || (st = upload(A, z, d.nz, &dz)) ||
+        (st = upload<double>(A, nullptr, 3 * d.nap, &d.ap_xyz)) ||
+        (st = upload<double>(A, nullptr, 3 * d.nap, &d.ap_force)) ||
+        (st = upload<double>(A, nullptr, d.nap, &d.ap_torque)) ||
+        (st = upload<double>(A, nullptr, d.nap, &d.ap_thrust)) ||
+        (st = upload<double>(A, nullptr, d.nt, &d.torque)) || (st = upload<double>(A, nullptr, d.nt, &d.thrust)) ||
+        (st = upload<int>(A, nullptr, 3 * d.nap, &d.ap_ijk)) || (st = upload<int>(A, nullptr, d.nap, &d.ap_rank))) {
+        falm_alm_destroy(A);
+        return st;
+    }
+    d.tb = tbd;
+    d.r = dr_;
+    d.chord = dch;
+    d.twist = dtw;
+    d.attack = dat;
+    d.cl = dcl;
+    d.cd = dcd;
+    d.twist_off = dto;
+    d.x = dx;
+    d.y = dy;
+    d.z = dz;
+    *out = A;
+    return FALM_OK;
+}
+
+falm_status falm_alm_set_twist_offset(falm_alm *A, const double *off) {
+    if (!A) return FALM_ERR_INVALID;
+    std::vector<double> v(A->napb, 0.0);
+    if (off) v.assign(off, off + A->napb);
+    FALM_HIP(hipMemcpy((void *)A->d.twist_off, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    return FALM_OK;
+}
+
+falm_status falm_alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, void *stream) {
+    if (!A || !u || !ff) return FALM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = ncell_of(&A->dom);
+    Geom g = make_geom(&A->dom);
+    Tiling tl = make_tiling(interior_box(g));
+    TimedLaunch tm(FALM_TIME_ALM, s);
+    if (prec == FALM_F32) {
+        hipLaunchKernelGGL((k_alm_points<float>), dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d, col<float>(u, n, 0),
+                           col<float>(u, n, 1), col<float>(u, n, 2), t);
+        FALM_CHECK_LAUNCH("k_alm_points");
+        hipLaunchKernelGGL((k_alm_project<float>), tiling_grid(tl), tiling_block(), 0, s, tl, A->d,
+                           col<float>(ff, n, 0), col<float>(ff, n, 1), col<float>(ff, n, 2), A->cut2);
+    } else {
+        hipLaunchKernelGGL((k_alm_points<double>), dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d,
+                           col<double>(u, n, 0), col<double>(u, n, 1), col<double>(u, n, 2), t);
+        FALM_CHECK_LAUNCH("k_alm_points");
+        hipLaunchKernelGGL((k_alm_project<double>), tiling_grid(tl), tiling_block(), 0, s, tl, A->d,
+                           col<double>(ff, n, 0), col<double>(ff, n, 1), col<double>(ff, n, 2), A->cut2);
+    }
+    FALM_CHECK_LAUNCH("k_alm_project");
+    hipLaunchKernelGGL(k_alm_totals, dim3(A->nt), dim3(64), 0, s, A->d);
+    FALM_CHECK_LAUNCH("k_alm_totals");
+    tm.done((double)tl.box.size() * 3.0 * prec_bytes(prec));
+    return FALM_OK;
+}
+
+falm_status falm_alm_get(falm_alm *A, double *ap_xyz, double *ap_force, double *torque, double *thrust,
+                         void *stream) {
+    if (!A) return FALM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    if (ap_xyz) FALM_HIP(hipMemcpyAsync(ap_xyz, A->d.ap_xyz, 3 * A->nap * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (ap_force)
+        FALM_HIP(hipMemcpyAsync(ap_force, A->d.ap_force, 3 * A->nap * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (torque) FALM_HIP(hipMemcpyAsync(torque, A->d.torque, A->nt * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (thrust) FALM_HIP(hipMemcpyAsync(thrust, A->d.thrust, A->nt * sizeof(double), hipMemcpyDeviceToHost, s));
+    FALM_HIP(hipStreamSynchronize(s));
+    return FALM_OK;
+}
+
+// BladeHandler::buildAP (src/alm/bladeHandler.h:16-104) without the JSON
+// round trip; the station radius column is used as an absolute radius, as
+// the reference does.
+falm_status falm_build_ap(int nst, int nat, const double *rr, const double *chord, const double *twist,
+                          const double *cl, const double *cd, int nt, int nb, int napb, double radius,
+                          double hub, double *ap_r, double *ap_chord, double *ap_twist, double *ap_cl,
+                          double *ap_cd) {
+    if (nst < 1 || nat < 1 || nt < 1 || nb < 1 || napb < 1 || !rr || !chord || !twist || !cl || !cd || !ap_r ||
+        !ap_chord || !ap_twist || !ap_cl || !ap_cd)
+        return FALM_ERR_INVALID;
+    const int nap = nt * nb * napb;
+    const double drr = (radius - hub) / napb;
+    for (int ap = 0; ap < nap; ap++) {
+        const double r = (ap % napb + 0.5) * drr + hub;
+        ap_r[ap] = r;
+        int i = 0, i1 = 0;
+        double p = 0.0;
+        if (r < rr[0]) {
+            i = i1 = 0;
+        } else if (r >= rr[nst - 1]) {
+            i = i1 = nst - 1;
+        } else {
+            for (i = 0; i < nst - 1; i++)
+                if (rr[i] <= r && rr[i + 1] > r) {
+                    p = (r - rr[i]) / (rr[i + 1] - rr[i]);
+                    break;
+                }
+            i1 = i + 1;
+        }
+        ap_chord[ap] = i == i1 ? chord[i] : (1. - p) * chord[i] + p * chord[i1];
+        ap_twist[ap] = i == i1 ? twist[i] : (1. - p) * twist[i] + p * twist[i1];
+        for (int a = 0; a < nat; a++) {
+            const double c0 = cl[(size_t)i * nat + a], c1 = cl[(size_t)i1 * nat + a];
+            const double d0 = cd[(size_t)i * nat + a], d1 = cd[(size_t)i1 * nat + a];
+            ap_cl[ap + (size_t)a * nap] = i == i1 ? c0 : (1. - p) * c0 + p * c1;
+            ap_cd[ap + (size_t)a * nap] = i == i1 ? d0 : (1. - p) * d0 + p * d1;
+        }
+    }
+    return FALM_OK;
+}
+
+}  // extern "C"

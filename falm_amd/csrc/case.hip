@@ -1,0 +1,377 @@
+// case.hip — the per-step orchestration of the reference apps
+// (app/nedo/main.cpp:150-204, app/windtunnel/main.cpp:94-165,
+// app/lid3d/main.cpp:94-175, app/ld2d/main.cpp:70-136) with all state on
+// the device.  Pseudo-velocity is computed out of place from the u_previous
+// snapshot the apps already take (the apps pass ua == u, a data race in the
+// reference kernel).
+//
+// Precision: FALM_CASE_F64 keeps everything in double.  FALM_CASE_MIXED keeps
+// velocity, face flux, eddy viscosity, force, divergence and metrics in
+// float (all CFD stencils run in float) while the pressure, the Poisson
+// right-hand side, the Poisson matrix and the residual stay double and the
+// Poisson solve runs as float Krylov passes inside a double
+// defect-correction loop.
+#include <cmath>
+#include <vector>
+
+#include "linalg.hpp"
+
+namespace falm {
+falm_status launch_poisson_rhs_mixed(const falm_domain *dom, int prec, const void *uu, double *rhs, const void *ja,
+                                     double scale, hipStream_t s);
+falm_status launch_divergence_mixed(const falm_domain *dom, int prec, const void *uu, void *div, const void *ja,
+                                    hipStream_t s);
+falm_status launch_project_mixed(const falm_domain *dom, int prec, double dt, void *u, const void *ua, void *uu,
+                                 const void *uua, const double *p, const void *kx, const void *gg, hipStream_t s);
+falm_status launch_bc_velocity(const falm_domain *dom, int prec, const falm_face_bc faces[6], void *u,
+                               const void *uprev, const void *xyz, double dt, const falm_inlet_noise *noise,
+                               hipStream_t s);
+falm_status launch_bc_pressure(const falm_domain *dom, int prec, const int faces[6], void *p, hipStream_t s);
+falm_status launch_bc_face_flux(const falm_domain *dom, int prec, const int faces[6], void *uu, hipStream_t s);
+falm_status launch_copy_z5(const falm_domain *dom, int prec, void *v, int ncol, hipStream_t s);
+}  // namespace falm
+
+using namespace falm;
+
+struct falm_case {
+    falm_case_config cfg;
+    falm_domain dom;
+    size_t n;
+    int prec;                 // FALM_F64 or FALM_F32 (CFD fields)
+    size_t tb;                // bytes per CFD element
+    hipStream_t stream = nullptr;
+    // CFD fields (prec)
+    void *u = nullptr, *uprev = nullptr, *uu = nullptr, *nut = nullptr, *ff = nullptr, *div = nullptr;
+    void *xyz = nullptr, *kx = nullptr, *g = nullptr, *ja = nullptr;
+    // Poisson fields (double)
+    double *p = nullptr, *rhs = nullptr, *res = nullptr, *a = nullptr;
+    double maxdiag = 1.0;
+    falm_solver *solver = nullptr;
+    falm_alm *alm = nullptr;
+    std::vector<double> x, y, z, hx, hy, hz;
+    int64_t it = 0;
+    falm_face_bc ubc[6];
+    int pbc[6], uubc[6];
+    // device scalar for the divergence norm + its reduction scratch
+    double *dnorm = nullptr, *partials = nullptr, *hnorm = nullptr;
+    unsigned *ticket = nullptr;
+    std::vector<void *> bufs;
+};
+
+namespace falm {
+
+static falm_status dalloc(falm_case *C, void **p, size_t bytes) {
+    FALM_HIP(hipMalloc(p, bytes));
+    C->bufs.push_back(*p);
+    FALM_HIP(hipMemset(*p, 0, bytes));
+    return FALM_OK;
+}
+
+static void case_faces(falm_case *C) {
+    const falm_case_config &c = C->cfg;
+    for (int f = 0; f < 6; f++) {
+        C->ubc[f].type = FALM_BC_NONE;
+        C->ubc[f].value[0] = C->ubc[f].value[1] = C->ubc[f].value[2] = 0.0;
+        C->pbc[f] = FALM_PBC_NONE;
+        C->uubc[f] = 0;
+    }
+    if (c.kind == FALM_CASE_WINDTUNNEL) {
+        // app/windtunnel/bc.h:6-68, bcdevcall.cu:8-211 (z- slip on request)
+        C->ubc[0].type = FALM_BC_INLET;
+        C->ubc[0].value[0] = c.u_inflow;
+        C->ubc[1].type = FALM_BC_CONVECTIVE;
+        C->ubc[2].type = FALM_BC_SLIP;
+        C->ubc[3].type = FALM_BC_SLIP;
+        C->ubc[4].type = c.zminus_slip ? FALM_BC_SLIP : FALM_BC_WALL0;
+        C->ubc[5].type = FALM_BC_SLIP;
+        for (int f = 0; f < 6; f++) C->pbc[f] = FALM_PBC_NEUMANN;
+        C->pbc[1] = FALM_PBC_ZERO;
+    } else {
+        // app/lid3d/bcdev.cu:11-357 and app/ld2d/bcdevcall.cu: no-slip
+        // reflection, lid on y+ moving in +x, Neumann p, zero wall flux
+        const int nf = c.kind == FALM_CASE_LID2D ? 4 : 6;
+        for (int f = 0; f < nf; f++) {
+            C->ubc[f].type = FALM_BC_REFLECT;
+            C->pbc[f] = FALM_PBC_NEUMANN;
+            C->uubc[f] = 1;
+        }
+        C->ubc[3].value[0] = c.lid_velocity;
+    }
+}
+
+template <class T>
+__global__ void k_fill3(T *u, size_t n, T a, T b, T c) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        u[t] = a;
+        u[t + n] = b;
+        u[t + 2 * n] = c;
+    }
+}
+
+template <class TS, class TD>
+__global__ void k_cast(const TS *s, TD *d, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        d[t] = (TD)s[t];
+}
+
+static falm_status case_init(falm_case *C) {
+    const falm_case_config &c = C->cfg;
+    hipStream_t s = C->stream;
+    const int gc = 2;
+    for (int m = 0; m < 3; m++) {
+        if (c.n[m] < 1 || !(c.h[m] > 0.0)) {
+            set_error("falm_case_create: bad grid");
+            return FALM_ERR_INVALID;
+        }
+        C->dom.shape[m] = c.n[m] + 2 * gc;
+        C->dom.offset[m] = 0;
+        C->dom.gshape[m] = C->dom.shape[m];
+    }
+    C->dom.gc = gc;
+    if (c.kind == FALM_CASE_LID2D && c.n[2] != 1) {
+        set_error("falm_case_create: the 2-D cavity needs n[2] == 1");
+        return FALM_ERR_INVALID;
+    }
+    FALM_TRY(check_domain(&C->dom));
+    C->n = ncell_of(&C->dom);
+    C->prec = c.precision == FALM_CASE_MIXED ? FALM_F32 : FALM_F64;
+    C->tb = prec_bytes(C->prec);
+    const size_t n = C->n, tb = C->tb;
+    FALM_TRY(dalloc(C, &C->u, 3 * n * tb));
+    FALM_TRY(dalloc(C, &C->uprev, 3 * n * tb));
+    FALM_TRY(dalloc(C, &C->uu, 3 * n * tb));
+    FALM_TRY(dalloc(C, &C->nut, n * tb));
+    FALM_TRY(dalloc(C, &C->ff, 3 * n * tb));
+    FALM_TRY(dalloc(C, &C->div, n * tb));
+    FALM_TRY(dalloc(C, &C->xyz, 3 * n * tb));
+    FALM_TRY(dalloc(C, &C->kx, 3 * n * tb));
+    FALM_TRY(dalloc(C, &C->g, 3 * n * tb));
+    FALM_TRY(dalloc(C, &C->ja, n * tb));
+    FALM_TRY(dalloc(C, (void **)&C->p, n * sizeof(double)));
+    FALM_TRY(dalloc(C, (void **)&C->rhs, n * sizeof(double)));
+    FALM_TRY(dalloc(C, (void **)&C->res, n * sizeof(double)));
+    FALM_TRY(dalloc(C, (void **)&C->a, 7 * n * sizeof(double)));
+    FALM_TRY(dalloc(C, (void **)&C->dnorm, 8 * sizeof(double)));
+    FALM_TRY(dalloc(C, (void **)&C->ticket, 64));
+    Tiling tl = make_tiling(interior_box(make_geom(&C->dom)));
+    FALM_TRY(dalloc(C, (void **)&C->partials, (size_t)(tl.tiles + 1) * sizeof(double)));
+    FALM_HIP(hipHostMalloc(&C->hnorm, sizeof(double), hipHostMallocDefault));
+    // uniform control-volume mesh: cell centres and widths of every padded
+    // cell (what Mesher::build_mesh + readControlVolumeFile produce for a
+    // uniform node list, src/mesher/mesher.hpp, src/FalmIO.h:99-116)
+    std::vector<double> *cen[3] = {&C->x, &C->y, &C->z}, *wid[3] = {&C->hx, &C->hy, &C->hz};
+    for (int m = 0; m < 3; m++) {
+        cen[m]->resize(C->dom.shape[m]);
+        wid[m]->resize(C->dom.shape[m]);
+        for (int i = 0; i < C->dom.shape[m]; i++) {
+            (*cen[m])[i] = c.origin[m] + (i - gc + 0.5) * c.h[m];
+            (*wid[m])[i] = c.h[m];
+        }
+    }
+    FALM_TRY(falm_metrics(&C->dom, C->prec, C->x.data(), C->y.data(), C->z.data(), C->hx.data(), C->hy.data(),
+                          C->hz.data(), C->xyz, C->kx, C->g, C->ja, s));
+    // Poisson matrix from double metrics
+    int kind = c.kind == FALM_CASE_WINDTUNNEL ? FALM_POISSON_OUTLET_XPLUS
+             : c.kind == FALM_CASE_LID2D     ? FALM_POISSON_2D
+                                             : FALM_POISSON_NEUMANN;
+    if (C->prec == FALM_F64) {
+        FALM_TRY(falm_poisson_matrix(&C->dom, FALM_F64, kind, C->g, C->ja, C->a, &C->maxdiag, s));
+    } else {
+        void *x64, *k64, *g64, *j64;
+        FALM_HIP(hipMalloc(&x64, 3 * n * 8));
+        FALM_HIP(hipMalloc(&k64, 3 * n * 8));
+        FALM_HIP(hipMalloc(&g64, 3 * n * 8));
+        FALM_HIP(hipMalloc(&j64, n * 8));
+        falm_status st = falm_metrics(&C->dom, FALM_F64, C->x.data(), C->y.data(), C->z.data(), C->hx.data(),
+                                      C->hy.data(), C->hz.data(), x64, k64, g64, j64, s);
+        if (st == FALM_OK) st = falm_poisson_matrix(&C->dom, FALM_F64, kind, g64, j64, C->a, &C->maxdiag, s);
+        FALM_HIP(hipStreamSynchronize(s));
+        (void)hipFree(x64);
+        (void)hipFree(k64);
+        (void)hipFree(g64);
+        (void)hipFree(j64);
+        FALM_TRY(st);
+    }
+    // initial field u = (u_inflow, 0, 0) in every cell, then UtoUU and SGS
+    // (app/windtunnel/main.cpp:120-129)
+    if (C->prec == FALM_F32)
+        hipLaunchKernelGGL((k_fill3<float>), ew_grid(n), dim3(EW_BLOCK), 0, s, (float *)C->u, n, (float)c.u_inflow,
+                           0.f, 0.f);
+    else
+        hipLaunchKernelGGL((k_fill3<double>), ew_grid(n), dim3(EW_BLOCK), 0, s, (double *)C->u, n, c.u_inflow, 0.0,
+                           0.0);
+    FALM_CHECK_LAUNCH("k_fill3");
+    FALM_TRY(falm_u_to_uu(&C->dom, C->prec, C->u, C->uu, C->kx, C->ja, s));
+    FALM_TRY(falm_sgs(&C->dom, C->prec, c.sgs_model, c.Cs, C->u, C->nut, C->kx, C->ja, s));
+    FALM_TRY(falm_solver_create(&C->dom, &c.solver, &C->solver));
+    FALM_TRY(falm_solver_set_matrix(C->solver, C->a, s));
+    if (c.alm) FALM_TRY(falm_alm_create(&C->dom, c.alm, C->x.data(), C->y.data(), C->z.data(), &C->alm));
+    case_faces(C);
+    FALM_HIP(hipStreamSynchronize(s));
+    return FALM_OK;
+}
+
+static falm_status case_step(falm_case *C, falm_solve_info *si) {
+    const falm_case_config &c = C->cfg;
+    hipStream_t s = C->stream;
+    const falm_domain *d = &C->dom;
+    const int pr = C->prec;
+    const bool is2d = c.kind == FALM_CASE_LID2D;
+    C->it++;
+    const double t = c.dt * (double)C->it;
+    FALM_HIP(hipMemcpyAsync(C->uprev, C->u, 3 * C->n * C->tb, hipMemcpyDeviceToDevice, s));
+    if (C->alm) FALM_TRY(falm_alm_step(C->alm, pr, C->u, C->ff, t, s));
+    FALM_TRY(falm_pseudo_u(d, pr, c.adv_scheme, 1.0 / c.Re, c.dt, C->uprev, C->uprev, C->uu, C->u, C->nut, C->kx,
+                           C->g, C->ja, C->ff, s));
+    FALM_TRY(falm_u_to_uu(d, pr, C->u, C->uu, C->kx, C->ja, s));
+    if (c.kind != FALM_CASE_WINDTUNNEL) FALM_TRY(launch_bc_face_flux(d, pr, C->uubc, C->uu, s));
+    FALM_TRY(launch_poisson_rhs_mixed(d, pr, C->uu, C->rhs, C->ja, 1.0 / (c.dt * C->maxdiag), s));
+    FALM_TRY(falm_solver_solve(C->solver, C->a, C->p, C->rhs, C->res, s, si));
+    FALM_TRY(launch_bc_pressure(d, FALM_F64, C->pbc, C->p, s));
+    if (is2d) FALM_TRY(launch_copy_z5(d, FALM_F64, C->p, 1, s));
+    FALM_TRY(launch_project_mixed(d, pr, c.dt, C->u, C->u, C->uu, C->uu, C->p, C->kx, C->g, s));
+    falm_inlet_noise nz{c.inlet_intensity, c.seed, C->it};
+    FALM_TRY(launch_bc_velocity(d, pr, C->ubc, C->u, C->uprev, C->xyz, c.dt, &nz, s));
+    if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->u, 3, s));
+    FALM_TRY(falm_sgs(d, pr, c.sgs_model, c.Cs, C->u, C->nut, C->kx, C->ja, s));
+    if (is2d && c.sgs_model != FALM_SGS_EMPTY) FALM_TRY(launch_copy_z5(d, pr, C->nut, 1, s));
+    FALM_TRY(launch_divergence_mixed(d, pr, C->uu, C->div, C->ja, s));
+    FALM_TRY(launch_dot(d, pr, C->div, C->div, 1, true, C->dnorm, C->partials, C->ticket, s));
+    return FALM_OK;
+}
+
+static void *case_field(falm_case *C, int field, int &ncol, int &prec) {
+    ncol = 1;
+    prec = C->prec;
+    switch (field) {
+    case FALM_FIELD_U: ncol = 3; return C->u;
+    case FALM_FIELD_UU: ncol = 3; return C->uu;
+    case FALM_FIELD_P: prec = FALM_F64; return C->p;
+    case FALM_FIELD_NUT: return C->nut;
+    case FALM_FIELD_FF: ncol = 3; return C->ff;
+    case FALM_FIELD_DIV: return C->div;
+    case FALM_FIELD_RHS: prec = FALM_F64; return C->rhs;
+    case FALM_FIELD_POI_A: ncol = 7; prec = FALM_F64; return C->a;
+    case FALM_FIELD_XYZ: ncol = 3; return C->xyz;
+    case FALM_FIELD_KX: ncol = 3; return C->kx;
+    case FALM_FIELD_G: ncol = 3; return C->g;
+    case FALM_FIELD_JA: return C->ja;
+    default: return nullptr;
+    }
+}
+
+}  // namespace falm
+
+extern "C" {
+
+falm_status falm_case_destroy(falm_case *C) {
+    if (!C) return FALM_OK;
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
+    if (C->solver) falm_solver_destroy(C->solver);
+    if (C->alm) falm_alm_destroy(C->alm);
+    for (void *p : C->bufs) (void)hipFree(p);
+    if (C->hnorm) (void)hipHostFree(C->hnorm);
+    if (C->stream) (void)hipStreamDestroy(C->stream);
+    delete C;
+    return FALM_OK;
+}
+
+falm_status falm_case_create(const falm_case_config *cfg, falm_case **out) {
+    if (!cfg || !out) return FALM_ERR_INVALID;
+    *out = nullptr;
+    if (cfg->kind < FALM_CASE_WINDTUNNEL || cfg->kind > FALM_CASE_LID2D) return FALM_ERR_INVALID;
+    if (cfg->precision != FALM_CASE_F64 && cfg->precision != FALM_CASE_MIXED) return FALM_ERR_UNSUPPORTED;
+    falm_case *C = new falm_case();
+    C->cfg = *cfg;
+    C->cfg.alm = cfg->alm;  // read during create only
+    hipError_t e = hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        falm_status st = hip_status(e, "hipStreamCreate");
+        delete C;
+        return st;
+    }
+    falm_status st = case_init(C);
+    C->cfg.alm = nullptr;
+    if (st != FALM_OK) {
+        falm_case_destroy(C);
+        return st;
+    }
+    *out = C;
+    return FALM_OK;
+}
+
+falm_status falm_case_step(falm_case *C, int nsteps, falm_step_info *info) {
+    if (!C || nsteps < 0) return FALM_ERR_INVALID;
+    falm_solve_info si{0, 0.0, 0, FALM_OK};
+    for (int k = 0; k < nsteps; k++) FALM_TRY(case_step(C, &si));
+    FALM_HIP(hipMemcpyAsync(C->hnorm, C->dnorm, sizeof(double), hipMemcpyDeviceToHost, C->stream));
+    FALM_HIP(hipStreamSynchronize(C->stream));
+    if (info) {
+        const falm_domain &d = C->dom;
+        const double nint = (double)(d.shape[0] - 2 * d.gc) * (double)(d.shape[1] - 2 * d.gc) *
+                            (double)(d.shape[2] - 2 * d.gc);
+        info->step = C->it;
+        info->t = C->cfg.dt * (double)C->it;
+        info->div = sqrt(C->hnorm[0]) / nint;  // app/windtunnel/main.cpp:185
+        info->solver_it = si.it;
+        info->solver_err = si.err;
+        info->outer_it = si.outer_it;
+        info->status = si.status;
+        if (!std::isfinite(info->div)) info->status = FALM_ERR_NONFINITE;
+    }
+    return FALM_OK;
+}
+
+falm_status falm_case_get_field(falm_case *C, int field, double *host_out, size_t n) {
+    if (!C || !host_out) return FALM_ERR_INVALID;
+    int ncol, prec;
+    void *src = case_field(C, field, ncol, prec);
+    if (!src || n != (size_t)ncol * C->n) {
+        set_error("falm_case_get_field: bad field or size");
+        return FALM_ERR_INVALID;
+    }
+    FALM_HIP(hipStreamSynchronize(C->stream));
+    if (prec == FALM_F64) {
+        FALM_HIP(hipMemcpy(host_out, src, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        FALM_HIP(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; q++) host_out[q] = (double)tmp[q];
+    }
+    return FALM_OK;
+}
+
+falm_status falm_case_set_field(falm_case *C, int field, const double *host_in, size_t n) {
+    if (!C || !host_in) return FALM_ERR_INVALID;
+    int ncol, prec;
+    void *dst = case_field(C, field, ncol, prec);
+    if (!dst || n != (size_t)ncol * C->n) {
+        set_error("falm_case_set_field: bad field or size");
+        return FALM_ERR_INVALID;
+    }
+    FALM_HIP(hipStreamSynchronize(C->stream));
+    if (prec == FALM_F64) {
+        FALM_HIP(hipMemcpy(dst, host_in, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t q = 0; q < n; q++) tmp[q] = (float)host_in[q];
+        FALM_HIP(hipMemcpy(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (field == FALM_FIELD_POI_A) FALM_TRY(falm_solver_set_matrix(C->solver, C->a, C->stream));
+    FALM_HIP(hipStreamSynchronize(C->stream));
+    return FALM_OK;
+}
+
+falm_status falm_case_domain(falm_case *C, falm_domain *dom) {
+    if (!C || !dom) return FALM_ERR_INVALID;
+    *dom = C->dom;
+    return FALM_OK;
+}
+
+falm_status falm_case_alm(falm_case *C, falm_alm **alm) {
+    if (!C || !alm) return FALM_ERR_INVALID;
+    *alm = C->alm;
+    return FALM_OK;
+}
+
+}  // extern "C"

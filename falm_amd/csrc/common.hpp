@@ -1,0 +1,266 @@
+// common.hpp — shared host/device helpers of the gfx950 library.
+//
+// Layout: the reference's Matrix layout (src/matrix.h:20), one column per
+// component, x fastest over the padded local shape.  Kernels receive one
+// pointer per column so that no 64-bit column arithmetic happens on device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/falm_amd.h"
+
+namespace falm {
+
+// ------------------------------------------------------------------ errors
+void set_error(const char *fmt, ...);
+falm_status hip_status(hipError_t e, const char *what);
+
+#define FALM_HIP(expr)                                                      \
+    do {                                                                    \
+        hipError_t _e = (expr);                                             \
+        if (_e != hipSuccess) return ::falm::hip_status(_e, #expr);         \
+    } while (0)
+
+#define FALM_TRY(expr)                                                      \
+    do {                                                                    \
+        falm_status _s = (expr);                                            \
+        if (_s != FALM_OK) return _s;                                       \
+    } while (0)
+
+#define FALM_CHECK_LAUNCH(name) FALM_HIP(hipGetLastError())
+
+// ----------------------------------------------------------------- domain
+// Device-side view of a falm_domain (src/region.h:8-31 + CPM::gc).
+struct Geom {
+    int nx, ny, nz;     // padded local shape
+    int gc;
+    int osum;           // sum of pdm.offset: parity term of the RB colour
+    int nxy;            // nx*ny
+    __host__ __device__ inline int id(int i, int j, int k) const { return i + nx * j + nxy * k; }
+};
+
+inline Geom make_geom(const falm_domain *d) {
+    Geom g;
+    g.nx = d->shape[0];
+    g.ny = d->shape[1];
+    g.nz = d->shape[2];
+    g.gc = d->gc;
+    g.osum = d->offset[0] + d->offset[1] + d->offset[2];
+    g.nxy = g.nx * g.ny;
+    return g;
+}
+
+inline size_t ncell_of(const falm_domain *d) {
+    return (size_t)d->shape[0] * (size_t)d->shape[1] * (size_t)d->shape[2];
+}
+
+falm_status check_domain(const falm_domain *d);
+
+// A box of cells [lo, lo+n) in padded local indices.
+struct Box {
+    int lo[3];
+    int n[3];
+    __host__ __device__ inline long long size() const { return (long long)n[0] * n[1] * n[2]; }
+};
+
+inline Box interior_box(const Geom &g) {
+    return Box{{g.gc, g.gc, g.gc}, {g.nx - 2 * g.gc, g.ny - 2 * g.gc, g.nz - 2 * g.gc}};
+}
+// Region(pdm.shape, gc) with shape += 1 and offset -= 1 (src/FalmCFD.cpp:119-126).
+inline Box face_box(const Geom &g) {
+    return Box{{g.gc - 1, g.gc - 1, g.gc - 1},
+               {g.nx - 2 * g.gc + 1, g.ny - 2 * g.gc + 1, g.nz - 2 * g.gc + 1}};
+}
+
+// ----------------------------------------------------------- launch shape
+// Stencil launches use 64x4 threads (one wavefront per x row segment) over a
+// 1-D grid.  The 1-D block id is remapped so that each of the 8 XCDs of an
+// MI355X (workgroups are dealt to XCDs round-robin) gets one contiguous
+// slab of tiles: neighbouring tiles then share that XCD's L2.
+constexpr int TX = 64;
+constexpr int TY = 4;
+constexpr int NXCD = 8;
+
+struct Tiling {
+    Box box;
+    int tx, ty;          // tiles along x, y
+    int tiles;           // tx*ty*box.n[2]
+    __host__ __device__ inline int ntiles() const { return tiles; }
+};
+
+inline Tiling make_tiling(const Box &b) {
+    Tiling t;
+    t.box = b;
+    t.tx = (b.n[0] + TX - 1) / TX;
+    t.ty = (b.n[1] + TY - 1) / TY;
+    t.tiles = t.tx * t.ty * b.n[2];
+    return t;
+}
+
+// physical block id -> logical tile id, XCD-contiguous
+__device__ inline int xcd_tile(int bid, int nblocks) {
+    int per = (nblocks + NXCD - 1) / NXCD;
+    int xcd = bid % NXCD;
+    int loc = bid / NXCD;
+    int full = nblocks % NXCD;          // XCDs that receive `per` blocks
+    // blocks are dealt round robin: XCDs [0, full) get `per` blocks and the
+    // rest get per-1 when nblocks is not a multiple of NXCD
+    if (full == 0) return xcd * per + loc;
+    if (xcd < full) return xcd * per + loc;
+    return full * per + (xcd - full) * (per - 1) + loc;
+}
+
+// cell coordinates of this thread; returns false outside the box
+__device__ inline bool tile_cell(const Tiling &t, int &i, int &j, int &k) {
+    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int txy = t.tx * t.ty;
+    k = tile / txy;
+    int r = tile - k * txy;
+    int by = r / t.tx;
+    int bx = r - by * t.tx;
+    i = bx * TX + (int)(threadIdx.x & (TX - 1));
+    j = by * TY + (int)(threadIdx.x >> 6);
+    if (i >= t.box.n[0] || j >= t.box.n[1]) return false;
+    i += t.box.lo[0];
+    j += t.box.lo[1];
+    k += t.box.lo[2];
+    return true;
+}
+
+inline dim3 tiling_grid(const Tiling &t) { return dim3((unsigned)t.tiles); }
+inline dim3 tiling_block() { return dim3(TX * TY); }
+
+// 1-D element-wise launches
+constexpr int EW_BLOCK = 256;
+inline dim3 ew_grid(size_t n) {
+    size_t b = (n + EW_BLOCK - 1) / EW_BLOCK;
+    if (b > 2147483647ull) b = 2147483647ull;
+    return dim3((unsigned)b);
+}
+
+// ------------------------------------------------------------- reductions
+// Wavefront (64 lanes) then block reduction of doubles.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+template <int NT>
+__device__ inline double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < NT / 64; w++) r += lds[w];
+    }
+    __syncthreads();
+    return r;  // valid in thread 0
+}
+
+// Multi-value block sum (NV values per thread), result valid in thread 0.
+template <int NT, int NV>
+__device__ inline void block_sum_n(double (&v)[NV], double *lds) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double x = wave_sum(v[q]);
+        if (lane == 0) lds[q * (NT / 64) + wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double r = 0.0;
+#pragma unroll
+            for (int w = 0; w < NT / 64; w++) r += lds[q * (NT / 64) + w];
+            v[q] = r;
+        }
+    }
+    __syncthreads();
+}
+
+// Deterministic grid reduction: every block writes its partials, the last
+// block to finish (atomic ticket) sums all partials in block order.
+// Returns true in the last block (all threads), with the totals in `tot`
+// (valid in every thread of that block).
+template <int NT, int NV>
+__device__ inline bool grid_sum_last(double (&v)[NV], double *partials, unsigned *ticket,
+                                     double (&tot)[NV], double *lds) {
+    block_sum_n<NT, NV>(v, lds);
+    __shared__ bool last;
+    const int nb = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) partials[(size_t)q * nb + blockIdx.x] = v[q];
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        last = (t == (unsigned)nb - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    // fixed-order sum of the partials: each thread a strided chunk, then a
+    // fixed tree across threads
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += NT) s += partials[(size_t)q * nb + b];
+        acc[q] = s;
+    }
+    block_sum_n<NT, NV>(acc, lds);
+    __shared__ double bc[NV];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) bc[q] = acc[q];
+        *ticket = 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; q++) tot[q] = bc[q];
+    return true;
+}
+
+// ------------------------------------------------------------ precision
+template <class T> struct PrecOf;
+template <> struct PrecOf<double> { static constexpr int value = FALM_F64; };
+template <> struct PrecOf<float> { static constexpr int value = FALM_F32; };
+
+inline size_t prec_bytes(int prec) { return prec == FALM_F32 ? 4 : 8; }
+
+// column pointer helper
+template <class T> inline T *col(void *p, size_t ncell, int c) { return (T *)p + (size_t)c * ncell; }
+template <class T> inline const T *col(const void *p, size_t ncell, int c) {
+    return (const T *)p + (size_t)c * ncell;
+}
+
+// -------------------------------------------------------------- scratch
+// Library-owned device scratch for one-shot reductions (per device).
+struct ReduceScratch {
+    double *partials = nullptr;  // 8 * max_blocks doubles
+    unsigned *ticket = nullptr;
+    double *result = nullptr;    // 8 doubles
+    double *host = nullptr;      // pinned host mirror of result
+    size_t max_blocks = 0;
+};
+falm_status reduce_scratch(size_t nblocks, ReduceScratch **out);
+
+// ------------------------------------------------------------ timing
+struct TimedLaunch {
+    int family;
+    hipStream_t stream;
+    bool on;
+    TimedLaunch(int fam, hipStream_t s);
+    void done(double bytes);
+};
+
+}  // namespace falm

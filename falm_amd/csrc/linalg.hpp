@@ -1,0 +1,49 @@
+// linalg.hpp — 7-point stencil helpers and launch declarations shared by
+// linalg.hip, solver.hip and the step driver.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace falm {
+
+// The 7 coefficient columns [c,w,e,s,n,b,t] (src/dev/MVDevCall.cu:25-31).
+template <class TA>
+struct Mat7 {
+    const TA *__restrict__ c[7];
+};
+
+template <class TA>
+inline Mat7<TA> mat7(const void *a, size_t ncell) {
+    Mat7<TA> m;
+    for (int q = 0; q < 7; q++) m.c[q] = col<TA>(a, ncell, q);
+    return m;
+}
+
+// ac*xc + ae*xe + aw*xw + an*xn + as*xs + at*xt + ab*xb, summed in the
+// reference's order (src/dev/MVDevCall.cu:39).
+template <class TA, class TX, class R>
+__device__ inline R apply7(const Mat7<TA> &a, const TX *__restrict__ x, int cc, int sy, int sz) {
+    const R ac = (R)a.c[0][cc], aw = (R)a.c[1][cc], ae = (R)a.c[2][cc], as = (R)a.c[3][cc], an = (R)a.c[4][cc],
+            ab = (R)a.c[5][cc], at = (R)a.c[6][cc];
+    const R xc = (R)x[cc], xe = (R)x[cc + 1], xw = (R)x[cc - 1], xn = (R)x[cc + sy], xs = (R)x[cc - sy],
+            xt = (R)x[cc + sz], xb = (R)x[cc - sz];
+    return ac * xc + ae * xe + aw * xw + an * xn + as * xs + at * xt + ab * xb;
+}
+
+template <class T>
+falm_status launch_sor(const falm_domain *dom, const void *a, void *x, const void *b, double omega, int color,
+                       const int *done, hipStream_t s);
+template <class T>
+falm_status launch_jacobi(const falm_domain *dom, const void *a, void *x, const void *xp, const void *b,
+                          const int *done, hipStream_t s);
+template <class T>
+falm_status launch_residual(const falm_domain *dom, const void *a, const void *x, const void *b, void *r,
+                            hipStream_t s);
+falm_status launch_dot(const falm_domain *dom, int prec, const void *a, const void *b, int ncol, bool norm,
+                       double *dev_out, double *partials, unsigned *ticket, hipStream_t s);
+falm_status host_minmax(const falm_domain *dom, int prec, int op, const void *a, int ncol, int col, double *out,
+                        hipStream_t s);
+
+}  // namespace falm

@@ -1,0 +1,111 @@
+"""Whole time steps (the apps' main loops) on the GPU against the CPU oracle,
+FP64 vs mixed precision, determinism, and size-independent properties of the
+512x128x128 rotor case."""
+import numpy as np
+import pytest
+
+from falm_amd import _abi as abi
+from falm_amd import cases
+from oracle import pyoracle as O
+from tests.helpers import interior_mask, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_pair(setup, steps):
+    ref = O.Case(setup)
+    from falm_amd import runtime as rt
+    dev = rt.Case(setup)
+    recs = []
+    for s in range(steps):
+        ir = ref.step(1)
+        ig = dev.step(1)
+        recs.append((ir.solver_it, ig.solver_it, ir.div, ig.div))
+    return ref, dev, recs
+
+
+def _check(ref, dev, recs, tol):
+    m3 = interior_mask(ref.dom, 3)
+    for field, mask in ((abi.FIELD_U, m3), (abi.FIELD_P, interior_mask(ref.dom)), (abi.FIELD_NUT, None),
+                        (abi.FIELD_UU, None)):
+        a, b = dev.get(field), ref.get(field)
+        err = rel_err(a, b, mask)
+        assert err < tol, (field, err, recs)
+    for ir, ig, dr, dg in recs:
+        assert abs(ir - ig) <= 1, recs
+
+
+def test_lid2d_steps(gpu):
+    setup = cases.lid2d(n=32)
+    ref, dev, recs = _run_pair(setup, 6)
+    _check(ref, dev, recs, 1e-9)
+
+
+def test_lid3d_steps(gpu):
+    setup = cases.lid3d(n=20, dt=2e-3, tol=1e-8)
+    ref, dev, recs = _run_pair(setup, 5)
+    _check(ref, dev, recs, 1e-9)
+
+
+def test_windtunnel_rotor_steps(gpu):
+    # a tight Poisson tolerance makes the pressure a property of the system
+    # rather than of the iteration at which BiCGStab stops (both sides stop
+    # at the same iteration, but reordered dot products drift by ~1e-8 over
+    # 30 iterations)
+    setup = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, tol=1e-12, rotor_kw=dict(n_ap_per_blade=8))
+    ref, dev, recs = _run_pair(setup, 4)
+    _check(ref, dev, recs, 1e-10)
+    xyz, f, tq, th = dev.alm_get()
+    xyz_r, f_r, tq_r, th_r = ref.alm_get()
+    assert rel_err(f, f_r) < 1e-9 and rel_err(tq, tq_r) < 1e-9
+    assert rel_err(dev.get(abi.FIELD_FF), ref.get(abi.FIELD_FF)) < 1e-9
+
+
+def test_deterministic_rerun(gpu):
+    from falm_amd import runtime as rt
+    setup = cases.windtunnel(n=(48, 16, 16), cells_per_d=8, rotor_kw=dict(n_ap_per_blade=8))
+    out = []
+    for _ in range(2):
+        c = rt.Case(setup)
+        c.step(3)
+        out.append((c.get(abi.FIELD_U), c.get(abi.FIELD_P)))
+        c.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("tol,bar", [(1e-6, 2e-3), (1e-10, 1e-5)])
+def test_mixed_tracks_fp64(gpu, tol, bar):
+    """Mixed precision against FP64 on the GPU after 10 steps.  At the
+    benchmark tolerance (1e-6) the difference is dominated by where each
+    solver stops; at 1e-10 it isolates the float stencils."""
+    from falm_amd import runtime as rt
+    s64 = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, rotor_kw=dict(n_ap_per_blade=8), tol=tol)
+    smx = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, rotor_kw=dict(n_ap_per_blade=8),
+                           precision=abi.CASE_MIXED, tol=tol)
+    a, b = rt.Case(s64), rt.Case(smx)
+    for _ in range(10):
+        ia = a.step(1)
+        ib = b.step(1)
+        assert ib.solver_err <= tol and ib.status == abi.FALM_OK
+    m3 = interior_mask(a.dom, 3)
+    ua, ub = a.get(abi.FIELD_U)[m3], b.get(abi.FIELD_U)[m3]
+    rel = np.linalg.norm(ua - ub) / np.linalg.norm(ua)
+    print(f"mixed vs fp64 rel L2 (tol {tol}): {rel:.3e}")
+    assert rel < bar
+
+
+@pytest.mark.parametrize("precision", [abi.CASE_F64, abi.CASE_MIXED])
+def test_full_size_rotor_case(gpu, precision):
+    """512x128x128 + rotor, 3 steps: finite, solver converged to 1e-6,
+    divergence at the solver tolerance's level, rotor thrust positive."""
+    from falm_amd import runtime as rt
+    setup = cases.windtunnel(precision=precision)
+    c = rt.Case(setup)
+    info = None
+    for _ in range(3):
+        info = c.step(1)
+        assert info.status == abi.FALM_OK, info.status
+        assert info.solver_err <= 1e-6
+        assert np.isfinite(info.div) and info.div < 1e-5
+    _, _, tq, th = c.alm_get()
+    assert th[0] > 0 and np.isfinite(tq[0])
