@@ -53,7 +53,25 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_examples(verbose)
     return LIB
+
+
+EXAMPLES = {"falm_windtunnel": "windtunnel.cpp"}
+
+
+def build_examples(verbose: bool = False) -> None:
+    """C++ applications written against include/falm/*.h, linked to the library."""
+    root = os.path.dirname(HERE)
+    for exe, src in EXAMPLES.items():
+        out = os.path.join(LIBDIR, exe)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"),
+               os.path.join(HERE, "examples", src), "-L", LIBDIR, "-lfalm_amd", "-Wl,-rpath,$ORIGIN",
+               "-o", out + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

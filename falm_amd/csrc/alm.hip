@@ -3,13 +3,15 @@
 // k_alm_points: one thread per actuator point does what the reference does
 //   in three launches (kernel_UpdateTurbineAngles, kernel_UpdateAPX,
 //   kernel_CalcAPForce, src/alm/almDevCall.cu:43-253).
-// k_alm_project: the Gaussian projection (kernel_DistributeAPForce,
-//   :255-294) as a cell-major gather.  Each 64x4 tile first culls, in LDS,
-//   the points whose kernel value is exactly zero in double precision over
-//   the whole tile (exp underflows to 0 for r^2/eps^2 > 746), so the sum over
-//   the remaining points in ascending id order is bit-identical to the
-//   all-pairs sum; a finite cutoff (radius in units of eps) trades that for
-//   speed.
+// k_alm_factors + k_alm_project: the Gaussian projection
+//   (kernel_DistributeAPForce, :255-294) as a cell-major gather.  The kernel
+//   is separable on the tensor-product grid, exp(-r^2/eps^2) =
+//   ex(x_i) ey(y_j) ez(z_k), so each point's three 1-D factor rows are
+//   evaluated once per step and a cell-point pair costs two products
+//   instead of an exp.  Each 64x4 tile first culls, in LDS, the points whose
+//   kernel value is exactly zero in double precision over the whole tile
+//   (r^2/eps^2 > 746), keeping the survivors in ascending id order; a finite
+//   cutoff (radius in units of eps) culls harder.
 // k_alm_totals: per-turbine torque/thrust in a fixed order (the reference
 //   uses thrust::reduce, src/alm/almDevCall.cu:362-370), kept on device.
 #include <cmath>
@@ -36,7 +38,32 @@ struct AlmDev {
     const double *x, *y, *z;
     double *ap_xyz, *ap_force, *ap_torque, *ap_thrust, *torque, *thrust;
     int *ap_ijk, *ap_rank;
+    double *fx, *fy, *fz;  // per-point axis factors exp(-(x_i - X)^2/eps^2): [nap][nx], [nap][ny], [nap][nz]
 };
+
+// per-point, per-axis Gaussian factors of the projection
+__global__ void k_alm_factors(AlmDev A) {
+    const int L = A.nx + A.ny + A.nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nap * L) return;
+    const int ap = t / L;
+    int r = t - ap * L;
+    const double eps2 = A.eps * A.eps;
+    double v;
+    double *out;
+    if (r < A.nx) {
+        v = A.x[r] - A.ap_xyz[3 * ap];
+        out = A.fx + (size_t)ap * A.nx + r;
+    } else if ((r -= A.nx) < A.ny) {
+        v = A.y[r] - A.ap_xyz[3 * ap + 1];
+        out = A.fy + (size_t)ap * A.ny + r;
+    } else {
+        r -= A.ny;
+        v = A.z[r] - A.ap_xyz[3 * ap + 2];
+        out = A.fz + (size_t)ap * A.nz + r;
+    }
+    *out = exp(-(v * v) / eps2);
+}
 
 // one_angle_frame_rotation, src/falmath.h:55-81
 __device__ inline void rot(const double x[3], const double ang[3], int type, double out[3]) {
@@ -73,13 +100,19 @@ __device__ inline void rot_dt(const double x[3], const double v[3], const double
     }
 }
 
-// find_index, src/alm/almDevCall.cu:12-26
+// find_index, src/alm/almDevCall.cu:12-26: the first i with
+// seq[i] <= v < seq[i+1].  The cell-centre arrays are strictly increasing,
+// so a bisection finds the same i as the reference's linear scan.
 __device__ inline int find_index(const double *seq, double v, int size) {
     if (v < seq[0]) return -1;
-    if (v >= seq[size - 1]) return size - 1;
-    for (int i = 0; i < size - 1; i++)
-        if (seq[i] <= v && v < seq[i + 1]) return i;
-    return size - 1;
+    if (v >= seq[size - 1] || v != v) return size - 1;  // NaN: the scan finds nothing
+    int lo = 0, hi = size - 1;  // seq[lo] <= v < seq[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (seq[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 // trilinear_interpolation, src/alm/almDevCall.cu:28-41
@@ -207,9 +240,10 @@ constexpr int kCullChunk = 256;
 
 template <class T>
 __global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0, T *ff1, T *ff2, double cut2) {
-    __shared__ double sx[kCullChunk], sy[kCullChunk], sz[kCullChunk], sf0[kCullChunk], sf1[kCullChunk],
-        sf2[kCullChunk];
-    __shared__ int nkeep;
+    __shared__ double sf0[kCullChunk], sf1[kCullChunk], sf2[kCullChunk], sez[kCullChunk];
+    __shared__ double sey[TY][kCullChunk];
+    __shared__ int sap[kCullChunk];
+    __shared__ int nkeep, tj0, tjn, tk;
     __shared__ double box[6];
     int i, j, k;
     const bool valid = tile_cell(tl, i, j, k);
@@ -226,18 +260,16 @@ __global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0
         box[0] = fmin(A.x[ilo], A.x[ihi]); box[1] = fmax(A.x[ilo], A.x[ihi]);
         box[2] = fmin(A.y[jlo], A.y[jhi]); box[3] = fmax(A.y[jlo], A.y[jhi]);
         box[4] = A.z[kc]; box[5] = A.z[kc];
+        tj0 = jlo;
+        tjn = jhi - jlo + 1;
+        tk = kc;
     }
     __syncthreads();
     const double eps = A.eps;
     const double eps2 = eps * eps;
     const double sp = eps * sqrt(kPi);
     const double eta = 1. / (sp * sp * sp);
-    double cx = 0, cy = 0, cz = 0;
-    if (valid) {
-        cx = A.x[i];
-        cy = A.y[j];
-        cz = A.z[k];
-    }
+    const int jr = (int)(threadIdx.x >> 6);  // row of this thread within the tile
     __shared__ int wcount[256 / 64];
     double fx = 0., fy = 0., fz = 0.;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -245,11 +277,8 @@ __global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0
         // keep the point unless every cell of the tile is beyond the cutoff
         const int ap = base + (int)threadIdx.x;
         bool keep = false;
-        double px = 0, py = 0, pz = 0;
         if (ap < A.nap) {
-            px = A.ap_xyz[3 * ap];
-            py = A.ap_xyz[3 * ap + 1];
-            pz = A.ap_xyz[3 * ap + 2];
+            const double px = A.ap_xyz[3 * ap], py = A.ap_xyz[3 * ap + 1], pz = A.ap_xyz[3 * ap + 2];
             const double dx = fmax(0.0, fmax(box[0] - px, px - box[1]));
             const double dy = fmax(0.0, fmax(box[2] - py, py - box[3]));
             const double dz = fmax(0.0, fmax(box[4] - pz, pz - box[5]));
@@ -269,24 +298,23 @@ __global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0
         }
         if (keep) {
             const int slot = off + before;
-            sx[slot] = px;
-            sy[slot] = py;
-            sz[slot] = pz;
-            sf0[slot] = A.ap_force[3 * ap];
-            sf1[slot] = A.ap_force[3 * ap + 1];
-            sf2[slot] = A.ap_force[3 * ap + 2];
+            sap[slot] = ap;
+            sf0[slot] = A.ap_force[3 * ap] * eta;
+            sf1[slot] = A.ap_force[3 * ap + 1] * eta;
+            sf2[slot] = A.ap_force[3 * ap + 2] * eta;
+            sez[slot] = A.fz[(size_t)ap * A.nz + tk];
+            for (int q = 0; q < tjn; q++) sey[q][slot] = A.fy[(size_t)ap * A.ny + tj0 + q];
         }
         if (threadIdx.x == 0) nkeep = tot;
         __syncthreads();
         const int nk = nkeep;
         if (valid) {
             for (int q = 0; q < nk; q++) {
-                const double ddx = cx - sx[q], ddy = cy - sy[q], ddz = cz - sz[q];
-                const double rr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                const double e = exp(-rr2 / eps2);
-                fx += sf0[q] * eta * e;
-                fy += sf1[q] * eta * e;
-                fz += sf2[q] * eta * e;
+                // exp(-r^2/eps^2) = ex*ey*ez (per-axis factors of the point)
+                const double e = (A.fx[(size_t)sap[q] * A.nx + i] * sey[jr][q]) * sez[q];
+                fx += sf0[q] * e;
+                fy += sf1[q] * e;
+                fz += sf2[q] * e;
             }
         }
         __syncthreads();
@@ -418,7 +446,10 @@ falm_status falm_alm_create(const falm_domain *dom, const falm_alm_config *c, co
         (st = upload<double>(A, nullptr, d.nap, &d.ap_torque)) ||
         (st = upload<double>(A, nullptr, d.nap, &d.ap_thrust)) ||
         (st = upload<double>(A, nullptr, d.nt, &d.torque)) || (st = upload<double>(A, nullptr, d.nt, &d.thrust)) ||
-        (st = upload<int>(A, nullptr, 3 * d.nap, &d.ap_ijk)) || (st = upload<int>(A, nullptr, d.nap, &d.ap_rank))) {
+        (st = upload<int>(A, nullptr, 3 * d.nap, &d.ap_ijk)) || (st = upload<int>(A, nullptr, d.nap, &d.ap_rank)) ||
+        (st = upload<double>(A, nullptr, (size_t)d.nap * d.nx, &d.fx)) ||
+        (st = upload<double>(A, nullptr, (size_t)d.nap * d.ny, &d.fy)) ||
+        (st = upload<double>(A, nullptr, (size_t)d.nap * d.nz, &d.fz))) {
         falm_alm_destroy(A);
         return st;
     }
@@ -452,16 +483,19 @@ falm_status falm_alm_step(falm_alm *A, int prec, const void *u, void *ff, double
     Geom g = make_geom(&A->dom);
     Tiling tl = make_tiling(interior_box(g));
     TimedLaunch tm(FALM_TIME_ALM, s);
+    const int nfac = A->nap * (A->d.nx + A->d.ny + A->d.nz);
     if (prec == FALM_F32) {
         hipLaunchKernelGGL((k_alm_points<float>), dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d, col<float>(u, n, 0),
                            col<float>(u, n, 1), col<float>(u, n, 2), t);
         FALM_CHECK_LAUNCH("k_alm_points");
+        hipLaunchKernelGGL(k_alm_factors, dim3((nfac + 255) / 256), dim3(256), 0, s, A->d);
         hipLaunchKernelGGL((k_alm_project<float>), tiling_grid(tl), tiling_block(), 0, s, tl, A->d,
                            col<float>(ff, n, 0), col<float>(ff, n, 1), col<float>(ff, n, 2), A->cut2);
     } else {
         hipLaunchKernelGGL((k_alm_points<double>), dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d,
                            col<double>(u, n, 0), col<double>(u, n, 1), col<double>(u, n, 2), t);
         FALM_CHECK_LAUNCH("k_alm_points");
+        hipLaunchKernelGGL(k_alm_factors, dim3((nfac + 255) / 256), dim3(256), 0, s, A->d);
         hipLaunchKernelGGL((k_alm_project<double>), tiling_grid(tl), tiling_block(), 0, s, tl, A->d,
                            col<double>(ff, n, 0), col<double>(ff, n, 1), col<double>(ff, n, 2), A->cut2);
     }

@@ -54,7 +54,6 @@ struct falm_case {
     int pbc[6], uubc[6];
     // device scalar for the divergence norm + its reduction scratch
     double *dnorm = nullptr, *partials = nullptr, *hnorm = nullptr;
-    unsigned *ticket = nullptr;
     std::vector<void *> bufs;
 };
 
@@ -152,7 +151,6 @@ static falm_status case_init(falm_case *C) {
     FALM_TRY(dalloc(C, (void **)&C->res, n * sizeof(double)));
     FALM_TRY(dalloc(C, (void **)&C->a, 7 * n * sizeof(double)));
     FALM_TRY(dalloc(C, (void **)&C->dnorm, 8 * sizeof(double)));
-    FALM_TRY(dalloc(C, (void **)&C->ticket, 64));
     Tiling tl = make_tiling(interior_box(make_geom(&C->dom)));
     FALM_TRY(dalloc(C, (void **)&C->partials, (size_t)(tl.tiles + 1) * sizeof(double)));
     FALM_HIP(hipHostMalloc(&C->hnorm, sizeof(double), hipHostMallocDefault));
@@ -236,7 +234,7 @@ static falm_status case_step(falm_case *C, falm_solve_info *si) {
     FALM_TRY(falm_sgs(d, pr, c.sgs_model, c.Cs, C->u, C->nut, C->kx, C->ja, s));
     if (is2d && c.sgs_model != FALM_SGS_EMPTY) FALM_TRY(launch_copy_z5(d, pr, C->nut, 1, s));
     FALM_TRY(launch_divergence_mixed(d, pr, C->uu, C->div, C->ja, s));
-    FALM_TRY(launch_dot(d, pr, C->div, C->div, 1, true, C->dnorm, C->partials, C->ticket, s));
+    FALM_TRY(launch_dot(d, pr, C->div, C->div, 1, true, C->dnorm, C->partials, s));
     return FALM_OK;
 }
 

@@ -115,6 +115,35 @@ __device__ inline int xcd_tile(int bid, int nblocks) {
     return full * per + (xcd - full) * (per - 1) + loc;
 }
 
+// cell coordinates of this thread in tile `tile`; false outside the box
+__device__ inline bool tile_cell_at(const Tiling &t, int tile, int &i, int &j, int &k) {
+    int txy = t.tx * t.ty;
+    k = tile / txy;
+    int r = tile - k * txy;
+    int by = r / t.tx;
+    int bx = r - by * t.tx;
+    i = bx * TX + (int)(threadIdx.x & (TX - 1));
+    j = by * TY + (int)(threadIdx.x >> 6);
+    if (i >= t.box.n[0] || j >= t.box.n[1]) return false;
+    i += t.box.lo[0];
+    j += t.box.lo[1];
+    k += t.box.lo[2];
+    return true;
+}
+
+// Reduction kernels run at most RED_BLOCKS workgroups; each walks a
+// contiguous run of tiles (of the XCD-contiguous order), so the finalize
+// kernel sums at most RED_BLOCKS partials.
+constexpr int RED_BLOCKS = 4096;
+inline dim3 red_grid(const Tiling &t) { return dim3((unsigned)(t.tiles < RED_BLOCKS ? t.tiles : RED_BLOCKS)); }
+inline int red_blocks(const Tiling &t) { return t.tiles < RED_BLOCKS ? t.tiles : RED_BLOCKS; }
+__device__ inline void tile_range(int ntiles, int &t0, int &t1) {
+    const int L = xcd_tile(blockIdx.x, gridDim.x);
+    const int C = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    t0 = L * C;
+    t1 = t0 + C < ntiles ? t0 + C : ntiles;
+}
+
 // cell coordinates of this thread; returns false outside the box
 __device__ inline bool tile_cell(const Tiling &t, int &i, int &j, int &k) {
     int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -188,46 +217,45 @@ __device__ inline void block_sum_n(double (&v)[NV], double *lds) {
     __syncthreads();
 }
 
-// Deterministic grid reduction: every block writes its partials, the last
-// block to finish (atomic ticket) sums all partials in block order.
-// Returns true in the last block (all threads), with the totals in `tot`
-// (valid in every thread of that block).
+// Deterministic two-phase grid reduction.  Phase 1 (in the producing
+// kernel): each workgroup reduces its values and writes one partial per
+// value with a plain store.  Phase 2 (a separate one-workgroup kernel that
+// the stream orders after phase 1): the partials are summed in a fixed
+// order.  No device-scope fences or atomics are needed: a per-workgroup
+// release fence would write back the XCD's L2 in every one of tens of
+// thousands of workgroups.
 template <int NT, int NV>
-__device__ inline bool grid_sum_last(double (&v)[NV], double *partials, unsigned *ticket,
-                                     double (&tot)[NV], double *lds) {
+__device__ inline void block_partials(double (&v)[NV], double *partials, double *lds) {
     block_sum_n<NT, NV>(v, lds);
-    __shared__ bool last;
-    const int nb = gridDim.x;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NV; q++) partials[(size_t)q * nb + blockIdx.x] = v[q];
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        last = (t == (unsigned)nb - 1);
+        for (int q = 0; q < NV; q++) partials[(size_t)q * gridDim.x + blockIdx.x] = v[q];
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    // fixed-order sum of the partials: each thread a strided chunk, then a
-    // fixed tree across threads
+}
+
+constexpr int FIN_NT = 1024;  // threads of a finalize workgroup
+
+// Phase 2 helper for a FIN_NT-thread workgroup: totals of nb partials per
+// value, valid in every thread.
+template <int NV>
+__device__ inline void sum_partials(const double *partials, int nb, double (&tot)[NV]) {
+    __shared__ double lds[NV * (FIN_NT / 64)];
+    __shared__ double bc[NV];
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; q++) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += NT) s += partials[(size_t)q * nb + b];
+        for (int b = threadIdx.x; b < nb; b += FIN_NT) s += partials[(size_t)q * nb + b];
         acc[q] = s;
     }
-    block_sum_n<NT, NV>(acc, lds);
-    __shared__ double bc[NV];
+    block_sum_n<FIN_NT, NV>(acc, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; q++) bc[q] = acc[q];
-        *ticket = 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NV; q++) tot[q] = bc[q];
-    return true;
 }
 
 // ------------------------------------------------------------ precision
@@ -247,7 +275,6 @@ template <class T> inline const T *col(const void *p, size_t ncell, int c) {
 // Library-owned device scratch for one-shot reductions (per device).
 struct ReduceScratch {
     double *partials = nullptr;  // 8 * max_blocks doubles
-    unsigned *ticket = nullptr;
     double *result = nullptr;    // 8 doubles
     double *host = nullptr;      // pinned host mirror of result
     size_t max_blocks = 0;

@@ -30,12 +30,13 @@ __global__ __launch_bounds__(256) void k_residual(Tiling tl, Geom g, Mat7<TA> a,
 // products accumulated in double, result in out[0].
 template <class T, bool NORM>
 __global__ __launch_bounds__(256) void k_dot(Tiling tl, Geom g, const T *__restrict__ a, const T *__restrict__ b,
-                                             int ncol, size_t ncell, double *partials, unsigned *ticket,
-                                             double *out) {
+                                             int ncol, size_t ncell, double *partials) {
     __shared__ double lds[4];
-    int i, j, k;
+    int i, j, k, t0, t1;
     double v[1] = {0.0};
-    if (tile_cell(tl, i, j, k)) {
+    tile_range(tl.tiles, t0, t1);
+    for (int tile = t0; tile < t1; tile++) {
+        if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
         if (NORM) {
             for (int d = 0; d < ncol; d++) {
@@ -43,11 +44,16 @@ __global__ __launch_bounds__(256) void k_dot(Tiling tl, Geom g, const T *__restr
                 v[0] += x * x;
             }
         } else {
-            v[0] = (double)a[cc] * (double)b[cc];
+            v[0] += (double)a[cc] * (double)b[cc];
         }
     }
+    block_partials<256, 1>(v, partials, lds);
+}
+
+__global__ __launch_bounds__(FIN_NT) void k_fin_sum(const double *partials, int nb, double *out) {
     double tot[1];
-    if (grid_sum_last<256, 1>(v, partials, ticket, tot, lds) && threadIdx.x == 0) out[0] = tot[0];
+    sum_partials<1>(partials, nb, tot);
+    if (threadIdx.x == 0) out[0] = tot[0];
 }
 
 // per-block max/min partials (kernels src/dev/MVDevCall.cu:216-598)
@@ -209,7 +215,7 @@ template falm_status launch_residual<float>(const falm_domain *, const void *, c
 
 // dot or squared norm into a device scalar (async) and optionally a host value
 falm_status launch_dot(const falm_domain *dom, int prec, const void *a, const void *b, int ncol, bool norm,
-                       double *dev_out, double *partials, unsigned *ticket, hipStream_t s) {
+                       double *dev_out, double *partials, hipStream_t s) {
     Geom g = make_geom(dom);
     Tiling tl = make_tiling(interior_box(g));
     if (tl.tiles == 0) {
@@ -217,9 +223,9 @@ falm_status launch_dot(const falm_domain *dom, int prec, const void *a, const vo
         return FALM_OK;
     }
     size_t n = ncell_of(dom);
-#define FALM_DOT(T, N)                                                                                    \
-    hipLaunchKernelGGL((k_dot<T, N>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (const T *)a, (const T *)b, \
-                       ncol, n, partials, ticket, dev_out)
+#define FALM_DOT(T, N)                                                                                          \
+    hipLaunchKernelGGL((k_dot<T, N>), red_grid(tl), tiling_block(), 0, s, tl, g, (const T *)a, (const T *)b, ncol, \
+                       n, partials)
     if (prec == FALM_F32) {
         if (norm) FALM_DOT(float, true); else FALM_DOT(float, false);
     } else {
@@ -227,6 +233,8 @@ falm_status launch_dot(const falm_domain *dom, int prec, const void *a, const vo
     }
 #undef FALM_DOT
     FALM_CHECK_LAUNCH("k_dot");
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, red_blocks(tl), dev_out);
+    FALM_CHECK_LAUNCH("k_fin_sum");
     return FALM_OK;
 }
 
@@ -236,7 +244,7 @@ static falm_status host_scalar(const falm_domain *dom, int prec, const void *a, 
     Tiling tl = make_tiling(interior_box(g));
     ReduceScratch *rs;
     FALM_TRY(reduce_scratch((size_t)tl.tiles, &rs));
-    FALM_TRY(launch_dot(dom, prec, a, b, ncol, norm, rs->result, rs->partials, rs->ticket, s));
+    FALM_TRY(launch_dot(dom, prec, a, b, ncol, norm, rs->result, rs->partials, s));
     FALM_HIP(hipMemcpyAsync(rs->host, rs->result, sizeof(double), hipMemcpyDeviceToHost, s));
     FALM_HIP(hipStreamSynchronize(s));
     *out = rs->host[0];

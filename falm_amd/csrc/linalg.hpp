@@ -1,4 +1,4 @@
-// linalg.hpp — 7-point stencil helpers and launch declarations shared by
+// linalg.hpp — 7-point stencil operators and launch declarations shared by
 // linalg.hip, solver.hip and the step driver.
 #pragma once
 
@@ -32,6 +32,43 @@ __device__ inline R apply7(const Mat7<TA> &a, const TX *__restrict__ x, int cc, 
     return ac * xc + ae * xe + aw * xw + an * xn + as * xs + at * xt + ab * xb;
 }
 
+// Operator views used by the solver kernels.  OpCols reads the seven stored
+// columns; OpSep reads a stored diagonal and six 1-D coefficient arrays
+// (west/east indexed by i, south/north by j, bottom/top by k) — the exact
+// stored values of a matrix whose off-diagonal columns each depend on one
+// index only, which is what the Poisson assembly gives on a uniform
+// tensor-product mesh.
+template <class TA>
+struct OpCols {
+    const TA *__restrict__ c[7];
+    __device__ inline void get(int cc, int, int, int, TA &ac, TA &aw, TA &ae, TA &as, TA &an, TA &ab,
+                               TA &at) const {
+        ac = c[0][cc]; aw = c[1][cc]; ae = c[2][cc]; as = c[3][cc]; an = c[4][cc]; ab = c[5][cc]; at = c[6][cc];
+    }
+    __device__ inline TA diag(int cc) const { return c[0][cc]; }
+};
+
+template <class TA>
+struct OpSep {
+    const TA *__restrict__ ac;
+    const TA *__restrict__ aw, *__restrict__ ae, *__restrict__ as, *__restrict__ an, *__restrict__ ab,
+        *__restrict__ at;
+    __device__ inline void get(int cc, int i, int j, int k, TA &c0, TA &w, TA &e, TA &s, TA &n, TA &b,
+                               TA &t) const {
+        c0 = ac[cc]; w = aw[i]; e = ae[i]; s = as[j]; n = an[j]; b = ab[k]; t = at[k];
+    }
+    __device__ inline TA diag(int cc) const { return ac[cc]; }
+};
+
+template <class OP, class TA, class TX, class R>
+__device__ inline R apply_op(const OP &op, const TX *__restrict__ x, int cc, int i, int j, int k, int sy, int sz) {
+    TA ac, aw, ae, as, an, ab, at;
+    op.get(cc, i, j, k, ac, aw, ae, as, an, ab, at);
+    const R xc = (R)x[cc], xe = (R)x[cc + 1], xw = (R)x[cc - 1], xn = (R)x[cc + sy], xs = (R)x[cc - sy],
+            xt = (R)x[cc + sz], xb = (R)x[cc - sz];
+    return (R)ac * xc + (R)ae * xe + (R)aw * xw + (R)an * xn + (R)as * xs + (R)at * xt + (R)ab * xb;
+}
+
 template <class T>
 falm_status launch_sor(const falm_domain *dom, const void *a, void *x, const void *b, double omega, int color,
                        const int *done, hipStream_t s);
@@ -42,7 +79,7 @@ template <class T>
 falm_status launch_residual(const falm_domain *dom, const void *a, const void *x, const void *b, void *r,
                             hipStream_t s);
 falm_status launch_dot(const falm_domain *dom, int prec, const void *a, const void *b, int ncol, bool norm,
-                       double *dev_out, double *partials, unsigned *ticket, hipStream_t s);
+                       double *dev_out, double *partials, hipStream_t s);
 falm_status host_minmax(const falm_domain *dom, int prec, int op, const void *a, int ncol, int col, double *out,
                         hipStream_t s);
 

@@ -64,9 +64,7 @@ falm_status reduce_scratch(size_t nblocks, ReduceScratch **out) {
         FALM_HIP(hipMalloc(&rs.partials, 8 * nb * sizeof(double)));
         rs.max_blocks = nb;
     }
-    if (!rs.ticket) {
-        FALM_HIP(hipMalloc(&rs.ticket, 64));
-        FALM_HIP(hipMemset(rs.ticket, 0, 64));
+    if (!rs.result) {
         FALM_HIP(hipMalloc(&rs.result, 8 * sizeof(double)));
         FALM_HIP(hipHostMalloc(&rs.host, 8 * sizeof(double), hipHostMallocDefault));
     }

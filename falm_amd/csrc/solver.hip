@@ -1,27 +1,35 @@
 // solver.hip — FalmEq on gfx950 (src/FalmEq.h:14-104, src/FalmEq.cpp:45-410).
 //
 // The whole Krylov loop is device resident: every scalar of BiCGStab
-// (rho, alpha, omega, beta, err, it) lives in a DevState in device memory,
-// reductions finish in the last workgroup of the kernel that produces them
-// (fixed summation order, so results do not depend on scheduling), and
-// every solver kernel returns immediately once `done` is set.  The host only
+// (rho, alpha, omega, beta, err, it) lives in a DevState in device memory.
+// Each reduction is two-phase: the producing kernel writes one partial per
+// workgroup, a one-workgroup finalize kernel sums them in a fixed order
+// (results do not depend on scheduling) and updates the scalars.  Every
+// solver kernel returns at once when `done` is set, so the host only
 // enqueues iterations and polls the state every `check_every` iterations
-// instead of copying partial sums back after every dot product.
+// instead of copying partial sums back after each dot product.
 //
 // Fusions relative to the reference kernels:
-//   residual + |r|^2 + rr = r copy + rho                      (init)
+//   residual + |r|^2 + rr = r + rho                              (init)
 //   p = r + beta(p - omega q) + first preconditioner sweep from zero
-//   q = A p^ + (rr, q) -> alpha
+//   q = A p^ + (rr, q)                                           -> alpha
 //   s = r - alpha q + first preconditioner sweep from zero
-//   t = A s^ + (t, s), (t, t) -> omega
-//   x += alpha p^ + omega s^, r = s - omega t, |r|^2, (r, rr) -> err, it, beta
+//   t = A s^ + (t, s), (t, t)                                    -> omega
+//   x += alpha p^ + omega s^, r = s - omega t, |r|^2, (r, rr)    -> err, it, beta
 //
-// Mixed precision (FALM_SOLVE_MIXED): the Krylov vectors, the matrix copy and
-// the preconditioner sweeps are float; dots and norms accumulate in double;
-// x (the pressure) is updated in double; each pass restarts from the true
-// residual b - A x computed in double with the double matrix, and the solve
-// ends when that true residual meets the tolerance.
+// Operator storage: falm_solver_set_matrix checks whether each off-diagonal
+// column of A depends on one index only (bitwise); if so the solver keeps the
+// diagonal column plus six 1-D arrays of the stored values (8 B/cell of
+// matrix traffic instead of 56) — the same coefficients, so the same
+// results.  Otherwise the seven columns are read as given.
+//
+// Mixed precision (FALM_SOLVE_MIXED): the Krylov vectors, a float copy of the
+// operator and the preconditioner sweeps are float; dots and norms
+// accumulate in double; x (the pressure) is updated in double; each pass
+// starts from the true residual b - A x computed in double with the double
+// operator.
 #include <cfloat>
+#include <cstring>
 #include <vector>
 
 #include "linalg.hpp"
@@ -45,11 +53,11 @@ struct DevState {
 constexpr int NT = 256;
 
 template <class T>
-__device__ inline T pc_first(int pc_type, T omega_pc, T b, T ac, bool mine) {
+__device__ inline T pc_first(int pc_type, T omega_pc, T b, T ac, bool black) {
     if (pc_type == FALM_LS_SOR) {
-        // kernel_Struct3d7p_SOR from x = 0: cc = (b - 0)/ac on the colour,
-        // x = 0 + omega*cc (src/dev/FalmEqDevCall.cu:204-209)
-        return mine ? T(0) + omega_pc * (b / ac) : T(0);
+        // kernel_Struct3d7p_SOR from x = 0: cc = (b - 0)/ac on the black
+        // colour, x = 0 + omega*cc (src/dev/FalmEqDevCall.cu:204-209)
+        return black ? T(0) + omega_pc * (b / ac) : T(0);
     } else if (pc_type == FALM_LS_JACOBI) {
         // kernel_Struct3d7p_Jacobi from xp = 0 (src/dev/FalmEqDevCall.cu:130)
         return T(0) + b / ac;
@@ -57,26 +65,35 @@ __device__ inline T pc_first(int pc_type, T omega_pc, T b, T ac, bool mine) {
     return b;  // identity (see falm_solver_create)
 }
 
-// residual in double + float/double Krylov copies + |r|^2 and (r, rr)
-template <class T>
-__global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, Mat7<double> a, const double *__restrict__ x,
+// ------------------------------------------------------------ kernels
+// r64 = b - A x in double; float/double Krylov copies r, rr; partials of
+// |r64|^2 and (r, rr)
+template <class T, class OP>
+__global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, OP a, const double *__restrict__ x,
                                                   const double *__restrict__ b, double *r64, T *r, T *rr,
-                                                  DevState *st, double *partials, unsigned *ticket) {
+                                                  double *partials) {
     __shared__ double lds[2 * (NT / 64)];
-    int i, j, k;
+    int i, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
-    if (tile_cell(tl, i, j, k)) {
+    tile_range(tl.tiles, t0, t1);
+    for (int tile = t0; tile < t1; tile++) {
+        if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
-        const double rv = b[cc] - apply7<double, double, double>(a, x, cc, g.nx, g.nxy);
+        const double rv = b[cc] - apply_op<OP, double, double, double>(a, x, cc, i, j, k, g.nx, g.nxy);
         r64[cc] = rv;
         const T rt = (T)rv;
         if ((void *)r != (void *)r64) r[cc] = rt;
         rr[cc] = rt;
-        v[0] = rv * rv;
-        v[1] = (double)rt * (double)rt;
+        v[0] += rv * rv;
+        v[1] += (double)rt * (double)rt;
     }
+    block_partials<NT, 2>(v, partials, lds);
+}
+
+__global__ __launch_bounds__(FIN_NT) void k_fin_init(DevState *st, const double *partials, int nb) {
     double tot[2];
-    if (grid_sum_last<NT, 2>(v, partials, ticket, tot, lds) && threadIdx.x == 0) {
+    sum_partials<2>(partials, nb, tot);
+    if (threadIdx.x == 0) {
         st->err = sqrt(tot[0]) / st->gN;
         st->rho = tot[1];
         st->rrho = 1.0;
@@ -97,10 +114,10 @@ __global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, Mat7<double
 
 // p = r (first) or r + beta(p - omega q) (kernel_PBiCGStab_1), then the
 // first preconditioner sweep pp = M0(p)
-template <class T>
-__global__ __launch_bounds__(NT) void k_bicg_p(Tiling tl, Geom g, const T *__restrict__ ac, const T *__restrict__ r,
-                                               T *__restrict__ p, const T *__restrict__ q, T *__restrict__ pp,
-                                               const DevState *st, int pc_type, T omega_pc) {
+template <class T, class OP>
+__global__ __launch_bounds__(NT) void k_bicg_p(Tiling tl, Geom g, OP a, const T *__restrict__ r, T *__restrict__ p,
+                                               const T *__restrict__ q, T *__restrict__ pp, const DevState *st,
+                                               int pc_type, T omega_pc) {
     if (st->done) return;
     int i, j, k;
     if (!tile_cell(tl, i, j, k)) return;
@@ -113,12 +130,12 @@ __global__ __launch_bounds__(NT) void k_bicg_p(Tiling tl, Geom g, const T *__res
         pv = r[cc] + beta * (p[cc] - omega * q[cc]);
     }
     p[cc] = pv;
-    pp[cc] = pc_first<T>(pc_type, omega_pc, pv, ac[cc], ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK);
+    pp[cc] = pc_first<T>(pc_type, omega_pc, pv, a.diag(cc), ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK);
 }
 
 // s = r - alpha q (kernel_PBiCGStab_2), then ss = M0(s)
-template <class T>
-__global__ __launch_bounds__(NT) void k_bicg_s(Tiling tl, Geom g, const T *__restrict__ ac, const T *__restrict__ r,
+template <class T, class OP>
+__global__ __launch_bounds__(NT) void k_bicg_s(Tiling tl, Geom g, OP a, const T *__restrict__ r,
                                                const T *__restrict__ q, T *__restrict__ s, T *__restrict__ ss,
                                                const DevState *st, int pc_type, T omega_pc) {
     if (st->done) return;
@@ -128,71 +145,135 @@ __global__ __launch_bounds__(NT) void k_bicg_s(Tiling tl, Geom g, const T *__res
     const T alpha = (T)st->alpha;
     const T sv = r[cc] - alpha * q[cc];
     s[cc] = sv;
-    ss[cc] = pc_first<T>(pc_type, omega_pc, sv, ac[cc], ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK);
+    ss[cc] = pc_first<T>(pc_type, omega_pc, sv, a.diag(cc), ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK);
 }
 
-// q = A pp ; alpha = rho / (rr, q)
+// red-black SOR half sweep on the cells of `color` (kernel_Struct3d7p_SOR,
+// src/dev/FalmEqDevCall.cu:175-211)
+template <class T, class OP>
+__global__ __launch_bounds__(NT) void k_sor_op(Tiling tl, Geom g, OP a, T *x, const T *__restrict__ b, T omega,
+                                               int color, const int *done) {
+    if (*done) return;
+    int i, j, k;
+    if (!tile_cell(tl, i, j, k)) return;
+    if (((i + j + k + g.osum) & 1) != color) return;
+    const int cc = g.id(i, j, k);
+    const T xc = x[cc];
+    const T s = apply_op<OP, T, T, T>(a, x, cc, i, j, k, g.nx, g.nxy);
+    const T cv = (b[cc] - s) / a.diag(cc);
+    x[cc] = xc + omega * cv;
+}
+
+// kernel_Struct3d7p_Jacobi (src/dev/FalmEqDevCall.cu:101-132)
+template <class T, class OP>
+__global__ __launch_bounds__(NT) void k_jacobi_op(Tiling tl, Geom g, OP a, T *__restrict__ x,
+                                                  const T *__restrict__ xp, const T *__restrict__ b,
+                                                  const int *done) {
+    if (*done) return;
+    int i, j, k;
+    if (!tile_cell(tl, i, j, k)) return;
+    const int cc = g.id(i, j, k);
+    const T xc = xp[cc];
+    x[cc] = xc + (b[cc] - apply_op<OP, T, T, T>(a, xp, cc, i, j, k, g.nx, g.nxy)) / a.diag(cc);
+}
+
 template <class T>
-__global__ __launch_bounds__(NT) void k_bicg_mvdot(Tiling tl, Geom g, Mat7<T> a, const T *__restrict__ pp,
-                                                   T *__restrict__ q, const T *__restrict__ rr, DevState *st,
-                                                   double *partials, unsigned *ticket) {
+__global__ __launch_bounds__(NT) void k_copy_interior(Tiling tl, Geom g, const T *__restrict__ src,
+                                                      T *__restrict__ dst, const int *done) {
+    if (*done) return;
+    int i, j, k;
+    if (!tile_cell(tl, i, j, k)) return;
+    const int cc = g.id(i, j, k);
+    dst[cc] = src[cc];
+}
+
+// q = A pp ; partial (rr, q)
+template <class T, class OP>
+__global__ __launch_bounds__(NT) void k_bicg_mvdot(Tiling tl, Geom g, OP a, const T *__restrict__ pp,
+                                                   T *__restrict__ q, const T *__restrict__ rr, const DevState *st,
+                                                   double *partials) {
     if (st->done) return;
     __shared__ double lds[NT / 64];
-    int i, j, k;
+    int i, j, k, t0, t1;
     double v[1] = {0.0};
-    if (tile_cell(tl, i, j, k)) {
+    tile_range(tl.tiles, t0, t1);
+    for (int tile = t0; tile < t1; tile++) {
+        if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
-        const T qv = apply7<T, T, T>(a, pp, cc, g.nx, g.nxy);
+        const T qv = apply_op<OP, T, T, T>(a, pp, cc, i, j, k, g.nx, g.nxy);
         q[cc] = qv;
-        v[0] = (double)rr[cc] * (double)qv;
+        v[0] += (double)rr[cc] * (double)qv;
     }
-    double tot[1];
-    if (grid_sum_last<NT, 1>(v, partials, ticket, tot, lds) && threadIdx.x == 0) st->alpha = st->rho / tot[0];
+    block_partials<NT, 1>(v, partials, lds);
 }
 
-// t = A ss ; omega = (t, s) / (t, t)
-template <class T>
-__global__ __launch_bounds__(NT) void k_bicg_mvdot2(Tiling tl, Geom g, Mat7<T> a, const T *__restrict__ ss,
-                                                    T *__restrict__ t, const T *__restrict__ s, DevState *st,
-                                                    double *partials, unsigned *ticket) {
+__global__ __launch_bounds__(FIN_NT) void k_fin_alpha(DevState *st, const double *partials, int nb) {
+    if (st->done) return;
+    double tot[1];
+    sum_partials<1>(partials, nb, tot);
+    if (threadIdx.x == 0) st->alpha = st->rho / tot[0];
+}
+
+// t = A ss ; partials (t, s), (t, t)
+template <class T, class OP>
+__global__ __launch_bounds__(NT) void k_bicg_mvdot2(Tiling tl, Geom g, OP a, const T *__restrict__ ss,
+                                                    T *__restrict__ t, const T *__restrict__ s, const DevState *st,
+                                                    double *partials) {
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
-    int i, j, k;
+    int i, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
-    if (tile_cell(tl, i, j, k)) {
+    tile_range(tl.tiles, t0, t1);
+    for (int tile = t0; tile < t1; tile++) {
+        if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
-        const T tv = apply7<T, T, T>(a, ss, cc, g.nx, g.nxy);
+        const T tv = apply_op<OP, T, T, T>(a, ss, cc, i, j, k, g.nx, g.nxy);
         t[cc] = tv;
-        v[0] = (double)tv * (double)s[cc];
-        v[1] = (double)tv * (double)tv;
+        v[0] += (double)tv * (double)s[cc];
+        v[1] += (double)tv * (double)tv;
     }
+    block_partials<NT, 2>(v, partials, lds);
+}
+
+__global__ __launch_bounds__(FIN_NT) void k_fin_omega(DevState *st, const double *partials, int nb) {
+    if (st->done) return;
     double tot[2];
-    if (grid_sum_last<NT, 2>(v, partials, ticket, tot, lds) && threadIdx.x == 0) st->omega = tot[0] / tot[1];
+    sum_partials<2>(partials, nb, tot);
+    if (threadIdx.x == 0) st->omega = tot[0] / tot[1];
 }
 
 // x += alpha pp + omega ss (kernel_PBiCGStab_3, x in double);
-// r = s - omega t (kernel_PBiCGStab_4); err, it, next rho and beta
+// r = s - omega t (kernel_PBiCGStab_4); partials |r|^2, (r, rr)
 template <class T>
 __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *__restrict__ x,
                                                     const T *__restrict__ pp, const T *__restrict__ ss,
                                                     const T *__restrict__ s, const T *__restrict__ t,
-                                                    T *__restrict__ r, const T *__restrict__ rr, DevState *st,
-                                                    double *partials, unsigned *ticket) {
+                                                    T *__restrict__ r, const T *__restrict__ rr, const DevState *st,
+                                                    double *partials) {
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
-    int i, j, k;
+    int i, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
-    if (tile_cell(tl, i, j, k)) {
+    const double alpha = st->alpha, omega = st->omega;
+    tile_range(tl.tiles, t0, t1);
+    for (int tile = t0; tile < t1; tile++) {
+        if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
-        const double alpha = st->alpha, omega = st->omega;
         x[cc] += alpha * (double)pp[cc] + omega * (double)ss[cc];
         const T rv = s[cc] - (T)omega * t[cc];
         r[cc] = rv;
-        v[0] = (double)rv * (double)rv;
-        v[1] = (double)rv * (double)rr[cc];
+        v[0] += (double)rv * (double)rv;
+        v[1] += (double)rv * (double)rr[cc];
     }
+    block_partials<NT, 2>(v, partials, lds);
+}
+
+// err = |r|/N, it++, stop test, next rho and beta (src/FalmEq.cpp:345-407)
+__global__ __launch_bounds__(FIN_NT) void k_fin_update(DevState *st, const double *partials, int nb) {
+    if (st->done) return;
     double tot[2];
-    if (grid_sum_last<NT, 2>(v, partials, ticket, tot, lds) && threadIdx.x == 0) {
+    sum_partials<2>(partials, nb, tot);
+    if (threadIdx.x == 0) {
         st->rrho = st->rho;
         st->err = sqrt(tot[0]) / st->gN;
         st->it_pass += 1;
@@ -212,36 +293,35 @@ __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *_
     }
 }
 
-// plain SOR / Jacobi solvers: residual norm -> err, it, done
-__global__ __launch_bounds__(NT) void k_res_norm(Tiling tl, Geom g, Mat7<double> a, const double *__restrict__ x,
-                                                 const double *__restrict__ b, double *__restrict__ r, DevState *st,
-                                                 double *partials, unsigned *ticket) {
+// plain SOR / Jacobi solvers: r = b - A x and partial |r|^2
+template <class OP>
+__global__ __launch_bounds__(NT) void k_res_norm(Tiling tl, Geom g, OP a, const double *__restrict__ x,
+                                                 const double *__restrict__ b, double *__restrict__ r,
+                                                 const DevState *st, double *partials) {
     if (st->done) return;
     __shared__ double lds[NT / 64];
-    int i, j, k;
+    int i, j, k, t0, t1;
     double v[1] = {0.0};
-    if (tile_cell(tl, i, j, k)) {
+    tile_range(tl.tiles, t0, t1);
+    for (int tile = t0; tile < t1; tile++) {
+        if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
-        const double rv = b[cc] - apply7<double, double, double>(a, x, cc, g.nx, g.nxy);
+        const double rv = b[cc] - apply_op<OP, double, double, double>(a, x, cc, i, j, k, g.nx, g.nxy);
         r[cc] = rv;
-        v[0] = rv * rv;
+        v[0] += rv * rv;
     }
+    block_partials<NT, 1>(v, partials, lds);
+}
+
+__global__ __launch_bounds__(FIN_NT) void k_fin_resnorm(DevState *st, const double *partials, int nb) {
+    if (st->done) return;
     double tot[1];
-    if (grid_sum_last<NT, 1>(v, partials, ticket, tot, lds) && threadIdx.x == 0) {
+    sum_partials<1>(partials, nb, tot);
+    if (threadIdx.x == 0) {
         st->err = sqrt(tot[0]) / st->gN;
         st->it_pass += 1;
         if (!(st->it_pass < st->maxit_pass && st->err > st->tol_inner)) st->done = 1;
     }
-}
-
-template <class T>
-__global__ __launch_bounds__(NT) void k_copy_interior(Tiling tl, Geom g, const T *__restrict__ src,
-                                                      T *__restrict__ dst, const DevState *st) {
-    if (st && st->done) return;
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    dst[cc] = src[cc];
 }
 
 template <class TS, class TD>
@@ -250,7 +330,7 @@ __global__ void k_convert(const TS *__restrict__ s, TD *__restrict__ d, size_t n
         d[t] = (TD)s[t];
 }
 
-// limits of the next pass (the init kernel leaves them alone)
+// limits of the next pass (the init kernels leave them alone)
 __global__ void k_state_arm(DevState *st, double tol, int maxit) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->tol_inner = tol;
@@ -271,6 +351,45 @@ __global__ void k_state_reset(DevState *st, double gN, int mixed, double tol, in
     }
 }
 
+// --------------------------------------------- separable operator check
+// 1-D candidates from one interior row per axis
+template <class TA>
+__global__ void k_sep_extract(Geom g, Mat7<TA> a, TA *aw, TA *ae, TA *as, TA *an, TA *ab, TA *at) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gc = g.gc;
+    if (t < g.nx) {
+        const int cc = g.id(t, gc, gc);
+        aw[t] = a.c[1][cc];
+        ae[t] = a.c[2][cc];
+    }
+    if (t < g.ny) {
+        const int cc = g.id(gc, t, gc);
+        as[t] = a.c[3][cc];
+        an[t] = a.c[4][cc];
+    }
+    if (t < g.nz) {
+        const int cc = g.id(gc, gc, t);
+        ab[t] = a.c[5][cc];
+        at[t] = a.c[6][cc];
+    }
+}
+
+__device__ inline bool same_bits(double x, double y) { return __double_as_longlong(x) == __double_as_longlong(y); }
+__device__ inline bool same_bits(float x, float y) { return __float_as_int(x) == __float_as_int(y); }
+
+// every interior cell's off-diagonals equal the 1-D candidates bit for bit
+template <class TA>
+__global__ __launch_bounds__(NT) void k_sep_check(Tiling tl, Geom g, Mat7<TA> a, const TA *aw, const TA *ae,
+                                                  const TA *as, const TA *an, const TA *ab, const TA *at,
+                                                  int *mismatch) {
+    int i, j, k;
+    if (!tile_cell(tl, i, j, k)) return;
+    const int cc = g.id(i, j, k);
+    const bool ok = same_bits(a.c[1][cc], aw[i]) && same_bits(a.c[2][cc], ae[i]) && same_bits(a.c[3][cc], as[j]) &&
+                    same_bits(a.c[4][cc], an[j]) && same_bits(a.c[5][cc], ab[k]) && same_bits(a.c[6][cc], at[k]);
+    if (!ok) *mismatch = 1;  // benign race: every writer stores 1
+}
+
 }  // namespace falm
 
 using namespace falm;
@@ -284,16 +403,24 @@ struct falm_solver {
     DevState *st = nullptr;
     DevState *hst = nullptr;  // pinned mirror
     double *partials = nullptr;
-    unsigned *ticket = nullptr;
+    int *flag = nullptr, *hflag = nullptr;
     // Krylov vectors: double (F64) or float (MIXED)
     void *rr = nullptr, *p = nullptr, *q = nullptr, *s = nullptr, *pp = nullptr, *ss = nullptr, *t = nullptr,
          *w = nullptr, *r32 = nullptr;
-    float *a32 = nullptr;
-    const void *a32_src = nullptr;
-    bool a32_pinned = false;
+    // operator copies set by falm_solver_set_matrix
+    const void *a_src = nullptr;  // the caller's matrix the copies were made from
+    bool sep = false;
+    double *sep64 = nullptr;      // 1-D arrays: aw,ae [nx], as,an [ny], ab,at [nz]
+    float *sep32 = nullptr;
+    float *a32 = nullptr;         // mixed: float columns (7 if not separable, else diagonal only)
 };
 
 namespace falm {
+
+static double interior_count(const falm_solver *S) {
+    return (double)(S->dom.gshape[0] - 2 * S->dom.gc) * (double)(S->dom.gshape[1] - 2 * S->dom.gc) *
+           (double)(S->dom.gshape[2] - 2 * S->dom.gc);
+}
 
 static falm_status solver_alloc(falm_solver *S) {
     const size_t vb = S->n * (S->cfg.precision == FALM_SOLVE_MIXED ? sizeof(float) : sizeof(double));
@@ -308,13 +435,16 @@ static falm_status solver_alloc(falm_solver *S) {
         FALM_HIP(hipMalloc(&S->a32, 7 * S->n * sizeof(float)));
         FALM_HIP(hipMemset(S->a32, 0, 7 * S->n * sizeof(float)));
     }
+    const size_t ns = 2 * ((size_t)S->g.nx + S->g.ny + S->g.nz);
+    FALM_HIP(hipMalloc(&S->sep64, ns * sizeof(double)));
+    FALM_HIP(hipMalloc(&S->sep32, ns * sizeof(float)));
     FALM_HIP(hipMalloc(&S->st, sizeof(DevState)));
     FALM_HIP(hipMemset(S->st, 0, sizeof(DevState)));
     FALM_HIP(hipHostMalloc(&S->hst, sizeof(DevState), hipHostMallocDefault));
-    size_t nb = (size_t)(S->tl.tiles > 0 ? S->tl.tiles : 1);
+    FALM_HIP(hipMalloc(&S->flag, sizeof(int)));
+    FALM_HIP(hipHostMalloc(&S->hflag, sizeof(int), hipHostMallocDefault));
+    size_t nb = (size_t)(S->tl.tiles > 0 ? red_blocks(S->tl) : 1);
     FALM_HIP(hipMalloc(&S->partials, 2 * nb * sizeof(double)));
-    FALM_HIP(hipMalloc(&S->ticket, 64));
-    FALM_HIP(hipMemset(S->ticket, 0, 64));
     return FALM_OK;
 }
 
@@ -331,105 +461,132 @@ static int check_period(falm_solver *S) {
     return c < 1 ? 1 : (c > 32 ? 32 : c);
 }
 
+// views of the operator the solver uses
+template <class TA>
+static OpSep<TA> sep_view(const falm_solver *S, const TA *diag, const TA *arr) {
+    OpSep<TA> o;
+    const int nx = S->g.nx, ny = S->g.ny, nz = S->g.nz;
+    o.ac = diag;
+    o.aw = arr;
+    o.ae = arr + nx;
+    o.as = arr + 2 * nx;
+    o.an = arr + 2 * nx + ny;
+    o.ab = arr + 2 * nx + 2 * ny;
+    o.at = arr + 2 * nx + 2 * ny + nz;
+    return o;
+}
+template <class TA>
+static OpCols<TA> cols_view(const falm_solver *S, const void *a) {
+    OpCols<TA> o;
+    for (int q = 0; q < 7; q++) o.c[q] = col<TA>(a, S->n, q);
+    return o;
+}
+
+#define FALM_GRID(S) tiling_grid((S)->tl), tiling_block(), 0
+#define FALM_RED(S) red_grid((S)->tl), tiling_block(), 0
+#define FALM_FIN dim3(1), dim3(FIN_NT), 0
+
 // preconditioner sweeps after the fused first one (SORPC: pc_maxit x
 // {black, red}; JacobiPC: pc_maxit sweeps; both run at least once,
 // src/FalmEq.cpp:147-170,258-302)
-template <class T>
-static falm_status pc_rest(falm_solver *S, const void *a, void *x, const void *b, hipStream_t s) {
+template <class T, class OP>
+static falm_status pc_rest(falm_solver *S, const OP &a, T *x, const T *b, hipStream_t s) {
     const falm_solver_config &c = S->cfg;
     const int m = c.pc_maxit < 1 ? 1 : c.pc_maxit;
     const int *done = &S->st->done;
+    const T w = (T)c.pc_relax;
     if (c.pc_type == FALM_LS_SOR) {
-        FALM_TRY(launch_sor<T>(&S->dom, a, x, b, c.pc_relax, FALM_COLOR_RED, done, s));
-        for (int it = 1; it < m; it++) {
-            FALM_TRY(launch_sor<T>(&S->dom, a, x, b, c.pc_relax, FALM_COLOR_BLACK, done, s));
-            FALM_TRY(launch_sor<T>(&S->dom, a, x, b, c.pc_relax, FALM_COLOR_RED, done, s));
+        for (int it = 0; it < m; it++) {
+            if (it > 0) {
+                TimedLaunch tm(FALM_TIME_SOR, s);
+                hipLaunchKernelGGL((k_sor_op<T, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, w, FALM_COLOR_BLACK,
+                                   done);
+                tm.done((double)S->tl.box.size() * 3.5 * sizeof(T));
+            }
+            TimedLaunch tm(FALM_TIME_SOR, s);
+            hipLaunchKernelGGL((k_sor_op<T, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, w, FALM_COLOR_RED, done);
+            tm.done((double)S->tl.box.size() * 3.5 * sizeof(T));
         }
+        FALM_CHECK_LAUNCH("k_sor_op");
     } else if (c.pc_type == FALM_LS_JACOBI) {
         for (int it = 1; it < m; it++) {
-            hipLaunchKernelGGL((k_copy_interior<T>), tiling_grid(S->tl), tiling_block(), 0, s, S->tl, S->g,
-                               (const T *)x, (T *)S->w, (const DevState *)S->st);
-            FALM_CHECK_LAUNCH("k_copy_interior");
-            FALM_TRY(launch_jacobi<T>(&S->dom, a, x, S->w, b, done, s));
+            hipLaunchKernelGGL((k_copy_interior<T>), FALM_GRID(S), s, S->tl, S->g, (const T *)x, (T *)S->w, done);
+            hipLaunchKernelGGL((k_jacobi_op<T, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, (const T *)S->w, b, done);
         }
+        FALM_CHECK_LAUNCH("k_jacobi_op");
     }
     return FALM_OK;
 }
 
 // one BiCGStab iteration (src/FalmEq.cpp:345-407)
-template <class T>
-static falm_status bicg_iteration(falm_solver *S, const void *aT, double *x, T *r, hipStream_t s) {
+template <class T, class OP>
+static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, hipStream_t s) {
     const falm_solver_config &c = S->cfg;
     const Tiling &tl = S->tl;
     const Geom &g = S->g;
-    const T *ac = (const T *)aT;
+    const int nb = red_blocks(tl);
     T *rr = (T *)S->rr, *p = (T *)S->p, *q = (T *)S->q, *sv = (T *)S->s, *pp = (T *)S->pp, *ss = (T *)S->ss,
       *t = (T *)S->t;
     const T wpc = (T)c.pc_relax;
-    hipLaunchKernelGGL((k_bicg_p<T>), tiling_grid(tl), tiling_block(), 0, s, tl, g, ac, r, p, q, pp,
+    hipLaunchKernelGGL((k_bicg_p<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, p, (const T *)q, pp,
                        (const DevState *)S->st, c.pc_type, wpc);
     FALM_CHECK_LAUNCH("k_bicg_p");
-    FALM_TRY(pc_rest<T>(S, aT, pp, p, s));
+    FALM_TRY((pc_rest<T, OP>(S, a, pp, p, s)));
     TimedLaunch tm1(FALM_TIME_MV, s);
-    hipLaunchKernelGGL((k_bicg_mvdot<T>), tiling_grid(tl), tiling_block(), 0, s, tl, g, mat7<T>(aT, S->n),
-                       (const T *)pp, q, (const T *)rr, S->st, S->partials, S->ticket);
+    hipLaunchKernelGGL((k_bicg_mvdot<T, OP>), FALM_RED(S), s, tl, g, a, (const T *)pp, q, (const T *)rr,
+                       (const DevState *)S->st, S->partials);
+    tm1.done((double)tl.box.size() * 4.0 * sizeof(T));
+    hipLaunchKernelGGL(k_fin_alpha, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot");
-    tm1.done((double)tl.box.size() * 10.0 * sizeof(T));
-    hipLaunchKernelGGL((k_bicg_s<T>), tiling_grid(tl), tiling_block(), 0, s, tl, g, ac, (const T *)r,
-                       (const T *)q, sv, ss, (const DevState *)S->st, c.pc_type, wpc);
+    hipLaunchKernelGGL((k_bicg_s<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, (const T *)q, sv, ss,
+                       (const DevState *)S->st, c.pc_type, wpc);
     FALM_CHECK_LAUNCH("k_bicg_s");
-    FALM_TRY(pc_rest<T>(S, aT, ss, sv, s));
-    hipLaunchKernelGGL((k_bicg_mvdot2<T>), tiling_grid(tl), tiling_block(), 0, s, tl, g, mat7<T>(aT, S->n),
-                       (const T *)ss, t, (const T *)sv, S->st, S->partials, S->ticket);
+    FALM_TRY((pc_rest<T, OP>(S, a, ss, sv, s)));
+    hipLaunchKernelGGL((k_bicg_mvdot2<T, OP>), FALM_RED(S), s, tl, g, a, (const T *)ss, t, (const T *)sv,
+                       (const DevState *)S->st, S->partials);
+    hipLaunchKernelGGL(k_fin_omega, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot2");
     TimedLaunch tm2(FALM_TIME_SOLVER_UPDATE, s);
-    hipLaunchKernelGGL((k_bicg_update<T>), tiling_grid(tl), tiling_block(), 0, s, tl, g, x, (const T *)pp,
-                       (const T *)ss, (const T *)sv, (const T *)t, r, (const T *)rr, S->st, S->partials, S->ticket);
-    FALM_CHECK_LAUNCH("k_bicg_update");
+    hipLaunchKernelGGL((k_bicg_update<T>), FALM_RED(S), s, tl, g, x, (const T *)pp, (const T *)ss, (const T *)sv,
+                       (const T *)t, r, (const T *)rr, (const DevState *)S->st, S->partials);
     tm2.done((double)tl.box.size() * (2.0 * sizeof(double) + 6.0 * sizeof(T)));
+    hipLaunchKernelGGL(k_fin_update, FALM_FIN, s, S->st, (const double *)S->partials, nb);
+    FALM_CHECK_LAUNCH("k_bicg_update");
     return FALM_OK;
 }
 
-// run iterations until the device says done; returns iterations of the pass
-template <class T>
-static falm_status bicg_pass(falm_solver *S, const void *aT, double *x, T *r, int maxit_pass, hipStream_t s) {
+// run iterations until the device says done
+template <class T, class OP>
+static falm_status bicg_pass(falm_solver *S, const OP &a, double *x, T *r, int maxit_pass, hipStream_t s) {
     const int period = check_period(S);
     int enqueued = 0;
     while (true) {
         for (int c = 0; c < period && enqueued < maxit_pass; c++, enqueued++)
-            FALM_TRY(bicg_iteration<T>(S, aT, x, r, s));
+            FALM_TRY((bicg_iteration<T, OP>(S, a, x, r, s)));
         FALM_TRY(read_state(S, s));
         if (S->hst->done || enqueued >= maxit_pass) break;
     }
     return FALM_OK;
 }
 
-static falm_status launch_init(falm_solver *S, const void *a, const double *x, const double *b, double *r64,
-                               void *rT, bool f32, hipStream_t s) {
-    if (f32)
-        hipLaunchKernelGGL((k_bicg_init<float>), tiling_grid(S->tl), tiling_block(), 0, s, S->tl, S->g,
-                           mat7<double>(a, S->n), x, b, r64, (float *)rT, (float *)S->rr, S->st, S->partials,
-                           S->ticket);
-    else
-        hipLaunchKernelGGL((k_bicg_init<double>), tiling_grid(S->tl), tiling_block(), 0, s, S->tl, S->g,
-                           mat7<double>(a, S->n), x, b, r64, (double *)rT, (double *)S->rr, S->st, S->partials,
-                           S->ticket);
+template <class T, class OP64>
+static falm_status launch_init(falm_solver *S, const OP64 &a, const double *x, const double *b, double *r64, T *rT,
+                               hipStream_t s) {
+    hipLaunchKernelGGL((k_bicg_init<T, OP64>), FALM_RED(S), s, S->tl, S->g, a, x, b, r64, rT, (T *)S->rr,
+                       S->partials);
+    hipLaunchKernelGGL(k_fin_init, FALM_FIN, s, S->st, (const double *)S->partials, red_blocks(S->tl));
     FALM_CHECK_LAUNCH("k_bicg_init");
     return FALM_OK;
 }
 
-static double interior_count(const falm_solver *S) {
-    return (double)(S->dom.gshape[0] - 2 * S->dom.gc) * (double)(S->dom.gshape[1] - 2 * S->dom.gc) *
-           (double)(S->dom.gshape[2] - 2 * S->dom.gc);
-}
-
-static falm_status solve_bicgstab_f64(falm_solver *S, const void *a, double *x, const double *b, double *r,
+template <class OP64>
+static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, const double *b, double *r,
                                       hipStream_t s, falm_solve_info *info) {
     const falm_solver_config &c = S->cfg;
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit);
     FALM_CHECK_LAUNCH("k_state_reset");
-    FALM_TRY(launch_init(S, a, x, b, r, r, false, s));
-    FALM_TRY(bicg_pass<double>(S, a, x, r, c.maxit, s));
+    FALM_TRY((launch_init<double, OP64>(S, a, x, b, r, r, s)));
+    FALM_TRY((bicg_pass<double, OP64>(S, a, x, r, c.maxit, s)));
     info->it = S->hst->it_pass;
     info->err = S->hst->err;
     info->outer_it = 1;
@@ -438,29 +595,24 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const void *a, double *x, 
     return FALM_OK;
 }
 
-static falm_status solve_bicgstab_mixed(falm_solver *S, const void *a, double *x, const double *b, double *r,
-                                        hipStream_t s, falm_solve_info *info) {
+template <class OP64, class OP32>
+static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const OP32 &a32, double *x,
+                                        const double *b, double *r, hipStream_t s, falm_solve_info *info) {
     const falm_solver_config &c = S->cfg;
-    const double gN = interior_count(S);
     const int inner_maxit = c.inner_maxit > 0 ? c.inner_maxit : 50;
     const double inner_rtol = c.inner_rtol > 0 ? c.inner_rtol : 1e-4;
-    if (!S->a32_pinned || S->a32_src != a) {
-        hipLaunchKernelGGL((k_convert<double, float>), ew_grid(7 * S->n), dim3(EW_BLOCK), 0, s, (const double *)a,
-                           S->a32, 7 * S->n);
-        FALM_CHECK_LAUNCH("k_convert");
-    }
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, gN, 1, c.tol, c.maxit);
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 1, c.tol, c.maxit);
     FALM_CHECK_LAUNCH("k_state_reset");
     int total = 0, pass = 0;
     double err = 0.0;
     int status = FALM_OK;
     while (true) {
         // true residual in double; the inner target follows from it
-        FALM_TRY(launch_init(S, a, x, b, r, S->r32, true, s));
+        FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, (float *)S->r32, s)));
         FALM_TRY(read_state(S, s));
         err = S->hst->err;
         if ((pass > 0 && err <= c.tol) || total >= c.maxit) break;
-        if (S->hst->done) {  // r32 vanished: nothing more a float pass can do
+        if (S->hst->done) {  // the float residual vanished: nothing more a float pass can do
             status = (err <= c.tol) ? FALM_OK : FALM_ERR_BREAKDOWN;
             break;
         }
@@ -469,10 +621,10 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const void *a, double *x
         int maxit_pass = c.maxit - total < inner_maxit ? c.maxit - total : inner_maxit;
         hipLaunchKernelGGL(k_state_arm, dim3(1), dim3(64), 0, s, S->st, target, maxit_pass);
         FALM_CHECK_LAUNCH("k_state_arm");
-        FALM_TRY(bicg_pass<float>(S, S->a32, x, (float *)S->r32, maxit_pass, s));
+        FALM_TRY((bicg_pass<float, OP32>(S, a32, x, (float *)S->r32, maxit_pass, s)));
         total += S->hst->it_pass;
         pass++;
-        if (S->hst->it_pass == 0) {  // breakdown at the start of the pass
+        if (S->hst->it_pass == 0) {
             status = FALM_ERR_BREAKDOWN;
             break;
         }
@@ -485,27 +637,32 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const void *a, double *x
 }
 
 // FalmEq::SOR (src/FalmEq.cpp:173-240) and FalmEq::Jacobi (:81-127)
-static falm_status solve_stationary(falm_solver *S, const void *a, double *x, const double *b, double *r,
+template <class OP>
+static falm_status solve_stationary(falm_solver *S, const OP &a, double *x, const double *b, double *r,
                                     hipStream_t s, falm_solve_info *info) {
     const falm_solver_config &c = S->cfg;
-    const double gN = interior_count(S);
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, gN, 0, c.tol, c.maxit);
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit);
     FALM_CHECK_LAUNCH("k_state_reset");
     const int period = check_period(S);
     const int *done = &S->st->done;
+    const int nb = red_blocks(S->tl);
     int enqueued = 0;
     while (true) {
         for (int k = 0; k < period && enqueued < c.maxit; k++, enqueued++) {
             if (c.type == FALM_LS_SOR) {
-                FALM_TRY(launch_sor<double>(&S->dom, a, x, b, c.relax, FALM_COLOR_BLACK, done, s));
-                FALM_TRY(launch_sor<double>(&S->dom, a, x, b, c.relax, FALM_COLOR_RED, done, s));
+                hipLaunchKernelGGL((k_sor_op<double, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, c.relax,
+                                   FALM_COLOR_BLACK, done);
+                hipLaunchKernelGGL((k_sor_op<double, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, c.relax,
+                                   FALM_COLOR_RED, done);
             } else {
-                hipLaunchKernelGGL((k_copy_interior<double>), tiling_grid(S->tl), tiling_block(), 0, s, S->tl, S->g,
-                                   (const double *)x, (double *)S->w, (const DevState *)S->st);
-                FALM_TRY(launch_jacobi<double>(&S->dom, a, x, S->w, b, done, s));
+                hipLaunchKernelGGL((k_copy_interior<double>), FALM_GRID(S), s, S->tl, S->g, (const double *)x,
+                                   (double *)S->w, done);
+                hipLaunchKernelGGL((k_jacobi_op<double, OP>), FALM_GRID(S), s, S->tl, S->g, a, x,
+                                   (const double *)S->w, b, done);
             }
-            hipLaunchKernelGGL(k_res_norm, tiling_grid(S->tl), tiling_block(), 0, s, S->tl, S->g,
-                               mat7<double>(a, S->n), (const double *)x, b, r, S->st, S->partials, S->ticket);
+            hipLaunchKernelGGL((k_res_norm<OP>), FALM_RED(S), s, S->tl, S->g, a, (const double *)x, b, r,
+                               (const DevState *)S->st, S->partials);
+            hipLaunchKernelGGL(k_fin_resnorm, FALM_FIN, s, S->st, (const double *)S->partials, nb);
             FALM_CHECK_LAUNCH("k_res_norm");
         }
         FALM_TRY(read_state(S, s));
@@ -515,6 +672,35 @@ static falm_status solve_stationary(falm_solver *S, const void *a, double *x, co
     info->err = S->hst->err;
     info->outer_it = 1;
     info->status = S->hst->err <= c.tol ? FALM_OK : FALM_ERR_NOT_CONVERGED;
+    return FALM_OK;
+}
+
+static falm_status prepare_matrix(falm_solver *S, const void *a, hipStream_t s) {
+    const Geom &g = S->g;
+    const int nmax = g.nx > g.ny ? (g.nx > g.nz ? g.nx : g.nz) : (g.ny > g.nz ? g.ny : g.nz);
+    double *d = S->sep64;
+    hipLaunchKernelGGL((k_sep_extract<double>), dim3((nmax + 255) / 256), dim3(256), 0, s, g,
+                       mat7<double>(a, S->n), d, d + g.nx, d + 2 * g.nx, d + 2 * g.nx + g.ny,
+                       d + 2 * g.nx + 2 * g.ny, d + 2 * g.nx + 2 * g.ny + g.nz);
+    FALM_CHECK_LAUNCH("k_sep_extract");
+    FALM_HIP(hipMemsetAsync(S->flag, 0, sizeof(int), s));
+    hipLaunchKernelGGL((k_sep_check<double>), FALM_GRID(S), s, S->tl, g, mat7<double>(a, S->n), d, d + g.nx,
+                       d + 2 * g.nx, d + 2 * g.nx + g.ny, d + 2 * g.nx + 2 * g.ny, d + 2 * g.nx + 2 * g.ny + g.nz,
+                       S->flag);
+    FALM_CHECK_LAUNCH("k_sep_check");
+    FALM_HIP(hipMemcpyAsync(S->hflag, S->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    FALM_HIP(hipStreamSynchronize(s));
+    S->sep = (*S->hflag == 0);
+    if (S->cfg.precision == FALM_SOLVE_MIXED) {
+        const size_t ncol = S->sep ? 1 : 7;
+        hipLaunchKernelGGL((k_convert<double, float>), ew_grid(ncol * S->n), dim3(EW_BLOCK), 0, s, (const double *)a,
+                           S->a32, ncol * S->n);
+        const size_t ns = 2 * ((size_t)g.nx + g.ny + g.nz);
+        hipLaunchKernelGGL((k_convert<double, float>), ew_grid(ns), dim3(EW_BLOCK), 0, s, (const double *)S->sep64,
+                           S->sep32, ns);
+        FALM_CHECK_LAUNCH("k_convert");
+    }
+    S->a_src = a;
     return FALM_OK;
 }
 
@@ -559,14 +745,7 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
 
 falm_status falm_solver_set_matrix(falm_solver *S, const void *a, void *stream) {
     if (!S || !a) return FALM_ERR_INVALID;
-    if (S->cfg.precision != FALM_SOLVE_MIXED) return FALM_OK;
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL((k_convert<double, float>), ew_grid(7 * S->n), dim3(EW_BLOCK), 0, s, (const double *)a, S->a32,
-                       7 * S->n);
-    FALM_CHECK_LAUNCH("k_convert");
-    S->a32_src = a;
-    S->a32_pinned = true;
-    return FALM_OK;
+    return prepare_matrix(S, a, (hipStream_t)stream);
 }
 
 falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void *b, void *r, void *stream,
@@ -582,21 +761,36 @@ falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void
     falm_solve_info tmp;
     if (!info) info = &tmp;
     hipStream_t s = (hipStream_t)stream;
-    if (S->cfg.type == FALM_LS_PBICGSTAB) {
-        if (S->cfg.precision == FALM_SOLVE_MIXED)
-            return solve_bicgstab_mixed(S, a, (double *)x, (const double *)b, (double *)r, s, info);
-        return solve_bicgstab_f64(S, a, (double *)x, (const double *)b, (double *)r, s, info);
+    double *xd = (double *)x, *rd = (double *)r;
+    const double *bd = (const double *)b;
+    const bool prepared = (S->a_src == a);
+    if (!prepared && S->cfg.precision == FALM_SOLVE_MIXED) {
+        // no declared matrix: build the float copy from this one now
+        FALM_TRY(prepare_matrix(S, a, s));
     }
-    return solve_stationary(S, a, (double *)x, (const double *)b, (double *)r, s, info);
+    const bool sep = (S->a_src == a) && S->sep;
+    if (S->cfg.type == FALM_LS_PBICGSTAB) {
+        if (S->cfg.precision == FALM_SOLVE_MIXED) {
+            if (sep)
+                return solve_bicgstab_mixed(S, sep_view<double>(S, (const double *)a, S->sep64),
+                                            sep_view<float>(S, S->a32, S->sep32), xd, bd, rd, s, info);
+            return solve_bicgstab_mixed(S, cols_view<double>(S, a), cols_view<float>(S, S->a32), xd, bd, rd, s, info);
+        }
+        if (sep) return solve_bicgstab_f64(S, sep_view<double>(S, (const double *)a, S->sep64), xd, bd, rd, s, info);
+        return solve_bicgstab_f64(S, cols_view<double>(S, a), xd, bd, rd, s, info);
+    }
+    if (sep) return solve_stationary(S, sep_view<double>(S, (const double *)a, S->sep64), xd, bd, rd, s, info);
+    return solve_stationary(S, cols_view<double>(S, a), xd, bd, rd, s, info);
 }
 
 falm_status falm_solver_destroy(falm_solver *S) {
     if (!S) return FALM_OK;
     void *bufs[] = {S->rr, S->p, S->q, S->s, S->pp, S->ss, S->t, S->w, S->r32, S->a32,
-                    S->st, S->partials, S->ticket};
+                    S->sep64, S->sep32, S->st, S->partials, S->flag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (S->hst) (void)hipHostFree(S->hst);
+    if (S->hflag) (void)hipHostFree(S->hflag);
     delete S;
     return FALM_OK;
 }

@@ -94,3 +94,18 @@ def test_missing_device_fails_loudly():
     from falm_amd import runtime
     with pytest.raises(runtime.FalmError):
         runtime.init(0)
+
+
+def test_cpp_class_api_compiles_and_links():
+    """include/falm/*.h (the FalmCFD / FalmEq / FalmMV / Matrix / CPM / Alm
+    class API) compiles warning-free and the wind-tunnel application links
+    against the library."""
+    from falm_amd import runtime
+    lib_dir = os.path.dirname(runtime.LIB_PATH)
+    src = os.path.join(ROOT, "falm_amd", "examples", "windtunnel.cpp")
+    with tempfile.TemporaryDirectory() as td:
+        exe = os.path.join(td, "wt")
+        r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I",
+                            os.path.join(ROOT, "include"), src, "-L", lib_dir, "-lfalm_amd", "-o", exe],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

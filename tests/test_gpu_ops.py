@@ -279,6 +279,38 @@ def test_bicgstab_solve(gpu, pc):
         assert rel_err(from_dev(dx), x_ref) < bar
 
 
+@pytest.mark.parametrize("stretch", [0.0, 0.02])
+@pytest.mark.parametrize("precision", [abi.SOLVE_F64, abi.SOLVE_MIXED])
+def test_declared_matrix_paths(gpu, stretch, precision):
+    """falm_solver_set_matrix: on a uniform mesh the off-diagonal columns are
+    one-index arrays and the solver switches to the compressed operator; on a
+    stretched mesh it keeps the seven columns.  Both reproduce the oracle's
+    first iterations and solve the system."""
+    dom = domain((18, 12, 10))
+    a, _, _, _ = _matrix(dom, abi.POISSON_OUTLET_XPLUS, stretch=stretch)
+    rng = np.random.default_rng(31)
+    b = rng.standard_normal(dom.ncell) * interior_mask(dom)
+    m = interior_mask(dom)
+    da, db = to_dev(a), to_dev(b)
+    if precision == abi.SOLVE_F64:
+        cfg = abi.solver_config(maxit=3, tol=1e-30, pc_type=abi.LS_SOR, pc_maxit=2, pc_relax=1.2)
+        x_ref, r_ref = np.zeros(dom.ncell), np.zeros(dom.ncell)
+        info_ref = O.Solver(dom, cfg).solve(a, x_ref, b, r_ref)
+        S = gpu.Solver(dom, cfg)
+        S.set_matrix(da)
+        dx, dr = to_dev(np.zeros(dom.ncell)), to_dev(np.zeros(dom.ncell))
+        info = S.solve(da, dx, db, dr)
+        assert info.it == info_ref.it == 3
+        assert rel_err(from_dev(dx), x_ref) < 1e-11
+    cfg = abi.solver_config(maxit=1000, tol=1e-12, pc_type=abi.LS_SOR, pc_maxit=2, pc_relax=1.2, precision=precision)
+    S = gpu.Solver(dom, cfg)
+    S.set_matrix(da)
+    dx, dr = to_dev(np.zeros(dom.ncell)), to_dev(np.zeros(dom.ncell))
+    info = S.solve(da, dx, db, dr)
+    assert info.status == abi.FALM_OK
+    assert np.sqrt(np.sum(O.residual(dom, a, from_dev(dx), b)[m] ** 2)) / m.sum() <= 1.01e-12
+
+
 def test_sor_solver(gpu):
     dom = domain((24, 20, 1))
     a, _, _, _ = _matrix(dom, abi.POISSON_2D, stretch=0.0)
