@@ -10,7 +10,13 @@ beside it on the same workload.  A step is one full time step (ALM, pseudo
 velocity, face flux, Poisson RHS, Poisson solve, projection, boundary fill,
 SGS, divergence monitor) with all fields resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+Per-kernel HBM rooflines come from HIP events around each launch of one
+kernel family at a time (extra steps after the timed ones, so the timed
+steps carry no event overhead), with algorithmic bytes per launch from the
+operand-list model (every array a kernel touches counted once per cell).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only mixed|fp64]
+                    [--no-cpu] [--calibrate]
 """
 import argparse
 import json
@@ -23,9 +29,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MLUPS + steps/s @512x128x128 ALM: mixed vs FP64 GPU at div<1e-6; HBM GB/s"
 PEAK_HBM_GBS = 8000.0
+DOMINANT = "pseudo_u"  # the largest kernel of the step
 
 
-def run_gpu(precision, steps, warmup, n, time_family):
+def families():
+    from falm_amd import _abi as abi
+    return {
+        "pseudo_u": (abi.TIME_PSEUDO_U, "k_pseudo_u (Upwind3 + diffusion + force)"),
+        "project": (abi.TIME_PROJECT, "k_project (pressure-gradient projection, cell + face)"),
+        "face_flux": (abi.TIME_FACE_FLUX, "k_u_to_uu (face flux)"),
+        "rhs": (abi.TIME_RHS, "k_divergence (Poisson right-hand side)"),
+        "sgs": (abi.TIME_SGS, "k_smagorinsky"),
+        "divergence": (abi.TIME_DIVERGENCE, "k_div_norm (divergence + |div|^2)"),
+        "solver_init": (abi.TIME_SOLVER_INIT, "k_bicg_init (b - A x in double + dots)"),
+        "sor": (abi.TIME_SOR, "k_sor_op (RB-SOR preconditioner half sweep)"),
+        "mv": (abi.TIME_MV, "k_bicg_mvdot (A p + dot)"),
+        "solver_update": (abi.TIME_SOLVER_UPDATE, "k_bicg_update (x, r update + dots)"),
+        "alm": (abi.TIME_ALM, "ALM step (points, box, factors, projection, totals)"),
+    }
+
+
+def run_gpu(precision, steps, warmup, n, fam_steps):
     """Every falm_case_step call returns after synchronising the case's
     stream, so the wall clock brackets completed device work."""
     from falm_amd import _abi as abi
@@ -35,7 +59,6 @@ def run_gpu(precision, steps, warmup, n, time_family):
     c = runtime.Case(setup)
     for _ in range(warmup):
         c.step(1)
-    runtime.timing_enable(time_family)
     its, outer = [], []
     t0 = time.perf_counter()
     info = None
@@ -44,9 +67,20 @@ def run_gpu(precision, steps, warmup, n, time_family):
         its.append(info.solver_it)
         outer.append(info.outer_it)
     dt = time.perf_counter() - t0
-    launches, kms, kbytes = runtime.timing_read()
-    runtime.timing_enable(abi.TIME_NONE)
     u = c.get(abi.FIELD_U)
+    kern = {}
+    for name, (fam, desc) in families().items():
+        runtime.timing_enable(fam)
+        for _ in range(fam_steps):
+            c.step(1)
+        launches, kms, kbytes = runtime.timing_read()
+        runtime.timing_enable(abi.TIME_NONE)
+        if launches == 0 or kms <= 0:
+            continue
+        gbs = kbytes / (kms * 1e-3) / 1e9
+        kern[name] = {"kernel": desc, "launches_per_step": launches / fam_steps,
+                      "avg_launch_us": 1e3 * kms / launches, "bytes_per_launch": kbytes / launches,
+                      "gbs": gbs, "frac": gbs / PEAK_HBM_GBS}
     c.close()
     cells = setup.n_interior
     return {
@@ -58,8 +92,24 @@ def run_gpu(precision, steps, warmup, n, time_family):
         "div": info.div,
         "solver_err": info.solver_err,
         "status": info.status,
-        "kernel": {"launches": launches, "ms": kms, "bytes": kbytes},
+        "kernels": kern,
     }, u, setup
+
+
+def calibrate_copy(nbytes=1 << 30, reps=3):
+    """A plain streaming kernel of known traffic (falm_scale over a double
+    array: read + write of nbytes each) for scaling HBM counter readings."""
+    import ctypes as C
+
+    from falm_amd import _abi as abi
+    from falm_amd import runtime
+    p = C.c_void_p()
+    runtime.call("falm_malloc", C.byref(p), nbytes)
+    for _ in range(reps):
+        runtime.call("falm_scale", abi.F64, p, nbytes // 8, 1.0, None)
+    runtime.call("falm_stream_sync", None)
+    runtime.call("falm_free", p)
+    return {"kernel": "k_scale", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps}
 
 
 def cpu_baseline(n, steps=2, warmup=1):
@@ -92,8 +142,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, nargs=3, default=[512, 128, 128])
+    ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--fam-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--calibrate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -107,65 +160,77 @@ def main():
 
     runtime.init(0)
     n = tuple(args.n)
-    r64, u64, setup = run_gpu(abi.CASE_F64, args.steps, args.warmup, n, abi.TIME_SOR)
-    rmx, umx, _ = run_gpu(abi.CASE_MIXED, args.steps, args.warmup, n, abi.TIME_SOR)
-    nx, ny, nz = setup.shape
-    mask = np.zeros((nz, ny, nx), dtype=bool)
-    mask[2:-2, 2:-2, 2:-2] = True
-    m3 = np.tile(mask.ravel(), 3)
-    rel_l2 = float(np.linalg.norm(umx[m3] - u64[m3]) / np.linalg.norm(u64[m3]))
+    calib = calibrate_copy() if args.calibrate else None
+    r64 = rmx = u64 = umx = None
+    if args.only in ("both", "fp64"):
+        r64, u64, setup = run_gpu(abi.CASE_F64, args.steps, args.warmup, n, args.fam_steps)
+    if args.only in ("both", "mixed"):
+        rmx, umx, setup = run_gpu(abi.CASE_MIXED, args.steps, args.warmup, n, args.fam_steps)
+    head = rmx if rmx is not None else r64
+    rel_l2 = None
+    if r64 is not None and rmx is not None:
+        nx, ny, nz = setup.shape
+        mask = np.zeros((nz, ny, nx), dtype=bool)
+        mask[2:-2, 2:-2, 2:-2] = True
+        m3 = np.tile(mask.ravel(), 3)
+        rel_l2 = float(np.linalg.norm(umx[m3] - u64[m3]) / np.linalg.norm(u64[m3]))
 
-    k = rmx["kernel"]
-    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else None
-    roofline = {
-        "bound": "hbm",
-        "kernel": "k_sor<float,float> (red-black SOR half sweep, float Krylov pass)",
-        "achieved": achieved,
-        "peak": PEAK_HBM_GBS,
-        "unit": "GB/s",
-        "frac": achieved / PEAK_HBM_GBS if achieved else None,
-        "traffic": None,
-        "launches": k["launches"],
-        "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1),
-        "algorithmic_bytes_per_launch": k["bytes"] / max(k["launches"], 1),
-    }
-    k64 = r64["kernel"]
-    r64_achieved = k64["bytes"] / (k64["ms"] * 1e-3) / 1e9 if k64["ms"] > 0 else None
-    tr_path = os.path.join(ROOT, "profiles", "traffic_sor.json")
-    if os.path.exists(tr_path):
-        with open(tr_path) as f:
-            roofline["traffic"] = json.load(f).get("bytes_per_launch")
+    dom = head["kernels"].get(DOMINANT)
+    roofline = None
+    if dom:
+        roofline = {
+            "bound": "hbm",
+            "kernel": dom["kernel"],
+            "achieved": dom["gbs"],
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": dom["frac"],
+            "traffic": None,
+            "avg_launch_us": dom["avg_launch_us"],
+            "algorithmic_bytes_per_launch": dom["bytes_per_launch"],
+        }
+        tr_path = os.path.join(ROOT, "profiles", f"traffic_{DOMINANT}.json")
+        if os.path.exists(tr_path):
+            with open(tr_path) as f:
+                tr = json.load(f)
+            key = "mixed" if rmx is not None else "fp64"
+            if key in tr:
+                roofline["traffic"] = tr[key].get("hbm_bytes_per_launch")
 
     cpu = None if args.no_cpu else cpu_baseline(n, steps=args.cpu_steps)
 
+    def strip(r):
+        return None if r is None else {k_: v for k_, v in r.items()}
+
     out = {
         "metric": METRIC,
-        "value": rmx["mlups"],
+        "value": head["mlups"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": rmx["ms_per_step"],
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 stencils+Krylov / f64 defect correction",
+        "dtype": "f32 stencils+Krylov / f64 defect correction" if rmx is not None else "f64",
         "data": "synthetic (uniform inflow + seeded inlet noise, synthetic blade polar)",
         "config": {
             "workload": f"windtunnel {n[0]}x{n[1]}x{n[2]} cells + 1 ALM rotor (3x32 points), dx=D/32, "
                         "Re 2.5e5, Cs 0.1, dt 0.3dx/U, BiCGStab+RB-SOR(2, w=1.5) tol 1e-6",
             "cells": setup.n_interior,
-            "precision": "mixed",
+            "precision": "mixed" if rmx is not None else "fp64",
         },
-        "steps_per_s": rmx["steps_per_s"],
-        "mixed": {k_: v for k_, v in rmx.items() if k_ != "kernel"},
-        "fp64": {k_: v for k_, v in r64.items() if k_ != "kernel"},
-        "fp64_sor_achieved_gbs": r64_achieved,
-        "speedup_mixed_vs_fp64": r64["ms_per_step"] / rmx["ms_per_step"],
+        "steps_per_s": head["steps_per_s"],
+        "mixed": strip(rmx),
+        "fp64": strip(r64),
+        "speedup_mixed_vs_fp64": (r64["ms_per_step"] / rmx["ms_per_step"]) if (r64 and rmx) else None,
         "velocity_rel_l2_mixed_vs_fp64": rel_l2,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if calib:
+        out["calibration"] = calib
     print(json.dumps(out))
 
 

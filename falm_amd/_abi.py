@@ -34,6 +34,7 @@ PBC_NONE, PBC_NEUMANN, PBC_ZERO = 0, 1, 2
 POISSON_NEUMANN, POISSON_OUTLET_XPLUS, POISSON_2D = 0, 1, 2
 
 ALM_FIX_DRAG_TANGENT = 1
+ALM_CUTOFF_ROUNDING = 6.1  # FALM_ALM_CUTOFF_ROUNDING
 
 CASE_WINDTUNNEL, CASE_LID3D, CASE_LID2D = 0, 1, 2
 CASE_F64, CASE_MIXED = 0, 1
@@ -43,7 +44,8 @@ FIELD_U, FIELD_UU, FIELD_P, FIELD_NUT, FIELD_FF, FIELD_DIV, FIELD_RHS, FIELD_POI
 FIELD_NCOL = {FIELD_U: 3, FIELD_UU: 3, FIELD_P: 1, FIELD_NUT: 1, FIELD_FF: 3, FIELD_DIV: 1,
               FIELD_RHS: 1, FIELD_POI_A: 7, FIELD_XYZ: 3, FIELD_KX: 3, FIELD_G: 3, FIELD_JA: 1}
 
-TIME_NONE, TIME_SOR, TIME_MV, TIME_PSEUDO_U, TIME_ALM, TIME_SOLVER_UPDATE = range(6)
+TIME_NONE, TIME_SOR, TIME_MV, TIME_PSEUDO_U, TIME_ALM, TIME_SOLVER_UPDATE, TIME_PROJECT, TIME_FACE_FLUX, \
+    TIME_SOLVER_INIT, TIME_SGS, TIME_RHS, TIME_DIVERGENCE = range(12)
 
 
 class Domain(C.Structure):

@@ -134,12 +134,14 @@ def build_ap_host(rr, chord, twist, cl, cd, n_turbine, n_blade, n_ap_per_blade, 
 
 
 def rotor(h, base, radius=0.5, tsr=7.0, u_inf=1.0, n_blade=3, n_ap_per_blade=32, eps_cells=2.0,
-          chord=(0.08, 0.02), twist=(20.0, 0.0), hub_radius=0.0, cutoff=0.0, flags=0,
+          chord=(0.08, 0.02), twist=(20.0, 0.0), hub_radius=0.0, cutoff=abi.ALM_CUTOFF_ROUNDING, flags=0,
           angle_type=0, motion=(0.0, 0.0, 0.0), surge=(0.0, 0.0, 0.0), pitch_deg=(),
           twist_offset=None, extra_bases=()):
     """One (or more, via extra_bases) three-bladed rotor of diameter 2*radius:
     chord and twist linear in r, synthetic polar, Gaussian width eps_cells*h,
-    tip rate Omega = tsr*u_inf/radius."""
+    tip rate Omega = tsr*u_inf/radius.  cutoff: culling radius in units of
+    eps (default: terms below 2^-53 of a point's peak are dropped; 0 keeps
+    every term, the reference's all-pairs sum)."""
     rr, ch, tw, attack, cl, cd = blade_table(radius, chord[0], chord[1], twist[0], twist[1])
     bases = [tuple(base)] + [tuple(b) for b in extra_bases]
     turbines = [make_turbine(b, tip_rate=tsr * u_inf / radius, angle_type=angle_type, motion=motion,

@@ -17,7 +17,7 @@
 #include <cmath>
 #include <vector>
 
-#include "common.hpp"
+#include "alm.hpp"
 
 namespace falm {
 
@@ -39,6 +39,11 @@ struct AlmDev {
     double *ap_xyz, *ap_force, *ap_torque, *ap_thrust, *torque, *thrust;
     int *ap_ijk, *ap_rank;
     double *fx, *fy, *fz;  // per-point axis factors exp(-(x_i - X)^2/eps^2): [nap][nx], [nap][ny], [nap][nz]
+    double cutr;           // culling radius (length)
+    // cell boxes (ilo, ihi, jlo, jhi, klo, khi; empty when ilo > ihi):
+    // [0,6) cells that may receive force this step, [6,12) its union with
+    // the previous step's, [12,18) the previous step's
+    int *box;
 };
 
 // per-point, per-axis Gaussian factors of the projection
@@ -126,8 +131,31 @@ __device__ inline double trilinear(double xd, double yd, double zd, const double
     return c0123 * (1. - zd) + c4567 * zd;
 }
 
+// The 1-D cell-centre arrays and the attack-angle table are staged in LDS
+// when they fit (kPointsLds bytes), so the searches below are LDS-latency
+// chains instead of dependent global loads.
+constexpr int kPointsLds = 48 * 1024;
+
 template <class T>
-__global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, double t) {
+__global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, double t, int staged) {
+    extern __shared__ double stage[];
+    const double *X = A.x, *Y = A.y, *Z = A.z, *AT = A.attack;
+    if (staged) {
+        const int L = A.nx + A.ny + A.nz + A.nat;
+        for (int q = threadIdx.x; q < L; q += blockDim.x) {
+            double v;
+            if (q < A.nx) v = A.x[q];
+            else if (q < A.nx + A.ny) v = A.y[q - A.nx];
+            else if (q < A.nx + A.ny + A.nz) v = A.z[q - A.nx - A.ny];
+            else v = A.attack[q - A.nx - A.ny - A.nz];
+            stage[q] = v;
+        }
+        __syncthreads();
+        X = stage;
+        Y = stage + A.nx;
+        Z = stage + A.nx + A.ny;
+        AT = stage + A.nx + A.ny + A.nz;
+    }
     const int ap = blockIdx.x * blockDim.x + threadIdx.x;
     if (ap >= A.nap) return;
     const int napt = A.napb * A.nb;
@@ -157,7 +185,7 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
     rot(c1, nang, T0.angle_type, P);
     for (int m = 0; m < 3; m++) P[m] += base[m];
     for (int m = 0; m < 3; m++) A.ap_xyz[3 * ap + m] = P[m];
-    const int I = find_index(A.x, P[0], A.nx), J = find_index(A.y, P[1], A.ny), K = find_index(A.z, P[2], A.nz);
+    const int I = find_index(X, P[0], A.nx), J = find_index(Y, P[1], A.ny), K = find_index(Z, P[2], A.nz);
     A.ap_ijk[3 * ap] = I;
     A.ap_ijk[3 * ap + 1] = J;
     A.ap_ijk[3 * ap + 2] = K;
@@ -174,7 +202,7 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
     }
     const double dr = (A.radius - A.hub_radius) / A.napb;
     const int i0 = I, i1 = I + 1, j0 = J, j1 = J + 1, k0 = K, k1 = K + 1;
-    const double x0 = A.x[i0], x1 = A.x[i1], y0 = A.y[j0], y1 = A.y[j1], z0 = A.z[k0], z1 = A.z[k1];
+    const double x0 = X[i0], x1 = X[i1], y0 = Y[j0], y1 = Y[j1], z0 = Z[k0], z1 = Z[k1];
     const double xd = fabs(P[0] - x0) / fabs(x1 - x0);
     const double yd = fabs(P[1] - y0) / fabs(y1 - y0);
     const double zd = fabs(P[2] - z0) / fabs(z1 - z0);
@@ -206,21 +234,25 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
     const double att = phi * 180. / kPi - twist;
     double cl = 0.0, cd = 0.0;
     const int nat = A.nat;
-    if (att < A.attack[0]) {
+    if (att < AT[0]) {
         cl = A.cl[ap];
         cd = A.cd[ap];
-    } else if (att >= A.attack[nat - 1]) {
+    } else if (att >= AT[nat - 1]) {
         cl = A.cl[ap + (nat - 1) * A.nap];
         cd = A.cd[ap + (nat - 1) * A.nap];
-    } else {
-        for (int a = 0; a < nat - 1; a++) {
-            if (A.attack[a] <= att && A.attack[a + 1] > att) {
-                const double p = (att - A.attack[a]) / (A.attack[a + 1] - A.attack[a]);
-                cl = (1. - p) * A.cl[ap + a * A.nap] + p * A.cl[ap + (a + 1) * A.nap];
-                cd = (1. - p) * A.cd[ap + a * A.nap] + p * A.cd[ap + (a + 1) * A.nap];
-                break;
-            }
+    } else if (att == att) {
+        // the interval AT[a] <= att < AT[a+1]: the reference scans for it
+        // (src/alm/apHandler.h:121-141); on an ascending table a bisection
+        // finds the same one.  NaN matches no interval (cl = cd = 0).
+        int a = 0, hi = nat - 1;
+        while (hi - a > 1) {
+            const int mid = (a + hi) >> 1;
+            if (AT[mid] <= att) a = mid;
+            else hi = mid;
         }
+        const double p = (att - AT[a]) / (AT[a + 1] - AT[a]);
+        cl = (1. - p) * A.cl[ap + a * A.nap] + p * A.cl[ap + (a + 1) * A.nap];
+        cd = (1. - p) * A.cd[ap + a * A.nap] + p * A.cd[ap + (a + 1) * A.nap];
     }
     const double fl = .5 * cl * urel2 * chord * dr;
     const double fd = .5 * cd * urel2 * chord * dr;
@@ -236,10 +268,112 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
     A.ap_thrust[ap] = fx;
 }
 
+// The cells that can receive force: centres within cutr (plus one cell) of
+// the bounding box of the points along every axis.  One workgroup.
+__global__ __launch_bounds__(256) void k_alm_box(AlmDev A) {
+    __shared__ double smn[3][4], smx[3][4];
+    __shared__ int slo[3][4], shi[3][4];
+    __shared__ double lim[6];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int ap = threadIdx.x; ap < A.nap; ap += blockDim.x)
+        for (int m = 0; m < 3; m++) {
+            mn[m] = fmin(mn[m], A.ap_xyz[3 * ap + m]);
+            mx[m] = fmax(mx[m], A.ap_xyz[3 * ap + m]);
+        }
+    for (int m = 0; m < 3; m++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[m] = fmin(mn[m], __shfl_down(mn[m], off, 64));
+            mx[m] = fmax(mx[m], __shfl_down(mx[m], off, 64));
+        }
+        if (lane == 0) {
+            smn[m][wid] = mn[m];
+            smx[m][wid] = mx[m];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int m = threadIdx.x;
+        double a = smn[m][0], b = smx[m][0];
+        for (int w = 1; w < 4; w++) {
+            a = fmin(a, smn[m][w]);
+            b = fmax(b, smx[m][w]);
+        }
+        lim[2 * m] = a - A.cutr;
+        lim[2 * m + 1] = b + A.cutr;
+    }
+    __syncthreads();
+    const double *C[3] = {A.x, A.y, A.z};
+    const int N[3] = {A.nx, A.ny, A.nz};
+    int lo[3], hi[3];
+    for (int m = 0; m < 3; m++) {
+        lo[m] = 0x7fffffff;
+        hi[m] = -1;
+        for (int i = threadIdx.x; i < N[m]; i += blockDim.x) {
+            const double c = C[m][i];
+            if (c >= lim[2 * m] && c <= lim[2 * m + 1]) {
+                lo[m] = min(lo[m], i);
+                hi[m] = max(hi[m], i);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[m] = min(lo[m], __shfl_down(lo[m], off, 64));
+            hi[m] = max(hi[m], __shfl_down(hi[m], off, 64));
+        }
+        if (lane == 0) {
+            slo[m][wid] = lo[m];
+            shi[m][wid] = hi[m];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int cur[6];
+        bool empty = false;
+        for (int m = 0; m < 3; m++) {
+            int a = slo[m][0], b = shi[m][0];
+            for (int w = 1; w < 4; w++) {
+                a = min(a, slo[m][w]);
+                b = max(b, shi[m][w]);
+            }
+            if (b < a) {
+                empty = true;
+                continue;
+            }
+            // one more cell on each side, within the interior
+            a = max(a - 1, A.gc);
+            b = min(b + 1, N[m] - A.gc - 1);
+            cur[2 * m] = a;
+            cur[2 * m + 1] = b;
+            if (b < a) empty = true;
+        }
+        if (empty)
+            for (int m = 0; m < 3; m++) {
+                cur[2 * m] = 1;
+                cur[2 * m + 1] = 0;
+            }
+        int *B = A.box;
+        for (int q = 0; q < 6; q++) B[12 + q] = B[q];
+        const bool pe = B[12] > B[13], ce = cur[0] > cur[1];
+        for (int m = 0; m < 3; m++) {
+            B[2 * m] = cur[2 * m];
+            B[2 * m + 1] = cur[2 * m + 1];
+            B[6 + 2 * m] = pe ? cur[2 * m] : ce ? B[12 + 2 * m] : min(cur[2 * m], B[12 + 2 * m]);
+            B[6 + 2 * m + 1] = pe ? cur[2 * m + 1] : ce ? B[12 + 2 * m + 1] : max(cur[2 * m + 1], B[12 + 2 * m + 1]);
+        }
+    }
+}
+
+__device__ inline bool box_meets(const int *b, int ilo, int ihi, int jlo, int jhi, int klo, int khi) {
+    return b[0] <= b[1] && ilo <= b[1] && ihi >= b[0] && jlo <= b[3] && jhi >= b[2] && klo <= b[5] && khi >= b[4];
+}
+
 constexpr int kCullChunk = 256;
 
+// union_only: the caller's ff is already zero outside the union box, so
+// tiles outside it are left alone; otherwise every interior cell is written.
 template <class T>
-__global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0, T *ff1, T *ff2, double cut2) {
+__global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0, T *ff1, T *ff2, double cut2,
+                                                     int union_only) {
     __shared__ double sf0[kCullChunk], sf1[kCullChunk], sf2[kCullChunk], sez[kCullChunk];
     __shared__ double sey[TY][kCullChunk];
     __shared__ int sap[kCullChunk];
@@ -247,13 +381,32 @@ __global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0
     __shared__ double box[6];
     int i, j, k;
     const bool valid = tile_cell(tl, i, j, k);
+    {
+        // tile extent in cells, against the force boxes
+        const int tile = xcd_tile(blockIdx.x, gridDim.x);
+        int bx, by, kk;
+        tile_coords(tl, tile, bx, by, kk);
+        const int ilo = tl.box.lo[0] + bx * TX, ihi = min(ilo + TX, tl.box.lo[0] + tl.box.n[0]) - 1;
+        const int jlo = tl.box.lo[1] + by * TY, jhi = min(jlo + TY, tl.box.lo[1] + tl.box.n[1]) - 1;
+        const int kc = tl.box.lo[2] + kk;
+        if (union_only && !box_meets(A.box + 6, ilo, ihi, jlo, jhi, kc, kc)) return;
+        if (!box_meets(A.box, ilo, ihi, jlo, jhi, kc, kc)) {
+            if (valid) {
+                const int cc = i + A.nx * j + A.nx * A.ny * k;
+                ff0[cc] = T(0);
+                ff1[cc] = T(0);
+                ff2[cc] = T(0);
+            }
+            return;
+        }
+    }
+    const int *B = A.box;
+    const bool inbox = valid && i >= B[0] && i <= B[1] && j >= B[2] && j <= B[3] && k >= B[4] && k <= B[5];
     // tile bounding box in coordinates (cell centres of the tile)
     if (threadIdx.x == 0) {
         const int tile = xcd_tile(blockIdx.x, gridDim.x);
-        const int txy = tl.tx * tl.ty;
-        const int kk = tile / txy;
-        const int r = tile - kk * txy;
-        const int by = r / tl.tx, bx = r - by * tl.tx;
+        int bx, by, kk;
+        tile_coords(tl, tile, bx, by, kk);
         const int ilo = tl.box.lo[0] + bx * TX, ihi = min(ilo + TX, tl.box.lo[0] + tl.box.n[0]) - 1;
         const int jlo = tl.box.lo[1] + by * TY, jhi = min(jlo + TY, tl.box.lo[1] + tl.box.n[1]) - 1;
         const int kc = tl.box.lo[2] + kk;
@@ -320,24 +473,42 @@ __global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0
         __syncthreads();
     }
     if (valid) {
+        // cells outside the box receive nothing (every term there is below
+        // the cutoff by more than a cell width)
         const int cc = i + A.nx * j + A.nx * A.ny * k;
-        ff0[cc] = (T)fx;
-        ff1[cc] = (T)fy;
-        ff2[cc] = (T)fz;
+        ff0[cc] = inbox ? (T)fx : T(0);
+        ff1[cc] = inbox ? (T)fy : T(0);
+        ff2[cc] = inbox ? (T)fz : T(0);
     }
 }
 
+// per-turbine sums in ascending point order; the point values are first
+// gathered into LDS by the whole workgroup
 __global__ void k_alm_totals(AlmDev A) {
+    __shared__ double stq[1024], sth[1024];
     const int tid = blockIdx.x;
-    if (tid >= A.nt || threadIdx.x != 0) return;
+    if (tid >= A.nt) return;
     const int napt = A.napb * A.nb;
     double tq = 0.0, th = 0.0;
-    for (int a = 0; a < napt; a++) {
-        tq += A.ap_torque[tid * napt + a];
-        th += A.ap_thrust[tid * napt + a];
+    for (int a0 = 0; a0 < napt; a0 += 1024) {
+        const int m = napt - a0 < 1024 ? napt - a0 : 1024;
+        for (int a = threadIdx.x; a < m; a += blockDim.x) {
+            stq[a] = A.ap_torque[tid * napt + a0 + a];
+            sth[a] = A.ap_thrust[tid * napt + a0 + a];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int a = 0; a < m; a++) {
+                tq += stq[a];
+                th += sth[a];
+            }
+        }
+        __syncthreads();
     }
-    A.torque[tid] = tq;
-    A.thrust[tid] = th;
+    if (threadIdx.x == 0) {
+        A.torque[tid] = tq;
+        A.thrust[tid] = th;
+    }
 }
 
 }  // namespace falm
@@ -350,6 +521,7 @@ struct falm_alm {
     std::vector<void *> bufs;
     double cut2;
     int nt, nap, napb;
+    bool full_next = true;  // the next owned-ff step writes every cell
 };
 
 namespace falm {
@@ -365,6 +537,49 @@ static falm_status upload(falm_alm *A, const V *src, size_t n, V **dst) {
     *dst = (V *)p;
     return FALM_OK;
 }
+
+// One ALM step.  owned_ff: ff belongs to the caller's step driver and is
+// only changed by these steps, so after the first full write only the
+// union of this and the previous step's force box needs rewriting.
+falm_status alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, hipStream_t s, bool owned_ff) {
+    const size_t n = ncell_of(&A->dom);
+    Geom g = make_geom(&A->dom);
+    Tiling tl = make_tiling(interior_box(g));
+    TimedLaunch tm(FALM_TIME_ALM, s);
+    const int nfac = A->nap * (A->d.nx + A->d.ny + A->d.nz);
+    const size_t stage_bytes = (size_t)(A->d.nx + A->d.ny + A->d.nz + A->d.nat) * sizeof(double);
+    const int staged = stage_bytes <= (size_t)kPointsLds;
+    const size_t lds = staged ? stage_bytes : 0;
+    const int union_only = (owned_ff && !A->full_next) ? 1 : 0;
+    A->full_next = !owned_ff;
+    if (prec == FALM_F32)
+        hipLaunchKernelGGL((k_alm_points<float>), dim3((A->nap + 255) / 256), dim3(256), lds, s, A->d,
+                           col<float>(u, n, 0), col<float>(u, n, 1), col<float>(u, n, 2), t, staged);
+    else
+        hipLaunchKernelGGL((k_alm_points<double>), dim3((A->nap + 255) / 256), dim3(256), lds, s, A->d,
+                           col<double>(u, n, 0), col<double>(u, n, 1), col<double>(u, n, 2), t, staged);
+    FALM_CHECK_LAUNCH("k_alm_points");
+    hipLaunchKernelGGL(k_alm_box, dim3(1), dim3(256), 0, s, A->d);
+    hipLaunchKernelGGL(k_alm_factors, dim3((nfac + 255) / 256), dim3(256), 0, s, A->d);
+    FALM_CHECK_LAUNCH("k_alm_factors");
+    if (tl.tiles > 0) {
+        if (prec == FALM_F32)
+            hipLaunchKernelGGL((k_alm_project<float>), one_tile_grid(tl), tiling_block(), 0, s, tl, A->d,
+                               col<float>(ff, n, 0), col<float>(ff, n, 1), col<float>(ff, n, 2), A->cut2, union_only);
+        else
+            hipLaunchKernelGGL((k_alm_project<double>), one_tile_grid(tl), tiling_block(), 0, s, tl, A->d,
+                               col<double>(ff, n, 0), col<double>(ff, n, 1), col<double>(ff, n, 2), A->cut2,
+                               union_only);
+        FALM_CHECK_LAUNCH("k_alm_project");
+    }
+    hipLaunchKernelGGL(k_alm_totals, dim3(A->nt), dim3(256), 0, s, A->d);
+    FALM_CHECK_LAUNCH("k_alm_totals");
+    tm.done((double)tl.box.size() * 3.0 * prec_bytes(prec));
+    return FALM_OK;
+}
+
+const int *alm_force_box(const falm_alm *A) { return A->d.box; }
+void alm_ff_reset(falm_alm *A) { A->full_next = true; }
 
 }  // namespace falm
 
@@ -449,7 +664,8 @@ falm_status falm_alm_create(const falm_domain *dom, const falm_alm_config *c, co
         (st = upload<int>(A, nullptr, 3 * d.nap, &d.ap_ijk)) || (st = upload<int>(A, nullptr, d.nap, &d.ap_rank)) ||
         (st = upload<double>(A, nullptr, (size_t)d.nap * d.nx, &d.fx)) ||
         (st = upload<double>(A, nullptr, (size_t)d.nap * d.ny, &d.fy)) ||
-        (st = upload<double>(A, nullptr, (size_t)d.nap * d.nz, &d.fz))) {
+        (st = upload<double>(A, nullptr, (size_t)d.nap * d.nz, &d.fz)) ||
+        (st = upload<int>(A, nullptr, 18, &d.box))) {
         falm_alm_destroy(A);
         return st;
     }
@@ -464,6 +680,17 @@ falm_status falm_alm_create(const falm_domain *dom, const falm_alm_config *c, co
     d.x = dx;
     d.y = dy;
     d.z = dz;
+    d.cutr = sqrt(A->cut2) * d.eps;
+    const int empty[18] = {1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0};
+    hipError_t e = hipMemcpy(d.box, empty, sizeof(empty), hipMemcpyHostToDevice);
+    // the uploads and zero fills ran on the null stream: finish them before
+    // the caller's (possibly non-blocking) stream uses the buffers
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        st = hip_status(e, "hipMemcpy");
+        falm_alm_destroy(A);
+        return st;
+    }
     *out = A;
     return FALM_OK;
 }
@@ -478,32 +705,7 @@ falm_status falm_alm_set_twist_offset(falm_alm *A, const double *off) {
 
 falm_status falm_alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, void *stream) {
     if (!A || !u || !ff) return FALM_ERR_INVALID;
-    hipStream_t s = (hipStream_t)stream;
-    const size_t n = ncell_of(&A->dom);
-    Geom g = make_geom(&A->dom);
-    Tiling tl = make_tiling(interior_box(g));
-    TimedLaunch tm(FALM_TIME_ALM, s);
-    const int nfac = A->nap * (A->d.nx + A->d.ny + A->d.nz);
-    if (prec == FALM_F32) {
-        hipLaunchKernelGGL((k_alm_points<float>), dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d, col<float>(u, n, 0),
-                           col<float>(u, n, 1), col<float>(u, n, 2), t);
-        FALM_CHECK_LAUNCH("k_alm_points");
-        hipLaunchKernelGGL(k_alm_factors, dim3((nfac + 255) / 256), dim3(256), 0, s, A->d);
-        hipLaunchKernelGGL((k_alm_project<float>), tiling_grid(tl), tiling_block(), 0, s, tl, A->d,
-                           col<float>(ff, n, 0), col<float>(ff, n, 1), col<float>(ff, n, 2), A->cut2);
-    } else {
-        hipLaunchKernelGGL((k_alm_points<double>), dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d,
-                           col<double>(u, n, 0), col<double>(u, n, 1), col<double>(u, n, 2), t);
-        FALM_CHECK_LAUNCH("k_alm_points");
-        hipLaunchKernelGGL(k_alm_factors, dim3((nfac + 255) / 256), dim3(256), 0, s, A->d);
-        hipLaunchKernelGGL((k_alm_project<double>), tiling_grid(tl), tiling_block(), 0, s, tl, A->d,
-                           col<double>(ff, n, 0), col<double>(ff, n, 1), col<double>(ff, n, 2), A->cut2);
-    }
-    FALM_CHECK_LAUNCH("k_alm_project");
-    hipLaunchKernelGGL(k_alm_totals, dim3(A->nt), dim3(64), 0, s, A->d);
-    FALM_CHECK_LAUNCH("k_alm_totals");
-    tm.done((double)tl.box.size() * 3.0 * prec_bytes(prec));
-    return FALM_OK;
+    return falm::alm_step(A, prec, u, ff, t, (hipStream_t)stream, false);
 }
 
 falm_status falm_alm_get(falm_alm *A, double *ap_xyz, double *ap_force, double *torque, double *thrust,

@@ -198,29 +198,29 @@ __global__ __launch_bounds__(256) void k_poisson_matrix(Tiling tl, Geom g, int k
                                                         const T *__restrict__ gy, const T *__restrict__ gz,
                                                         const T *__restrict__ ja, T *a0, T *a1, T *a2, T *a3, T *a4,
                                                         T *a5, T *a6) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    const int gi = ox + i, gj = oy + j, gk = oz + k, gc = g.gc;
-    const int xe_lim = kind == FALM_POISSON_OUTLET_XPLUS ? gnx - gc : gnx - gc - 1;
-    T ac = 0, ae = 0, aw = 0, an = 0, as = 0, at = 0, ab = 0;
-    const T jacob = ja[cc];
-    T coef;
-    coef = T(0.5) * (gx[cc] + gx[cc + 1]) / jacob;
-    if (gi < xe_lim) { ac -= coef; ae = coef; }
-    coef = T(0.5) * (gx[cc] + gx[cc - 1]) / jacob;
-    if (gi > gc) { ac -= coef; aw = coef; }
-    coef = T(0.5) * (gy[cc] + gy[cc + g.nx]) / jacob;
-    if (gj < gny - gc - 1) { ac -= coef; an = coef; }
-    coef = T(0.5) * (gy[cc] + gy[cc - g.nx]) / jacob;
-    if (gj > gc) { ac -= coef; as = coef; }
-    if (kind != FALM_POISSON_2D) {
-        coef = T(0.5) * (gz[cc] + gz[cc + g.nxy]) / jacob;
-        if (gk < gnz - gc - 1) { ac -= coef; at = coef; }
-        coef = T(0.5) * (gz[cc] + gz[cc - g.nxy]) / jacob;
-        if (gk > gc) { ac -= coef; ab = coef; }
-    }
-    a0[cc] = ac; a1[cc] = aw; a2[cc] = ae; a3[cc] = as; a4[cc] = an; a5[cc] = ab; a6[cc] = at;
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        const int gi = ox + i, gj = oy + j, gk = oz + k, gc = g.gc;
+        const int xe_lim = kind == FALM_POISSON_OUTLET_XPLUS ? gnx - gc : gnx - gc - 1;
+        T ac = 0, ae = 0, aw = 0, an = 0, as = 0, at = 0, ab = 0;
+        const T jacob = ja[cc];
+        T coef;
+        coef = T(0.5) * (gx[cc] + gx[cc + 1]) / jacob;
+        if (gi < xe_lim) { ac -= coef; ae = coef; }
+        coef = T(0.5) * (gx[cc] + gx[cc - 1]) / jacob;
+        if (gi > gc) { ac -= coef; aw = coef; }
+        coef = T(0.5) * (gy[cc] + gy[cc + g.nx]) / jacob;
+        if (gj < gny - gc - 1) { ac -= coef; an = coef; }
+        coef = T(0.5) * (gy[cc] + gy[cc - g.nx]) / jacob;
+        if (gj > gc) { ac -= coef; as = coef; }
+        if (kind != FALM_POISSON_2D) {
+            coef = T(0.5) * (gz[cc] + gz[cc + g.nxy]) / jacob;
+            if (gk < gnz - gc - 1) { ac -= coef; at = coef; }
+            coef = T(0.5) * (gz[cc] + gz[cc - g.nxy]) / jacob;
+            if (gk > gc) { ac -= coef; ab = coef; }
+        }
+        a0[cc] = ac; a1[cc] = aw; a2[cc] = ae; a3[cc] = as; a4[cc] = an; a5[cc] = ab; a6[cc] = at;
+    });
 }
 
 template <class T>

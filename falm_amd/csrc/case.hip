@@ -11,18 +11,20 @@
 // right-hand side, the Poisson matrix and the residual stay double and the
 // Poisson solve runs as float Krylov passes inside a double
 // defect-correction loop.
+//
+// The stencils take their metrics from the 1-D cell widths of the mesh
+// (bit-identical to the per-cell metric fields, which are still kept for
+// falm_case_get_field), and the velocity snapshot is a buffer swap plus a
+// copy of the ghost shell instead of a full copy.
 #include <cmath>
+#include <utility>
 #include <vector>
 
+#include "alm.hpp"
+#include "cfd.hpp"
 #include "linalg.hpp"
 
 namespace falm {
-falm_status launch_poisson_rhs_mixed(const falm_domain *dom, int prec, const void *uu, double *rhs, const void *ja,
-                                     double scale, hipStream_t s);
-falm_status launch_divergence_mixed(const falm_domain *dom, int prec, const void *uu, void *div, const void *ja,
-                                    hipStream_t s);
-falm_status launch_project_mixed(const falm_domain *dom, int prec, double dt, void *u, const void *ua, void *uu,
-                                 const void *uua, const double *p, const void *kx, const void *gg, hipStream_t s);
 falm_status launch_bc_velocity(const falm_domain *dom, int prec, const falm_face_bc faces[6], void *u,
                                const void *uprev, const void *xyz, double dt, const falm_inlet_noise *noise,
                                hipStream_t s);
@@ -49,11 +51,13 @@ struct falm_case {
     falm_solver *solver = nullptr;
     falm_alm *alm = nullptr;
     std::vector<double> x, y, z, hx, hy, hz;
+    SepMesh sm{};             // device 1-D widths and reciprocals
     int64_t it = 0;
     falm_face_bc ubc[6];
     int pbc[6], uubc[6];
     // device scalar for the divergence norm + its reduction scratch
     double *dnorm = nullptr, *partials = nullptr, *hnorm = nullptr;
+    int *nobox = nullptr;     // empty cell box: no rotor, ff stays zero
     std::vector<void *> bufs;
 };
 
@@ -107,10 +111,55 @@ __global__ void k_fill3(T *u, size_t n, T a, T b, T c) {
     }
 }
 
-template <class TS, class TD>
-__global__ void k_cast(const TS *s, TD *d, size_t n) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-        d[t] = (TD)s[t];
+// Copy the ghost shell (every padded cell outside the interior) of an
+// ncol-column field.  Cells on shell edges belong to two slabs and are
+// copied twice with the same value.
+template <class T>
+__global__ void k_copy_shell(Geom g, const T *__restrict__ src, T *__restrict__ dst, size_t n, int ncol) {
+    const int G = 2 * g.gc;
+    const long long cx = (long long)G * g.ny * g.nz, cy = (long long)G * g.nx * g.nz;
+    const long long cz = (long long)G * g.nx * g.ny;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < cx + cy + cz;
+         t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        long long r = t;
+        if (r < cx) {
+            const int q = (int)(r % G);
+            const long long rest = r / G;
+            j = (int)(rest % g.ny);
+            k = (int)(rest / g.ny);
+            i = q < g.gc ? q : g.nx - G + q;
+        } else if ((r -= cx) < cy) {
+            i = (int)(r % g.nx);
+            const long long rest = r / g.nx;
+            const int q = (int)(rest % G);
+            k = (int)(rest / G);
+            j = q < g.gc ? q : g.ny - G + q;
+        } else {
+            r -= cy;
+            i = (int)(r % g.nx);
+            const long long rest = r / g.nx;
+            j = (int)(rest % g.ny);
+            const int q = (int)(rest / g.ny);
+            k = q < g.gc ? q : g.nz - G + q;
+        }
+        const size_t c = (size_t)g.id(i, j, k);
+        for (int m = 0; m < ncol; m++) dst[c + m * n] = src[c + m * n];
+    }
+}
+
+static falm_status copy_shell(falm_case *C, const void *src, void *dst, hipStream_t s) {
+    const Geom g = make_geom(&C->dom);
+    const int G = 2 * g.gc;
+    const size_t total = (size_t)G * ((size_t)g.ny * g.nz + (size_t)g.nx * g.nz + (size_t)g.nx * g.ny);
+    if (C->prec == FALM_F32)
+        hipLaunchKernelGGL((k_copy_shell<float>), ew_grid(total), dim3(EW_BLOCK), 0, s, g, (const float *)src,
+                           (float *)dst, C->n, 3);
+    else
+        hipLaunchKernelGGL((k_copy_shell<double>), ew_grid(total), dim3(EW_BLOCK), 0, s, g, (const double *)src,
+                           (double *)dst, C->n, 3);
+    FALM_CHECK_LAUNCH("k_copy_shell");
+    return FALM_OK;
 }
 
 static falm_status case_init(falm_case *C) {
@@ -153,7 +202,15 @@ static falm_status case_init(falm_case *C) {
     FALM_TRY(dalloc(C, (void **)&C->dnorm, 8 * sizeof(double)));
     Tiling tl = make_tiling(interior_box(make_geom(&C->dom)));
     FALM_TRY(dalloc(C, (void **)&C->partials, (size_t)(tl.tiles + 1) * sizeof(double)));
+    // the zero fills ran on the null stream, which the case's non-blocking
+    // stream does not wait for
+    FALM_HIP(hipDeviceSynchronize());
     FALM_HIP(hipHostMalloc(&C->hnorm, sizeof(double), hipHostMallocDefault));
+    {
+        const int empty[6] = {1, 0, 1, 0, 1, 0};
+        FALM_TRY(dalloc(C, (void **)&C->nobox, sizeof(empty)));
+        FALM_HIP(hipMemcpy(C->nobox, empty, sizeof(empty), hipMemcpyHostToDevice));
+    }
     // uniform control-volume mesh: cell centres and widths of every padded
     // cell (what Mesher::build_mesh + readControlVolumeFile produce for a
     // uniform node list, src/mesher/mesher.hpp, src/FalmIO.h:99-116)
@@ -168,6 +225,19 @@ static falm_status case_init(falm_case *C) {
     }
     FALM_TRY(falm_metrics(&C->dom, C->prec, C->x.data(), C->y.data(), C->z.data(), C->hx.data(), C->hy.data(),
                           C->hz.data(), C->xyz, C->kx, C->g, C->ja, s));
+    for (int m = 0; m < 3; m++) {
+        const int L = C->dom.shape[m];
+        std::vector<double> r(L);
+        for (int i = 0; i < L; i++) r[i] = 1.0 / (*wid[m])[i];  // Real(1)/pitch, src/falm.h:395
+        double *dh = nullptr, *dr = nullptr;
+        FALM_TRY(dalloc(C, (void **)&dh, L * sizeof(double)));
+        FALM_TRY(dalloc(C, (void **)&dr, L * sizeof(double)));
+        FALM_HIP(hipMemcpy(dh, wid[m]->data(), L * sizeof(double), hipMemcpyHostToDevice));
+        FALM_HIP(hipMemcpy(dr, r.data(), L * sizeof(double), hipMemcpyHostToDevice));
+        C->sm.h[m] = dh;
+        C->sm.rh[m] = dr;
+    }
+    FALM_HIP(hipDeviceSynchronize());
     // Poisson matrix from double metrics
     int kind = c.kind == FALM_CASE_WINDTUNNEL ? FALM_POISSON_OUTLET_XPLUS
              : c.kind == FALM_CASE_LID2D     ? FALM_POISSON_2D
@@ -199,8 +269,13 @@ static falm_status case_init(falm_case *C) {
         hipLaunchKernelGGL((k_fill3<double>), ew_grid(n), dim3(EW_BLOCK), 0, s, (double *)C->u, n, c.u_inflow, 0.0,
                            0.0);
     FALM_CHECK_LAUNCH("k_fill3");
-    FALM_TRY(falm_u_to_uu(&C->dom, C->prec, C->u, C->uu, C->kx, C->ja, s));
-    FALM_TRY(falm_sgs(&C->dom, C->prec, c.sgs_model, c.Cs, C->u, C->nut, C->kx, C->ja, s));
+    FALM_HIP(hipMemcpyAsync(C->uprev, C->u, 3 * n * tb, hipMemcpyDeviceToDevice, s));
+    FALM_TRY(sep_u_to_uu(&C->dom, C->prec, C->u, C->uu, C->sm, s));
+    if (c.sgs_model == FALM_SGS_SMAGORINSKY) FALM_TRY(sep_sgs(&C->dom, C->prec, c.Cs, C->u, C->nut, C->sm, s));
+    else if (c.sgs_model != FALM_SGS_EMPTY) {
+        set_error("falm_case_create: only the Smagorinsky model is implemented");
+        return FALM_ERR_UNSUPPORTED;
+    }
     FALM_TRY(falm_solver_create(&C->dom, &c.solver, &C->solver));
     FALM_TRY(falm_solver_set_matrix(C->solver, C->a, s));
     if (c.alm) FALM_TRY(falm_alm_create(&C->dom, c.alm, C->x.data(), C->y.data(), C->z.data(), &C->alm));
@@ -217,24 +292,29 @@ static falm_status case_step(falm_case *C, falm_solve_info *si) {
     const bool is2d = c.kind == FALM_CASE_LID2D;
     C->it++;
     const double t = c.dt * (double)C->it;
-    FALM_HIP(hipMemcpyAsync(C->uprev, C->u, 3 * C->n * C->tb, hipMemcpyDeviceToDevice, s));
-    if (C->alm) FALM_TRY(falm_alm_step(C->alm, pr, C->u, C->ff, t, s));
-    FALM_TRY(falm_pseudo_u(d, pr, c.adv_scheme, 1.0 / c.Re, c.dt, C->uprev, C->uprev, C->uu, C->u, C->nut, C->kx,
-                           C->g, C->ja, C->ff, s));
-    FALM_TRY(falm_u_to_uu(d, pr, C->u, C->uu, C->kx, C->ja, s));
+    // u_previous <- u (app/windtunnel/main.cpp:136): swap the buffers, then
+    // restore the ghost shell of the new u, which u_to_uu reads before the
+    // boundary fill rewrites it
+    std::swap(C->u, C->uprev);
+    FALM_TRY(copy_shell(C, C->uprev, C->u, s));
+    if (C->alm) FALM_TRY(alm_step(C->alm, pr, C->uprev, C->ff, t, s, true));
+    FALM_TRY(sep_pseudo_u(d, pr, c.adv_scheme, 1.0 / c.Re, c.dt, C->uprev, C->uprev, C->uu, C->u, C->nut, C->sm,
+                          C->ff, C->alm ? alm_force_box(C->alm) : C->nobox, s));
+    FALM_TRY(sep_u_to_uu(d, pr, C->u, C->uu, C->sm, s));
     if (c.kind != FALM_CASE_WINDTUNNEL) FALM_TRY(launch_bc_face_flux(d, pr, C->uubc, C->uu, s));
-    FALM_TRY(launch_poisson_rhs_mixed(d, pr, C->uu, C->rhs, C->ja, 1.0 / (c.dt * C->maxdiag), s));
-    FALM_TRY(falm_solver_solve(C->solver, C->a, C->p, C->rhs, C->res, s, si));
+    FALM_TRY(sep_poisson_rhs(d, pr, C->uu, C->rhs, C->sm, 1.0 / (c.dt * C->maxdiag), s));
+    FALM_TRY(falm_solver_solve(C->solver, C->a, C->p, C->rhs, C->prec == FALM_F64 ? C->res : nullptr, s, si));
     FALM_TRY(launch_bc_pressure(d, FALM_F64, C->pbc, C->p, s));
     if (is2d) FALM_TRY(launch_copy_z5(d, FALM_F64, C->p, 1, s));
-    FALM_TRY(launch_project_mixed(d, pr, c.dt, C->u, C->u, C->uu, C->uu, C->p, C->kx, C->g, s));
+    FALM_TRY(sep_project(d, pr, c.dt, C->u, C->uu, C->p, C->sm, s));
     falm_inlet_noise nz{c.inlet_intensity, c.seed, C->it};
     FALM_TRY(launch_bc_velocity(d, pr, C->ubc, C->u, C->uprev, C->xyz, c.dt, &nz, s));
     if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->u, 3, s));
-    FALM_TRY(falm_sgs(d, pr, c.sgs_model, c.Cs, C->u, C->nut, C->kx, C->ja, s));
-    if (is2d && c.sgs_model != FALM_SGS_EMPTY) FALM_TRY(launch_copy_z5(d, pr, C->nut, 1, s));
-    FALM_TRY(launch_divergence_mixed(d, pr, C->uu, C->div, C->ja, s));
-    FALM_TRY(launch_dot(d, pr, C->div, C->div, 1, true, C->dnorm, C->partials, s));
+    if (c.sgs_model == FALM_SGS_SMAGORINSKY) {
+        FALM_TRY(sep_sgs(d, pr, c.Cs, C->u, C->nut, C->sm, s));
+        if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->nut, 1, s));
+    }
+    FALM_TRY(sep_div_norm(d, pr, C->uu, C->div, C->sm, C->dnorm, C->partials, s));
     return FALM_OK;
 }
 
@@ -356,6 +436,7 @@ falm_status falm_case_set_field(falm_case *C, int field, const double *host_in, 
         FALM_HIP(hipMemcpy(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
     }
     if (field == FALM_FIELD_POI_A) FALM_TRY(falm_solver_set_matrix(C->solver, C->a, C->stream));
+    if (field == FALM_FIELD_FF && C->alm) alm_ff_reset(C->alm);
     FALM_HIP(hipStreamSynchronize(C->stream));
     return FALM_OK;
 }

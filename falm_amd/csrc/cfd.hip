@@ -7,7 +7,7 @@
 // from L1/L2 (the x-neighbours of a wavefront are in the same cache lines,
 // the y/z-neighbours are re-read by the adjacent rows/planes of the same
 // XCD slab while they are still in its L2).
-#include "common.hpp"
+#include "cfd.hpp"
 
 namespace falm {
 
@@ -95,146 +95,425 @@ struct Vec3W {
     T *__restrict__ c[3];
 };
 
+// Metric access.  MetField reads the per-cell metric fields kx, g, ja of the
+// reference (FalmCore::computation_init, src/falm.h:383-404).  MetSep
+// evaluates the same expressions from the 1-D cell widths of the
+// tensor-product mesh (volume = hx*hy*hz, kx = 1/h, g = volume*kx*kx, in
+// double, then rounded to T), so both give the same bits and MetSep reads
+// no per-cell metric array.
+template <class T>
+struct MetField {
+    const T *__restrict__ kx[3];
+    const T *__restrict__ g[3];
+    const T *__restrict__ ja;
+    __device__ inline T KX(int d, int cc, int, int, int) const { return kx[d][cc]; }
+    __device__ inline T G(int d, int cc, int, int, int) const { return g[d][cc]; }
+    __device__ inline T JA(int cc, int, int, int) const { return ja[cc]; }
+    static constexpr bool per_cell = true;  // metric arrays count as traffic
+};
+template <class T>
+struct MetSep {
+    const double *__restrict__ h[3];   // cell widths along x, y, z
+    const double *__restrict__ rh[3];  // 1.0/h along x, y, z
+    __device__ inline double vol(int i, int j, int k) const { return h[0][i] * h[1][j] * h[2][k]; }
+    __device__ inline double r(int d, int i, int j, int k) const { return rh[d][d == 0 ? i : d == 1 ? j : k]; }
+    __device__ inline T KX(int d, int, int i, int j, int k) const { return (T)r(d, i, j, k); }
+    __device__ inline T G(int d, int, int i, int j, int k) const {
+        const double q = r(d, i, j, k);
+        return (T)(vol(i, j, k) * q * q);
+    }
+    __device__ inline T JA(int, int i, int j, int k) const { return (T)vol(i, j, k); }
+    static constexpr bool per_cell = false;
+};
+
 // kernel_Cartesian_CalcPseudoU, src/dev/FalmCFDDevCall.cu:10-301
-template <int SCHEME, class T>
+// ffbox (optional): ff is known to be zero outside this cell box
+// (ilo, ihi, jlo, jhi, klo, khi) and is not read there.
+template <int SCHEME, class T, class M>
 __global__ __launch_bounds__(256) void k_pseudo_u(Tiling tl, Geom g, T ReI, T dt, Vec3<T> un, Vec3<T> u,
-                                                  Vec3<T> uu, Vec3W<T> ua, const T *__restrict__ nut,
-                                                  Vec3<T> kx, Vec3<T> gg, const T *__restrict__ ja, Vec3<T> ff) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int sx = 1, sy = g.nx, sz = g.nxy;
-    const int cc = g.id(i, j, k);
-    const T uc = u.c[0][cc], vc = u.c[1][cc], wc = u.c[2][cc];
-    const T Uabs = fabs(uc * kx.c[0][cc]);
-    const T Vabs = fabs(vc * kx.c[1][cc]);
-    const T Wabs = fabs(wc * kx.c[2][cc]);
-    const T UE = uu.c[0][cc], UW = uu.c[0][cc - sx];
-    const T VN = uu.c[1][cc], VS = uu.c[1][cc - sy];
-    const T WT = uu.c[2][cc], WB = uu.c[2][cc - sz];
-    const T nc = nut[cc], ne = nut[cc + sx], nw = nut[cc - sx], nn = nut[cc + sy], ns = nut[cc - sy],
-            nt = nut[cc + sz], nb = nut[cc - sz];
-    const T gxc = gg.c[0][cc], gxe = gg.c[0][cc + sx], gxw = gg.c[0][cc - sx];
-    const T gyc = gg.c[1][cc], gyn = gg.c[1][cc + sy], gys = gg.c[1][cc - sy];
-    const T gzc = gg.c[2][cc], gzt = gg.c[2][cc + sz], gzb = gg.c[2][cc - sz];
-    const T jac = ja[cc];
+                                                  Vec3<T> uu, Vec3W<T> ua, const T *__restrict__ nut, M met,
+                                                  Vec3<T> ff, const int *__restrict__ ffbox) {
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const bool useff = !ffbox || (i >= ffbox[0] && i <= ffbox[1] && j >= ffbox[2] && j <= ffbox[3] &&
+                                      k >= ffbox[4] && k <= ffbox[5]);
+        const int sx = 1, sy = g.nx, sz = g.nxy;
+        const int cc = g.id(i, j, k);
+        const T uc = u.c[0][cc], vc = u.c[1][cc], wc = u.c[2][cc];
+        const T Uabs = fabs(uc * met.KX(0, cc, i, j, k));
+        const T Vabs = fabs(vc * met.KX(1, cc, i, j, k));
+        const T Wabs = fabs(wc * met.KX(2, cc, i, j, k));
+        const T UE = uu.c[0][cc], UW = uu.c[0][cc - sx];
+        const T VN = uu.c[1][cc], VS = uu.c[1][cc - sy];
+        const T WT = uu.c[2][cc], WB = uu.c[2][cc - sz];
+        const T nc = nut[cc], ne = nut[cc + sx], nw = nut[cc - sx], nn = nut[cc + sy], ns = nut[cc - sy],
+                nt = nut[cc + sz], nb = nut[cc - sz];
+        const T gxc = met.G(0, cc, i, j, k), gxe = met.G(0, cc + sx, i + 1, j, k), gxw = met.G(0, cc - sx, i - 1, j, k);
+        const T gyc = met.G(1, cc, i, j, k), gyn = met.G(1, cc + sy, i, j + 1, k), gys = met.G(1, cc - sy, i, j - 1, k);
+        const T gzc = met.G(2, cc, i, j, k), gzt = met.G(2, cc + sz, i, j, k + 1), gzb = met.G(2, cc - sz, i, j, k - 1);
+        const T jac = met.JA(cc, i, j, k);
+    #pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T *__restrict__ q = u.c[d];
+            const T ucc = d == 0 ? uc : d == 1 ? vc : wc;
+            const T ue1 = q[cc + sx], uw1 = q[cc - sx], un1 = q[cc + sy], us1 = q[cc - sy], ut1 = q[cc + sz],
+                    ub1 = q[cc - sz];
+            const T ue2 = q[cc + 2 * sx], uw2 = q[cc - 2 * sx], un2 = q[cc + 2 * sy], us2 = q[cc - 2 * sy],
+                    ut2 = q[cc + 2 * sz], ub2 = q[cc - 2 * sz];
+            T adv = advection<SCHEME, T>(ucc, ue1, ue2, uw1, uw2, un1, un2, us1, us2, ut1, ut2, ub1, ub2, Uabs,
+                                         Vabs, Wabs, UE, UW, VN, VS, WT, WB, jac);
+            T vis = diffusion<T>(ReI, ucc, ue1, uw1, un1, us1, ut1, ub1, nc, ne, nw, nn, ns, nt, nb, gxc, gxe,
+                                 gxw, gyc, gyn, gys, gzc, gzt, gzb, jac);
+            const T f = useff ? ff.c[d][cc] : T(0);
+            ua.c[d][cc] = un.c[d][cc] + dt * (-adv + vis + f);
+        }
+    });
+}
+
+// ---------------------------------------------------------------------
+// Pseudo-velocity on a tensor-product mesh, k-marching form.  A workgroup
+// owns a KM_BX x KM_BY column of cells and walks KM_KC planes in k.  The
+// in-plane neighbours (x, y within +-2) come from an LDS copy of the plane
+// with a two-cell halo, the k neighbours from a per-thread register queue,
+// and the metrics from the 1-D widths (per thread once, per plane as
+// workgroup-uniform values).  Each cell then costs ~10 vector loads instead
+// of ~60; the arithmetic is the same expression by expression as
+// k_pseudo_u, so the results are identical.  The next plane's loads are
+// issued before the current plane is computed, and the LDS planes are
+// double-buffered, so one barrier per plane suffices.
+constexpr int KM_BX = 64, KM_BY = 8, KM_KC = 16;
+constexpr int KM_NT = KM_BX * KM_BY;
+constexpr int KM_UX = 4 * KM_BY, KM_UY = 4 * KM_BX;   // u halo cells per component
+constexpr int KM_NX = 2 * KM_BY, KM_NY = 2 * KM_BX;   // nut halo cells
+constexpr int KM_HALO = 3 * (KM_UX + KM_UY) + KM_NX + KM_NY;
+constexpr int KM_HPT = (KM_HALO + KM_NT - 1) / KM_NT;  // halo cells per thread
+
+struct KmTiling {
+    Box box;
+    int tx, ty, tk;  // column tiles along x, y and k chunks
+    int blocks;
+};
+inline KmTiling make_km_tiling(const Box &b) {
+    KmTiling t;
+    t.box = b;
+    t.tx = (b.n[0] + KM_BX - 1) / KM_BX;
+    t.ty = (b.n[1] + KM_BY - 1) / KM_BY;
+    t.tk = (b.n[2] + KM_KC - 1) / KM_KC;
+    t.blocks = t.tx * t.ty * t.tk;
+    return t;
+}
+
+// halo slot -> (LDS array: 0..2 u components, 3 nut; LDS row, column)
+__device__ inline void km_halo_slot(int s, int &arr, int &row, int &col) {
+    if (s < 3 * (KM_UX + KM_UY)) {
+        arr = s / (KM_UX + KM_UY);
+        int r = s - arr * (KM_UX + KM_UY);
+        if (r < KM_UX) {  // x halo: rows 2..BY+1, columns 0,1,BX+2,BX+3
+            row = 2 + r / 4;
+            const int q = r & 3;
+            col = q < 2 ? q : KM_BX + q;
+        } else {          // y halo: rows 0,1,BY+2,BY+3, columns 2..BX+1
+            r -= KM_UX;
+            const int q = r / KM_BX;
+            row = q < 2 ? q : KM_BY + q;
+            col = 2 + (r - q * KM_BX);
+        }
+    } else {
+        arr = 3;
+        int r = s - 3 * (KM_UX + KM_UY);
+        if (r < KM_NX) {  // rows 1..BY, columns 0, BX+1
+            row = 1 + (r >> 1);
+            col = (r & 1) ? KM_BX + 1 : 0;
+        } else {          // rows 0, BY+1, columns 1..BX
+            r -= KM_NX;
+            const int q = r / KM_BX;
+            row = q ? KM_BY + 1 : 0;
+            col = 1 + (r - q * KM_BX);
+        }
+    }
+}
+
+template <int SCHEME, class T>
+__global__ __launch_bounds__(KM_NT) void k_pseudo_u_km(KmTiling kt, Geom g, T ReI, T dt, Vec3<T> u, Vec3<T> uu,
+                                                       Vec3W<T> ua, const T *__restrict__ nut, MetSep<T> met,
+                                                       Vec3<T> ff, const int *__restrict__ ffbox) {
+    __shared__ T su[2][3][KM_BY + 4][KM_BX + 4];
+    __shared__ T sn[2][KM_BY + 2][KM_BX + 2];
+    const int lane = threadIdx.x & (KM_BX - 1), wy = threadIdx.x / KM_BX;
+    const int L = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = L % kt.tx, by = (L / kt.tx) % kt.ty, kc = L / (kt.tx * kt.ty);
+    const int i0 = kt.box.lo[0] + bx * KM_BX, j0 = kt.box.lo[1] + by * KM_BY;
+    const int i = i0 + lane, j = j0 + wy;
+    const bool act = i < kt.box.lo[0] + kt.box.n[0] && j < kt.box.lo[1] + kt.box.n[1];
+    const int k0 = kt.box.lo[2] + kc * KM_KC;
+    const int k1 = min(k0 + KM_KC, kt.box.lo[2] + kt.box.n[2]);
+    const int sy = g.nx, sz = g.nxy;
+    // Threads past the box still load their own cell into the LDS plane
+    // when it lies in the array: the last active columns and rows read it
+    // as a neighbour.  Threads past the array load a valid cell instead.
+    const bool ld = i < g.nx && j < g.ny;
+    const int col0 = (ld ? i : kt.box.lo[0]) + sy * (ld ? j : kt.box.lo[1]);
+    // per-column metric factors (cells of the box only)
+    const int ia = act ? i : kt.box.lo[0], ja = act ? j : kt.box.lo[1];
+    const double hxc = met.h[0][ia], hxe = met.h[0][ia + 1], hxw = met.h[0][ia - 1];
+    const double rxc = met.rh[0][ia], rxe = met.rh[0][ia + 1], rxw = met.rh[0][ia - 1];
+    const double hyc = met.h[1][ja], hyn = met.h[1][ja + 1], hys = met.h[1][ja - 1];
+    const double ryc = met.rh[1][ja], ryn = met.rh[1][ja + 1], rys = met.rh[1][ja - 1];
+    bool inx = true, iny = true;
+    if (ffbox) {
+        inx = i >= ffbox[0] && i <= ffbox[1];
+        iny = j >= ffbox[2] && j <= ffbox[3];
+    }
+    // halo slots of this thread
+    int harr[KM_HPT], hrow[KM_HPT], hcol[KM_HPT], hoff[KM_HPT];
+    bool hok[KM_HPT];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const T *__restrict__ q = u.c[d];
-        const T ucc = d == 0 ? uc : d == 1 ? vc : wc;
-        const T ue1 = q[cc + sx], uw1 = q[cc - sx], un1 = q[cc + sy], us1 = q[cc - sy], ut1 = q[cc + sz],
-                ub1 = q[cc - sz];
-        const T ue2 = q[cc + 2 * sx], uw2 = q[cc - 2 * sx], un2 = q[cc + 2 * sy], us2 = q[cc - 2 * sy],
-                ut2 = q[cc + 2 * sz], ub2 = q[cc - 2 * sz];
-        T adv = advection<SCHEME, T>(ucc, ue1, ue2, uw1, uw2, un1, un2, us1, us2, ut1, ut2, ub1, ub2, Uabs,
-                                     Vabs, Wabs, UE, UW, VN, VS, WT, WB, jac);
-        T vis = diffusion<T>(ReI, ucc, ue1, uw1, un1, us1, ut1, ub1, nc, ne, nw, nn, ns, nt, nb, gxc, gxe,
-                             gxw, gyc, gyn, gys, gzc, gzt, gzb, jac);
-        ua.c[d][cc] = un.c[d][cc] + dt * (-adv + vis + ff.c[d][cc]);
+    for (int q = 0; q < KM_HPT; q++) {
+        const int s = threadIdx.x + q * KM_NT;
+        hok[q] = false;
+        harr[q] = hrow[q] = hcol[q] = hoff[q] = 0;
+        if (s < KM_HALO) {
+            km_halo_slot(s, harr[q], hrow[q], hcol[q]);
+            const int gi = i0 - (harr[q] < 3 ? 2 : 1) + hcol[q];
+            const int gj = j0 - (harr[q] < 3 ? 2 : 1) + hrow[q];
+            hok[q] = gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny;
+            hoff[q] = gi + sy * gj;
+        }
+    }
+    auto halo_load = [&](int k, T (&hv)[KM_HPT]) {
+#pragma unroll
+        for (int q = 0; q < KM_HPT; q++) {
+            hv[q] = T(0);
+            if (hok[q]) {
+                const T *src = harr[q] < 3 ? u.c[harr[q]] : nut;
+                hv[q] = src[hoff[q] + sz * k];
+            }
+        }
+    };
+    auto lds_fill = [&](int b, const T (&hv)[KM_HPT], T u0, T u1, T u2, T nv) {
+        su[b][0][wy + 2][lane + 2] = u0;
+        su[b][1][wy + 2][lane + 2] = u1;
+        su[b][2][wy + 2][lane + 2] = u2;
+        sn[b][wy + 1][lane + 1] = nv;
+#pragma unroll
+        for (int q = 0; q < KM_HPT; q++) {
+            if (threadIdx.x + q * KM_NT >= KM_HALO) continue;
+            if (harr[q] < 3) su[b][harr[q]][hrow[q]][hcol[q]] = hv[q];
+            else sn[b][hrow[q]][hcol[q]] = hv[q];
+        }
+    };
+    // prologue: k queue, uu_z below, LDS plane k0
+    T qu[3][5], qn[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int d = 0; d < 4; d++) qu[c][d] = u.c[c][col0 + sz * (k0 - 2 + d)];
+#pragma unroll
+    for (int c = 0; c < 3; c++) qu[c][4] = T(0);
+    qn[0] = nut[col0 + sz * (k0 - 1)];
+    qn[1] = nut[col0 + sz * k0];
+    qn[2] = T(0);
+    T wb = uu.c[2][col0 + sz * (k0 - 1)];
+    T hv[KM_HPT];
+    halo_load(k0, hv);
+    lds_fill(0, hv, qu[0][2], qu[1][2], qu[2][2], qn[1]);
+    __syncthreads();
+    int b = 0;
+    for (int k = k0; k < k1; k++) {
+        const int cc = col0 + sz * k;
+        // loads: this plane's k+2 / k+1 values, the next plane's halo
+        qu[0][4] = u.c[0][cc + 2 * sz];
+        qu[1][4] = u.c[1][cc + 2 * sz];
+        qu[2][4] = u.c[2][cc + 2 * sz];
+        qn[2] = nut[cc + sz];
+        const bool more = k + 1 < k1;
+        if (more) halo_load(k + 1, hv);
+        const T UE = uu.c[0][cc], UW = uu.c[0][cc - 1];
+        const T VN = uu.c[1][cc], VS = uu.c[1][cc - sy];
+        const T WT = uu.c[2][cc], WB = wb;
+        const bool useff = inx && iny && (!ffbox || (k >= ffbox[4] && k <= ffbox[5]));
+        T fv[3] = {T(0), T(0), T(0)};
+        if (useff && act) {
+            fv[0] = ff.c[0][cc];
+            fv[1] = ff.c[1][cc];
+            fv[2] = ff.c[2][cc];
+        }
+        // metrics of this plane
+        const double hzc = met.h[2][k], hzt = met.h[2][k + 1], hzb = met.h[2][k - 1];
+        const double rzc = met.rh[2][k], rzt = met.rh[2][k + 1], rzb = met.rh[2][k - 1];
+        const double volc = hxc * hyc * hzc;
+        const T gxc = (T)(volc * rxc * rxc), gxe = (T)(hxe * hyc * hzc * rxe * rxe), gxw = (T)(hxw * hyc * hzc * rxw * rxw);
+        const T gyc = (T)(volc * ryc * ryc), gyn = (T)(hxc * hyn * hzc * ryn * ryn), gys = (T)(hxc * hys * hzc * rys * rys);
+        const T gzc = (T)(volc * rzc * rzc), gzt = (T)(hxc * hyc * hzt * rzt * rzt), gzb = (T)(hxc * hyc * hzb * rzb * rzb);
+        const T jac = (T)volc;
+        const T Uabs = fabs(qu[0][2] * (T)rxc);
+        const T Vabs = fabs(qu[1][2] * (T)ryc);
+        const T Wabs = fabs(qu[2][2] * (T)rzc);
+        const T nc = qn[1], ne = sn[b][wy + 1][lane + 2], nw = sn[b][wy + 1][lane], nn = sn[b][wy + 2][lane + 1],
+                ns = sn[b][wy][lane + 1], nt = qn[2], nb = qn[0];
+        if (act) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const T ucc = qu[d][2];
+                const T ue1 = su[b][d][wy + 2][lane + 3], ue2 = su[b][d][wy + 2][lane + 4];
+                const T uw1 = su[b][d][wy + 2][lane + 1], uw2 = su[b][d][wy + 2][lane];
+                const T un1 = su[b][d][wy + 3][lane + 2], un2 = su[b][d][wy + 4][lane + 2];
+                const T us1 = su[b][d][wy + 1][lane + 2], us2 = su[b][d][wy][lane + 2];
+                const T ut1 = qu[d][3], ut2 = qu[d][4], ub1 = qu[d][1], ub2 = qu[d][0];
+                T adv = advection<SCHEME, T>(ucc, ue1, ue2, uw1, uw2, un1, un2, us1, us2, ut1, ut2, ub1, ub2, Uabs,
+                                             Vabs, Wabs, UE, UW, VN, VS, WT, WB, jac);
+                T vis = diffusion<T>(ReI, ucc, ue1, uw1, un1, us1, ut1, ub1, nc, ne, nw, nn, ns, nt, nb, gxc, gxe,
+                                     gxw, gyc, gyn, gys, gzc, gzt, gzb, jac);
+                ua.c[d][cc] = ucc + dt * (-adv + vis + fv[d]);
+            }
+        }
+        // next plane: LDS from the queue and the halo registers, shift
+        if (more) lds_fill(b ^ 1, hv, qu[0][3], qu[1][3], qu[2][3], qn[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            qu[c][0] = qu[c][1];
+            qu[c][1] = qu[c][2];
+            qu[c][2] = qu[c][3];
+            qu[c][3] = qu[c][4];
+        }
+        qn[0] = qn[1];
+        qn[1] = qn[2];
+        wb = WT;
+        b ^= 1;
+        __syncthreads();
     }
 }
 
 // UtoCU + InterpolateCU fused (src/dev/FalmCFDDevCall.cu:303-349): the
 // face flux of face (i+1/2) is the mean of the two cell-centre fluxes
 // ja*kx*u, each evaluated exactly as the reference does.
-template <class T>
-__global__ __launch_bounds__(256) void k_u_to_uu(Tiling tl, Geom g, Vec3<T> u, Vec3W<T> uu, Vec3<T> kx,
-                                                 const T *__restrict__ ja) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    const int nb[3] = {1, g.nx, g.nxy};
-    const T jc = ja[cc];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const int cn = cc + nb[d];
-        const T ucc = jc * kx.c[d][cc] * u.c[d][cc];
-        const T ucn = ja[cn] * kx.c[d][cn] * u.c[d][cn];
-        uu.c[d][cc] = T(0.5) * (ucc + ucn);
-    }
+template <class T, class M>
+__global__ __launch_bounds__(256) void k_u_to_uu(Tiling tl, Geom g, Vec3<T> u, Vec3W<T> uu, M met) {
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        const int nb[3] = {1, g.nx, g.nxy};
+        const T jc = met.JA(cc, i, j, k);
+    #pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int cn = cc + nb[d];
+            const int in = i + (d == 0), jn = j + (d == 1), kn = k + (d == 2);
+            const T ucc = jc * met.KX(d, cc, i, j, k) * u.c[d][cc];
+            const T ucn = met.JA(cn, in, jn, kn) * met.KX(d, cn, in, jn, kn) * u.c[d][cn];
+            uu.c[d][cc] = T(0.5) * (ucc + ucn);
+        }
+    });
 }
 
 // Divergence (src/dev/FalmCFDDevCall.cu:557-581), optionally followed by the
 // ScaleMatrix pass of MACCalcPoissonRHS (src/FalmCFD.h:96-109): the launch
 // covers every padded cell; interior cells get div*scale, the others are
 // multiplied by scale as ScaleMatrix does over the whole array.
-template <class T, class TO, bool SCALE>
-__global__ __launch_bounds__(256) void k_divergence(Tiling tl, Geom g, Vec3<T> uu, TO *__restrict__ div,
-                                                    const T *__restrict__ ja, double scale) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    const bool inner = i >= g.gc && i < g.nx - g.gc && j >= g.gc && j < g.ny - g.gc && k >= g.gc &&
-                       k < g.nz - g.gc;
-    if (inner) {
-        TO UE = uu.c[0][cc], UW = uu.c[0][cc - 1];
-        TO VN = uu.c[1][cc], VS = uu.c[1][cc - g.nx];
-        TO WT = uu.c[2][cc], WB = uu.c[2][cc - g.nxy];
-        TO d = (UE - UW + VN - VS + WT - WB) / (TO)ja[cc];
-        if (SCALE) d *= (TO)scale;
-        div[cc] = d;
-    } else if (SCALE) {
-        div[cc] *= (TO)scale;
-    }
+template <class T, class TO, bool SCALE, class M>
+__global__ __launch_bounds__(256) void k_divergence(Tiling tl, Geom g, Vec3<T> uu, TO *__restrict__ div, M met,
+                                                    double scale) {
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        const bool inner = i >= g.gc && i < g.nx - g.gc && j >= g.gc && j < g.ny - g.gc && k >= g.gc &&
+                           k < g.nz - g.gc;
+        if (inner) {
+            TO UE = uu.c[0][cc], UW = uu.c[0][cc - 1];
+            TO VN = uu.c[1][cc], VS = uu.c[1][cc - g.nx];
+            TO WT = uu.c[2][cc], WB = uu.c[2][cc - g.nxy];
+            TO d = (UE - UW + VN - VS + WT - WB) / (TO)met.JA(cc, i, j, k);
+            if (SCALE) d *= (TO)scale;
+            div[cc] = d;
+        } else if (SCALE) {
+            div[cc] *= (TO)scale;
+        }
+    });
 }
 
 // ProjectPGrid + ProjectPFace (src/dev/FalmCFDDevCall.cu:351-407) in one
 // launch over the face region (which contains the interior): cell update on
 // interior cells, face update on every face-region cell.  u may alias ua and
 // uu may alias uua.  The pressure may be kept in a wider type P.
-template <class T, class P>
+template <class T, class P, class M>
 __global__ __launch_bounds__(256) void k_project(Tiling tl, Geom g, P dt, T *u0, T *u1, T *u2, const T *ua0,
                                                  const T *ua1, const T *ua2, T *uu0, T *uu1, T *uu2,
                                                  const T *uua0, const T *uua1, const T *uua2,
-                                                 const P *__restrict__ p, Vec3<T> kx, Vec3<T> gg) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int sy = g.nx, sz = g.nxy;
-    const int cc = g.id(i, j, k);
-    const P pcc = p[cc];
-    if (i >= g.gc && j >= g.gc && k >= g.gc) {  // face region upper bound == interior upper bound
-        P dpx = P(0.5) * (P)kx.c[0][cc] * (p[cc + 1] - p[cc - 1]);
-        P dpy = P(0.5) * (P)kx.c[1][cc] * (p[cc + sy] - p[cc - sy]);
-        P dpz = P(0.5) * (P)kx.c[2][cc] * (p[cc + sz] - p[cc - sz]);
-        T a0 = ua0[cc], a1 = ua1[cc], a2 = ua2[cc];
-        u0[cc] = (T)((P)a0 - dt * dpx);
-        u1[cc] = (T)((P)a1 - dt * dpy);
-        u2[cc] = (T)((P)a2 - dt * dpz);
-    }
-    P dfx = P(0.5) * ((P)gg.c[0][cc] + (P)gg.c[0][cc + 1]) * (p[cc + 1] - pcc);
-    P dfy = P(0.5) * ((P)gg.c[1][cc] + (P)gg.c[1][cc + sy]) * (p[cc + sy] - pcc);
-    P dfz = P(0.5) * ((P)gg.c[2][cc] + (P)gg.c[2][cc + sz]) * (p[cc + sz] - pcc);
-    T b0 = uua0[cc], b1 = uua1[cc], b2 = uua2[cc];
-    uu0[cc] = (T)((P)b0 - dt * dfx);
-    uu1[cc] = (T)((P)b1 - dt * dfy);
-    uu2[cc] = (T)((P)b2 - dt * dfz);
+                                                 const P *__restrict__ p, M met) {
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int sy = g.nx, sz = g.nxy;
+        const int cc = g.id(i, j, k);
+        const P pcc = p[cc];
+        if (i >= g.gc && j >= g.gc && k >= g.gc) {  // face region upper bound == interior upper bound
+            P dpx = P(0.5) * (P)met.KX(0, cc, i, j, k) * (p[cc + 1] - p[cc - 1]);
+            P dpy = P(0.5) * (P)met.KX(1, cc, i, j, k) * (p[cc + sy] - p[cc - sy]);
+            P dpz = P(0.5) * (P)met.KX(2, cc, i, j, k) * (p[cc + sz] - p[cc - sz]);
+            T a0 = ua0[cc], a1 = ua1[cc], a2 = ua2[cc];
+            u0[cc] = (T)((P)a0 - dt * dpx);
+            u1[cc] = (T)((P)a1 - dt * dpy);
+            u2[cc] = (T)((P)a2 - dt * dpz);
+        }
+        P dfx = P(0.5) * ((P)met.G(0, cc, i, j, k) + (P)met.G(0, cc + 1, i + 1, j, k)) * (p[cc + 1] - pcc);
+        P dfy = P(0.5) * ((P)met.G(1, cc, i, j, k) + (P)met.G(1, cc + sy, i, j + 1, k)) * (p[cc + sy] - pcc);
+        P dfz = P(0.5) * ((P)met.G(2, cc, i, j, k) + (P)met.G(2, cc + sz, i, j, k + 1)) * (p[cc + sz] - pcc);
+        T b0 = uua0[cc], b1 = uua1[cc], b2 = uua2[cc];
+        uu0[cc] = (T)((P)b0 - dt * dfx);
+        uu1[cc] = (T)((P)b1 - dt * dfy);
+        uu2[cc] = (T)((P)b2 - dt * dfz);
+    });
 }
 
 // kernel_Cartesian_Smagorinsky, src/dev/FalmCFDDevCall.cu:409-476
-template <class T>
+template <class T, class M>
 __global__ __launch_bounds__(256) void k_smagorinsky(Tiling tl, Geom g, T Cs, Vec3<T> u, T *__restrict__ nut,
-                                                     Vec3<T> kx, const T *__restrict__ ja) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int sy = g.nx, sz = g.nxy;
-    const int cc = g.id(i, j, k);
-    const T kxx = kx.c[0][cc], kxy = kx.c[1][cc], kxz = kx.c[2][cc];
-    T gr[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const T *__restrict__ q = u.c[d];
-        gr[d][0] = T(0.5) * kxx * (q[cc + 1] - q[cc - 1]);
-        gr[d][1] = T(0.5) * kxy * (q[cc + sy] - q[cc - sy]);
-        gr[d][2] = T(0.5) * kxz * (q[cc + sz] - q[cc - sz]);
+                                                     M met) {
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int sy = g.nx, sz = g.nxy;
+        const int cc = g.id(i, j, k);
+        const T kxx = met.KX(0, cc, i, j, k), kxy = met.KX(1, cc, i, j, k), kxz = met.KX(2, cc, i, j, k);
+        T gr[3][3];
+    #pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T *__restrict__ q = u.c[d];
+            gr[d][0] = T(0.5) * kxx * (q[cc + 1] - q[cc - 1]);
+            gr[d][1] = T(0.5) * kxy * (q[cc + sy] - q[cc - sy]);
+            gr[d][2] = T(0.5) * kxz * (q[cc + sz] - q[cc - sz]);
+        }
+        const T dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
+        const T dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
+        const T dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
+        T d1 = T(2) * dux * dux, d2 = T(2) * dvy * dvy, d3 = T(2) * dwz * dwz;
+        T d4 = (duy + dvx) * (duy + dvx), d5 = (dvz + dwy) * (dvz + dwy), d6 = (duz + dwx) * (duz + dwx);
+        T Du = sqrt(d1 + d2 + d3 + d4 + d5 + d6);
+        T De = cbrt(met.JA(cc, i, j, k));
+        T lc = Cs * De;
+        nut[cc] = lc * lc * Du;
+    });
+}
+
+// |div|^2 of the step monitor fused into the divergence pass: the interior
+// tiles are walked in the chunked order of the reduction kernels, so the
+// partial sums equal those of a separate dot product over the stored field.
+template <class T, class M>
+__global__ __launch_bounds__(256) void k_div_norm(Tiling tl, Geom g, Vec3<T> uu, T *__restrict__ div, M met,
+                                                  double *partials) {
+    __shared__ double lds[256 / 64];
+    int i, j, k, t0, t1;
+    double v[1] = {0.0};
+    tile_range(tl.tiles, t0, t1);
+    for (int tile = t0; tile < t1; tile++) {
+        if (!tile_cell_at(tl, tile, i, j, k)) continue;
+        const int cc = g.id(i, j, k);
+        T UE = uu.c[0][cc], UW = uu.c[0][cc - 1];
+        T VN = uu.c[1][cc], VS = uu.c[1][cc - g.nx];
+        T WT = uu.c[2][cc], WB = uu.c[2][cc - g.nxy];
+        T d = (UE - UW + VN - VS + WT - WB) / met.JA(cc, i, j, k);
+        div[cc] = d;
+        v[0] += (double)d * (double)d;
     }
-    const T dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
-    const T dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
-    const T dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
-    T d1 = T(2) * dux * dux, d2 = T(2) * dvy * dvy, d3 = T(2) * dwz * dwz;
-    T d4 = (duy + dvx) * (duy + dvx), d5 = (dvz + dwy) * (dvz + dwy), d6 = (duz + dwx) * (duz + dwx);
-    T Du = sqrt(d1 + d2 + d3 + d4 + d5 + d6);
-    T De = cbrt(ja[cc]);
-    T lc = Cs * De;
-    nut[cc] = lc * lc * Du;
+    block_partials<256, 1>(v, partials, lds);
+}
+
+__global__ __launch_bounds__(FIN_NT) void k_fin_norm(const double *partials, int nb, double *out) {
+    double tot[1];
+    sum_partials<1>(partials, nb, tot);
+    if (threadIdx.x == 0) out[0] = tot[0];
 }
 
 // ------------------------------------------------------------ host side
@@ -250,25 +529,44 @@ static Vec3W<T> v3w(void *p, size_t n) {
     for (int d = 0; d < 3; d++) v.c[d] = col<T>(p, n, d);
     return v;
 }
+template <class T>
+static MetField<T> met_field(const void *kx, const void *gg, const void *ja, size_t n) {
+    MetField<T> m;
+    for (int d = 0; d < 3; d++) {
+        m.kx[d] = kx ? col<T>(kx, n, d) : nullptr;
+        m.g[d] = gg ? col<T>(gg, n, d) : nullptr;
+    }
+    m.ja = (const T *)ja;
+    return m;
+}
+template <class T>
+static MetSep<T> met_sep(const SepMesh &sm) {
+    MetSep<T> m;
+    for (int d = 0; d < 3; d++) {
+        m.h[d] = sm.h[d];
+        m.rh[d] = sm.rh[d];
+    }
+    return m;
+}
 
 static bool overlaps(const void *a, size_t abytes, const void *b, size_t bbytes) {
     const char *x = (const char *)a, *y = (const char *)b;
     return x < y + bbytes && y < x + abytes;
 }
 
-template <class T>
+template <class T, class M>
 static falm_status pseudo_u_t(const falm_domain *dom, int scheme, double ReI, double dt, const void *un,
-                              const void *u, const void *uu, void *ua, const void *nut, const void *kx,
-                              const void *gg, const void *ja, const void *ff, hipStream_t s) {
+                              const void *u, const void *uu, void *ua, const void *nut, const M &met,
+                              const void *ff, const int *ffbox, double words, hipStream_t s) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     Tiling tl = make_tiling(interior_box(g));
     if (tl.tiles == 0) return FALM_OK;
     TimedLaunch tm(FALM_TIME_PSEUDO_U, s);
-#define FALM_PU(S)                                                                                   \
-    hipLaunchKernelGGL((k_pseudo_u<S, T>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (T)ReI, (T)dt, \
-                       v3<T>(un, n), v3<T>(u, n), v3<T>(uu, n), v3w<T>(ua, n), (const T *)nut, v3<T>(kx, n), \
-                       v3<T>(gg, n), (const T *)ja, v3<T>(ff, n))
+#define FALM_PU(S)                                                                                      \
+    hipLaunchKernelGGL((k_pseudo_u<S, T, M>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (T)ReI, (T)dt, \
+                       v3<T>(un, n), v3<T>(u, n), v3<T>(uu, n), v3w<T>(ua, n), (const T *)nut, met, v3<T>(ff, n), \
+                       ffbox)
     switch (scheme) {
     case FALM_ADV_UPWIND1: FALM_PU(FALM_ADV_UPWIND1); break;
     case FALM_ADV_UPWIND3: FALM_PU(FALM_ADV_UPWIND3); break;
@@ -279,81 +577,160 @@ static falm_status pseudo_u_t(const falm_domain *dom, int scheme, double ReI, do
     }
 #undef FALM_PU
     FALM_CHECK_LAUNCH("k_pseudo_u");
-    tm.done((double)tl.box.size() * 23.0 * sizeof(T));
+    tm.done((double)tl.box.size() * words * sizeof(T));
     return FALM_OK;
 }
 
-template <class T>
-static falm_status u_to_uu_t(const falm_domain *dom, const void *u, void *uu, const void *kx, const void *ja,
-                             hipStream_t s) {
+template <class T, class M>
+static falm_status u_to_uu_t(const falm_domain *dom, const void *u, void *uu, const M &met, hipStream_t s) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     Tiling tl = make_tiling(face_box(g));
     if (tl.tiles == 0) return FALM_OK;
-    hipLaunchKernelGGL((k_u_to_uu<T>), tiling_grid(tl), tiling_block(), 0, s, tl, g, v3<T>(u, n), v3w<T>(uu, n),
-                       v3<T>(kx, n), (const T *)ja);
+    TimedLaunch tm(FALM_TIME_FACE_FLUX, s);
+    hipLaunchKernelGGL((k_u_to_uu<T, M>), tiling_grid(tl), tiling_block(), 0, s, tl, g, v3<T>(u, n), v3w<T>(uu, n),
+                       met);
     FALM_CHECK_LAUNCH("k_u_to_uu");
+    // u read, uu written (+ kx, ja)
+    tm.done((double)tl.box.size() * (6.0 + (M::per_cell ? 4.0 : 0.0)) * sizeof(T));
     return FALM_OK;
 }
 
-template <class T, class TO, bool SCALE>
-static falm_status divergence_t(const falm_domain *dom, const void *uu, void *div, const void *ja,
-                                double scale, hipStream_t s) {
+// whole_array: the ScaleMatrix pass of MACCalcPoissonRHS covers every padded
+// cell; the step driver keeps ghost rhs at zero and passes false
+template <class T, class TO, bool SCALE, class M>
+static falm_status divergence_t(const falm_domain *dom, const void *uu, void *div, const M &met, double scale,
+                                bool whole_array, hipStream_t s) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
-    Box b = SCALE ? Box{{0, 0, 0}, {g.nx, g.ny, g.nz}} : interior_box(g);
+    Box b = (SCALE && whole_array) ? Box{{0, 0, 0}, {g.nx, g.ny, g.nz}} : interior_box(g);
     Tiling tl = make_tiling(b);
     if (tl.tiles == 0) return FALM_OK;
-    hipLaunchKernelGGL((k_divergence<T, TO, SCALE>), tiling_grid(tl), tiling_block(), 0, s, tl, g, v3<T>(uu, n),
-                       (TO *)div, (const T *)ja, scale);
+    TimedLaunch tm(SCALE ? FALM_TIME_RHS : FALM_TIME_DIVERGENCE, s);
+    hipLaunchKernelGGL((k_divergence<T, TO, SCALE, M>), tiling_grid(tl), tiling_block(), 0, s, tl, g, v3<T>(uu, n),
+                       (TO *)div, met, scale);
     FALM_CHECK_LAUNCH("k_divergence");
+    // uu read, div written (+ ja)
+    tm.done((double)interior_box(g).size() * ((3.0 + (M::per_cell ? 1.0 : 0.0)) * sizeof(T) + sizeof(TO)));
     return FALM_OK;
 }
 
-template <class T, class P>
+template <class T, class P, class M>
 static falm_status project_t(const falm_domain *dom, double dt, void *u, const void *ua, void *uu,
-                             const void *uua, const void *p, const void *kx, const void *gg, hipStream_t s) {
+                             const void *uua, const void *p, const M &met, hipStream_t s) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     Tiling tl = make_tiling(face_box(g));
     if (tl.tiles == 0) return FALM_OK;
-    hipLaunchKernelGGL((k_project<T, P>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (P)dt, col<T>(u, n, 0),
+    TimedLaunch tm(FALM_TIME_PROJECT, s);
+    hipLaunchKernelGGL((k_project<T, P, M>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (P)dt, col<T>(u, n, 0),
                        col<T>(u, n, 1), col<T>(u, n, 2), col<T>(ua, n, 0), col<T>(ua, n, 1), col<T>(ua, n, 2),
                        col<T>(uu, n, 0), col<T>(uu, n, 1), col<T>(uu, n, 2), col<T>(uua, n, 0), col<T>(uua, n, 1),
-                       col<T>(uua, n, 2), (const P *)p, v3<T>(kx, n), v3<T>(gg, n));
+                       col<T>(uua, n, 2), (const P *)p, met);
     FALM_CHECK_LAUNCH("k_project");
+    // ua, uua read, u, uu written, p read (+ kx, g)
+    tm.done((double)tl.box.size() * ((12.0 + (M::per_cell ? 6.0 : 0.0)) * sizeof(T) + sizeof(P)));
     return FALM_OK;
 }
 
-template <class T>
-static falm_status smag_t(const falm_domain *dom, double Cs, const void *u, void *nut, const void *kx,
-                          const void *ja, hipStream_t s) {
+template <class T, class M>
+static falm_status smag_t(const falm_domain *dom, double Cs, const void *u, void *nut, const M &met,
+                          hipStream_t s) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     Tiling tl = make_tiling(interior_box(g));
     if (tl.tiles == 0) return FALM_OK;
-    hipLaunchKernelGGL((k_smagorinsky<T>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (T)Cs, v3<T>(u, n),
-                       (T *)nut, v3<T>(kx, n), (const T *)ja);
+    TimedLaunch tm(FALM_TIME_SGS, s);
+    hipLaunchKernelGGL((k_smagorinsky<T, M>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (T)Cs, v3<T>(u, n),
+                       (T *)nut, met);
     FALM_CHECK_LAUNCH("k_smagorinsky");
+    // u read, nut written (+ kx, ja)
+    tm.done((double)tl.box.size() * (4.0 + (M::per_cell ? 4.0 : 0.0)) * sizeof(T));
     return FALM_OK;
 }
 
-// Entry points shared with the step driver (mixed-type forms).
-falm_status launch_poisson_rhs_mixed(const falm_domain *dom, int prec, const void *uu, double *rhs,
-                                     const void *ja, double scale, hipStream_t s) {
-    if (prec == FALM_F32) return divergence_t<float, double, true>(dom, uu, rhs, ja, scale, s);
-    return divergence_t<double, double, true>(dom, uu, rhs, ja, scale, s);
+template <class T>
+static falm_status div_norm_t(const falm_domain *dom, const void *uu, void *div, const SepMesh &sm,
+                              double *dev_out, double *partials, hipStream_t s) {
+    Geom g = make_geom(dom);
+    size_t n = ncell_of(dom);
+    Tiling tl = make_tiling(interior_box(g));
+    if (tl.tiles == 0) return FALM_OK;
+    TimedLaunch tm(FALM_TIME_DIVERGENCE, s);
+    hipLaunchKernelGGL((k_div_norm<T, MetSep<T>>), red_grid(tl), tiling_block(), 0, s, tl, g, v3<T>(uu, n),
+                       (T *)div, met_sep<T>(sm), partials);
+    tm.done((double)tl.box.size() * 4.0 * sizeof(T));  // uu read, div written
+    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, red_blocks(tl), dev_out);
+    FALM_CHECK_LAUNCH("k_div_norm");
+    return FALM_OK;
 }
-falm_status launch_divergence_mixed(const falm_domain *dom, int prec, const void *uu, void *div,
-                                    const void *ja, hipStream_t s) {
-    if (prec == FALM_F32) return divergence_t<float, float, false>(dom, uu, div, ja, 1.0, s);
-    return divergence_t<double, double, false>(dom, uu, div, ja, 1.0, s);
+
+// ---------------------------------------------- step-driver entry points
+// Tensor-product mesh metrics (MetSep); CFD fields in prec, pressure and
+// Poisson right-hand side in double.
+template <class T>
+static falm_status pseudo_u_km_t(const falm_domain *dom, int scheme, double ReI, double dt, const void *u,
+                                 const void *uu, void *ua, const void *nut, const SepMesh &sm, const void *ff,
+                                 const int *ffbox, hipStream_t s) {
+    Geom g = make_geom(dom);
+    size_t n = ncell_of(dom);
+    KmTiling kt = make_km_tiling(interior_box(g));
+    if (kt.blocks == 0) return FALM_OK;
+    TimedLaunch tm(FALM_TIME_PSEUDO_U, s);
+#define FALM_PUK(S)                                                                                        \
+    hipLaunchKernelGGL((k_pseudo_u_km<S, T>), dim3(kt.blocks), dim3(KM_NT), 0, s, kt, g, (T)ReI, (T)dt,   \
+                       v3<T>(u, n), v3<T>(uu, n), v3w<T>(ua, n), (const T *)nut, met_sep<T>(sm), v3<T>(ff, n), \
+                       ffbox)
+    switch (scheme) {
+    case FALM_ADV_UPWIND1: FALM_PUK(FALM_ADV_UPWIND1); break;
+    case FALM_ADV_UPWIND3: FALM_PUK(FALM_ADV_UPWIND3); break;
+    case FALM_ADV_QUICK: FALM_PUK(FALM_ADV_QUICK); break;
+    case FALM_ADV_UTOPIA: FALM_PUK(FALM_ADV_UTOPIA); break;
+    case FALM_ADV_KK: FALM_PUK(FALM_ADV_KK); break;
+    default: FALM_PUK(FALM_ADV_UPWIND1); break;
+    }
+#undef FALM_PUK
+    FALM_CHECK_LAUNCH("k_pseudo_u_km");
+    // u, uu, nut read, ua written (ff only inside the rotor's force box)
+    tm.done((double)interior_box(g).size() * 10.0 * sizeof(T));
+    return FALM_OK;
 }
-falm_status launch_project_mixed(const falm_domain *dom, int prec, double dt, void *u, const void *ua,
-                                 void *uu, const void *uua, const double *p, const void *kx, const void *gg,
-                                 hipStream_t s) {
-    if (prec == FALM_F32) return project_t<float, double>(dom, dt, u, ua, uu, uua, p, kx, gg, s);
-    return project_t<double, double>(dom, dt, u, ua, uu, uua, p, kx, gg, s);
+
+falm_status sep_pseudo_u(const falm_domain *dom, int prec, int scheme, double ReI, double dt, const void *un,
+                         const void *u, const void *uu, void *ua, const void *nut, const SepMesh &sm, const void *ff,
+                         const int *ffbox, hipStream_t s) {
+    if (un == u) {  // the step driver's form: ua = u + dt(...)
+        if (prec == FALM_F32) return pseudo_u_km_t<float>(dom, scheme, ReI, dt, u, uu, ua, nut, sm, ff, ffbox, s);
+        return pseudo_u_km_t<double>(dom, scheme, ReI, dt, u, uu, ua, nut, sm, ff, ffbox, s);
+    }
+    if (prec == FALM_F32)
+        return pseudo_u_t<float>(dom, scheme, ReI, dt, un, u, uu, ua, nut, met_sep<float>(sm), ff, ffbox, 13.0, s);
+    return pseudo_u_t<double>(dom, scheme, ReI, dt, un, u, uu, ua, nut, met_sep<double>(sm), ff, ffbox, 13.0, s);
+}
+falm_status sep_u_to_uu(const falm_domain *dom, int prec, const void *u, void *uu, const SepMesh &sm,
+                        hipStream_t s) {
+    if (prec == FALM_F32) return u_to_uu_t<float>(dom, u, uu, met_sep<float>(sm), s);
+    return u_to_uu_t<double>(dom, u, uu, met_sep<double>(sm), s);
+}
+falm_status sep_poisson_rhs(const falm_domain *dom, int prec, const void *uu, double *rhs, const SepMesh &sm,
+                            double scale, hipStream_t s) {
+    if (prec == FALM_F32) return divergence_t<float, double, true>(dom, uu, rhs, met_sep<float>(sm), scale, false, s);
+    return divergence_t<double, double, true>(dom, uu, rhs, met_sep<double>(sm), scale, false, s);
+}
+falm_status sep_project(const falm_domain *dom, int prec, double dt, void *u, void *uu, const double *p,
+                        const SepMesh &sm, hipStream_t s) {
+    if (prec == FALM_F32) return project_t<float, double>(dom, dt, u, u, uu, uu, p, met_sep<float>(sm), s);
+    return project_t<double, double>(dom, dt, u, u, uu, uu, p, met_sep<double>(sm), s);
+}
+falm_status sep_sgs(const falm_domain *dom, int prec, double Cs, const void *u, void *nut, const SepMesh &sm,
+                    hipStream_t s) {
+    if (prec == FALM_F32) return smag_t<float>(dom, Cs, u, nut, met_sep<float>(sm), s);
+    return smag_t<double>(dom, Cs, u, nut, met_sep<double>(sm), s);
+}
+falm_status sep_div_norm(const falm_domain *dom, int prec, const void *uu, void *div, const SepMesh &sm,
+                         double *dev_out, double *partials, hipStream_t s) {
+    if (prec == FALM_F32) return div_norm_t<float>(dom, uu, div, sm, dev_out, partials, s);
+    return div_norm_t<double>(dom, uu, div, sm, dev_out, partials, s);
 }
 
 }  // namespace falm
@@ -376,8 +753,12 @@ falm_status falm_pseudo_u(const falm_domain *dom, int prec, int adv_scheme, doub
         return FALM_ERR_ALIAS;
     }
     hipStream_t s = (hipStream_t)stream;
-    if (prec == FALM_F32) return pseudo_u_t<float>(dom, adv_scheme, ReI, dt, un, u, uu, ua, nut, kx, g, ja, ff, s);
-    return pseudo_u_t<double>(dom, adv_scheme, ReI, dt, un, u, uu, ua, nut, kx, g, ja, ff, s);
+    const size_t n = ncell_of(dom);
+    if (prec == FALM_F32)
+        return pseudo_u_t<float>(dom, adv_scheme, ReI, dt, un, u, uu, ua, nut, met_field<float>(kx, g, ja, n), ff,
+                                 nullptr, 23.0, s);
+    return pseudo_u_t<double>(dom, adv_scheme, ReI, dt, un, u, uu, ua, nut, met_field<double>(kx, g, ja, n), ff,
+                              nullptr, 23.0, s);
 }
 
 falm_status falm_u_to_uu(const falm_domain *dom, int prec, const void *u, void *uu, const void *kx,
@@ -388,8 +769,9 @@ falm_status falm_u_to_uu(const falm_domain *dom, int prec, const void *u, void *
         return FALM_ERR_INVALID;
     }
     hipStream_t s = (hipStream_t)stream;
-    if (prec == FALM_F32) return u_to_uu_t<float>(dom, u, uu, kx, ja, s);
-    return u_to_uu_t<double>(dom, u, uu, kx, ja, s);
+    const size_t n = ncell_of(dom);
+    if (prec == FALM_F32) return u_to_uu_t<float>(dom, u, uu, met_field<float>(kx, nullptr, ja, n), s);
+    return u_to_uu_t<double>(dom, u, uu, met_field<double>(kx, nullptr, ja, n), s);
 }
 
 falm_status falm_poisson_rhs(const falm_domain *dom, int prec, const void *uu, void *rhs, const void *ja,
@@ -401,8 +783,11 @@ falm_status falm_poisson_rhs(const falm_domain *dom, int prec, const void *uu, v
     }
     hipStream_t s = (hipStream_t)stream;
     double scale = 1.0 / (dt * maxdiag);
-    if (prec == FALM_F32) return divergence_t<float, float, true>(dom, uu, rhs, ja, scale, s);
-    return divergence_t<double, double, true>(dom, uu, rhs, ja, scale, s);
+    const size_t n = ncell_of(dom);
+    if (prec == FALM_F32)
+        return divergence_t<float, float, true>(dom, uu, rhs, met_field<float>(nullptr, nullptr, ja, n), scale, true, s);
+    return divergence_t<double, double, true>(dom, uu, rhs, met_field<double>(nullptr, nullptr, ja, n), scale, true,
+                                              s);
 }
 
 falm_status falm_project_p(const falm_domain *dom, int prec, double dt, void *u, const void *ua, void *uu,
@@ -413,8 +798,9 @@ falm_status falm_project_p(const falm_domain *dom, int prec, double dt, void *u,
         return FALM_ERR_INVALID;
     }
     hipStream_t s = (hipStream_t)stream;
-    if (prec == FALM_F32) return project_t<float, float>(dom, dt, u, ua, uu, uua, p, kx, g, s);
-    return project_t<double, double>(dom, dt, u, ua, uu, uua, p, kx, g, s);
+    const size_t n = ncell_of(dom);
+    if (prec == FALM_F32) return project_t<float, float>(dom, dt, u, ua, uu, uua, p, met_field<float>(kx, g, nullptr, n), s);
+    return project_t<double, double>(dom, dt, u, ua, uu, uua, p, met_field<double>(kx, g, nullptr, n), s);
 }
 
 falm_status falm_sgs(const falm_domain *dom, int prec, int model, double Cs, const void *u, void *nut,
@@ -430,8 +816,9 @@ falm_status falm_sgs(const falm_domain *dom, int prec, int model, double Cs, con
         return FALM_ERR_INVALID;
     }
     hipStream_t s = (hipStream_t)stream;
-    if (prec == FALM_F32) return smag_t<float>(dom, Cs, u, nut, kx, ja, s);
-    return smag_t<double>(dom, Cs, u, nut, kx, ja, s);
+    const size_t n = ncell_of(dom);
+    if (prec == FALM_F32) return smag_t<float>(dom, Cs, u, nut, met_field<float>(kx, nullptr, ja, n), s);
+    return smag_t<double>(dom, Cs, u, nut, met_field<double>(kx, nullptr, ja, n), s);
 }
 
 falm_status falm_divergence(const falm_domain *dom, int prec, const void *uu, void *div, const void *ja,
@@ -442,8 +829,11 @@ falm_status falm_divergence(const falm_domain *dom, int prec, const void *uu, vo
         return FALM_ERR_INVALID;
     }
     hipStream_t s = (hipStream_t)stream;
-    if (prec == FALM_F32) return divergence_t<float, float, false>(dom, uu, div, ja, 1.0, s);
-    return divergence_t<double, double, false>(dom, uu, div, ja, 1.0, s);
+    const size_t n = ncell_of(dom);
+    if (prec == FALM_F32)
+        return divergence_t<float, float, false>(dom, uu, div, met_field<float>(nullptr, nullptr, ja, n), 1.0, false, s);
+    return divergence_t<double, double, false>(dom, uu, div, met_field<double>(nullptr, nullptr, ja, n), 1.0, false,
+                                               s);
 }
 
 }  // extern "C"

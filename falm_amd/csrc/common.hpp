@@ -115,15 +115,28 @@ __device__ inline int xcd_tile(int bid, int nblocks) {
     return full * per + (xcd - full) * (per - 1) + loc;
 }
 
-// cell coordinates of this thread in tile `tile`; false outside the box
+// Tile order: x tile fastest, then y, then k, so consecutive workgroups
+// stream through contiguous memory (a k-fastest order that keeps stencil
+// neighbours in flight together measured slower: every tile then starts a
+// new DRAM page).
+__host__ __device__ inline void tile_coords(const Tiling &t, int tile, int &bx, int &by, int &kk) {
+    const int txy = t.tx * t.ty;
+    kk = tile / txy;
+    const int r = tile - kk * txy;
+    by = r / t.tx;
+    bx = r - by * t.tx;
+}
+
+// cell coordinates of this thread in tile `tile`; false outside the box.
+// A wavefront covers one x row of the tile, so j and k are wave-uniform:
+// readfirstlane makes that visible to the compiler, which then keeps them
+// (and loads indexed by them alone) in scalar registers.
 __device__ inline bool tile_cell_at(const Tiling &t, int tile, int &i, int &j, int &k) {
-    int txy = t.tx * t.ty;
-    k = tile / txy;
-    int r = tile - k * txy;
-    int by = r / t.tx;
-    int bx = r - by * t.tx;
+    int bx, by;
+    tile = __builtin_amdgcn_readfirstlane(tile);
+    tile_coords(t, tile, bx, by, k);
     i = bx * TX + (int)(threadIdx.x & (TX - 1));
-    j = by * TY + (int)(threadIdx.x >> 6);
+    j = by * TY + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (i >= t.box.n[0] || j >= t.box.n[1]) return false;
     i += t.box.lo[0];
     j += t.box.lo[1];
@@ -144,24 +157,25 @@ __device__ inline void tile_range(int ntiles, int &t0, int &t1) {
     t1 = t0 + C < ntiles ? t0 + C : ntiles;
 }
 
-// cell coordinates of this thread; returns false outside the box
-__device__ inline bool tile_cell(const Tiling &t, int &i, int &j, int &k) {
-    int tile = xcd_tile(blockIdx.x, gridDim.x);
-    int txy = t.tx * t.ty;
-    k = tile / txy;
-    int r = tile - k * txy;
-    int by = r / t.tx;
-    int bx = r - by * t.tx;
-    i = bx * TX + (int)(threadIdx.x & (TX - 1));
-    j = by * TY + (int)(threadIdx.x >> 6);
-    if (i >= t.box.n[0] || j >= t.box.n[1]) return false;
-    i += t.box.lo[0];
-    j += t.box.lo[1];
-    k += t.box.lo[2];
-    return true;
+// The non-reducing kernels take one tile per workgroup (one_tile_grid):
+// measured on MI355X, walking several tiles per workgroup made the stencil
+// kernels slower.  f(i, j, k) is called for this thread's cell of the tile
+// (return in f ends the thread's work).
+template <class F>
+__device__ inline void for_each_cell(const Tiling &t, F f) {
+    int i, j, k;
+    if (tile_cell_at(t, xcd_tile(blockIdx.x, gridDim.x), i, j, k)) f(i, j, k);
 }
 
+// cell coordinates of this thread for one tile per workgroup; false outside
+// the box
+__device__ inline bool tile_cell(const Tiling &t, int &i, int &j, int &k) {
+    return tile_cell_at(t, xcd_tile(blockIdx.x, gridDim.x), i, j, k);
+}
+
+// one tile per workgroup (for_each_cell, tile_cell)
 inline dim3 tiling_grid(const Tiling &t) { return dim3((unsigned)t.tiles); }
+inline dim3 one_tile_grid(const Tiling &t) { return dim3((unsigned)t.tiles); }
 inline dim3 tiling_block() { return dim3(TX * TY); }
 
 // 1-D element-wise launches

@@ -11,19 +11,19 @@ namespace falm {
 template <class TA, class TX>
 __global__ __launch_bounds__(256) void k_mv(Tiling tl, Geom g, Mat7<TA> a, const TX *__restrict__ x,
                                             TX *__restrict__ ax) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    ax[cc] = apply7<TA, TX, TX>(a, x, cc, g.nx, g.nxy);
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        ax[cc] = apply7<TA, TX, TX>(a, x, cc, g.nx, g.nxy);
+    });
 }
 
 template <class TA, class TX>
 __global__ __launch_bounds__(256) void k_residual(Tiling tl, Geom g, Mat7<TA> a, const TX *__restrict__ x,
                                                   const TX *__restrict__ b, TX *__restrict__ r) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    r[cc] = b[cc] - apply7<TA, TX, TX>(a, x, cc, g.nx, g.nxy);
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        r[cc] = b[cc] - apply7<TA, TX, TX>(a, x, cc, g.nx, g.nxy);
+    });
 }
 
 // kernel_DotProduct / kernel_EuclideanNormSq (src/dev/MVDevCall.cu:57-179):
@@ -126,14 +126,14 @@ template <class TA, class TX>
 __global__ __launch_bounds__(256) void k_sor(Tiling tl, Geom g, Mat7<TA> a, TX *x, const TX *__restrict__ b,
                                              TX omega, int color, const int *done) {
     if (done && *done) return;
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    if (((i + j + k + g.osum) & 1) != color) return;
-    const int cc = g.id(i, j, k);
-    const TX xc = x[cc];
-    const TX s = apply7<TA, TX, TX>(a, x, cc, g.nx, g.nxy);
-    const TX cv = (b[cc] - s) / (TX)a.c[0][cc];
-    x[cc] = xc + omega * cv;
+    for_each_cell(tl, [&](int i, int j, int k) {
+        if (((i + j + k + g.osum) & 1) != color) return;
+        const int cc = g.id(i, j, k);
+        const TX xc = x[cc];
+        const TX s = apply7<TA, TX, TX>(a, x, cc, g.nx, g.nxy);
+        const TX cv = (b[cc] - s) / (TX)a.c[0][cc];
+        x[cc] = xc + omega * cv;
+    });
 }
 
 // kernel_Struct3d7p_Jacobi (src/dev/FalmEqDevCall.cu:101-132)
@@ -142,11 +142,11 @@ __global__ __launch_bounds__(256) void k_jacobi(Tiling tl, Geom g, Mat7<TA> a, T
                                                 const TX *__restrict__ xp, const TX *__restrict__ b,
                                                 const int *done) {
     if (done && *done) return;
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    const TX xc = xp[cc];
-    x[cc] = xc + (b[cc] - apply7<TA, TX, TX>(a, xp, cc, g.nx, g.nxy)) / (TX)a.c[0][cc];
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        const TX xc = xp[cc];
+        x[cc] = xc + (b[cc] - apply7<TA, TX, TX>(a, xp, cc, g.nx, g.nxy)) / (TX)a.c[0][cc];
+    });
 }
 
 // ------------------------------------------------------------ launches
@@ -263,10 +263,10 @@ falm_status host_minmax(const falm_domain *dom, int prec, int op, const void *a,
     FALM_TRY(reduce_scratch((size_t)tl.tiles, &rs));
     size_t n = ncell_of(dom);
     if (prec == FALM_F32)
-        hipLaunchKernelGGL((k_minmax<float>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (const float *)a, op,
+        hipLaunchKernelGGL((k_minmax<float>), one_tile_grid(tl), tiling_block(), 0, s, tl, g, (const float *)a, op,
                            ncol, col, n, rs->partials);
     else
-        hipLaunchKernelGGL((k_minmax<double>), tiling_grid(tl), tiling_block(), 0, s, tl, g, (const double *)a, op,
+        hipLaunchKernelGGL((k_minmax<double>), one_tile_grid(tl), tiling_block(), 0, s, tl, g, (const double *)a, op,
                            ncol, col, n, rs->partials);
     FALM_CHECK_LAUNCH("k_minmax");
     std::vector<double> h(2 * (size_t)tl.tiles);

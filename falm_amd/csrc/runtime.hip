@@ -20,6 +20,7 @@ void set_error(const char *fmt, ...) {
 }
 
 falm_status hip_status(hipError_t e, const char *what) {
+    if (e == hipSuccess) return FALM_OK;
     set_error("%s failed: %s (%d)", what, hipGetErrorString(e), (int)e);
     if (e == hipErrorOutOfMemory) return FALM_ERR_MALLOC;
     if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return FALM_ERR_NO_DEVICE;
@@ -90,7 +91,8 @@ static hipEvent_t get_event() {
         return e;
     }
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    // timing only: no system-scope fence (cache write-back) at the record
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 
@@ -167,6 +169,9 @@ falm_status falm_malloc(void **ptr, size_t bytes) {
     if (!bytes) return FALM_OK;
     FALM_HIP(hipMalloc(ptr, bytes));
     FALM_HIP(hipMemset(*ptr, 0, bytes));  // Matrix::alloc zero-fills (src/matrix.h:85-98)
+    // complete the fill before work on any stream (non-blocking ones do not
+    // wait for the null stream)
+    FALM_HIP(hipDeviceSynchronize());
     return FALM_OK;
 }
 

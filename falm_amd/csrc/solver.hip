@@ -38,16 +38,22 @@ namespace falm {
 
 struct DevState {
     double rho, rrho, alpha, omega, beta;
-    double err;        // FalmEq::err of the current pass
-    double tol_inner;  // the pass stops when err <= tol_inner
-    double gN;         // global interior cell count
-    int it_pass;       // iterations of the current pass
+    double err;         // FalmEq::err: recursive residual of the pass; in
+                        // mixed mode the init kernels store the true one
+    double tol_inner;   // the pass stops when err <= tol_inner
+    double gN;          // global interior cell count
+    double tol, inner_rtol;
+    int it_pass;        // iterations of the current pass
     int maxit_pass;
-    int done;
+    int done;           // the pass is over: iteration kernels do nothing
+    int final;          // the solve is over: init kernels do nothing
     int status;
-    int first;         // next p update is p = r
+    int first;          // next p update is p = r
     int mixed;
-    int pad[2];
+    int total;          // iterations over all passes
+    int passes;         // passes started
+    int maxit, inner_maxit;
+    int pad;
 };
 
 constexpr int NT = 256;
@@ -71,7 +77,8 @@ __device__ inline T pc_first(int pc_type, T omega_pc, T b, T ac, bool black) {
 template <class T, class OP>
 __global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, OP a, const double *__restrict__ x,
                                                   const double *__restrict__ b, double *r64, T *r, T *rr,
-                                                  double *partials) {
+                                                  const DevState *st, double *partials) {
+    if (st->final || !st->done) return;  // a pass is still running, or the solve is over
     __shared__ double lds[2 * (NT / 64)];
     int i, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
@@ -80,7 +87,7 @@ __global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, OP a, const
         if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
         const double rv = b[cc] - apply_op<OP, double, double, double>(a, x, cc, i, j, k, g.nx, g.nxy);
-        r64[cc] = rv;
+        if (r64) r64[cc] = rv;
         const T rt = (T)rv;
         if ((void *)r != (void *)r64) r[cc] = rt;
         rr[cc] = rt;
@@ -90,25 +97,46 @@ __global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, OP a, const
     block_partials<NT, 2>(v, partials, lds);
 }
 
+// Start of a pass (src/FalmEq.cpp:318-351).  In mixed mode this is also the
+// outer loop's test: the true residual of the accumulated x ends the solve
+// once a pass has run, and otherwise sets the float pass's target.
 __global__ __launch_bounds__(FIN_NT) void k_fin_init(DevState *st, const double *partials, int nb) {
+    if (st->final || !st->done) return;
     double tot[2];
     sum_partials<2>(partials, nb, tot);
     if (threadIdx.x == 0) {
-        st->err = sqrt(tot[0]) / st->gN;
+        const double err = sqrt(tot[0]) / st->gN;
+        st->err = err;
         st->rho = tot[1];
+        if (st->mixed && ((st->passes > 0 && err <= st->tol) || st->total >= st->maxit)) {
+            st->final = 1;
+            return;
+        }
         st->rrho = 1.0;
         st->alpha = 0.0;
         st->omega = 1.0;
         st->beta = 0.0;
         st->it_pass = 0;
         st->first = 1;
-        st->done = 0;
-        st->status = FALM_OK;
         if (fabs(st->rho) < (double)FLT_MIN) {  // src/FalmEq.cpp:345-351
-            if (!st->mixed) st->err = st->rho;
-            st->status = FALM_ERR_BREAKDOWN;
-            st->done = 1;
+            if (st->mixed) {
+                // the float residual vanished: no float pass can improve x
+                st->status = err <= st->tol ? FALM_OK : FALM_ERR_BREAKDOWN;
+            } else {
+                st->err = st->rho;
+                st->status = FALM_ERR_BREAKDOWN;
+            }
+            st->final = 1;
+            return;
         }
+        if (st->mixed) {
+            const double target = st->inner_rtol * err;
+            st->tol_inner = target > 0.5 * st->tol ? target : 0.5 * st->tol;
+            const int left = st->maxit - st->total;
+            st->maxit_pass = left < st->inner_maxit ? left : st->inner_maxit;
+        }
+        st->passes += 1;
+        st->done = 0;
     }
 }
 
@@ -119,18 +147,18 @@ __global__ __launch_bounds__(NT) void k_bicg_p(Tiling tl, Geom g, OP a, const T 
                                                const T *__restrict__ q, T *__restrict__ pp, const DevState *st,
                                                int pc_type, T omega_pc) {
     if (st->done) return;
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    T pv;
-    if (st->first) {
-        pv = r[cc];
-    } else {
-        const T beta = (T)st->beta, omega = (T)st->omega;
-        pv = r[cc] + beta * (p[cc] - omega * q[cc]);
-    }
-    p[cc] = pv;
-    pp[cc] = pc_first<T>(pc_type, omega_pc, pv, a.diag(cc), ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK);
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        T pv;
+        if (st->first) {
+            pv = r[cc];
+        } else {
+            const T beta = (T)st->beta, omega = (T)st->omega;
+            pv = r[cc] + beta * (p[cc] - omega * q[cc]);
+        }
+        p[cc] = pv;
+        pp[cc] = pc_first<T>(pc_type, omega_pc, pv, a.diag(cc), ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK);
+    });
 }
 
 // s = r - alpha q (kernel_PBiCGStab_2), then ss = M0(s)
@@ -139,13 +167,13 @@ __global__ __launch_bounds__(NT) void k_bicg_s(Tiling tl, Geom g, OP a, const T 
                                                const T *__restrict__ q, T *__restrict__ s, T *__restrict__ ss,
                                                const DevState *st, int pc_type, T omega_pc) {
     if (st->done) return;
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    const T alpha = (T)st->alpha;
-    const T sv = r[cc] - alpha * q[cc];
-    s[cc] = sv;
-    ss[cc] = pc_first<T>(pc_type, omega_pc, sv, a.diag(cc), ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK);
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        const T alpha = (T)st->alpha;
+        const T sv = r[cc] - alpha * q[cc];
+        s[cc] = sv;
+        ss[cc] = pc_first<T>(pc_type, omega_pc, sv, a.diag(cc), ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK);
+    });
 }
 
 // red-black SOR half sweep on the cells of `color` (kernel_Struct3d7p_SOR,
@@ -154,14 +182,14 @@ template <class T, class OP>
 __global__ __launch_bounds__(NT) void k_sor_op(Tiling tl, Geom g, OP a, T *x, const T *__restrict__ b, T omega,
                                                int color, const int *done) {
     if (*done) return;
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    if (((i + j + k + g.osum) & 1) != color) return;
-    const int cc = g.id(i, j, k);
-    const T xc = x[cc];
-    const T s = apply_op<OP, T, T, T>(a, x, cc, i, j, k, g.nx, g.nxy);
-    const T cv = (b[cc] - s) / a.diag(cc);
-    x[cc] = xc + omega * cv;
+    for_each_cell(tl, [&](int i, int j, int k) {
+        if (((i + j + k + g.osum) & 1) != color) return;
+        const int cc = g.id(i, j, k);
+        const T xc = x[cc];
+        const T s = apply_op<OP, T, T, T>(a, x, cc, i, j, k, g.nx, g.nxy);
+        const T cv = (b[cc] - s) / a.diag(cc);
+        x[cc] = xc + omega * cv;
+    });
 }
 
 // kernel_Struct3d7p_Jacobi (src/dev/FalmEqDevCall.cu:101-132)
@@ -170,21 +198,21 @@ __global__ __launch_bounds__(NT) void k_jacobi_op(Tiling tl, Geom g, OP a, T *__
                                                   const T *__restrict__ xp, const T *__restrict__ b,
                                                   const int *done) {
     if (*done) return;
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    const T xc = xp[cc];
-    x[cc] = xc + (b[cc] - apply_op<OP, T, T, T>(a, xp, cc, i, j, k, g.nx, g.nxy)) / a.diag(cc);
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        const T xc = xp[cc];
+        x[cc] = xc + (b[cc] - apply_op<OP, T, T, T>(a, xp, cc, i, j, k, g.nx, g.nxy)) / a.diag(cc);
+    });
 }
 
 template <class T>
 __global__ __launch_bounds__(NT) void k_copy_interior(Tiling tl, Geom g, const T *__restrict__ src,
                                                       T *__restrict__ dst, const int *done) {
     if (*done) return;
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    dst[cc] = src[cc];
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        dst[cc] = src[cc];
+    });
 }
 
 // q = A pp ; partial (rr, q)
@@ -277,14 +305,18 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_update(DevState *st, const doubl
         st->rrho = st->rho;
         st->err = sqrt(tot[0]) / st->gN;
         st->it_pass += 1;
+        st->total += 1;
         st->first = 0;
         if (!(st->it_pass < st->maxit_pass && st->err > st->tol_inner)) {
             st->done = 1;
         } else {
             st->rho = tot[1];
             if (fabs(st->rho) < (double)FLT_MIN) {
-                if (!st->mixed) st->err = st->rho;
-                st->status = FALM_ERR_BREAKDOWN;
+                // mixed: the pass ends and the outer loop decides
+                if (!st->mixed) {
+                    st->err = st->rho;
+                    st->status = FALM_ERR_BREAKDOWN;
+                }
                 st->done = 1;
             } else {
                 st->beta = (st->rho * st->alpha) / (st->rrho * st->omega);
@@ -330,24 +362,26 @@ __global__ void k_convert(const TS *__restrict__ s, TD *__restrict__ d, size_t n
         d[t] = (TD)s[t];
 }
 
-// limits of the next pass (the init kernels leave them alone)
-__global__ void k_state_arm(DevState *st, double tol, int maxit) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->tol_inner = tol;
-        st->maxit_pass = maxit;
-    }
-}
-
-__global__ void k_state_reset(DevState *st, double gN, int mixed, double tol, int maxit) {
+// A new solve.  Krylov solves start with no pass running (done = 1) and the
+// init kernels armed (final = 0); the stationary solvers start iterating.
+__global__ void k_state_reset(DevState *st, double gN, int mixed, double tol, int maxit, double inner_rtol,
+                              int inner_maxit, int done0) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->gN = gN;
         st->mixed = mixed;
-        st->done = 0;
+        st->done = done0;
+        st->final = 0;
         st->status = FALM_OK;
         st->it_pass = 0;
+        st->total = 0;
+        st->passes = 0;
         st->err = 0.0;
+        st->tol = tol;
         st->tol_inner = tol;
+        st->maxit = maxit;
         st->maxit_pass = maxit;
+        st->inner_rtol = inner_rtol;
+        st->inner_maxit = inner_maxit;
     }
 }
 
@@ -382,12 +416,12 @@ template <class TA>
 __global__ __launch_bounds__(NT) void k_sep_check(Tiling tl, Geom g, Mat7<TA> a, const TA *aw, const TA *ae,
                                                   const TA *as, const TA *an, const TA *ab, const TA *at,
                                                   int *mismatch) {
-    int i, j, k;
-    if (!tile_cell(tl, i, j, k)) return;
-    const int cc = g.id(i, j, k);
-    const bool ok = same_bits(a.c[1][cc], aw[i]) && same_bits(a.c[2][cc], ae[i]) && same_bits(a.c[3][cc], as[j]) &&
-                    same_bits(a.c[4][cc], an[j]) && same_bits(a.c[5][cc], ab[k]) && same_bits(a.c[6][cc], at[k]);
-    if (!ok) *mismatch = 1;  // benign race: every writer stores 1
+    for_each_cell(tl, [&](int i, int j, int k) {
+        const int cc = g.id(i, j, k);
+        const bool ok = same_bits(a.c[1][cc], aw[i]) && same_bits(a.c[2][cc], ae[i]) && same_bits(a.c[3][cc], as[j]) &&
+                        same_bits(a.c[4][cc], an[j]) && same_bits(a.c[5][cc], ab[k]) && same_bits(a.c[6][cc], at[k]);
+        if (!ok) *mismatch = 1;  // benign race: every writer stores 1
+    });
 }
 
 }  // namespace falm
@@ -413,6 +447,8 @@ struct falm_solver {
     double *sep64 = nullptr;      // 1-D arrays: aw,ae [nx], as,an [ny], ab,at [nz]
     float *sep32 = nullptr;
     float *a32 = nullptr;         // mixed: float columns (7 if not separable, else diagonal only)
+    double *r64own = nullptr;     // residual when the caller passes r = NULL (double solvers)
+    int last_it = 0;              // iterations of the previous solve
 };
 
 namespace falm {
@@ -445,6 +481,9 @@ static falm_status solver_alloc(falm_solver *S) {
     FALM_HIP(hipHostMalloc(&S->hflag, sizeof(int), hipHostMallocDefault));
     size_t nb = (size_t)(S->tl.tiles > 0 ? red_blocks(S->tl) : 1);
     FALM_HIP(hipMalloc(&S->partials, 2 * nb * sizeof(double)));
+    // the zero fills above run on the null stream, which the caller's
+    // (possibly non-blocking) stream does not wait for
+    FALM_HIP(hipDeviceSynchronize());
     return FALM_OK;
 }
 
@@ -454,7 +493,7 @@ static falm_status read_state(falm_solver *S, hipStream_t s) {
     return FALM_OK;
 }
 
-static int check_period(falm_solver *S) {
+static int check_period(const falm_solver *S) {
     if (S->cfg.check_every > 0) return S->cfg.check_every;
     double n = (double)S->tl.box.size();
     int c = (int)(4.0e6 / (n > 1 ? n : 1));
@@ -501,11 +540,11 @@ static falm_status pc_rest(falm_solver *S, const OP &a, T *x, const T *b, hipStr
                 TimedLaunch tm(FALM_TIME_SOR, s);
                 hipLaunchKernelGGL((k_sor_op<T, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, w, FALM_COLOR_BLACK,
                                    done);
-                tm.done((double)S->tl.box.size() * 3.5 * sizeof(T));
+                tm.done((double)S->tl.box.size() * 4.0 * sizeof(T));
             }
             TimedLaunch tm(FALM_TIME_SOR, s);
             hipLaunchKernelGGL((k_sor_op<T, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, w, FALM_COLOR_RED, done);
-            tm.done((double)S->tl.box.size() * 3.5 * sizeof(T));
+            tm.done((double)S->tl.box.size() * 4.0 * sizeof(T));
         }
         FALM_CHECK_LAUNCH("k_sor_op");
     } else if (c.pc_type == FALM_LS_JACOBI) {
@@ -555,25 +594,23 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     return FALM_OK;
 }
 
-// run iterations until the device says done
-template <class T, class OP>
-static falm_status bicg_pass(falm_solver *S, const OP &a, double *x, T *r, int maxit_pass, hipStream_t s) {
-    const int period = check_period(S);
-    int enqueued = 0;
-    while (true) {
-        for (int c = 0; c < period && enqueued < maxit_pass; c++, enqueued++)
-            FALM_TRY((bicg_iteration<T, OP>(S, a, x, r, s)));
-        FALM_TRY(read_state(S, s));
-        if (S->hst->done || enqueued >= maxit_pass) break;
-    }
-    return FALM_OK;
+// Iterations enqueued before the first poll: the previous solve's count
+// (time steps in sequence need similar counts), else the polling period.
+static int first_batch(const falm_solver *S) {
+    const int k = S->last_it > 0 ? S->last_it : check_period(S);
+    return k < 1 ? 1 : (k > 64 ? 64 : k);
 }
 
 template <class T, class OP64>
 static falm_status launch_init(falm_solver *S, const OP64 &a, const double *x, const double *b, double *r64, T *rT,
                                hipStream_t s) {
+    TimedLaunch tm(FALM_TIME_SOLVER_INIT, s);
     hipLaunchKernelGGL((k_bicg_init<T, OP64>), FALM_RED(S), s, S->tl, S->g, a, x, b, r64, rT, (T *)S->rr,
-                       S->partials);
+                       (const DevState *)S->st, S->partials);
+    // x, b, diagonal read; r (double, if kept), r and rr (T) written
+    tm.done((double)S->tl.box.size() *
+            (3.0 * sizeof(double) + (r64 ? sizeof(double) : 0.0) + ((void *)rT != (void *)r64 ? sizeof(T) : 0.0) +
+             sizeof(T)));
     hipLaunchKernelGGL(k_fin_init, FALM_FIN, s, S->st, (const double *)S->partials, red_blocks(S->tl));
     FALM_CHECK_LAUNCH("k_bicg_init");
     return FALM_OK;
@@ -583,56 +620,63 @@ template <class OP64>
 static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, const double *b, double *r,
                                       hipStream_t s, falm_solve_info *info) {
     const falm_solver_config &c = S->cfg;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit);
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit, 0.0, 0,
+                       1);
     FALM_CHECK_LAUNCH("k_state_reset");
     FALM_TRY((launch_init<double, OP64>(S, a, x, b, r, r, s)));
-    FALM_TRY((bicg_pass<double, OP64>(S, a, x, r, c.maxit, s)));
+    int batch = first_batch(S), enqueued = 0;
+    while (true) {
+        for (int k = 0; k < batch && enqueued < c.maxit; k++, enqueued++)
+            FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s)));
+        FALM_TRY(read_state(S, s));
+        if (S->hst->done || enqueued >= c.maxit) break;
+        batch = check_period(S);
+    }
     info->it = S->hst->it_pass;
     info->err = S->hst->err;
     info->outer_it = 1;
     info->status = S->hst->status;
     if (info->status == FALM_OK && !(S->hst->err <= c.tol)) info->status = FALM_ERR_NOT_CONVERGED;
+    S->last_it = info->it;
     return FALM_OK;
 }
 
+// Mixed precision: float BiCGStab passes on the defect b - A x, x
+// accumulated in double.  Every pass ends with the init kernels, which
+// compute the true residual in double and either end the solve or start the
+// next pass, all on the device; the host polls once per batch.
 template <class OP64, class OP32>
 static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const OP32 &a32, double *x,
                                         const double *b, double *r, hipStream_t s, falm_solve_info *info) {
     const falm_solver_config &c = S->cfg;
     const int inner_maxit = c.inner_maxit > 0 ? c.inner_maxit : 50;
     const double inner_rtol = c.inner_rtol > 0 ? c.inner_rtol : 1e-4;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 1, c.tol, c.maxit);
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 1, c.tol, c.maxit,
+                       inner_rtol, inner_maxit, 1);
     FALM_CHECK_LAUNCH("k_state_reset");
-    int total = 0, pass = 0;
-    double err = 0.0;
-    int status = FALM_OK;
+    float *r32 = (float *)S->r32;
+    FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
+    // every round either advances the running pass by at least one
+    // iteration or ends/starts a pass; the device ends the solve at maxit
+    int batch = first_batch(S), polls = 0;
     while (true) {
-        // true residual in double; the inner target follows from it
-        FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, (float *)S->r32, s)));
+        for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s)));
+        FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
         FALM_TRY(read_state(S, s));
-        err = S->hst->err;
-        if ((pass > 0 && err <= c.tol) || total >= c.maxit) break;
-        if (S->hst->done) {  // the float residual vanished: nothing more a float pass can do
-            status = (err <= c.tol) ? FALM_OK : FALM_ERR_BREAKDOWN;
-            break;
-        }
-        double target = inner_rtol * err;
-        if (target < 0.5 * c.tol) target = 0.5 * c.tol;
-        int maxit_pass = c.maxit - total < inner_maxit ? c.maxit - total : inner_maxit;
-        hipLaunchKernelGGL(k_state_arm, dim3(1), dim3(64), 0, s, S->st, target, maxit_pass);
-        FALM_CHECK_LAUNCH("k_state_arm");
-        FALM_TRY((bicg_pass<float, OP32>(S, a32, x, (float *)S->r32, maxit_pass, s)));
-        total += S->hst->it_pass;
-        pass++;
-        if (S->hst->it_pass == 0) {
-            status = FALM_ERR_BREAKDOWN;
-            break;
+        polls++;
+        if (S->hst->final) break;
+        batch = check_period(S);
+        if (polls > 2 * c.maxit + 2) {
+            set_error("falm_solver_solve: mixed solve made no progress");
+            return FALM_ERR_BREAKDOWN;
         }
     }
-    info->it = total;
-    info->err = err;
-    info->outer_it = pass;
-    info->status = status != FALM_OK ? status : (err <= c.tol ? FALM_OK : FALM_ERR_NOT_CONVERGED);
+    const DevState &h = *S->hst;
+    info->it = h.total;
+    info->err = h.err;
+    info->outer_it = h.passes;
+    info->status = h.status != FALM_OK ? h.status : (h.err <= c.tol ? FALM_OK : FALM_ERR_NOT_CONVERGED);
+    S->last_it = h.total;
     return FALM_OK;
 }
 
@@ -641,7 +685,8 @@ template <class OP>
 static falm_status solve_stationary(falm_solver *S, const OP &a, double *x, const double *b, double *r,
                                     hipStream_t s, falm_solve_info *info) {
     const falm_solver_config &c = S->cfg;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit);
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit, 0.0, 0,
+                       0);
     FALM_CHECK_LAUNCH("k_state_reset");
     const int period = check_period(S);
     const int *done = &S->st->done;
@@ -750,7 +795,7 @@ falm_status falm_solver_set_matrix(falm_solver *S, const void *a, void *stream) 
 
 falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void *b, void *r, void *stream,
                               falm_solve_info *info) {
-    if (!S || !a || !x || !b || !r) {
+    if (!S || !a || !x || !b) {
         set_error("falm_solver_solve: null argument");
         return FALM_ERR_INVALID;
     }
@@ -761,6 +806,14 @@ falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void
     falm_solve_info tmp;
     if (!info) info = &tmp;
     hipStream_t s = (hipStream_t)stream;
+    if (!r && S->cfg.precision != FALM_SOLVE_MIXED) {
+        // the double solvers iterate on r itself: use the solver's own vector
+        if (!S->r64own) {
+            FALM_HIP(hipMalloc(&S->r64own, S->n * sizeof(double)));
+            FALM_HIP(hipMemsetAsync(S->r64own, 0, S->n * sizeof(double), s));
+        }
+        r = S->r64own;
+    }
     double *xd = (double *)x, *rd = (double *)r;
     const double *bd = (const double *)b;
     const bool prepared = (S->a_src == a);
@@ -786,7 +839,7 @@ falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void
 falm_status falm_solver_destroy(falm_solver *S) {
     if (!S) return FALM_OK;
     void *bufs[] = {S->rr, S->p, S->q, S->s, S->pp, S->ss, S->t, S->w, S->r32, S->a32,
-                    S->sep64, S->sep32, S->st, S->partials, S->flag};
+                    S->sep64, S->sep32, S->st, S->partials, S->flag, S->r64own};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (S->hst) (void)hipHostFree(S->hst);

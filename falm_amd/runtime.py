@@ -11,7 +11,8 @@ import os
 from . import _abi as abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libfalm_amd.so")
+# FALM_AMD_LIB selects another build of the library (same ABI)
+LIB_PATH = os.environ.get("FALM_AMD_LIB") or os.path.join(HERE, "lib", "libfalm_amd.so")
 
 _lib = None
 

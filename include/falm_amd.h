@@ -178,8 +178,9 @@ typedef struct falm_solver_config {
     double pc_relax;
     int    precision;    /* FALM_SOLVE_*                                */
     int    inner_maxit;  /* mixed: Krylov iterations per correction (0 -> 50) */
-    double inner_rtol;   /* mixed: inner residual reduction (0 -> 1e-3)  */
-    int    check_every;  /* host convergence poll period (0 -> 1)       */
+    double inner_rtol;   /* mixed: inner residual reduction (0 -> 1e-4)  */
+    int    check_every;  /* iterations between host polls (0: the previous
+                            solve's count first, then max(1, 4e6/N) up to 32) */
 } falm_solver_config;
 
 typedef struct falm_solve_info {
@@ -193,7 +194,9 @@ typedef struct falm_solver falm_solver;
 falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config *cfg,
                                falm_solver **out);
 /* FalmEq::Solve (src/FalmEq.h:56-64).  a, x, b, r are double fields; x is the
- * warm-started solution (pressure), r receives the final residual. */
+ * warm-started solution (pressure), r receives the final residual (the
+ * recursive one of BiCGStab in double, b - A x in mixed precision).  r may be
+ * NULL when the caller does not need it. */
 falm_status falm_solver_solve(falm_solver *s, const void *a, void *x, const void *b,
                               void *r, void *stream, falm_solve_info *info);
 /* Mixed precision: declare the matrix once so its float copy is built once
@@ -278,6 +281,11 @@ enum { FALM_ALM_FIX_DRAG_TANGENT = 1 };  /* use cos(phi) for the drag share of
                                             the tangential force instead of the
                                             reference's sin(phi)
                                             (src/alm/almDevCall.cu:239)       */
+/* A culling radius at which exp(-r^2/eps^2) < 2^-53: every dropped term is
+ * below double rounding of the point's peak contribution (exp(-6.1^2) =
+ * 6.9e-17).  cutoff = 0 keeps every term that is non-zero in double, which
+ * equals the reference's all-pairs sum (src/alm/almDevCall.cu:255-294). */
+#define FALM_ALM_CUTOFF_ROUNDING 6.1
 typedef struct falm_alm_config {
     int    n_turbine, n_blade, n_ap_per_blade, n_attack;
     double radius, hub_radius;
@@ -372,7 +380,9 @@ falm_status falm_case_destroy(falm_case *c);
  * roofline figure; off (FALM_TIME_NONE) by default.  bytes is the sum of the
  * algorithmic bytes of the timed launches. */
 enum { FALM_TIME_NONE = 0, FALM_TIME_SOR = 1, FALM_TIME_MV = 2, FALM_TIME_PSEUDO_U = 3,
-       FALM_TIME_ALM = 4, FALM_TIME_SOLVER_UPDATE = 5 };
+       FALM_TIME_ALM = 4, FALM_TIME_SOLVER_UPDATE = 5, FALM_TIME_PROJECT = 6,
+       FALM_TIME_FACE_FLUX = 7, FALM_TIME_SOLVER_INIT = 8, FALM_TIME_SGS = 9,
+       FALM_TIME_RHS = 10, FALM_TIME_DIVERGENCE = 11 };
 falm_status falm_timing_enable(int family);
 falm_status falm_timing_read(int *launches, double *ms, double *bytes);
 

@@ -61,6 +61,27 @@ def test_windtunnel_rotor_steps(gpu):
     assert rel_err(dev.get(abi.FIELD_FF), ref.get(abi.FIELD_FF)) < 1e-9
 
 
+def test_windtunnel_moving_rotor(gpu):
+    """A surging, pitching rotor: the force region moves across cells from
+    step to step (the step driver rewrites only the union of the old and new
+    force boxes), and ff set from outside is fully rewritten by the next
+    step."""
+    setup = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, tol=1e-12,
+                             rotor_kw=dict(n_ap_per_blade=8, angle_type=2, motion=(np.radians(5.0), 6.0, 0.2),
+                                           surge=(0.25, 12.0, 0.0)))
+    ref, dev, recs = _run_pair(setup, 5)
+    _check(ref, dev, recs, 1e-10)
+    assert rel_err(dev.get(abi.FIELD_FF), ref.get(abi.FIELD_FF)) < 1e-9
+    # ff overwritten from outside: the next step must not keep stale values
+    junk = np.full(3 * ref.dom.ncell, 0.5)
+    dev.set(abi.FIELD_FF, junk)
+    ref.set(abi.FIELD_FF, junk)
+    ref.step(1)
+    dev.step(1)
+    assert rel_err(dev.get(abi.FIELD_FF), ref.get(abi.FIELD_FF), interior_mask(ref.dom, 3)) < 1e-9
+    assert rel_err(dev.get(abi.FIELD_U), ref.get(abi.FIELD_U), interior_mask(ref.dom, 3)) < 1e-10
+
+
 def test_deterministic_rerun(gpu):
     from falm_amd import runtime as rt
     setup = cases.windtunnel(n=(48, 16, 16), cells_per_d=8, rotor_kw=dict(n_ap_per_blade=8))
