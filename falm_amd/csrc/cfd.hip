@@ -18,14 +18,20 @@ __device__ inline T upwind_flux1(T ul, T ur, T UF) {
     return T(0.5) * (UF * (ur + ul) - fabs(UF) * (ur - ul));
 }
 
+// The scheme helpers take rjac = 1/J and multiply where the reference
+// divides (by 24, 16, J, 2J): a division costs ~10 VALU instructions and the
+// pseudo-velocity kernel is VALU-bound; results move by at most an ulp or two
+// per term.
+
 // One advection term of the Upwind3 family (Upwind3rd, UTOPIA, KK differ
 // only in the fourth-difference weight ALPHA; src/FalmCFDScheme.h:68-165).
 template <class T>
-__device__ inline T u3_dir(T ALPHA, T ucc, T up1, T up2, T um1, T um2, T Uabs, T JP, T JM, T jac) {
-    T UDL = JP * (-up2 + T(27) * up1 - T(27) * ucc + um1) / T(24.);
-    T UDR = JM * (-up1 + T(27) * ucc - T(27) * um1 + um2) / T(24.);
+__device__ inline T u3_dir(T ALPHA, T ucc, T up1, T up2, T um1, T um2, T Uabs, T JP, T JM, T rjac) {
+    const T r24 = T(1. / 24.);
+    T UDL = JP * (-up2 + T(27) * up1 - T(27) * ucc + um1) * r24;
+    T UDR = JM * (-up1 + T(27) * ucc - T(27) * um1 + um2) * r24;
     T UD4 = Uabs * (up2 - T(4) * up1 + T(6) * ucc - T(4) * um1 + um2) * ALPHA;
-    return T(.5) * (UDL + UDR) / jac + UD4;
+    return T(.5) * (UDL + UDR) * rjac + UD4;
 }
 
 // QUICK, src/FalmCFDScheme.h:33-66 (one direction pair, before /16J)
@@ -42,20 +48,20 @@ __device__ inline T quick_dir(T ucc, T up1, T up2, T um1, T um2, T JP, T JM) {
 template <int SCHEME, class T>
 __device__ inline T advection(T ucc, T ue1, T ue2, T uw1, T uw2, T un1, T un2, T us1, T us2, T ut1,
                               T ut2, T ub1, T ub2, T Uabs, T Vabs, T Wabs, T UE, T UW, T VN, T VS,
-                              T WT, T WB, T jac) {
+                              T WT, T WB, T rjac) {
     if (SCHEME == FALM_ADV_UPWIND3 || SCHEME == FALM_ADV_UTOPIA || SCHEME == FALM_ADV_KK) {
         const T ALPHA = SCHEME == FALM_ADV_UPWIND3 ? T(1. / 24.) : SCHEME == FALM_ADV_UTOPIA ? T(1. / 12.) : T(1. / 4.);
         T adv = T(0.0);
-        adv += u3_dir<T>(ALPHA, ucc, ue1, ue2, uw1, uw2, Uabs, UE, UW, jac);
-        adv += u3_dir<T>(ALPHA, ucc, un1, un2, us1, us2, Vabs, VN, VS, jac);
-        adv += u3_dir<T>(ALPHA, ucc, ut1, ut2, ub1, ub2, Wabs, WT, WB, jac);
+        adv += u3_dir<T>(ALPHA, ucc, ue1, ue2, uw1, uw2, Uabs, UE, UW, rjac);
+        adv += u3_dir<T>(ALPHA, ucc, un1, un2, us1, us2, Vabs, VN, VS, rjac);
+        adv += u3_dir<T>(ALPHA, ucc, ut1, ut2, ub1, ub2, Wabs, WT, WB, rjac);
         return adv;
     } else if (SCHEME == FALM_ADV_QUICK) {
         T adv = T(0.);
         adv += quick_dir<T>(ucc, ue1, ue2, uw1, uw2, UE, UW);
         adv += quick_dir<T>(ucc, un1, un2, us1, us2, VN, VS);
         adv += quick_dir<T>(ucc, ut1, ut2, ub1, ub2, WT, WB);
-        adv /= (T(16) * jac);
+        adv *= T(1. / 16.) * rjac;
         return adv;
     } else {  // Upwind1st, src/FalmCFDScheme.h:12-31
         T adv = T(0.0);
@@ -65,7 +71,7 @@ __device__ inline T advection(T ucc, T ue1, T ue2, T uw1, T uw2, T un1, T un2, T
         adv -= upwind_flux1<T>(us1, ucc, VS);
         adv += upwind_flux1<T>(ucc, ut1, WT);
         adv -= upwind_flux1<T>(ub1, ucc, WB);
-        adv /= jac;
+        adv *= rjac;
         return adv;
     }
 }
@@ -74,7 +80,7 @@ __device__ inline T advection(T ucc, T ue1, T ue2, T uw1, T uw2, T un1, T un2, T
 template <class T>
 __device__ inline T diffusion(T ReI, T ucc, T ue1, T uw1, T un1, T us1, T ut1, T ub1, T nc, T ne, T nw,
                               T nn, T ns, T nt, T nb, T gxc, T gxe, T gxw, T gyc, T gyn, T gys, T gzc,
-                              T gzt, T gzb, T jac) {
+                              T gzt, T gzb, T rjac) {
     T vis = T(0.0);
     vis += (gxc + gxe) * (ReI + T(0.5) * (nc + ne)) * (ue1 - ucc);
     vis -= (gxc + gxw) * (ReI + T(0.5) * (nc + nw)) * (ucc - uw1);
@@ -82,7 +88,7 @@ __device__ inline T diffusion(T ReI, T ucc, T ue1, T uw1, T un1, T us1, T ut1, T
     vis -= (gyc + gys) * (ReI + T(0.5) * (nc + ns)) * (ucc - us1);
     vis += (gzc + gzt) * (ReI + T(0.5) * (nc + nt)) * (ut1 - ucc);
     vis -= (gzc + gzb) * (ReI + T(0.5) * (nc + nb)) * (ucc - ub1);
-    vis /= (T(2) * jac);
+    vis *= T(0.5) * rjac;
     return vis;
 }
 
@@ -151,6 +157,7 @@ __global__ __launch_bounds__(256) void k_pseudo_u(Tiling tl, Geom g, T ReI, T dt
         const T gyc = met.G(1, cc, i, j, k), gyn = met.G(1, cc + sy, i, j + 1, k), gys = met.G(1, cc - sy, i, j - 1, k);
         const T gzc = met.G(2, cc, i, j, k), gzt = met.G(2, cc + sz, i, j, k + 1), gzb = met.G(2, cc - sz, i, j, k - 1);
         const T jac = met.JA(cc, i, j, k);
+    const T rjac = T(1) / jac;
     #pragma unroll
         for (int d = 0; d < 3; d++) {
             const T *__restrict__ q = u.c[d];
@@ -160,9 +167,9 @@ __global__ __launch_bounds__(256) void k_pseudo_u(Tiling tl, Geom g, T ReI, T dt
             const T ue2 = q[cc + 2 * sx], uw2 = q[cc - 2 * sx], un2 = q[cc + 2 * sy], us2 = q[cc - 2 * sy],
                     ut2 = q[cc + 2 * sz], ub2 = q[cc - 2 * sz];
             T adv = advection<SCHEME, T>(ucc, ue1, ue2, uw1, uw2, un1, un2, us1, us2, ut1, ut2, ub1, ub2, Uabs,
-                                         Vabs, Wabs, UE, UW, VN, VS, WT, WB, jac);
+                                         Vabs, Wabs, UE, UW, VN, VS, WT, WB, rjac);
             T vis = diffusion<T>(ReI, ucc, ue1, uw1, un1, us1, ut1, ub1, nc, ne, nw, nn, ns, nt, nb, gxc, gxe,
-                                 gxw, gyc, gyn, gys, gzc, gzt, gzb, jac);
+                                 gxw, gyc, gyn, gys, gzc, gzt, gzb, rjac);
             const T f = useff ? ff.c[d][cc] : T(0);
             ua.c[d][cc] = un.c[d][cc] + dt * (-adv + vis + f);
         }
@@ -301,18 +308,29 @@ __global__ __launch_bounds__(KM_NT) void k_pseudo_u_km(KmTiling kt, Geom g, T Re
             else sn[b][hrow[q]][hcol[q]] = hv[q];
         }
     };
-    // prologue: k queue, uu_z below, LDS plane k0
-    T qu[3][5], qn[3];
+    // Registers carry everything plane k needs (the k queue, its uu and ff
+    // values); the loads for plane k+1 are issued before plane k is computed
+    // (halo values first, so waiting for them leaves the rest in flight).
+    auto ffin = [&](int k) { return act && inx && iny && (!ffbox || (k >= ffbox[4] && k <= ffbox[5])); };
+    T qu[3][5], qn[3], cuu[5], cff[3];
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int d = 0; d < 4; d++) qu[c][d] = u.c[c][col0 + sz * (k0 - 2 + d)];
-#pragma unroll
-    for (int c = 0; c < 3; c++) qu[c][4] = T(0);
+        for (int d = 0; d < 5; d++) qu[c][d] = u.c[c][col0 + sz * (k0 - 2 + d)];
     qn[0] = nut[col0 + sz * (k0 - 1)];
     qn[1] = nut[col0 + sz * k0];
-    qn[2] = T(0);
+    qn[2] = nut[col0 + sz * (k0 + 1)];
     T wb = uu.c[2][col0 + sz * (k0 - 1)];
+    {
+        const int cc = col0 + sz * k0;
+        cuu[0] = uu.c[0][cc];
+        cuu[1] = uu.c[0][cc - 1];
+        cuu[2] = uu.c[1][cc];
+        cuu[3] = uu.c[1][cc - sy];
+        cuu[4] = uu.c[2][cc];
+        const bool f = ffin(k0);
+        for (int d = 0; d < 3; d++) cff[d] = f ? ff.c[d][cc] : T(0);
+    }
     T hv[KM_HPT];
     halo_load(k0, hv);
     lds_fill(0, hv, qu[0][2], qu[1][2], qu[2][2], qn[1]);
@@ -320,24 +338,30 @@ __global__ __launch_bounds__(KM_NT) void k_pseudo_u_km(KmTiling kt, Geom g, T Re
     int b = 0;
     for (int k = k0; k < k1; k++) {
         const int cc = col0 + sz * k;
-        // loads: this plane's k+2 / k+1 values, the next plane's halo
-        qu[0][4] = u.c[0][cc + 2 * sz];
-        qu[1][4] = u.c[1][cc + 2 * sz];
-        qu[2][4] = u.c[2][cc + 2 * sz];
-        qn[2] = nut[cc + sz];
         const bool more = k + 1 < k1;
-        if (more) halo_load(k + 1, hv);
-        const T UE = uu.c[0][cc], UW = uu.c[0][cc - 1];
-        const T VN = uu.c[1][cc], VS = uu.c[1][cc - sy];
-        const T WT = uu.c[2][cc], WB = wb;
-        const bool useff = inx && iny && (!ffbox || (k >= ffbox[4] && k <= ffbox[5]));
-        T fv[3] = {T(0), T(0), T(0)};
-        if (useff && act) {
-            fv[0] = ff.c[0][cc];
-            fv[1] = ff.c[1][cc];
-            fv[2] = ff.c[2][cc];
+        // loads for plane k+1
+        T nu[3] = {T(0), T(0), T(0)}, nn = T(0), nuu[5] = {T(0), T(0), T(0), T(0), T(0)};
+        T nff[3] = {T(0), T(0), T(0)};
+        if (more) {
+            halo_load(k + 1, hv);
+            const int cn = cc + sz;
+            nu[0] = u.c[0][cn + 2 * sz];
+            nu[1] = u.c[1][cn + 2 * sz];
+            nu[2] = u.c[2][cn + 2 * sz];
+            nn = nut[cn + sz];
+            nuu[0] = uu.c[0][cn];
+            nuu[1] = uu.c[0][cn - 1];
+            nuu[2] = uu.c[1][cn];
+            nuu[3] = uu.c[1][cn - sy];
+            nuu[4] = uu.c[2][cn];
+            if (ffin(k + 1)) {
+                nff[0] = ff.c[0][cn];
+                nff[1] = ff.c[1][cn];
+                nff[2] = ff.c[2][cn];
+            }
         }
-        // metrics of this plane
+        // plane k
+        const T UE = cuu[0], UW = cuu[1], VN = cuu[2], VS = cuu[3], WT = cuu[4], WB = wb;
         const double hzc = met.h[2][k], hzt = met.h[2][k + 1], hzb = met.h[2][k - 1];
         const double rzc = met.rh[2][k], rzt = met.rh[2][k + 1], rzb = met.rh[2][k - 1];
         const double volc = hxc * hyc * hzc;
@@ -345,10 +369,11 @@ __global__ __launch_bounds__(KM_NT) void k_pseudo_u_km(KmTiling kt, Geom g, T Re
         const T gyc = (T)(volc * ryc * ryc), gyn = (T)(hxc * hyn * hzc * ryn * ryn), gys = (T)(hxc * hys * hzc * rys * rys);
         const T gzc = (T)(volc * rzc * rzc), gzt = (T)(hxc * hyc * hzt * rzt * rzt), gzb = (T)(hxc * hyc * hzb * rzb * rzb);
         const T jac = (T)volc;
+        const T rjac = T(1) / jac;
         const T Uabs = fabs(qu[0][2] * (T)rxc);
         const T Vabs = fabs(qu[1][2] * (T)ryc);
         const T Wabs = fabs(qu[2][2] * (T)rzc);
-        const T nc = qn[1], ne = sn[b][wy + 1][lane + 2], nw = sn[b][wy + 1][lane], nn = sn[b][wy + 2][lane + 1],
+        const T nc = qn[1], ne = sn[b][wy + 1][lane + 2], nw = sn[b][wy + 1][lane], nn1 = sn[b][wy + 2][lane + 1],
                 ns = sn[b][wy][lane + 1], nt = qn[2], nb = qn[0];
         if (act) {
 #pragma unroll
@@ -360,13 +385,13 @@ __global__ __launch_bounds__(KM_NT) void k_pseudo_u_km(KmTiling kt, Geom g, T Re
                 const T us1 = su[b][d][wy + 1][lane + 2], us2 = su[b][d][wy][lane + 2];
                 const T ut1 = qu[d][3], ut2 = qu[d][4], ub1 = qu[d][1], ub2 = qu[d][0];
                 T adv = advection<SCHEME, T>(ucc, ue1, ue2, uw1, uw2, un1, un2, us1, us2, ut1, ut2, ub1, ub2, Uabs,
-                                             Vabs, Wabs, UE, UW, VN, VS, WT, WB, jac);
-                T vis = diffusion<T>(ReI, ucc, ue1, uw1, un1, us1, ut1, ub1, nc, ne, nw, nn, ns, nt, nb, gxc, gxe,
-                                     gxw, gyc, gyn, gys, gzc, gzt, gzb, jac);
-                ua.c[d][cc] = ucc + dt * (-adv + vis + fv[d]);
+                                             Vabs, Wabs, UE, UW, VN, VS, WT, WB, rjac);
+                T vis = diffusion<T>(ReI, ucc, ue1, uw1, un1, us1, ut1, ub1, nc, ne, nw, nn1, ns, nt, nb, gxc, gxe,
+                                     gxw, gyc, gyn, gys, gzc, gzt, gzb, rjac);
+                ua.c[d][cc] = ucc + dt * (-adv + vis + cff[d]);
             }
         }
-        // next plane: LDS from the queue and the halo registers, shift
+        // plane k+1 into the other LDS buffer; rotate the registers
         if (more) lds_fill(b ^ 1, hv, qu[0][3], qu[1][3], qu[2][3], qn[2]);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -374,10 +399,16 @@ __global__ __launch_bounds__(KM_NT) void k_pseudo_u_km(KmTiling kt, Geom g, T Re
             qu[c][1] = qu[c][2];
             qu[c][2] = qu[c][3];
             qu[c][3] = qu[c][4];
+            qu[c][4] = nu[c];
         }
         qn[0] = qn[1];
         qn[1] = qn[2];
+        qn[2] = nn;
         wb = WT;
+#pragma unroll
+        for (int d = 0; d < 5; d++) cuu[d] = nuu[d];
+#pragma unroll
+        for (int d = 0; d < 3; d++) cff[d] = nff[d];
         b ^= 1;
         __syncthreads();
     }
