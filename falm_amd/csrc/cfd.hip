@@ -8,6 +8,7 @@
 // the y/z-neighbours are re-read by the adjacent rows/planes of the same
 // XCD slab while they are still in its L2).
 #include "cfd.hpp"
+#include "km.hpp"
 
 namespace falm {
 
@@ -518,6 +519,212 @@ __global__ __launch_bounds__(256) void k_smagorinsky(Tiling tl, Geom g, T Cs, Ve
     });
 }
 
+// ------------------------------------------- k-marching forms (MetSep)
+// Projection on the interior cells: cell update and face update, the
+// pressure walked as a Plane7 (same expressions as k_project).
+template <class T, class P>
+__global__ __launch_bounds__(K7_NT) void k_project_km(K7Tiling kt, Geom g, P dt, Vec3W<T> u, Vec3W<T> uu,
+                                                      const P *__restrict__ p, MetSep<T> met) {
+    __shared__ K7Lds<P> lp;
+    const K7Pos Q = k7_pos(kt, g);
+    const int sz = g.nxy;
+    Plane7<P> pp;
+    pp.setup(p, &lp, Q, g);
+    pp.prologue(Q, sz);
+    __syncthreads();
+    const int i = Q.act ? Q.i : kt.box.lo[0], j = Q.act ? Q.j : kt.box.lo[1];
+    int b = 0;
+    for (int k = Q.k0; k < Q.k1; k++) {
+        const bool more = k + 1 < Q.k1;
+        if (more) pp.prefetch(Q, k, sz);
+        if (Q.act) {
+            const int cc = Q.col0 + sz * k;
+            const P pcc = pp.c();
+            P dpx = P(0.5) * (P)met.KX(0, cc, i, j, k) * (pp.xp(b) - pp.xm(b));
+            P dpy = P(0.5) * (P)met.KX(1, cc, i, j, k) * (pp.yp(b) - pp.ym(b));
+            P dpz = P(0.5) * (P)met.KX(2, cc, i, j, k) * (pp.zp() - pp.zm());
+            const T a0 = u.c[0][cc], a1 = u.c[1][cc], a2 = u.c[2][cc];
+            u.c[0][cc] = (T)((P)a0 - dt * dpx);
+            u.c[1][cc] = (T)((P)a1 - dt * dpy);
+            u.c[2][cc] = (T)((P)a2 - dt * dpz);
+            P dfx = P(0.5) * ((P)met.G(0, cc, i, j, k) + (P)met.G(0, cc + 1, i + 1, j, k)) * (pp.xp(b) - pcc);
+            P dfy = P(0.5) * ((P)met.G(1, cc, i, j, k) + (P)met.G(1, cc + g.nx, i, j + 1, k)) * (pp.yp(b) - pcc);
+            P dfz = P(0.5) * ((P)met.G(2, cc, i, j, k) + (P)met.G(2, cc + sz, i, j, k + 1)) * (pp.zp() - pcc);
+            const T b0 = uu.c[0][cc], b1 = uu.c[1][cc], b2 = uu.c[2][cc];
+            uu.c[0][cc] = (T)((P)b0 - dt * dfx);
+            uu.c[1][cc] = (T)((P)b1 - dt * dfy);
+            uu.c[2][cc] = (T)((P)b2 - dt * dfz);
+        }
+        pp.advance(b, more);
+        b ^= 1;
+        __syncthreads();
+    }
+}
+
+// The face-region cells outside the interior: the layer at index gc-1 of
+// each axis (three disjoint slabs), face update only.
+__device__ inline bool face_layer_cell(const Geom &g, long long t, int &i, int &j, int &k) {
+    const int gc = g.gc, lo = gc - 1;
+    const int fx = g.nx - 2 * gc + 1, fy = g.ny - 2 * gc + 1, fz = g.nz - 2 * gc + 1;  // face box sizes
+    const long long sx = (long long)fy * fz, sy = (long long)(fx - 1) * fz, sz = (long long)(fx - 1) * (fy - 1);
+    if (t < sx) {  // i = gc-1, all j, k of the face box
+        i = lo;
+        j = lo + (int)(t % fy);
+        k = lo + (int)(t / fy);
+        return true;
+    }
+    t -= sx;
+    if (t < sy) {  // j = gc-1, i >= gc
+        i = gc + (int)(t % (fx - 1));
+        j = lo;
+        k = lo + (int)(t / (fx - 1));
+        return true;
+    }
+    t -= sy;
+    if (t < sz) {  // k = gc-1, i, j >= gc
+        i = gc + (int)(t % (fx - 1));
+        j = gc + (int)(t / (fx - 1));
+        k = lo;
+        return true;
+    }
+    return false;
+}
+inline long long face_layer_count(const Geom &g) {
+    const int gc = g.gc;
+    const long long fx = g.nx - 2 * gc + 1, fy = g.ny - 2 * gc + 1, fz = g.nz - 2 * gc + 1;
+    return fy * fz + (fx - 1) * fz + (fx - 1) * (fy - 1);
+}
+
+template <class T, class P>
+__global__ __launch_bounds__(256) void k_project_layer(Geom g, P dt, Vec3W<T> uu, const P *__restrict__ p,
+                                                       MetSep<T> met) {
+    int i, j, k;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (!face_layer_cell(g, t, i, j, k)) return;
+    const int sy = g.nx, sz = g.nxy;
+    const int cc = g.id(i, j, k);
+    const P pcc = p[cc];
+    P dfx = P(0.5) * ((P)met.G(0, cc, i, j, k) + (P)met.G(0, cc + 1, i + 1, j, k)) * (p[cc + 1] - pcc);
+    P dfy = P(0.5) * ((P)met.G(1, cc, i, j, k) + (P)met.G(1, cc + sy, i, j + 1, k)) * (p[cc + sy] - pcc);
+    P dfz = P(0.5) * ((P)met.G(2, cc, i, j, k) + (P)met.G(2, cc + sz, i, j, k + 1)) * (p[cc + sz] - pcc);
+    const T b0 = uu.c[0][cc], b1 = uu.c[1][cc], b2 = uu.c[2][cc];
+    uu.c[0][cc] = (T)((P)b0 - dt * dfx);
+    uu.c[1][cc] = (T)((P)b1 - dt * dfy);
+    uu.c[2][cc] = (T)((P)b2 - dt * dfz);
+}
+
+// Face flux on the interior cells (each needs u at c and c+e_d: the x and y
+// neighbours from LDS, the k neighbour from the column registers).
+template <class T>
+__global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<T> u, Vec3W<T> uu, MetSep<T> met) {
+    __shared__ K7Lds<T> l0, l1, l2;
+    const K7Pos Q = k7_pos(kt, g);
+    const int sz = g.nxy;
+    Plane7<T> px, py, pz;
+    px.setup(u.c[0], &l0, Q, g);
+    py.setup(u.c[1], &l1, Q, g);
+    pz.setup(u.c[2], &l2, Q, g);
+    px.prologue(Q, sz);
+    py.prologue(Q, sz);
+    pz.prologue(Q, sz);
+    __syncthreads();
+    const int i = Q.act ? Q.i : kt.box.lo[0], j = Q.act ? Q.j : kt.box.lo[1];
+    int b = 0;
+    for (int k = Q.k0; k < Q.k1; k++) {
+        const bool more = k + 1 < Q.k1;
+        if (more) {
+            px.prefetch(Q, k, sz);
+            py.prefetch(Q, k, sz);
+            pz.prefetch(Q, k, sz);
+        }
+        if (Q.act) {
+            const int cc = Q.col0 + sz * k;
+            const T jc = met.JA(cc, i, j, k);
+            const T v[3] = {px.c(), py.c(), pz.c()};
+            const T vn[3] = {px.xp(b), py.yp(b), pz.zp()};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int in = i + (d == 0), jn = j + (d == 1), kn = k + (d == 2);
+                const int cn = cc + (d == 0 ? 1 : d == 1 ? g.nx : sz);
+                const T ucc = jc * met.KX(d, cc, i, j, k) * v[d];
+                const T ucn = met.JA(cn, in, jn, kn) * met.KX(d, cn, in, jn, kn) * vn[d];
+                uu.c[d][cc] = T(0.5) * (ucc + ucn);
+            }
+        }
+        px.advance(b, more);
+        py.advance(b, more);
+        pz.advance(b, more);
+        b ^= 1;
+        __syncthreads();
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_u_to_uu_layer(Geom g, Vec3<T> u, Vec3W<T> uu, MetSep<T> met) {
+    int i, j, k;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (!face_layer_cell(g, t, i, j, k)) return;
+    const int cc = g.id(i, j, k);
+    const int nb[3] = {1, g.nx, g.nxy};
+    const T jc = met.JA(cc, i, j, k);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int cn = cc + nb[d];
+        const int in = i + (d == 0), jn = j + (d == 1), kn = k + (d == 2);
+        const T ucc = jc * met.KX(d, cc, i, j, k) * u.c[d][cc];
+        const T ucn = met.JA(cn, in, jn, kn) * met.KX(d, cn, in, jn, kn) * u.c[d][cn];
+        uu.c[d][cc] = T(0.5) * (ucc + ucn);
+    }
+}
+
+// Smagorinsky on the interior (same expressions as k_smagorinsky).
+template <class T>
+__global__ __launch_bounds__(K7_NT) void k_smagorinsky_km(K7Tiling kt, Geom g, T Cs, Vec3<T> u, T *__restrict__ nut,
+                                                          MetSep<T> met) {
+    __shared__ K7Lds<T> l0, l1, l2;
+    const K7Pos Q = k7_pos(kt, g);
+    const int sz = g.nxy;
+    Plane7<T> pu[3];
+    pu[0].setup(u.c[0], &l0, Q, g);
+    pu[1].setup(u.c[1], &l1, Q, g);
+    pu[2].setup(u.c[2], &l2, Q, g);
+#pragma unroll
+    for (int d = 0; d < 3; d++) pu[d].prologue(Q, sz);
+    __syncthreads();
+    const int i = Q.act ? Q.i : kt.box.lo[0], j = Q.act ? Q.j : kt.box.lo[1];
+    int b = 0;
+    for (int k = Q.k0; k < Q.k1; k++) {
+        const bool more = k + 1 < Q.k1;
+        if (more)
+#pragma unroll
+            for (int d = 0; d < 3; d++) pu[d].prefetch(Q, k, sz);
+        if (Q.act) {
+            const int cc = Q.col0 + sz * k;
+            const T kxx = met.KX(0, cc, i, j, k), kxy = met.KX(1, cc, i, j, k), kxz = met.KX(2, cc, i, j, k);
+            T gr[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                gr[d][0] = T(0.5) * kxx * (pu[d].xp(b) - pu[d].xm(b));
+                gr[d][1] = T(0.5) * kxy * (pu[d].yp(b) - pu[d].ym(b));
+                gr[d][2] = T(0.5) * kxz * (pu[d].zp() - pu[d].zm());
+            }
+            const T dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
+            const T dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
+            const T dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
+            T d1 = T(2) * dux * dux, d2 = T(2) * dvy * dvy, d3 = T(2) * dwz * dwz;
+            T d4 = (duy + dvx) * (duy + dvx), d5 = (dvz + dwy) * (dvz + dwy), d6 = (duz + dwx) * (duz + dwx);
+            T Du = sqrt(d1 + d2 + d3 + d4 + d5 + d6);
+            T De = cbrt(met.JA(cc, i, j, k));
+            T lc = Cs * De;
+            nut[cc] = lc * lc * Du;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) pu[d].advance(b, more);
+        b ^= 1;
+        __syncthreads();
+    }
+}
+
 // |div|^2 of the step monitor fused into the divergence pass: the interior
 // tiles are walked in the chunked order of the reduction kernels, so the
 // partial sums equal those of a separate dot product over the stored field.
@@ -738,25 +945,76 @@ falm_status sep_pseudo_u(const falm_domain *dom, int prec, int scheme, double Re
         return pseudo_u_t<float>(dom, scheme, ReI, dt, un, u, uu, ua, nut, met_sep<float>(sm), ff, ffbox, 13.0, s);
     return pseudo_u_t<double>(dom, scheme, ReI, dt, un, u, uu, ua, nut, met_sep<double>(sm), ff, ffbox, 13.0, s);
 }
+template <class T>
+static falm_status u_to_uu_km_t(const falm_domain *dom, const void *u, void *uu, const SepMesh &sm, hipStream_t s) {
+    Geom g = make_geom(dom);
+    size_t n = ncell_of(dom);
+    K7Tiling kt = make_k7_tiling(interior_box(g));
+    if (kt.blocks == 0) return FALM_OK;
+    TimedLaunch tm(FALM_TIME_FACE_FLUX, s);
+    hipLaunchKernelGGL((k_u_to_uu_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, v3<T>(u, n), v3w<T>(uu, n),
+                       met_sep<T>(sm));
+    const long long nl = face_layer_count(g);
+    hipLaunchKernelGGL((k_u_to_uu_layer<T>), ew_grid((size_t)nl), dim3(EW_BLOCK), 0, s, g, v3<T>(u, n),
+                       v3w<T>(uu, n), met_sep<T>(sm));
+    FALM_CHECK_LAUNCH("k_u_to_uu_km");
+    tm.done((double)make_tiling(face_box(g)).box.size() * 6.0 * sizeof(T));
+    return FALM_OK;
+}
+
 falm_status sep_u_to_uu(const falm_domain *dom, int prec, const void *u, void *uu, const SepMesh &sm,
                         hipStream_t s) {
-    if (prec == FALM_F32) return u_to_uu_t<float>(dom, u, uu, met_sep<float>(sm), s);
-    return u_to_uu_t<double>(dom, u, uu, met_sep<double>(sm), s);
+    if (prec == FALM_F32) return u_to_uu_km_t<float>(dom, u, uu, sm, s);
+    return u_to_uu_km_t<double>(dom, u, uu, sm, s);
 }
 falm_status sep_poisson_rhs(const falm_domain *dom, int prec, const void *uu, double *rhs, const SepMesh &sm,
                             double scale, hipStream_t s) {
     if (prec == FALM_F32) return divergence_t<float, double, true>(dom, uu, rhs, met_sep<float>(sm), scale, false, s);
     return divergence_t<double, double, true>(dom, uu, rhs, met_sep<double>(sm), scale, false, s);
 }
+template <class T>
+static falm_status project_km_t(const falm_domain *dom, double dt, void *u, void *uu, const double *p,
+                                const SepMesh &sm, hipStream_t s) {
+    Geom g = make_geom(dom);
+    size_t n = ncell_of(dom);
+    K7Tiling kt = make_k7_tiling(interior_box(g));
+    if (kt.blocks == 0) return FALM_OK;
+    TimedLaunch tm(FALM_TIME_PROJECT, s);
+    hipLaunchKernelGGL((k_project_km<T, double>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, dt, v3w<T>(u, n),
+                       v3w<T>(uu, n), p, met_sep<T>(sm));
+    const long long nl = face_layer_count(g);
+    hipLaunchKernelGGL((k_project_layer<T, double>), ew_grid((size_t)nl), dim3(EW_BLOCK), 0, s, g, dt, v3w<T>(uu, n),
+                       p, met_sep<T>(sm));
+    FALM_CHECK_LAUNCH("k_project_km");
+    // u, uu read and written, p read
+    tm.done((double)make_tiling(face_box(g)).box.size() * (12.0 * sizeof(T) + sizeof(double)));
+    return FALM_OK;
+}
+
 falm_status sep_project(const falm_domain *dom, int prec, double dt, void *u, void *uu, const double *p,
                         const SepMesh &sm, hipStream_t s) {
-    if (prec == FALM_F32) return project_t<float, double>(dom, dt, u, u, uu, uu, p, met_sep<float>(sm), s);
-    return project_t<double, double>(dom, dt, u, u, uu, uu, p, met_sep<double>(sm), s);
+    if (prec == FALM_F32) return project_km_t<float>(dom, dt, u, uu, p, sm, s);
+    return project_km_t<double>(dom, dt, u, uu, p, sm, s);
 }
+template <class T>
+static falm_status smag_km_t(const falm_domain *dom, double Cs, const void *u, void *nut, const SepMesh &sm,
+                             hipStream_t s) {
+    Geom g = make_geom(dom);
+    size_t n = ncell_of(dom);
+    K7Tiling kt = make_k7_tiling(interior_box(g));
+    if (kt.blocks == 0) return FALM_OK;
+    TimedLaunch tm(FALM_TIME_SGS, s);
+    hipLaunchKernelGGL((k_smagorinsky_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, (T)Cs, v3<T>(u, n),
+                       (T *)nut, met_sep<T>(sm));
+    FALM_CHECK_LAUNCH("k_smagorinsky_km");
+    tm.done((double)interior_box(g).size() * 4.0 * sizeof(T));  // u read, nut written
+    return FALM_OK;
+}
+
 falm_status sep_sgs(const falm_domain *dom, int prec, double Cs, const void *u, void *nut, const SepMesh &sm,
                     hipStream_t s) {
-    if (prec == FALM_F32) return smag_t<float>(dom, Cs, u, nut, met_sep<float>(sm), s);
-    return smag_t<double>(dom, Cs, u, nut, met_sep<double>(sm), s);
+    if (prec == FALM_F32) return smag_km_t<float>(dom, Cs, u, nut, sm, s);
+    return smag_km_t<double>(dom, Cs, u, nut, sm, s);
 }
 falm_status sep_div_norm(const falm_domain *dom, int prec, const void *uu, void *div, const SepMesh &sm,
                          double *dev_out, double *partials, hipStream_t s) {

@@ -1,0 +1,143 @@
+// km.hpp — k-marching column tiles for 7-point stencils.
+//
+// One-cell-per-thread stencil kernels on gfx950 are bound by vector-memory
+// instruction issue: every neighbour load costs about as much as a streamed
+// one, and double-precision neighbour loads twice that.  Here a workgroup of
+// K7_BX x K7_BY threads owns that column of cells and walks a chunk of planes
+// in k.  For each stencil array a Plane7 keeps
+//   * the column's values at k-1, k, k+1 in registers (the k neighbours), and
+//   * plane k with a one-cell halo in LDS (the x and y neighbours),
+// so a cell costs one streamed load per array plus a share of the halo.  The
+// next plane's loads are issued before the current plane is computed and the
+// LDS planes are double-buffered: one barrier per plane.
+#pragma once
+
+#include "common.hpp"
+
+namespace falm {
+
+constexpr int K7_BX = 64, K7_BY = 8, K7_NT = K7_BX * K7_BY;
+constexpr int K7_HALO = 2 * K7_BY + 2 * K7_BX;  // one halo cell per thread at most
+static_assert(K7_HALO <= K7_NT, "halo slots exceed threads");
+
+struct K7Tiling {
+    Box box;
+    int tx, ty, tk;  // column tiles along x, y and k chunks
+    int kc;          // planes per chunk
+    int blocks;
+};
+
+// planes per chunk: 16, doubled until the grid fits max_blocks workgroups
+inline K7Tiling make_k7_tiling(const Box &b, int max_blocks = RED_BLOCKS) {
+    K7Tiling t;
+    t.box = b;
+    t.tx = (b.n[0] + K7_BX - 1) / K7_BX;
+    t.ty = (b.n[1] + K7_BY - 1) / K7_BY;
+    t.kc = 16;
+    for (;;) {
+        t.tk = (b.n[2] + t.kc - 1) / t.kc;
+        t.blocks = t.tx * t.ty * t.tk;
+        if (t.blocks <= max_blocks || t.kc >= b.n[2]) break;
+        t.kc *= 2;
+    }
+    if (b.n[0] <= 0 || b.n[1] <= 0 || b.n[2] <= 0) t.blocks = 0;
+    return t;
+}
+
+struct K7Pos {
+    int lane, wy;   // thread's column within the tile
+    int i0, j0;     // tile origin
+    int i, j;       // thread's cell column
+    int k0, k1;     // plane range [k0, k1)
+    int col0;       // offset of the column's plane-0 cell to load from
+    bool act;       // the column is inside the box
+};
+
+__device__ inline K7Pos k7_pos(const K7Tiling &kt, const Geom &g) {
+    K7Pos P;
+    P.lane = (int)(threadIdx.x & (K7_BX - 1));
+    P.wy = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / K7_BX));
+    const int L = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x));
+    const int bx = L % kt.tx, by = (L / kt.tx) % kt.ty, kc = L / (kt.tx * kt.ty);
+    P.i0 = kt.box.lo[0] + bx * K7_BX;
+    P.j0 = kt.box.lo[1] + by * K7_BY;
+    P.i = P.i0 + P.lane;
+    P.j = P.j0 + P.wy;
+    P.act = P.i < kt.box.lo[0] + kt.box.n[0] && P.j < kt.box.lo[1] + kt.box.n[1];
+    // a thread past the box still provides its cell to the LDS plane (its
+    // neighbours inside the box read it); one past the array loads a valid
+    // cell instead (no box cell reads it)
+    const bool ld = P.i < g.nx && P.j < g.ny;
+    P.col0 = (ld ? P.i : kt.box.lo[0]) + g.nx * (ld ? P.j : kt.box.lo[1]);
+    P.k0 = kt.box.lo[2] + kc * kt.kc;
+    P.k1 = min(P.k0 + kt.kc, kt.box.lo[2] + kt.box.n[2]);
+    return P;
+}
+
+template <class T>
+using K7Lds = T[2][K7_BY + 2][K7_BX + 2];
+
+// One stencil array walked through the planes of a K7 column tile.
+template <class T>
+struct Plane7 {
+    const T *__restrict__ a;
+    K7Lds<T> *lds;
+    T q0, q1, q2;  // column values at k-1, k, k+1
+    T nq, hv;      // prefetched: column at k+2, this thread's halo cell at k+1
+    int hoff, hrow, hcol, lane, wy;
+    bool hslot, hok;
+
+    __device__ inline void setup(const T *arr, K7Lds<T> *l, const K7Pos &P, const Geom &g) {
+        a = arr;
+        lds = l;
+        lane = P.lane;
+        wy = P.wy;
+        const int s = (int)threadIdx.x;
+        hslot = s < K7_HALO;
+        hrow = hcol = 0;
+        if (s < 2 * K7_BY) {  // x halo: rows 1..BY, columns 0 and BX+1
+            hrow = 1 + (s >> 1);
+            hcol = (s & 1) ? K7_BX + 1 : 0;
+        } else if (hslot) {   // y halo: rows 0 and BY+1, columns 1..BX
+            const int r = s - 2 * K7_BY, q = r / K7_BX;
+            hrow = q ? K7_BY + 1 : 0;
+            hcol = 1 + (r - q * K7_BX);
+        }
+        const int gi = P.i0 - 1 + hcol, gj = P.j0 - 1 + hrow;
+        hok = hslot && gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny;
+        hoff = hok ? gi + g.nx * gj : 0;
+    }
+    // registers for plane k0 and LDS buffer 0 (the caller synchronises)
+    __device__ inline void prologue(const K7Pos &P, int sz) {
+        q0 = a[P.col0 + sz * (P.k0 - 1)];
+        q1 = a[P.col0 + sz * P.k0];
+        q2 = a[P.col0 + sz * (P.k0 + 1)];
+        hv = hok ? a[hoff + sz * P.k0] : T(0);
+        fill(0, q1);
+    }
+    // loads for plane k+1 (call only when k+1 < k1)
+    __device__ inline void prefetch(const K7Pos &P, int k, int sz) {
+        nq = a[P.col0 + sz * (k + 2)];
+        hv = hok ? a[hoff + sz * (k + 1)] : T(0);
+    }
+    __device__ inline void fill(int b, T centre) {
+        lds[0][b][wy + 1][lane + 1] = centre;
+        if (hslot) lds[0][b][hrow][hcol] = hv;
+    }
+    // after plane k: LDS for plane k+1 and the register shift
+    __device__ inline void advance(int b, bool more) {
+        if (more) fill(b ^ 1, q2);
+        q0 = q1;
+        q1 = q2;
+        q2 = nq;
+    }
+    __device__ inline T c() const { return q1; }
+    __device__ inline T zm() const { return q0; }
+    __device__ inline T zp() const { return q2; }
+    __device__ inline T xm(int b) const { return lds[0][b][wy + 1][lane]; }
+    __device__ inline T xp(int b) const { return lds[0][b][wy + 1][lane + 2]; }
+    __device__ inline T ym(int b) const { return lds[0][b][wy][lane + 1]; }
+    __device__ inline T yp(int b) const { return lds[0][b][wy + 2][lane + 1]; }
+};
+
+}  // namespace falm

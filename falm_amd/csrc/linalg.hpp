@@ -69,6 +69,14 @@ __device__ inline R apply_op(const OP &op, const TX *__restrict__ x, int cc, int
     return (R)ac * xc + (R)ae * xe + (R)aw * xw + (R)an * xn + (R)as * xs + (R)at * xt + (R)ab * xb;
 }
 
+// the same sum with the seven x values given (k-marching kernels)
+template <class OP, class TA, class R>
+__device__ inline R apply_vals(const OP &op, int cc, int i, int j, int k, R xc, R xe, R xw, R xn, R xs, R xt, R xb) {
+    TA ac, aw, ae, as, an, ab, at;
+    op.get(cc, i, j, k, ac, aw, ae, as, an, ab, at);
+    return (R)ac * xc + (R)ae * xe + (R)aw * xw + (R)an * xn + (R)as * xs + (R)at * xt + (R)ab * xb;
+}
+
 template <class T>
 falm_status launch_sor(const falm_domain *dom, const void *a, void *x, const void *b, double omega, int color,
                        const int *done, hipStream_t s);

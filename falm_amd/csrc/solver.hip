@@ -32,6 +32,7 @@
 #include <cstring>
 #include <vector>
 
+#include "km.hpp"
 #include "linalg.hpp"
 
 namespace falm {
@@ -100,6 +101,116 @@ __global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, OP a, const
 // Start of a pass (src/FalmEq.cpp:318-351).  In mixed mode this is also the
 // outer loop's test: the true residual of the accumulated x ends the solve
 // once a pass has run, and otherwise sets the float pass's target.
+// k-marching forms of the solver's stencil kernels (x walked as a Plane7):
+// same arithmetic per cell, partial sums per workgroup in the K7 order.
+template <class T, class OP>
+__global__ __launch_bounds__(K7_NT) void k_bicg_init_km(K7Tiling kt, Geom g, OP a, const double *__restrict__ x,
+                                                        const double *__restrict__ b, double *r64, T *r, T *rr,
+                                                        const DevState *st, double *partials) {
+    if (st->final || !st->done) return;
+    __shared__ K7Lds<double> lx;
+    __shared__ double lds[2 * (K7_NT / 64)];
+    const K7Pos Q = k7_pos(kt, g);
+    const int sz = g.nxy;
+    Plane7<double> px;
+    px.setup(x, &lx, Q, g);
+    px.prologue(Q, sz);
+    __syncthreads();
+    double v[2] = {0.0, 0.0};
+    int bf = 0;
+    for (int k = Q.k0; k < Q.k1; k++) {
+        const bool more = k + 1 < Q.k1;
+        if (more) px.prefetch(Q, k, sz);
+        if (Q.act) {
+            const int cc = Q.col0 + sz * k;
+            const double rv = b[cc] - apply_vals<OP, double, double>(a, cc, Q.i, Q.j, k, px.c(), px.xp(bf), px.xm(bf),
+                                                                       px.yp(bf), px.ym(bf), px.zp(), px.zm());
+            if (r64) r64[cc] = rv;
+            const T rt = (T)rv;
+            if ((void *)r != (void *)r64) r[cc] = rt;
+            rr[cc] = rt;
+            v[0] += rv * rv;
+            v[1] += (double)rt * (double)rt;
+        }
+        px.advance(bf, more);
+        bf ^= 1;
+        __syncthreads();
+    }
+    block_partials<K7_NT, 2>(v, partials, lds);
+}
+
+template <class T, class OP>
+__global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, T *x, const T *__restrict__ b,
+                                                     T omega, int color, const int *done) {
+    if (*done) return;
+    __shared__ K7Lds<T> lx;
+    const K7Pos Q = k7_pos(kt, g);
+    const int sz = g.nxy;
+    Plane7<T> px;
+    px.setup(x, &lx, Q, g);
+    px.prologue(Q, sz);
+    __syncthreads();
+    int bf = 0;
+    for (int k = Q.k0; k < Q.k1; k++) {
+        const bool more = k + 1 < Q.k1;
+        if (more) px.prefetch(Q, k, sz);
+        if (Q.act && ((Q.i + Q.j + k + g.osum) & 1) == color) {
+            const int cc = Q.col0 + sz * k;
+            const T xc = px.c();
+            const T s = apply_vals<OP, T, T>(a, cc, Q.i, Q.j, k, xc, px.xp(bf), px.xm(bf), px.yp(bf), px.ym(bf),
+                                              px.zp(), px.zm());
+            const T cv = (b[cc] - s) / a.diag(cc);
+            x[cc] = xc + omega * cv;
+        }
+        px.advance(bf, more);
+        bf ^= 1;
+        __syncthreads();
+    }
+}
+
+// q = A pp with (rr, q) (mode 1) or t = A ss with (t, s), (t, t) (mode 2)
+template <class T, class OP, int MODE>
+__global__ __launch_bounds__(K7_NT) void k_bicg_mv_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ xin,
+                                                      T *__restrict__ out, const T *__restrict__ w, const DevState *st,
+                                                      double *partials) {
+    if (st->done) return;
+    __shared__ K7Lds<T> lx;
+    __shared__ double lds[2 * (K7_NT / 64)];
+    const K7Pos Q = k7_pos(kt, g);
+    const int sz = g.nxy;
+    Plane7<T> px;
+    px.setup(xin, &lx, Q, g);
+    px.prologue(Q, sz);
+    __syncthreads();
+    double v[2] = {0.0, 0.0};
+    int bf = 0;
+    for (int k = Q.k0; k < Q.k1; k++) {
+        const bool more = k + 1 < Q.k1;
+        if (more) px.prefetch(Q, k, sz);
+        if (Q.act) {
+            const int cc = Q.col0 + sz * k;
+            const T qv = apply_vals<OP, T, T>(a, cc, Q.i, Q.j, k, px.c(), px.xp(bf), px.xm(bf), px.yp(bf), px.ym(bf),
+                                               px.zp(), px.zm());
+            out[cc] = qv;
+            if (MODE == 1) {
+                v[0] += (double)w[cc] * (double)qv;
+            } else {
+                v[0] += (double)qv * (double)w[cc];
+                v[1] += (double)qv * (double)qv;
+            }
+        }
+        px.advance(bf, more);
+        bf ^= 1;
+        __syncthreads();
+    }
+    if (MODE == 1) {
+        double v1[1] = {v[0]};
+        block_partials<K7_NT, 1>(v1, partials, lds);
+    } else {
+        block_partials<K7_NT, 2>(v, partials, lds);
+    }
+}
+
 __global__ __launch_bounds__(FIN_NT) void k_fin_init(DevState *st, const double *partials, int nb) {
     if (st->final || !st->done) return;
     double tot[2];
@@ -434,6 +545,7 @@ struct falm_solver {
     size_t n;
     Geom g;
     Tiling tl;
+    K7Tiling kt;   // k-marching column tiles of the interior (stencil kernels)
     DevState *st = nullptr;
     DevState *hst = nullptr;  // pinned mirror
     double *partials = nullptr;
@@ -480,6 +592,7 @@ static falm_status solver_alloc(falm_solver *S) {
     FALM_HIP(hipMalloc(&S->flag, sizeof(int)));
     FALM_HIP(hipHostMalloc(&S->hflag, sizeof(int), hipHostMallocDefault));
     size_t nb = (size_t)(S->tl.tiles > 0 ? red_blocks(S->tl) : 1);
+    if ((size_t)S->kt.blocks > nb) nb = (size_t)S->kt.blocks;
     FALM_HIP(hipMalloc(&S->partials, 2 * nb * sizeof(double)));
     // the zero fills above run on the null stream, which the caller's
     // (possibly non-blocking) stream does not wait for
@@ -523,6 +636,7 @@ static OpCols<TA> cols_view(const falm_solver *S, const void *a) {
 
 #define FALM_GRID(S) tiling_grid((S)->tl), tiling_block(), 0
 #define FALM_RED(S) red_grid((S)->tl), tiling_block(), 0
+#define FALM_KM(S) dim3((S)->kt.blocks), dim3(K7_NT), 0
 #define FALM_FIN dim3(1), dim3(FIN_NT), 0
 
 // preconditioner sweeps after the fused first one (SORPC: pc_maxit x
@@ -538,12 +652,12 @@ static falm_status pc_rest(falm_solver *S, const OP &a, T *x, const T *b, hipStr
         for (int it = 0; it < m; it++) {
             if (it > 0) {
                 TimedLaunch tm(FALM_TIME_SOR, s);
-                hipLaunchKernelGGL((k_sor_op<T, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, w, FALM_COLOR_BLACK,
+                hipLaunchKernelGGL((k_sor_op_km<T, OP>), FALM_KM(S), s, S->kt, S->g, a, x, b, w, FALM_COLOR_BLACK,
                                    done);
                 tm.done((double)S->tl.box.size() * 4.0 * sizeof(T));
             }
             TimedLaunch tm(FALM_TIME_SOR, s);
-            hipLaunchKernelGGL((k_sor_op<T, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, w, FALM_COLOR_RED, done);
+            hipLaunchKernelGGL((k_sor_op_km<T, OP>), FALM_KM(S), s, S->kt, S->g, a, x, b, w, FALM_COLOR_RED, done);
             tm.done((double)S->tl.box.size() * 4.0 * sizeof(T));
         }
         FALM_CHECK_LAUNCH("k_sor_op");
@@ -563,7 +677,8 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     const falm_solver_config &c = S->cfg;
     const Tiling &tl = S->tl;
     const Geom &g = S->g;
-    const int nb = red_blocks(tl);
+    const int nb = S->kt.blocks;
+    const int nbu = red_blocks(tl);
     T *rr = (T *)S->rr, *p = (T *)S->p, *q = (T *)S->q, *sv = (T *)S->s, *pp = (T *)S->pp, *ss = (T *)S->ss,
       *t = (T *)S->t;
     const T wpc = (T)c.pc_relax;
@@ -572,7 +687,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     FALM_CHECK_LAUNCH("k_bicg_p");
     FALM_TRY((pc_rest<T, OP>(S, a, pp, p, s)));
     TimedLaunch tm1(FALM_TIME_MV, s);
-    hipLaunchKernelGGL((k_bicg_mvdot<T, OP>), FALM_RED(S), s, tl, g, a, (const T *)pp, q, (const T *)rr,
+    hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)pp, q, (const T *)rr,
                        (const DevState *)S->st, S->partials);
     tm1.done((double)tl.box.size() * 4.0 * sizeof(T));
     hipLaunchKernelGGL(k_fin_alpha, FALM_FIN, s, S->st, (const double *)S->partials, nb);
@@ -581,7 +696,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
                        (const DevState *)S->st, c.pc_type, wpc);
     FALM_CHECK_LAUNCH("k_bicg_s");
     FALM_TRY((pc_rest<T, OP>(S, a, ss, sv, s)));
-    hipLaunchKernelGGL((k_bicg_mvdot2<T, OP>), FALM_RED(S), s, tl, g, a, (const T *)ss, t, (const T *)sv,
+    hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 2>), FALM_KM(S), s, S->kt, g, a, (const T *)ss, t, (const T *)sv,
                        (const DevState *)S->st, S->partials);
     hipLaunchKernelGGL(k_fin_omega, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot2");
@@ -589,7 +704,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     hipLaunchKernelGGL((k_bicg_update<T>), FALM_RED(S), s, tl, g, x, (const T *)pp, (const T *)ss, (const T *)sv,
                        (const T *)t, r, (const T *)rr, (const DevState *)S->st, S->partials);
     tm2.done((double)tl.box.size() * (2.0 * sizeof(double) + 6.0 * sizeof(T)));
-    hipLaunchKernelGGL(k_fin_update, FALM_FIN, s, S->st, (const double *)S->partials, nb);
+    hipLaunchKernelGGL(k_fin_update, FALM_FIN, s, S->st, (const double *)S->partials, nbu);
     FALM_CHECK_LAUNCH("k_bicg_update");
     return FALM_OK;
 }
@@ -605,13 +720,13 @@ template <class T, class OP64>
 static falm_status launch_init(falm_solver *S, const OP64 &a, const double *x, const double *b, double *r64, T *rT,
                                hipStream_t s) {
     TimedLaunch tm(FALM_TIME_SOLVER_INIT, s);
-    hipLaunchKernelGGL((k_bicg_init<T, OP64>), FALM_RED(S), s, S->tl, S->g, a, x, b, r64, rT, (T *)S->rr,
+    hipLaunchKernelGGL((k_bicg_init_km<T, OP64>), FALM_KM(S), s, S->kt, S->g, a, x, b, r64, rT, (T *)S->rr,
                        (const DevState *)S->st, S->partials);
     // x, b, diagonal read; r (double, if kept), r and rr (T) written
     tm.done((double)S->tl.box.size() *
             (3.0 * sizeof(double) + (r64 ? sizeof(double) : 0.0) + ((void *)rT != (void *)r64 ? sizeof(T) : 0.0) +
              sizeof(T)));
-    hipLaunchKernelGGL(k_fin_init, FALM_FIN, s, S->st, (const double *)S->partials, red_blocks(S->tl));
+    hipLaunchKernelGGL(k_fin_init, FALM_FIN, s, S->st, (const double *)S->partials, S->kt.blocks);
     FALM_CHECK_LAUNCH("k_bicg_init");
     return FALM_OK;
 }
@@ -779,6 +894,7 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
     S->n = ncell_of(dom);
     S->g = make_geom(dom);
     S->tl = make_tiling(interior_box(S->g));
+    S->kt = make_k7_tiling(interior_box(S->g));
     falm_status st = solver_alloc(S);
     if (st != FALM_OK) {
         falm_solver_destroy(S);
