@@ -1423,6 +1423,605 @@ int orc_case_alm(void *h, void **alm) {
     return FALM_OK;
 }
 
+/* ============================================================ adjoint
+ *
+ * Discrete adjoint of the wind-tunnel step with respect to the blade
+ * pitch and per-section twist offsets, for the objective J = mean over a
+ * window of steps of the rotor power P = sum_t Omega_t * torque_t
+ * (AlmHandler::writePowerThrust, src/alm/alm.h:126-138).  No reference
+ * counterpart exists: the reference has no adjoint.  Each stage below is
+ * the transpose of the forward code above it refers to, written as a
+ * scatter over the forward loop (sequential, FP64).  The Poisson solve is
+ * taken as exact: its adjoint is a solve with the transposed operator.
+ * Scope: wind-tunnel case, Upwind3-family advection, Smagorinsky, FP64. */
+
+/* orc_sgs transposed: nut = (Cs J^(1/3))^2 sqrt(S(grad u)) */
+static void adj_sgs(const Dom &D, double Cs, const double *u_, const double *kx_, const double *ja_,
+                    const double *nutbar, double *ubar_) {
+    Col u{u_, D.ncell}, kx{kx_, D.ncell}, ja{ja_, D.ncell};
+    ColW ub{ubar_, D.ncell};
+    const int gc = D.gc;
+    for (int k = gc; k < D.nz - gc; k++)
+        for (int j = gc; j < D.ny - gc; j++)
+            for (int i = gc; i < D.nx - gc; i++) {
+                size_t c = D.id(i, j, k);
+                const double w = nutbar[c];
+                if (w == 0.0) continue;
+                size_t nb[3][2] = {{D.id(i + 1, j, k), D.id(i - 1, j, k)}, {D.id(i, j + 1, k), D.id(i, j - 1, k)},
+                                   {D.id(i, j, k + 1), D.id(i, j, k - 1)}};
+                double gr[3][3];
+                for (int d = 0; d < 3; d++)
+                    for (int e = 0; e < 3; e++) gr[d][e] = 0.5 * kx(c, e) * (u(nb[e][0], d) - u(nb[e][1], d));
+                const double dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
+                const double dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
+                const double dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
+                const double S = 2 * SQ(dux) + 2 * SQ(dvy) + 2 * SQ(dwz) + SQ(duy + dvx) + SQ(dvz + dwy) + SQ(duz + dwx);
+                const double Du = sqrt(S);
+                if (!(Du > 0.0)) continue;  /* sqrt at 0: zero subgradient */
+                const double lc = Cs * cbrt(ja(c));
+                const double coef = w * SQ(lc) / (2.0 * Du);
+                double dS[3][3];
+                dS[0][0] = 4 * dux; dS[1][1] = 4 * dvy; dS[2][2] = 4 * dwz;
+                dS[0][1] = dS[1][0] = 2 * (duy + dvx);
+                dS[1][2] = dS[2][1] = 2 * (dvz + dwy);
+                dS[0][2] = dS[2][0] = 2 * (duz + dwx);
+                for (int d = 0; d < 3; d++)
+                    for (int e = 0; e < 3; e++) {
+                        const double gb = coef * dS[d][e] * 0.5 * kx(c, e);
+                        ub(nb[e][0], d) += gb;
+                        ub(nb[e][1], d) -= gb;
+                    }
+            }
+}
+
+/* orc_pseudo_u transposed (Upwind3 family, un == u): accumulates into
+ * ubar, uubar, nutbar; writes ffbar = dt * uabar on the interior */
+static void adj_pseudo_u(const Dom &D, double ALPHA, double ReI, double dt, const double *u_, const double *uu_,
+                         const double *nut, const double *kx_, const double *g_, const double *ja_,
+                         const double *uabar_, double *ubar_, double *uubar_, double *nutbar, double *ffbar_) {
+    Col u{u_, D.ncell}, uu{uu_, D.ncell}, kx{kx_, D.ncell}, g{g_, D.ncell}, ja{ja_, D.ncell}, uab{uabar_, D.ncell};
+    ColW ub{ubar_, D.ncell}, uub{uubar_, D.ncell}, ffb{ffbar_, D.ncell};
+    const int gc = D.gc;
+    const long st[3] = {1, (long)D.nx, (long)D.nx * D.ny};
+    const double lco[5] = {0.0, 1. / 24., -27. / 24., 27. / 24., -1. / 24.};  /* m = -2..2 in UDL */
+    const double rco[5] = {1. / 24., -27. / 24., 27. / 24., -1. / 24., 0.0};  /* in UDR */
+    const double dco[5] = {1., -4., 6., -4., 1.};
+    for (int k = gc; k < D.nz - gc; k++)
+        for (int j = gc; j < D.ny - gc; j++)
+            for (int i = gc; i < D.nx - gc; i++) {
+                const size_t c = D.id(i, j, k);
+                const double J = ja(c);
+                for (int d = 0; d < 3; d++) {
+                    const double w = uab(c, d);
+                    ffb(c, d) = dt * w;
+                    if (w == 0.0) continue;
+                    ub(c, d) += w;  /* ua = un + ... */
+                    const double cw = -dt * w;
+                    for (int e = 0; e < 3; e++) {
+                        const long s_ = st[e];
+                        const double U = u(c, e) * kx(c, e), Uabs = fabs(U);
+                        const double JP = uu(c, e), JM = uu(c - s_, e);
+                        double v[5];
+                        for (int m = 0; m < 5; m++) v[m] = u(c + (m - 2) * s_, d);
+                        const double L = (-v[4] + 27 * v[3] - 27 * v[2] + v[1]) / 24.;
+                        const double R = (-v[3] + 27 * v[2] - 27 * v[1] + v[0]) / 24.;
+                        const double D4 = v[4] - 4 * v[3] + 6 * v[2] - 4 * v[1] + v[0];
+                        for (int m = 0; m < 5; m++)
+                            ub(c + (m - 2) * s_, d) += cw * (0.5 * (JP * lco[m] + JM * rco[m]) / J + Uabs * ALPHA * dco[m]);
+                        uub(c, e) += cw * 0.5 * L / J;
+                        uub(c - s_, e) += cw * 0.5 * R / J;
+                        const double sg = (U > 0) - (U < 0);
+                        ub(c, e) += cw * ALPHA * D4 * sg * kx(c, e);
+                    }
+                    const double cv = dt * w;
+                    for (int e = 0; e < 3; e++)
+                        for (int sgn = -1; sgn <= 1; sgn += 2) {
+                            const size_t f = c + sgn * st[e];
+                            const double a = (g(c, e) + g(f, e)) / (2 * J);
+                            const double b = ReI + 0.5 * (nut[c] + nut[f]);
+                            const double diff = u(f, d) - u(c, d);
+                            ub(f, d) += cv * a * b;
+                            ub(c, d) -= cv * a * b;
+                            nutbar[c] += cv * a * 0.5 * diff;
+                            nutbar[f] += cv * a * 0.5 * diff;
+                        }
+                }
+            }
+}
+
+/* orc_u_to_uu transposed: uu_d(c) = (ja kx_d ua_d)(c)/2 + (ja kx_d ua_d)(c+e_d)/2 on the face region */
+static void adj_u_to_uu(const Dom &D, const double *kx_, const double *ja_, const double *uubar_, double *uabar_) {
+    Col kx{kx_, D.ncell}, ja{ja_, D.ncell}, uub{uubar_, D.ncell};
+    ColW uab{uabar_, D.ncell};
+    const long st[3] = {1, (long)D.nx, (long)D.nx * D.ny};
+    const int lo = D.gc - 1;
+    for (int k = lo; k < D.nz - D.gc; k++)
+        for (int j = lo; j < D.ny - D.gc; j++)
+            for (int i = lo; i < D.nx - D.gc; i++) {
+                const size_t c = D.id(i, j, k);
+                for (int d = 0; d < 3; d++) {
+                    const double w = uub(c, d);
+                    const size_t cn = c + st[d];
+                    uab(c, d) += 0.5 * ja(c) * kx(c, d) * w;
+                    uab(cn, d) += 0.5 * ja(cn) * kx(cn, d) * w;
+                }
+            }
+}
+
+/* orc_poisson_rhs transposed (interior rhs; ghost rhs are 0 * scale) */
+static void adj_rhs(const Dom &D, const double *ja_, double scale, const double *rhsbar, double *uubar_) {
+    Col ja{ja_, D.ncell};
+    ColW uub{uubar_, D.ncell};
+    const long st[3] = {1, (long)D.nx, (long)D.nx * D.ny};
+    const int gc = D.gc;
+    for (int k = gc; k < D.nz - gc; k++)
+        for (int j = gc; j < D.ny - gc; j++)
+            for (int i = gc; i < D.nx - gc; i++) {
+                const size_t c = D.id(i, j, k);
+                const double w = rhsbar[c] * scale / ja(c);
+                for (int d = 0; d < 3; d++) {
+                    uub(c, d) += w;
+                    uub(c - st[d], d) -= w;
+                }
+            }
+}
+
+/* orc_project_p transposed: the cell part (interior) and the face part
+ * (face region); pressure adjoint accumulates in pbar */
+static void adj_project(const Dom &D, double dt, const double *kx_, const double *g_, const double *ustarbar_,
+                        const double *uunbar_, double *uabar_, double *uustarbar_, double *pbar) {
+    Col kx{kx_, D.ncell}, g{g_, D.ncell}, usb{ustarbar_, D.ncell}, uunb{uunbar_, D.ncell};
+    ColW uab{uabar_, D.ncell}, uusb{uustarbar_, D.ncell};
+    const long st[3] = {1, (long)D.nx, (long)D.nx * D.ny};
+    const int gc = D.gc;
+    for (int k = gc; k < D.nz - gc; k++)
+        for (int j = gc; j < D.ny - gc; j++)
+            for (int i = gc; i < D.nx - gc; i++) {
+                const size_t c = D.id(i, j, k);
+                for (int d = 0; d < 3; d++) {
+                    const double w = usb(c, d);
+                    uab(c, d) += w;
+                    const double q = dt * 0.5 * kx(c, d) * w;
+                    pbar[c + st[d]] -= q;
+                    pbar[c - st[d]] += q;
+                }
+            }
+    const int lo = gc - 1;
+    for (int k = lo; k < D.nz - gc; k++)
+        for (int j = lo; j < D.ny - gc; j++)
+            for (int i = lo; i < D.nx - gc; i++) {
+                const size_t c = D.id(i, j, k);
+                for (int d = 0; d < 3; d++) {
+                    const double w = uunb(c, d);
+                    uusb(c, d) += w;
+                    const double q = dt * 0.5 * (g(c, d) + g(c + st[d], d)) * w;
+                    pbar[c + st[d]] -= q;
+                    pbar[c] += q;
+                }
+            }
+}
+
+/* orc_bc_pressure transposed: Neumann ghost = interior neighbour, ZERO ghost = 0 */
+static void adj_pbc(const Dom &D, const int faces[6], double *pbar) {
+    const int gc = D.gc;
+    const int shp[3] = {D.nx, D.ny, D.nz};
+    for (int f = 0; f < 6; f++) {
+        if (faces[f] == FALM_PBC_NONE) continue;
+        int d, ax, bx;
+        face_axes(f, d, ax, bx);
+        for (int kb = gc; kb < shp[bx] - gc; kb++)
+            for (int ka = gc; ka < shp[ax] - gc; ka++) {
+                int o1[3], i1[3];
+                o1[ax] = i1[ax] = ka;
+                o1[bx] = i1[bx] = kb;
+                if (f & 1) { o1[d] = shp[d] - gc; i1[d] = shp[d] - gc - 1; }
+                else       { o1[d] = gc - 1;      i1[d] = gc; }
+                const size_t co = D.id(o1[0], o1[1], o1[2]), ci = D.id(i1[0], i1[1], i1[2]);
+                if (faces[f] == FALM_PBC_NEUMANN) pbar[ci] += pbar[co];
+                pbar[co] = 0.0;
+            }
+    }
+}
+
+/* orc_bc_velocity transposed for the wind-tunnel face types.  unp1bar: the
+ * adjoint of u^{n+1} (all cells); ustarbar: out, its interior part plus the
+ * ghost cells' dependence on the interior; unbar: accumulates the
+ * convective outlet's dependence on u^n.  Ghost cells no face writes keep
+ * their initial value and carry no dependence. */
+static void adj_ubc(const Dom &D, const falm_face_bc faces[6], const double *xyz_, double dt, const double *unp1bar_,
+                    double *ustarbar_, double *unbar_) {
+    Col xyz{xyz_, D.ncell}, unb1{unp1bar_, D.ncell};
+    ColW usb{ustarbar_, D.ncell}, unb{unbar_, D.ncell};
+    const int gc = D.gc;
+    const int shp[3] = {D.nx, D.ny, D.nz};
+    for (int k = gc; k < D.nz - gc; k++)
+        for (int j = gc; j < D.ny - gc; j++)
+            for (int i = gc; i < D.nx - gc; i++) {
+                const size_t c = D.id(i, j, k);
+                for (int m = 0; m < 3; m++) usb(c, m) = unb1(c, m);
+            }
+    for (int f = 0; f < 6; f++) {
+        const falm_face_bc &bc = faces[f];
+        if (bc.type == FALM_BC_NONE || bc.type == FALM_BC_INLET) continue;
+        int d, ax, bx;
+        face_axes(f, d, ax, bx);
+        const int plus = f & 1;
+        for (int kb = gc; kb < shp[bx] - gc; kb++)
+            for (int ka = gc; ka < shp[ax] - gc; ka++) {
+                int o1[3], o2[3], i1[3], i2[3];
+                o1[ax] = o2[ax] = i1[ax] = i2[ax] = ka;
+                o1[bx] = o2[bx] = i1[bx] = i2[bx] = kb;
+                if (plus) {
+                    o1[d] = shp[d] - gc; o2[d] = shp[d] - gc + 1; i1[d] = shp[d] - gc - 1; i2[d] = shp[d] - gc - 2;
+                } else {
+                    o1[d] = gc - 1; o2[d] = gc - 2; i1[d] = gc; i2[d] = gc + 1;
+                }
+                const size_t co1 = D.id(o1[0], o1[1], o1[2]), co2 = D.id(o2[0], o2[1], o2[2]);
+                const size_t ci1 = D.id(i1[0], i1[1], i1[2]), ci2 = D.id(i2[0], i2[1], i2[2]);
+                for (int m = 0; m < 3; m++) {
+                    const double w1 = unb1(co1, m), w2 = unb1(co2, m);
+                    if (bc.type == FALM_BC_CONVECTIVE) {
+                        const double dxi = 1.0 / (xyz(co1, 0) - xyz(ci1, 0));
+                        const double W = w1 + 2 * w2;   /* u(o2) = 2 u(o1) - u*(i1) */
+                        usb(ci1, m) -= w2;
+                        unb(co1, m) += (1.0 - dt * 0.5 * dxi * 3) * W;
+                        unb(ci1, m) += (dt * 0.5 * dxi * 4) * W;
+                        unb(ci2, m) += (-dt * 0.5 * dxi) * W;
+                    } else if (bc.type == FALM_BC_SLIP) {
+                        if (m == d) usb(ci1, m) -= w2;      /* u(o1) = 0, u(o2) = -u*(i1) */
+                        else usb(ci1, m) += w1 + w2;        /* u(o1) = u(o2) = u*(i1) */
+                    } else if (bc.type == FALM_BC_WALL0) {
+                        usb(ci1, m) -= w2;
+                    }
+                }
+            }
+    }
+}
+
+static void rot_T(const double y[3], const double ang[3], int type, double x[3]) {
+    /* x = R^T y for the rotation of rot() */
+    if (type == 1) {
+        double C = cos(ang[0]), S = sin(ang[0]);
+        x[0] = y[0]; x[1] = C * y[1] - S * y[2]; x[2] = S * y[1] + C * y[2];
+    } else if (type == 2) {
+        double C = cos(ang[1]), S = sin(ang[1]);
+        x[0] = C * y[0] + S * y[2]; x[1] = y[1]; x[2] = -S * y[0] + C * y[2];
+    } else if (type == 3) {
+        double C = cos(ang[2]), S = sin(ang[2]);
+        x[0] = C * y[0] - S * y[1]; x[1] = S * y[0] + C * y[1]; x[2] = y[2];
+    } else {
+        x[0] = y[0]; x[1] = y[1]; x[2] = y[2];
+    }
+}
+
+/* orc_alm_step transposed at state u, time t: ffbar is the adjoint of the
+ * projected force field, wP the weight of this step's power in J.
+ * Accumulates ubar and the parameter adjoints (per degree). */
+static void adj_alm(OrcAlm *A, const double *u_, double t, const double *ffbar_, double wP, double *ubar_,
+                    double *gpitch /* [nt*8] */, double *gtwist /* [napb] */) {
+    Dom D(&A->dom);
+    const int gc = D.gc;
+    Col uvw{u_, D.ncell}, ffb{ffbar_, D.ncell};
+    ColW ub{ubar_, D.ncell};
+    const int napt = A->napb * A->nb;
+    const double eps = A->eps;
+    const double eta = 1. / ((eps * sqrt(PI)) * (eps * sqrt(PI)) * (eps * sqrt(PI)));
+    const double dr = (A->radius - A->hub_radius) / A->napb;
+    for (int ap = 0; ap < A->nap; ap++) {
+        const int tid = ap / napt, bid = (ap % napt) / A->napb, sec = ap % A->napb;
+        const falm_turbine &T = A->tb[tid];
+        double ang[3] = {0, 0, 0}, om[3] = {0, 0, 0};
+        if (T.angle_type) {
+            ang[T.angle_type - 1] = T.motion[0] * sin(T.motion[1] * t + T.motion[2]);
+            om[T.angle_type - 1] = T.motion[0] * T.motion[1] * cos(T.motion[1] * t + T.motion[2]);
+        }
+        double base[3] = {T.base[0], T.base[1], T.base[2]}, bvel[3] = {0, 0, 0};
+        if (T.surge[0] != 0.0) {
+            base[0] = T.base[0] + T.surge[0] * sin(T.surge[1] * t + T.surge[2]);
+            bvel[0] = T.surge[0] * T.surge[1] * cos(T.surge[1] * t + T.surge[2]);
+        }
+        const double nang[3] = {-ang[0], -ang[1], -ang[2]};
+        const double theta = T.tip_rate * t + (2 * PI / A->nb) * bid;
+        double c1[3] = {T.hub[0], T.hub[1] + A->r[ap] * cos(theta), T.hub[2] + A->r[ap] * sin(theta)};
+        double P[3];
+        rot(c1, nang, T.angle_type, P);
+        for (int m = 0; m < 3; m++) P[m] += base[m];
+        const int I = find_index(A->x.data(), P[0], D.nx), Jj = find_index(A->y.data(), P[1], D.ny),
+                  K = find_index(A->z.data(), P[2], D.nz);
+        const bool in = I >= gc - 1 && I < D.nx - gc - 1 && Jj >= gc - 1 && Jj < D.ny - gc - 1 && K >= gc - 1 &&
+                        K < D.nz - gc - 1;
+        if (!in) continue;
+        /* forward values */
+        const int i0 = I, i1 = I + 1, j0 = Jj, j1 = Jj + 1, k0 = K, k1 = K + 1;
+        const double xd = fabs(P[0] - A->x[i0]) / fabs(A->x[i1] - A->x[i0]);
+        const double yd = fabs(P[1] - A->y[j0]) / fabs(A->y[j1] - A->y[j0]);
+        const double zd = fabs(P[2] - A->z[k0]) / fabs(A->z[k1] - A->z[k0]);
+        const size_t id[8] = {D.id(i0, j0, k0), D.id(i1, j0, k0), D.id(i0, j1, k0), D.id(i1, j1, k0),
+                              D.id(i0, j0, k1), D.id(i1, j0, k1), D.id(i0, j1, k1), D.id(i1, j1, k1)};
+        const double wt[8] = {(1 - xd) * (1 - yd) * (1 - zd), xd * (1 - yd) * (1 - zd), (1 - xd) * yd * (1 - zd),
+                              xd * yd * (1 - zd), (1 - xd) * (1 - yd) * zd, xd * (1 - yd) * zd,
+                              (1 - xd) * yd * zd, xd * yd * zd};
+        double uap[3];
+        for (int m = 0; m < 3; m++)
+            uap[m] = trilinear(xd, yd, zd, uvw(id[0], m), uvw(id[1], m), uvw(id[2], m), uvw(id[3], m),
+                               uvw(id[4], m), uvw(id[5], m), uvw(id[6], m), uvw(id[7], m));
+        double rel[3], vrel[3], utt[3];
+        for (int m = 0; m < 3; m++) {
+            rel[m] = P[m] - base[m];
+            vrel[m] = uap[m] - bvel[m];
+        }
+        rot_dt(rel, vrel, ang, om, T.angle_type, utt);
+        const double ux = utt[0];
+        const double ut = T.tip_rate * A->r[ap] + utt[1] * sin(theta) - utt[2] * cos(theta);
+        const double urel2 = ux * ux + ut * ut;
+        const double phi = atan(ux / ut);
+        const double chord = A->chord[ap];
+        const double twist = A->twist[ap] + T.pitch_deg[bid < 8 ? bid : 7] + A->twist_off[sec];
+        const double att = rad2deg(phi) - twist;
+        double cl = 0, cd = 0, scl = 0, scd = 0;
+        orc_polar_lookup(A->attack.data(), A->nat, A->cl.data(), A->cd.data(), A->nap, ap, att, &cl, &cd);
+        if (att >= A->attack[0] && att < A->attack[A->nat - 1])
+            for (int a = 0; a < A->nat - 1; a++)
+                if (A->attack[a] <= att && A->attack[a + 1] > att) {
+                    const double h = A->attack[a + 1] - A->attack[a];
+                    scl = (A->cl[ap + (size_t)(a + 1) * A->nap] - A->cl[ap + (size_t)a * A->nap]) / h;
+                    scd = (A->cd[ap + (size_t)(a + 1) * A->nap] - A->cd[ap + (size_t)a * A->nap]) / h;
+                    break;
+                }
+        const double fl = .5 * cl * urel2 * chord * dr;
+        const double fd = .5 * cd * urel2 * chord * dr;
+        const bool fix = (A->flags & FALM_ALM_FIX_DRAG_TANGENT) != 0;
+        const double ft0 = fix ? fl * sin(phi) - fd * cos(phi) : fl * sin(phi) - fd * sin(phi);
+        const int sg = (T.tip_rate > 0) - (T.tip_rate < 0);
+        const double ft = sg * ft0;
+        /* adjoint of the projected force: F_bar = sum_c ff_bar(c) eta exp(-r^2/eps^2) */
+        double Fb[3] = {0, 0, 0};
+        for (int k = gc; k < D.nz - gc; k++)
+            for (int j = gc; j < D.ny - gc; j++)
+                for (int i = gc; i < D.nx - gc; i++) {
+                    const size_t c = D.id(i, j, k);
+                    const double rr2 = SQ(A->x[i] - P[0]) + SQ(A->y[j] - P[1]) + SQ(A->z[k] - P[2]);
+                    const double e = eta * exp(-rr2 / SQ(eps));
+                    for (int m = 0; m < 3; m++) Fb[m] += ffb(c, m) * e;
+                }
+        /* F = R(nang) (-fx, ft sin(theta), -ft cos(theta)) */
+        double fttb[3];
+        rot_T(Fb, nang, T.angle_type, fttb);
+        double fxb = -fttb[0];
+        double ftb = fttb[1] * sin(theta) - fttb[2] * cos(theta);
+        ftb += wP * T.tip_rate * A->r[ap] * ((ft > 0) - (ft < 0));  /* power: Omega |ft| r */
+        const double ft0b = sg * ftb;
+        const double flb = cos(phi) * fxb + sin(phi) * ft0b;
+        const double fdb = sin(phi) * fxb + (fix ? -cos(phi) : -sin(phi)) * ft0b;
+        double phib = (-fl * sin(phi) + fd * cos(phi)) * fxb + (fix ? (fl * cos(phi) + fd * sin(phi))
+                                                                    : (fl * cos(phi) - fd * cos(phi))) * ft0b;
+        const double clb = .5 * urel2 * chord * dr * flb, cdb = .5 * urel2 * chord * dr * fdb;
+        const double urel2b = .5 * cl * chord * dr * flb + .5 * cd * chord * dr * fdb;
+        const double attb = scl * clb + scd * cdb;
+        phib += attb * 180. / PI;
+        gpitch[tid * 8 + (bid < 8 ? bid : 7)] += -attb;
+        gtwist[sec] += -attb;
+        const double den = ux * ux + ut * ut;
+        double uxb = phib * ut / den + 2 * ux * urel2b;
+        double utb = -phib * ux / den + 2 * ut * urel2b;
+        const double uttb[3] = {uxb, sin(theta) * utb, -cos(theta) * utb};
+        double vrelb[3];
+        rot_T(uttb, ang, T.angle_type, vrelb);
+        for (int q = 0; q < 8; q++)
+            for (int m = 0; m < 3; m++) ub(id[q], m) += wt[q] * vrelb[m];
+    }
+}
+
+
+/* stage adjoints with C signatures (tests) */
+int orc_adj_sgs(const falm_domain *dom, double Cs, const double *u, const double *kx, const double *ja,
+                const double *nutbar, double *ubar) {
+    adj_sgs(Dom(dom), Cs, u, kx, ja, nutbar, ubar);
+    return FALM_OK;
+}
+int orc_adj_pseudo_u(const falm_domain *dom, double ALPHA, double ReI, double dt, const double *u, const double *uu,
+                     const double *nut, const double *kx, const double *g, const double *ja, const double *uabar,
+                     double *ubar, double *uubar, double *nutbar, double *ffbar) {
+    adj_pseudo_u(Dom(dom), ALPHA, ReI, dt, u, uu, nut, kx, g, ja, uabar, ubar, uubar, nutbar, ffbar);
+    return FALM_OK;
+}
+int orc_adj_u_to_uu(const falm_domain *dom, const double *kx, const double *ja, const double *uubar, double *uabar) {
+    adj_u_to_uu(Dom(dom), kx, ja, uubar, uabar);
+    return FALM_OK;
+}
+int orc_adj_project(const falm_domain *dom, double dt, const double *kx, const double *g, const double *ustarbar,
+                    const double *uunbar, double *uabar, double *uustarbar, double *pbar) {
+    adj_project(Dom(dom), dt, kx, g, ustarbar, uunbar, uabar, uustarbar, pbar);
+    return FALM_OK;
+}
+int orc_adj_ubc(const falm_domain *dom, const falm_face_bc faces[6], const double *xyz, double dt,
+                const double *unp1bar, double *ustarbar, double *unbar) {
+    adj_ubc(Dom(dom), faces, xyz, dt, unp1bar, ustarbar, unbar);
+    return FALM_OK;
+}
+
+static double step_power(OrcCase *C) {
+    if (!C->alm) return 0.0;
+    OrcAlm *A = (OrcAlm *)C->alm;
+    double P = 0.0;
+    for (int t = 0; t < A->nt; t++) P += A->tb[t].tip_rate * A->torque[t];
+    return P;
+}
+
+struct CaseSnap {
+    std::vector<double> u, uu, p, nut;
+    int64_t it;
+};
+static CaseSnap snap(OrcCase *C) { return CaseSnap{C->u, C->uu, C->p, C->nut, C->it}; }
+static void restore(OrcCase *C, const CaseSnap &S) {
+    C->u = S.u; C->uu = S.uu; C->p = S.p; C->nut = S.nut; C->it = S.it;
+}
+
+/* J = mean rotor power over the next nsteps steps; the case is left as it was */
+int orc_case_window(void *h, int nsteps, double *J) {
+    OrcCase *C = (OrcCase *)h;
+    if (nsteps < 1 || !C->alm) return FALM_ERR_INVALID;
+    CaseSnap S0 = snap(C);
+    double sum = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        orc_case_step(h, 1, nullptr);
+        sum += step_power(C);
+    }
+    restore(C, S0);
+    *J = sum / nsteps;
+    return FALM_OK;
+}
+
+/* the transposed Poisson operator (same [c,w,e,s,n,b,t] layout) */
+static void transpose7(const Dom &D, const double *a_, double *at_) {
+    Col a{a_, D.ncell};
+    ColW at{at_, D.ncell};
+    const int gc = D.gc;
+    const long st[3] = {1, (long)D.nx, (long)D.nx * D.ny};
+    std::fill(at_, at_ + 7 * D.ncell, 0.0);
+    auto interior = [&](size_t c) {
+        const int k = (int)(c / ((size_t)D.nx * D.ny)), r = (int)(c % ((size_t)D.nx * D.ny));
+        const int j = r / D.nx, i = r % D.nx;
+        return i >= gc && i < D.nx - gc && j >= gc && j < D.ny - gc && k >= gc && k < D.nz - gc;
+    };
+    for (int k = gc; k < D.nz - gc; k++)
+        for (int j = gc; j < D.ny - gc; j++)
+            for (int i = gc; i < D.nx - gc; i++) {
+                const size_t c = D.id(i, j, k);
+                at(c, 0) = a(c, 0);
+                for (int d = 0; d < 3; d++) {
+                    /* column (1 + 2d) couples c to c - st (w/s/b), column (2 + 2d) to c + st (e/n/t) */
+                    const size_t cm = c - st[d], cp = c + st[d];
+                    at(c, 1 + 2 * d) = interior(cm) ? a(cm, 2 + 2 * d) : 0.0;
+                    at(c, 2 + 2 * d) = interior(cp) ? a(cp, 1 + 2 * d) : 0.0;
+                }
+            }
+}
+
+/* J and dJ/d(pitch, twist offset) in 1/degree by the checkpointed discrete
+ * adjoint over the next nsteps steps (checkpoints every ckpt steps); the
+ * adjoint Poisson solves stop at |r|/N <= adj_rtol * |rhs|/N.  The case is
+ * left as it was. */
+int orc_case_adjoint(void *h, int nsteps, int ckpt, double adj_rtol, double *J, double *g_pitch /* [nt*8] */,
+                     double *g_twist /* [napb] */, double *u0bar /* [3*ncell] or NULL: dJ/du at the start */) {
+    OrcCase *C = (OrcCase *)h;
+    const falm_case_config &c = C->cfg;
+    if (nsteps < 1 || ckpt < 1 || !C->alm || c.kind != FALM_CASE_WINDTUNNEL || c.precision != FALM_CASE_F64)
+        return FALM_ERR_INVALID;
+    if (c.adv_scheme != FALM_ADV_UPWIND3 && c.adv_scheme != FALM_ADV_UTOPIA && c.adv_scheme != FALM_ADV_KK)
+        return FALM_ERR_UNSUPPORTED;
+    if (c.sgs_model != FALM_SGS_SMAGORINSKY && c.sgs_model != FALM_SGS_EMPTY) return FALM_ERR_UNSUPPORTED;
+    const double ALPHA = c.adv_scheme == FALM_ADV_UPWIND3 ? 1. / 24. : c.adv_scheme == FALM_ADV_UTOPIA ? 1. / 12. : 1. / 4.;
+    OrcAlm *A = (OrcAlm *)C->alm;
+    Dom D(&C->dom);
+    const size_t n = C->ncell;
+    CaseSnap S0 = snap(C);
+    /* forward with checkpoints */
+    std::vector<CaseSnap> cps;
+    double sum = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        if (s % ckpt == 0) cps.push_back(snap(C));
+        orc_case_step(h, 1, nullptr);
+        sum += step_power(C);
+    }
+    *J = sum / nsteps;
+    const double wP = 1.0 / nsteps;
+    /* transposed operator and its solver */
+    std::vector<double> at(7 * n);
+    transpose7(D, C->a.data(), at.data());
+    falm_solver_config scfg = c.solver;
+    scfg.type = FALM_LS_PBICGSTAB;
+    scfg.maxit = 100000;
+    void *solverT = nullptr;
+    orc_solver_create(&C->dom, &scfg, &solverT);
+    OrcSolver *ST = (OrcSolver *)solverT;
+    std::vector<double> gp(A->nt * 8, 0.0), gt(A->napb, 0.0);
+    std::vector<double> ub(3 * n, 0.0), uub(3 * n, 0.0);  /* adjoints of u^{n+1}, uu^{n+1} */
+    std::vector<double> usb(3 * n), uab(3 * n), uusb(3 * n), pb(n), rhsb(n), lam(n), res(n), nutb(n), ffb(3 * n);
+    std::vector<double> unb(3 * n), uunb(3 * n);
+    const int nseg = (int)cps.size();
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int s0 = sg * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+        restore(C, cps[sg]);
+        std::vector<CaseSnap> states;
+        for (int s = s0; s < s1; s++) {
+            states.push_back(snap(C));
+            if (s + 1 < s1) orc_case_step(h, 1, nullptr);
+        }
+        for (int s = s1 - 1; s >= s0; s--) {
+            const CaseSnap &Sn = states[s - s0];
+            const double t = c.dt * (double)(Sn.it + 1);
+            /* nut^n = SGS(u^n) */
+            std::vector<double> nutn(n, 0.0);
+            orc_sgs(&C->dom, c.sgs_model, c.Cs, Sn.u.data(), nutn.data(), C->kx.data(), C->ja.data());
+            std::fill(unb.begin(), unb.end(), 0.0);
+            std::fill(uunb.begin(), uunb.end(), 0.0);
+            std::fill(usb.begin(), usb.end(), 0.0);
+            std::fill(uab.begin(), uab.end(), 0.0);
+            std::fill(uusb.begin(), uusb.end(), 0.0);
+            std::fill(pb.begin(), pb.end(), 0.0);
+            std::fill(nutb.begin(), nutb.end(), 0.0);
+            /* ubc, projection, pbc */
+            adj_ubc(D, C->ubc, C->xyz.data(), c.dt, ub.data(), usb.data(), unb.data());
+            adj_project(D, c.dt, C->kx.data(), C->g.data(), usb.data(), uub.data(), uab.data(), uusb.data(), pb.data());
+            adj_pbc(D, C->pbc, pb.data());
+            /* Poisson: rhs_bar = A^-T p_bar (interior) */
+            double nrm = 0.0;
+            for (int k = D.gc; k < D.nz - D.gc; k++)
+                for (int j = D.gc; j < D.ny - D.gc; j++)
+                    for (int i = D.gc; i < D.nx - D.gc; i++) nrm += SQ(pb[D.id(i, j, k)]);
+            std::fill(rhsb.begin(), rhsb.end(), 0.0);
+            if (nrm > 0.0) {
+                const double nint = (double)(D.nx - 2 * D.gc) * (D.ny - 2 * D.gc) * (D.nz - 2 * D.gc);
+                ST->cfg.tol = adj_rtol * sqrt(nrm) / nint;
+                std::fill(lam.begin(), lam.end(), 0.0);
+                std::vector<double> pbi(n, 0.0);
+                for (int k = D.gc; k < D.nz - D.gc; k++)
+                    for (int j = D.gc; j < D.ny - D.gc; j++)
+                        for (int i = D.gc; i < D.nx - D.gc; i++) pbi[D.id(i, j, k)] = pb[D.id(i, j, k)];
+                falm_solve_info si{};
+                orc_solver_solve(solverT, at.data(), lam.data(), pbi.data(), res.data(), &si);
+                rhsb = lam;
+            }
+            /* rhs, face flux, ghost copy of ua, pseudo velocity */
+            adj_rhs(D, C->ja.data(), 1.0 / (c.dt * C->maxdiag), rhsb.data(), uusb.data());
+            adj_u_to_uu(D, C->kx.data(), C->ja.data(), uusb.data(), uab.data());
+            for (size_t q = 0; q < n; q++) {
+                const int k = (int)(q / ((size_t)D.nx * D.ny)), r = (int)(q % ((size_t)D.nx * D.ny));
+                const int j = r / D.nx, i = r % D.nx;
+                const bool inner = i >= D.gc && i < D.nx - D.gc && j >= D.gc && j < D.ny - D.gc && k >= D.gc &&
+                                   k < D.nz - D.gc;
+                if (!inner)
+                    for (int m = 0; m < 3; m++) {
+                        unb[q + m * n] += uab[q + m * n];  /* ua ghosts are u^n ghosts */
+                        uab[q + m * n] = 0.0;
+                    }
+            }
+            adj_pseudo_u(D, ALPHA, 1.0 / c.Re, c.dt, Sn.u.data(), Sn.uu.data(), nutn.data(), C->kx.data(),
+                         C->g.data(), C->ja.data(), uab.data(), unb.data(), uunb.data(), nutb.data(), ffb.data());
+            /* ALM at u^n and this step's power */
+            adj_alm(A, Sn.u.data(), t, ffb.data(), wP, unb.data(), gp.data(), gt.data());
+            /* nut^n = SGS(u^n); ghost nut are constant */
+            if (c.sgs_model == FALM_SGS_SMAGORINSKY) adj_sgs(D, c.Cs, Sn.u.data(), C->kx.data(), C->ja.data(), nutb.data(), unb.data());
+            ub.swap(unb);
+            uub.swap(uunb);
+        }
+    }
+    orc_solver_destroy(solverT);
+    restore(C, S0);
+    if (u0bar) std::copy(ub.begin(), ub.end(), u0bar);
+    for (int q = 0; q < A->nt * 8; q++) g_pitch[q] = gp[q];
+    for (int q = 0; q < A->napb; q++) g_twist[q] = gt[q];
+    return FALM_OK;
+}
+
+int orc_alm_set_pitch(void *h, int turbine, const double *pitch_deg) {
+    OrcAlm *A = (OrcAlm *)h;
+    if (turbine < 0 || turbine >= A->nt) return FALM_ERR_INVALID;
+    for (int b = 0; b < 8; b++) A->tb[turbine].pitch_deg[b] = pitch_deg ? pitch_deg[b] : 0.0;
+    return FALM_OK;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

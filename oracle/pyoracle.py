@@ -66,6 +66,14 @@ def lib():
             "orc_case_get_field": [vp, ip, vp, C.c_size_t],
             "orc_case_set_field": [vp, ip, vp, C.c_size_t],
             "orc_case_alm": [vp, C.POINTER(vp)],
+            "orc_case_window": [vp, ip, dp],
+            "orc_case_adjoint": [vp, ip, ip, C.c_double, dp, dp, dp, dp],
+            "orc_alm_set_pitch": [vp, ip, dp],
+            "orc_adj_sgs": [dom, C.c_double, vp, vp, vp, vp, vp],
+            "orc_adj_pseudo_u": [dom, C.c_double, C.c_double, C.c_double] + [vp] * 11,
+            "orc_adj_u_to_uu": [dom, vp, vp, vp, vp],
+            "orc_adj_project": [dom, C.c_double, vp, vp, vp, vp, vp, vp, vp],
+            "orc_adj_ubc": [dom, C.POINTER(abi.FaceBC), vp, C.c_double, vp, vp, vp],
             "orc_num_threads": [],
         }
         for name, args in sig.items():
@@ -295,6 +303,40 @@ class Case:
         v = np.ascontiguousarray(values, dtype=np.float64)
         check(lib().orc_case_set_field(self.h, field, ptr(v), v.size))
 
+    def _alm_handle(self):
+        h = C.c_void_p()
+        check(lib().orc_case_alm(self.h, C.byref(h)))
+        return h
+
+    def set_pitch(self, turbine, pitch_deg):
+        v = np.zeros(8)
+        v[:len(pitch_deg)] = pitch_deg
+        check(lib().orc_alm_set_pitch(self._alm_handle(), turbine, v.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def set_twist_offset(self, off):
+        check(lib().orc_alm_set_twist_offset(self._alm_handle(), ptr(np.ascontiguousarray(off, dtype=np.float64))))
+
+    def window(self, nsteps):
+        """Mean rotor power over the next nsteps steps (state unchanged)."""
+        J = C.c_double()
+        check(lib().orc_case_window(self.h, nsteps, C.byref(J)))
+        return J.value
+
+    def adjoint(self, nsteps, ckpt, adj_rtol=1e-12, with_state=False):
+        """J and dJ/d(pitch [n_turbine x 8], twist offset [napb]) per degree
+        (and dJ/du at the window start with with_state)."""
+        J = C.c_double()
+        gp = np.zeros(8 * self.cfg.alm.n_turbine)
+        gt = np.zeros(self.cfg.alm.n_ap_per_blade)
+        ub = np.zeros(3 * self.dom.ncell) if with_state else None
+        check(lib().orc_case_adjoint(self.h, nsteps, ckpt, adj_rtol, C.byref(J),
+                                     gp.ctypes.data_as(C.POINTER(C.c_double)),
+                                     gt.ctypes.data_as(C.POINTER(C.c_double)),
+                                     ub.ctypes.data_as(C.POINTER(C.c_double)) if with_state else None))
+        if with_state:
+            return J.value, gp.reshape(-1, 8), gt, ub
+        return J.value, gp.reshape(-1, 8), gt
+
     def alm_get(self):
         h = C.c_void_p()
         check(lib().orc_case_alm(self.h, C.byref(h)))
@@ -312,3 +354,38 @@ class Case:
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+# ---------------------------------------------------------------- adjoints
+def adj_sgs(dom, Cs, u, kx, ja, nutbar, ubar=None):
+    ubar = np.zeros(3 * dom.ncell) if ubar is None else ubar
+    check(lib().orc_adj_sgs(C.byref(dom), Cs, ptr(u), ptr(kx), ptr(ja), ptr(nutbar), ptr(ubar)))
+    return ubar
+
+
+def adj_pseudo_u(dom, alpha, ReI, dt, u, uu, nut, kx, g, ja, uabar):
+    ubar, uubar = np.zeros(3 * dom.ncell), np.zeros(3 * dom.ncell)
+    nutbar, ffbar = np.zeros(dom.ncell), np.zeros(3 * dom.ncell)
+    check(lib().orc_adj_pseudo_u(C.byref(dom), alpha, ReI, dt, ptr(u), ptr(uu), ptr(nut), ptr(kx), ptr(g),
+                                 ptr(ja), ptr(uabar), ptr(ubar), ptr(uubar), ptr(nutbar), ptr(ffbar)))
+    return ubar, uubar, nutbar, ffbar
+
+
+def adj_u_to_uu(dom, kx, ja, uubar):
+    uabar = np.zeros(3 * dom.ncell)
+    check(lib().orc_adj_u_to_uu(C.byref(dom), ptr(kx), ptr(ja), ptr(uubar), ptr(uabar)))
+    return uabar
+
+
+def adj_project(dom, dt, kx, g, ustarbar, uunbar):
+    uabar, uustarbar, pbar = np.zeros(3 * dom.ncell), np.zeros(3 * dom.ncell), np.zeros(dom.ncell)
+    check(lib().orc_adj_project(C.byref(dom), dt, ptr(kx), ptr(g), ptr(ustarbar), ptr(uunbar), ptr(uabar),
+                                ptr(uustarbar), ptr(pbar)))
+    return uabar, uustarbar, pbar
+
+
+def adj_ubc(dom, faces, xyz, dt, unp1bar):
+    fa = (abi.FaceBC * 6)(*faces)
+    ustarbar, unbar = np.zeros(3 * dom.ncell), np.zeros(3 * dom.ncell)
+    check(lib().orc_adj_ubc(C.byref(dom), fa, ptr(xyz), dt, ptr(unp1bar), ptr(ustarbar), ptr(unbar)))
+    return ustarbar, unbar

@@ -1,0 +1,187 @@
+"""The oracle's discrete adjoint (no reference counterpart: the reference has
+no adjoint) pinned by finite differences and transposition identities:
+
+* every stage's vector-Jacobian product against central differences of the
+  oracle's forward stage (linear stages: <F v, w> = <v, F^T w> to rounding);
+* the gradient of the window-mean rotor power with respect to blade pitch
+  and section twist against central differences (h = 1e-4 rad) in seeded
+  random directions, with and without platform motion;
+* the gradient with respect to the initial velocity (without the SGS model,
+  whose |S| is not differentiable where the flow is uniform).
+"""
+import numpy as np
+import pytest
+
+from falm_amd import _abi as abi
+from falm_amd import cases
+from oracle import pyoracle as O
+from tests.helpers import domain, interior_mask, smooth_field, uniform_metrics
+
+
+def _fields(seed=3, n=(12, 10, 8)):
+    dom = domain(n)
+    rng = np.random.default_rng(seed)
+    _, (xyz, kx, g, ja) = uniform_metrics(dom, (0.1, 0.1, 0.1), stretch=0.02)
+    return dom, rng, xyz, kx, g, ja
+
+
+def test_sgs_adjoint_matches_finite_differences():
+    dom, rng, xyz, kx, g, ja = _fields()
+    u = smooth_field(dom, 3, rng, base=1.0, amp=0.3)
+    m = interior_mask(dom)
+    w = rng.standard_normal(dom.ncell) * m
+    ub = O.adj_sgs(dom, 0.1, u, kx, ja, w)
+    for _ in range(3):
+        v = rng.standard_normal(3 * dom.ncell)
+        eps = 1e-6
+        fp = O.sgs(dom, abi.SGS_SMAGORINSKY, 0.1, u + eps * v, kx, ja)
+        fm = O.sgs(dom, abi.SGS_SMAGORINSKY, 0.1, u - eps * v, kx, ja)
+        fd = ((fp - fm) / (2 * eps)) @ w
+        assert abs(fd - ub @ v) <= 1e-7 * abs(fd)
+
+
+@pytest.mark.parametrize("scheme,alpha", [(abi.ADV_UPWIND3, 1. / 24.), (abi.ADV_UTOPIA, 1. / 12.)])
+def test_pseudo_u_adjoint_matches_finite_differences(scheme, alpha):
+    dom, rng, xyz, kx, g, ja = _fields(5)
+    m = interior_mask(dom)
+    u = smooth_field(dom, 3, rng, base=1.0, amp=0.3)
+    uu = smooth_field(dom, 3, rng, base=1.0, amp=0.2)
+    nut = np.abs(smooth_field(dom, 1, rng, base=0.01, amp=0.005)) * m
+    ff = smooth_field(dom, 3, rng, base=0.0, amp=0.1)
+    wa = rng.standard_normal(3 * dom.ncell) * np.tile(m, 3)
+    ReI, dt = 1e-3, 0.01
+    ubar, uubar, nutbar, ffbar = O.adj_pseudo_u(dom, alpha, ReI, dt, u, uu, nut, kx, g, ja, wa)
+    assert np.allclose(ffbar, dt * wa, rtol=0, atol=1e-15)
+    for _ in range(2):
+        v, vu, vn = (rng.standard_normal(3 * dom.ncell), rng.standard_normal(3 * dom.ncell),
+                     rng.standard_normal(dom.ncell))
+        eps = 1e-6
+
+        def f(e):
+            x = u + e * v
+            return O.pseudo_u(dom, scheme, ReI, dt, x, x, uu + e * vu, nut + e * vn, kx, g, ja, ff)
+
+        fd = ((f(eps) - f(-eps)) / (2 * eps)) @ wa
+        ad = ubar @ v + uubar @ vu + nutbar @ vn
+        assert abs(fd - ad) <= 1e-7 * abs(fd)
+
+
+def test_linear_stage_adjoints_are_transposes():
+    dom, rng, xyz, kx, g, ja = _fields(7)
+    n = dom.ncell
+    # face flux: uu = A ua on the face region
+    v = rng.standard_normal(3 * n)
+    w = rng.standard_normal(3 * n)
+    Av = O.u_to_uu(dom, v, kx, ja)
+    lo, m = 1, np.zeros((dom.shape[2], dom.shape[1], dom.shape[0]), bool)
+    m[lo:-2, lo:-2, lo:-2] = True
+    wf = w * np.tile(m.ravel(), 3)
+    assert abs(Av @ wf - v @ O.adj_u_to_uu(dom, kx, ja, wf)) <= 1e-12 * np.abs(Av * wf).sum()
+    # projection: (u*, uu_new) = (ua - dt kx grad p, uua - dt g grad_f p)
+    dt = 0.02
+    ua, uua, p = rng.standard_normal(3 * n), rng.standard_normal(3 * n), rng.standard_normal(n)
+    us, uun = rng.standard_normal(3 * n) * np.tile(interior_mask(dom), 3), rng.standard_normal(3 * n) * np.tile(m.ravel(), 3)
+    u_out, uu_out = ua.copy(), uua.copy()
+    O.project_p(dom, dt, u_out, ua, uu_out, uua, p, kx, g)
+    uab, uusb, pb = O.adj_project(dom, dt, kx, g, us, uun)
+    lhs = (u_out * np.tile(interior_mask(dom), 3)) @ us + (uu_out * np.tile(m.ravel(), 3)) @ uun
+    rhs = ua @ uab + uua @ uusb + p @ pb
+    assert abs(lhs - rhs) <= 1e-11 * (np.abs(u_out @ us) + np.abs(uu_out @ uun))
+
+
+def test_boundary_fill_adjoint_is_transpose():
+    dom, rng, xyz, kx, g, ja = _fields(9)
+    n = dom.ncell
+    faces = [abi.FaceBC() for _ in range(6)]
+    faces[0].type = abi.BC_INLET
+    faces[0].value[0] = 1.0
+    faces[1].type = abi.BC_CONVECTIVE
+    faces[2].type = faces[3].type = faces[5].type = abi.BC_SLIP
+    faces[4].type = abi.BC_WALL0
+    dt = 0.03
+    ustar = rng.standard_normal(3 * n)
+    uprev = rng.standard_normal(3 * n)
+    w = rng.standard_normal(3 * n)
+    # u_new = B(u*, u_prev): the inlet values are constants, so compare the
+    # linear part by differencing two fills
+    def fill(us, up):
+        out = us.copy()
+        O.bc_velocity(dom, faces, out, up, xyz, dt)
+        return out
+    f0 = fill(np.zeros(3 * n), np.zeros(3 * n))
+    lin = fill(ustar, uprev) - f0
+    # outputs of the fill: the interior (u*) and the two ghost layers of
+    # each face over the face's interior range; other ghost cells keep
+    # their value and depend on nothing
+    nx, ny, nz = dom.shape
+    out = np.zeros((nz, ny, nx), bool)
+    out[2:-2, 2:-2, 2:-2] = True
+    out[2:-2, 2:-2, :2] = out[2:-2, 2:-2, -2:] = True
+    out[2:-2, :2, 2:-2] = out[2:-2, -2:, 2:-2] = True
+    out[:2, 2:-2, 2:-2] = out[-2:, 2:-2, 2:-2] = True
+    wo = w * np.tile(out.ravel(), 3)
+    usb, unb = O.adj_ubc(dom, faces, xyz, dt, wo)
+    lhs = lin @ wo
+    rhs = ustar @ usb + uprev @ unb
+    assert abs(lhs - rhs) <= 1e-11 * np.abs(lin * wo).sum()
+
+
+def _rotor_case(spin, motion=False, sgs=abi.SGS_SMAGORINSKY):
+    kw = dict(n_ap_per_blade=4)
+    if motion:
+        kw.update(angle_type=2, motion=(np.radians(4.0), 3.0, 0.3), surge=(0.1, 5.0, 0.2))
+    setup = cases.windtunnel(n=(24, 12, 12), cells_per_d=6, hub_x_d=1.5, re=1e3, tol=1e-13, rotor_kw=kw)
+    setup.struct.sgs_model = sgs
+    c = O.Case(setup)
+    c.step(spin)
+    return c
+
+
+@pytest.mark.parametrize("window,ckpt,spin,motion", [(6, 2, 4, False), (8, 3, 6, True)])
+def test_power_gradient_matches_finite_differences(window, ckpt, spin, motion):
+    c = _rotor_case(spin, motion)
+    J, gp, gt = c.adjoint(window, ckpt, 1e-13)
+    assert J > 0
+    rng = np.random.default_rng(2024)
+    h = np.degrees(1e-4)  # 1e-4 rad in the API's degrees
+    gnorm = np.sqrt(gp[0, :3] @ gp[0, :3] + gt @ gt)
+    for _ in range(4):
+        vp, vt = rng.standard_normal(3), rng.standard_normal(4)
+        nrm = np.sqrt(vp @ vp + vt @ vt)
+        vp, vt = vp / nrm, vt / nrm
+
+        def Jat(e):
+            c.set_pitch(0, e * vp)
+            c.set_twist_offset(e * vt)
+            return c.window(window)
+
+        fd = (Jat(h) - Jat(-h)) / (2 * h)
+        Jat(0.0)
+        ad = gp[0, :3] @ vp + gt @ vt
+        # relative to the gradient's norm: a direction can nearly cancel it
+        assert abs(fd - ad) <= 1e-5 * gnorm, (fd, ad, gnorm)
+    # checkpoint spacing does not change the result
+    J2, gp2, gt2 = c.adjoint(window, 1, 1e-13)
+    assert abs(J2 - J) <= 1e-14 * abs(J)
+    assert np.allclose(gp2, gp, rtol=1e-10, atol=0) and np.allclose(gt2, gt, rtol=1e-10, atol=0)
+
+
+def test_initial_state_gradient_matches_finite_differences():
+    c = _rotor_case(8, sgs=abi.SGS_EMPTY)
+    W = 5
+    J, gp, gt, ub = c.adjoint(W, 2, 1e-13, with_state=True)
+    u0 = c.get(abi.FIELD_U)
+    m3 = interior_mask(c.dom, 3)
+    rng = np.random.default_rng(11)
+    for _ in range(2):
+        du = rng.standard_normal(u0.size) * m3
+        du /= np.linalg.norm(du)
+        eps = 1e-6
+
+        def Jat(e):
+            c.set(abi.FIELD_U, u0 + e * du)
+            return c.window(W)
+
+        fd = (Jat(eps) - Jat(-eps)) / (2 * eps)
+        c.set(abi.FIELD_U, u0)
+        assert abs(fd - ub @ du) <= 1e-5 * abs(fd)
