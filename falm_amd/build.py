@@ -18,7 +18,7 @@ LIB = os.path.join(LIBDIR, "libfalm_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FALM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["runtime.hip", "cfd.hip", "linalg.hip", "solver.hip", "bc.hip", "alm.hip", "case.hip"]
+SOURCES = ["runtime.hip", "cfd.hip", "linalg.hip", "solver.hip", "bc.hip", "alm.hip", "case.hip", "adjoint.hip"]
 
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",

@@ -511,6 +511,222 @@ __global__ void k_alm_totals(AlmDev A) {
     }
 }
 
+// rotor power of the last step, accumulated: acc += sum_t Omega_t torque_t
+__global__ void k_power_accum(AlmDev A, double *acc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double p = 0.0;
+    for (int t = 0; t < A.nt; t++) p += A.tb[t].tip_rate * A.torque[t];
+    acc[0] += p;
+}
+
+// ------------------------------------------------------------- adjoint
+// x = R^T y for the rotation of rot()
+__device__ inline void rot_T(const double y[3], const double ang[3], int type, double x[3]) {
+    if (type == 1) {
+        const double C = cos(ang[0]), S = sin(ang[0]);
+        x[0] = y[0]; x[1] = C * y[1] - S * y[2]; x[2] = S * y[1] + C * y[2];
+    } else if (type == 2) {
+        const double C = cos(ang[1]), S = sin(ang[1]);
+        x[0] = C * y[0] + S * y[2]; x[1] = y[1]; x[2] = -S * y[0] + C * y[2];
+    } else if (type == 3) {
+        const double C = cos(ang[2]), S = sin(ang[2]);
+        x[0] = C * y[0] - S * y[1]; x[1] = S * y[0] + C * y[1]; x[2] = y[2];
+    } else {
+        x[0] = y[0]; x[1] = y[1]; x[2] = y[2];
+    }
+}
+
+// first index i with seq[i] >= v (seq ascending, size n)
+__device__ inline int lower_index(const double *seq, double v, int n) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (seq[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Adjoint of the projection with respect to the point forces: one
+// workgroup per point sums ffbar(c) eta exp(-r^2/eps^2) over the cells within
+// the culling radius (the cells the forward projection gives its force).
+__global__ __launch_bounds__(256) void k_alm_adj_force(AlmDev A, const double *ffb0, const double *ffb1,
+                                                       const double *ffb2, double *Fb) {
+    const int ap = blockIdx.x;
+    if (ap >= A.nap) return;
+    __shared__ int rng[6];
+    __shared__ double lds[3 * 4];
+    const double X = A.ap_xyz[3 * ap], Y = A.ap_xyz[3 * ap + 1], Z = A.ap_xyz[3 * ap + 2];
+    const double R = A.cutr;
+    if (threadIdx.x == 0) {
+        const int gc = A.gc;
+        rng[0] = max(gc, lower_index(A.x, X - R, A.nx));
+        rng[1] = min(A.nx - gc, lower_index(A.x, X + R, A.nx) + 1);
+        rng[2] = max(gc, lower_index(A.y, Y - R, A.ny));
+        rng[3] = min(A.ny - gc, lower_index(A.y, Y + R, A.ny) + 1);
+        rng[4] = max(gc, lower_index(A.z, Z - R, A.nz));
+        rng[5] = min(A.nz - gc, lower_index(A.z, Z + R, A.nz) + 1);
+    }
+    __syncthreads();
+    const int ni = max(0, rng[1] - rng[0]), nj = max(0, rng[3] - rng[2]), nk = max(0, rng[5] - rng[4]);
+    const long long tot = (long long)ni * nj * nk;
+    const double eps = A.eps, eps2 = eps * eps;
+    const double sp = eps * sqrt(kPi);
+    const double eta = 1. / (sp * sp * sp);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (long long q = threadIdx.x; q < tot; q += blockDim.x) {
+        const int i = rng[0] + (int)(q % ni);
+        const long long r = q / ni;
+        const int j = rng[2] + (int)(r % nj), k = rng[4] + (int)(r / nj);
+        const double dx = A.x[i] - X, dy = A.y[j] - Y, dz = A.z[k] - Z;
+        const double e = eta * exp(-(dx * dx + dy * dy + dz * dz) / eps2);
+        const size_t c = (size_t)i + (size_t)A.nx * ((size_t)j + (size_t)A.ny * k);
+        v[0] += ffb0[c] * e;
+        v[1] += ffb1[c] * e;
+        v[2] += ffb2[c] * e;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int m = 0; m < 3; m++) {
+        double x = v[m];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) lds[m * 4 + wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double x = 0.0;
+        for (int w = 0; w < 4; w++) x += lds[threadIdx.x * 4 + w];
+        Fb[3 * ap + threadIdx.x] = x;
+    }
+}
+
+// Adjoint of the point pipeline (kernel_UpdateTurbineAngles, UpdateAPX,
+// CalcAPForce, src/alm/almDevCall.cu:43-253) at (u, t): given the force
+// adjoints Fb and the power weight wP, accumulate ubar at the interpolation
+// corners and the pitch / twist-offset adjoints (per degree).
+__global__ void k_alm_adj_points(AlmDev A, const double *u0, const double *u1, const double *u2, double t,
+                                 const double *Fb, double wP, double *ub0, double *ub1, double *ub2,
+                                 double *gpitch, double *gtwist) {
+    const int ap = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ap >= A.nap) return;
+    const int napt = A.napb * A.nb;
+    const int tid = ap / napt, bid = (ap % napt) / A.napb, sec = ap % A.napb;
+    const TurbDev &T0 = A.tb[tid];
+    double ang[3] = {0, 0, 0}, om[3] = {0, 0, 0};
+    if (T0.angle_type) {
+        ang[T0.angle_type - 1] = T0.motion[0] * sin(T0.motion[1] * t + T0.motion[2]);
+        om[T0.angle_type - 1] = T0.motion[0] * T0.motion[1] * cos(T0.motion[1] * t + T0.motion[2]);
+    }
+    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {0, 0, 0};
+    if (T0.surge[0] != 0.0) {
+        base[0] = T0.base[0] + T0.surge[0] * sin(T0.surge[1] * t + T0.surge[2]);
+        bvel[0] = T0.surge[0] * T0.surge[1] * cos(T0.surge[1] * t + T0.surge[2]);
+    }
+    const double nang[3] = {-ang[0], -ang[1], -ang[2]};
+    const double tip = T0.tip_rate;
+    const double theta = tip * t + (2 * kPi / A.nb) * bid;
+    const double apr = A.r[ap];
+    double c1[3] = {T0.hub[0], T0.hub[1], T0.hub[2]};
+    c1[1] += apr * cos(theta);
+    c1[2] += apr * sin(theta);
+    double P[3];
+    rot(c1, nang, T0.angle_type, P);
+    for (int m = 0; m < 3; m++) P[m] += base[m];
+    const int I = find_index(A.x, P[0], A.nx), J = find_index(A.y, P[1], A.ny), K = find_index(A.z, P[2], A.nz);
+    const int gc = A.gc;
+    const bool in = I >= gc - 1 && I < A.nx - gc - 1 && J >= gc - 1 && J < A.ny - gc - 1 && K >= gc - 1 &&
+                    K < A.nz - gc - 1;
+    if (!in) return;
+    const double dr = (A.radius - A.hub_radius) / A.napb;
+    const int i0 = I, i1 = I + 1, j0 = J, j1 = J + 1, k0 = K, k1 = K + 1;
+    const double xd = fabs(P[0] - A.x[i0]) / fabs(A.x[i1] - A.x[i0]);
+    const double yd = fabs(P[1] - A.y[j0]) / fabs(A.y[j1] - A.y[j0]);
+    const double zd = fabs(P[2] - A.z[k0]) / fabs(A.z[k1] - A.z[k0]);
+    const int nx = A.nx, nxy = A.nx * A.ny;
+    const int id[8] = {i0 + nx * j0 + nxy * k0, i1 + nx * j0 + nxy * k0, i0 + nx * j1 + nxy * k0,
+                       i1 + nx * j1 + nxy * k0, i0 + nx * j0 + nxy * k1, i1 + nx * j0 + nxy * k1,
+                       i0 + nx * j1 + nxy * k1, i1 + nx * j1 + nxy * k1};
+    const double wt[8] = {(1 - xd) * (1 - yd) * (1 - zd), xd * (1 - yd) * (1 - zd), (1 - xd) * yd * (1 - zd),
+                          xd * yd * (1 - zd), (1 - xd) * (1 - yd) * zd, xd * (1 - yd) * zd,
+                          (1 - xd) * yd * zd, xd * yd * zd};
+    const double *uc[3] = {u0, u1, u2};
+    double uap[3];
+    for (int m = 0; m < 3; m++) {
+        double c[8];
+        for (int q = 0; q < 8; q++) c[q] = uc[m][id[q]];
+        uap[m] = trilinear(xd, yd, zd, c);
+    }
+    double rel[3], vrel[3], utt[3];
+    for (int m = 0; m < 3; m++) {
+        rel[m] = P[m] - base[m];
+        vrel[m] = uap[m] - bvel[m];
+    }
+    rot_dt(rel, vrel, ang, om, T0.angle_type, utt);
+    const double ux = utt[0];
+    const double ut = tip * apr + utt[1] * sin(theta) - utt[2] * cos(theta);
+    const double urel2 = ux * ux + ut * ut;
+    const double phi = atan(ux / ut);
+    const double chord = A.chord[ap];
+    const double twist = A.twist[ap] + T0.pitch_deg[bid < 8 ? bid : 7] + A.twist_off[sec];
+    const double att = phi * 180. / kPi - twist;
+    const int nat = A.nat;
+    double cl = 0.0, cd = 0.0, scl = 0.0, scd = 0.0;
+    if (att < A.attack[0]) {
+        cl = A.cl[ap];
+        cd = A.cd[ap];
+    } else if (att >= A.attack[nat - 1]) {
+        cl = A.cl[ap + (nat - 1) * A.nap];
+        cd = A.cd[ap + (nat - 1) * A.nap];
+    } else if (att == att) {
+        int a = 0, hi = nat - 1;
+        while (hi - a > 1) {
+            const int mid = (a + hi) >> 1;
+            if (A.attack[mid] <= att) a = mid;
+            else hi = mid;
+        }
+        const double h = A.attack[a + 1] - A.attack[a];
+        const double p = (att - A.attack[a]) / h;
+        const double cl0 = A.cl[ap + a * A.nap], cl1 = A.cl[ap + (a + 1) * A.nap];
+        const double cd0 = A.cd[ap + a * A.nap], cd1 = A.cd[ap + (a + 1) * A.nap];
+        cl = (1. - p) * cl0 + p * cl1;
+        cd = (1. - p) * cd0 + p * cd1;
+        scl = (cl1 - cl0) / h;
+        scd = (cd1 - cd0) / h;
+    }
+    const double fl = .5 * cl * urel2 * chord * dr;
+    const double fd = .5 * cd * urel2 * chord * dr;
+    const bool fix = (A.flags & FALM_ALM_FIX_DRAG_TANGENT) != 0;
+    const double ft0 = fix ? fl * sin(phi) - fd * cos(phi) : fl * sin(phi) - fd * sin(phi);
+    const int sg = (tip > 0) - (tip < 0);
+    const double ft = sg * ft0;
+    // F = R(nang) (-fx, ft sin(theta), -ft cos(theta))
+    const double Fbv[3] = {Fb[3 * ap], Fb[3 * ap + 1], Fb[3 * ap + 2]};
+    double fttb[3];
+    rot_T(Fbv, nang, T0.angle_type, fttb);
+    const double fxb = -fttb[0];
+    double ftb = fttb[1] * sin(theta) - fttb[2] * cos(theta);
+    ftb += wP * tip * apr * (double)((ft > 0) - (ft < 0));  // power: Omega |ft| r
+    const double ft0b = sg * ftb;
+    const double flb = cos(phi) * fxb + sin(phi) * ft0b;
+    const double fdb = sin(phi) * fxb + (fix ? -cos(phi) : -sin(phi)) * ft0b;
+    double phib = (-fl * sin(phi) + fd * cos(phi)) * fxb +
+                  (fix ? (fl * cos(phi) + fd * sin(phi)) : (fl * cos(phi) - fd * cos(phi))) * ft0b;
+    const double clb = .5 * urel2 * chord * dr * flb, cdb = .5 * urel2 * chord * dr * fdb;
+    const double urel2b = .5 * cl * chord * dr * flb + .5 * cd * chord * dr * fdb;
+    const double attb = scl * clb + scd * cdb;
+    phib += attb * 180. / kPi;
+    unsafeAtomicAdd(&gpitch[tid * 8 + (bid < 8 ? bid : 7)], -attb);
+    unsafeAtomicAdd(&gtwist[sec], -attb);
+    const double den = ux * ux + ut * ut;
+    const double uxb = phib * ut / den + 2 * ux * urel2b;
+    const double utb = -phib * ux / den + 2 * ut * urel2b;
+    const double uttb[3] = {uxb, sin(theta) * utb, -cos(theta) * utb};
+    double vrelb[3];
+    rot_T(uttb, ang, T0.angle_type, vrelb);
+    double *ubc[3] = {ub0, ub1, ub2};
+    for (int q = 0; q < 8; q++)
+        for (int m = 0; m < 3; m++) unsafeAtomicAdd(&ubc[m][id[q]], wt[q] * vrelb[m]);
+}
+
 }  // namespace falm
 
 using namespace falm;
@@ -522,6 +738,7 @@ struct falm_alm {
     double cut2;
     int nt, nap, napb;
     bool full_next = true;  // the next owned-ff step writes every cell
+    double *Fb = nullptr;   // adjoint: force adjoint per point
 };
 
 namespace falm {
@@ -580,6 +797,34 @@ falm_status alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, h
 
 const int *alm_force_box(const falm_alm *A) { return A->d.box; }
 void alm_ff_reset(falm_alm *A) { A->full_next = true; }
+int alm_n_turbine(const falm_alm *A) { return A->nt; }
+int alm_n_ap_per_blade(const falm_alm *A) { return A->napb; }
+
+falm_status alm_accumulate_power(falm_alm *A, double *acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_power_accum, dim3(1), dim3(64), 0, s, A->d, acc);
+    FALM_CHECK_LAUNCH("k_power_accum");
+    return FALM_OK;
+}
+
+falm_status alm_adjoint(falm_alm *A, const double *u, double t, const double *ffbar, double wP, double *ubar,
+                        double *gpitch, double *gtwist, hipStream_t s) {
+    const size_t n = ncell_of(&A->dom);
+    if (!A->Fb) {
+        FALM_HIP(hipMalloc(&A->Fb, 3 * (size_t)A->nap * sizeof(double)));
+        A->bufs.push_back(A->Fb);
+    }
+    // point positions at time t (the forward's k_alm_points, which also
+    // refreshes the point state)
+    const size_t stage_bytes = (size_t)(A->d.nx + A->d.ny + A->d.nz + A->d.nat) * sizeof(double);
+    const int staged = stage_bytes <= (size_t)kPointsLds;
+    hipLaunchKernelGGL((k_alm_points<double>), dim3((A->nap + 255) / 256), dim3(256), staged ? stage_bytes : 0, s,
+                       A->d, u, u + n, u + 2 * n, t, staged);
+    hipLaunchKernelGGL(k_alm_adj_force, dim3(A->nap), dim3(256), 0, s, A->d, ffbar, ffbar + n, ffbar + 2 * n, A->Fb);
+    hipLaunchKernelGGL(k_alm_adj_points, dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d, u, u + n, u + 2 * n, t,
+                       (const double *)A->Fb, wP, ubar, ubar + n, ubar + 2 * n, gpitch, gtwist);
+    FALM_CHECK_LAUNCH("k_alm_adj_points");
+    return FALM_OK;
+}
 
 }  // namespace falm
 
@@ -700,6 +945,16 @@ falm_status falm_alm_set_twist_offset(falm_alm *A, const double *off) {
     std::vector<double> v(A->napb, 0.0);
     if (off) v.assign(off, off + A->napb);
     FALM_HIP(hipMemcpy((void *)A->d.twist_off, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    return FALM_OK;
+}
+
+falm_status falm_alm_set_pitch(falm_alm *A, int turbine, const double *pitch_deg) {
+    if (!A || turbine < 0 || turbine >= A->nt) return FALM_ERR_INVALID;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (pitch_deg)
+        for (int b = 0; b < 8; b++) v[b] = pitch_deg[b];
+    TurbDev *dst = const_cast<TurbDev *>(A->d.tb) + turbine;
+    FALM_HIP(hipMemcpy(dst->pitch_deg, v, sizeof(v), hipMemcpyHostToDevice));
     return FALM_OK;
 }
 

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "alm.hpp"
+#include "case.hpp"
 #include "cfd.hpp"
 #include "linalg.hpp"
 
@@ -35,35 +36,9 @@ falm_status launch_copy_z5(const falm_domain *dom, int prec, void *v, int ncol, 
 
 using namespace falm;
 
-struct falm_case {
-    falm_case_config cfg;
-    falm_domain dom;
-    size_t n;
-    int prec;                 // FALM_F64 or FALM_F32 (CFD fields)
-    size_t tb;                // bytes per CFD element
-    hipStream_t stream = nullptr;
-    // CFD fields (prec)
-    void *u = nullptr, *uprev = nullptr, *uu = nullptr, *nut = nullptr, *ff = nullptr, *div = nullptr;
-    void *xyz = nullptr, *kx = nullptr, *g = nullptr, *ja = nullptr;
-    // Poisson fields (double)
-    double *p = nullptr, *rhs = nullptr, *res = nullptr, *a = nullptr;
-    double maxdiag = 1.0;
-    falm_solver *solver = nullptr;
-    falm_alm *alm = nullptr;
-    std::vector<double> x, y, z, hx, hy, hz;
-    SepMesh sm{};             // device 1-D widths and reciprocals
-    int64_t it = 0;
-    falm_face_bc ubc[6];
-    int pbc[6], uubc[6];
-    // device scalar for the divergence norm + its reduction scratch
-    double *dnorm = nullptr, *partials = nullptr, *hnorm = nullptr;
-    int *nobox = nullptr;     // empty cell box: no rotor, ff stays zero
-    std::vector<void *> bufs;
-};
-
 namespace falm {
 
-static falm_status dalloc(falm_case *C, void **p, size_t bytes) {
+falm_status case_dalloc(falm_case *C, void **p, size_t bytes) {
     FALM_HIP(hipMalloc(p, bytes));
     C->bufs.push_back(*p);
     FALM_HIP(hipMemset(*p, 0, bytes));
@@ -185,30 +160,30 @@ static falm_status case_init(falm_case *C) {
     C->prec = c.precision == FALM_CASE_MIXED ? FALM_F32 : FALM_F64;
     C->tb = prec_bytes(C->prec);
     const size_t n = C->n, tb = C->tb;
-    FALM_TRY(dalloc(C, &C->u, 3 * n * tb));
-    FALM_TRY(dalloc(C, &C->uprev, 3 * n * tb));
-    FALM_TRY(dalloc(C, &C->uu, 3 * n * tb));
-    FALM_TRY(dalloc(C, &C->nut, n * tb));
-    FALM_TRY(dalloc(C, &C->ff, 3 * n * tb));
-    FALM_TRY(dalloc(C, &C->div, n * tb));
-    FALM_TRY(dalloc(C, &C->xyz, 3 * n * tb));
-    FALM_TRY(dalloc(C, &C->kx, 3 * n * tb));
-    FALM_TRY(dalloc(C, &C->g, 3 * n * tb));
-    FALM_TRY(dalloc(C, &C->ja, n * tb));
-    FALM_TRY(dalloc(C, (void **)&C->p, n * sizeof(double)));
-    FALM_TRY(dalloc(C, (void **)&C->rhs, n * sizeof(double)));
-    FALM_TRY(dalloc(C, (void **)&C->res, n * sizeof(double)));
-    FALM_TRY(dalloc(C, (void **)&C->a, 7 * n * sizeof(double)));
-    FALM_TRY(dalloc(C, (void **)&C->dnorm, 8 * sizeof(double)));
+    FALM_TRY(case_dalloc(C, &C->u, 3 * n * tb));
+    FALM_TRY(case_dalloc(C, &C->uprev, 3 * n * tb));
+    FALM_TRY(case_dalloc(C, &C->uu, 3 * n * tb));
+    FALM_TRY(case_dalloc(C, &C->nut, n * tb));
+    FALM_TRY(case_dalloc(C, &C->ff, 3 * n * tb));
+    FALM_TRY(case_dalloc(C, &C->div, n * tb));
+    FALM_TRY(case_dalloc(C, &C->xyz, 3 * n * tb));
+    FALM_TRY(case_dalloc(C, &C->kx, 3 * n * tb));
+    FALM_TRY(case_dalloc(C, &C->g, 3 * n * tb));
+    FALM_TRY(case_dalloc(C, &C->ja, n * tb));
+    FALM_TRY(case_dalloc(C, (void **)&C->p, n * sizeof(double)));
+    FALM_TRY(case_dalloc(C, (void **)&C->rhs, n * sizeof(double)));
+    FALM_TRY(case_dalloc(C, (void **)&C->res, n * sizeof(double)));
+    FALM_TRY(case_dalloc(C, (void **)&C->a, 7 * n * sizeof(double)));
+    FALM_TRY(case_dalloc(C, (void **)&C->dnorm, 8 * sizeof(double)));
     Tiling tl = make_tiling(interior_box(make_geom(&C->dom)));
-    FALM_TRY(dalloc(C, (void **)&C->partials, (size_t)(tl.tiles + 1) * sizeof(double)));
+    FALM_TRY(case_dalloc(C, (void **)&C->partials, (size_t)(tl.tiles + 1) * sizeof(double)));
     // the zero fills ran on the null stream, which the case's non-blocking
     // stream does not wait for
     FALM_HIP(hipDeviceSynchronize());
     FALM_HIP(hipHostMalloc(&C->hnorm, sizeof(double), hipHostMallocDefault));
     {
         const int empty[6] = {1, 0, 1, 0, 1, 0};
-        FALM_TRY(dalloc(C, (void **)&C->nobox, sizeof(empty)));
+        FALM_TRY(case_dalloc(C, (void **)&C->nobox, sizeof(empty)));
         FALM_HIP(hipMemcpy(C->nobox, empty, sizeof(empty), hipMemcpyHostToDevice));
     }
     // uniform control-volume mesh: cell centres and widths of every padded
@@ -230,8 +205,8 @@ static falm_status case_init(falm_case *C) {
         std::vector<double> r(L);
         for (int i = 0; i < L; i++) r[i] = 1.0 / (*wid[m])[i];  // Real(1)/pitch, src/falm.h:395
         double *dh = nullptr, *dr = nullptr;
-        FALM_TRY(dalloc(C, (void **)&dh, L * sizeof(double)));
-        FALM_TRY(dalloc(C, (void **)&dr, L * sizeof(double)));
+        FALM_TRY(case_dalloc(C, (void **)&dh, L * sizeof(double)));
+        FALM_TRY(case_dalloc(C, (void **)&dr, L * sizeof(double)));
         FALM_HIP(hipMemcpy(dh, wid[m]->data(), L * sizeof(double), hipMemcpyHostToDevice));
         FALM_HIP(hipMemcpy(dr, r.data(), L * sizeof(double), hipMemcpyHostToDevice));
         C->sm.h[m] = dh;
@@ -284,7 +259,7 @@ static falm_status case_init(falm_case *C) {
     return FALM_OK;
 }
 
-static falm_status case_step(falm_case *C, falm_solve_info *si) {
+falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
     const falm_case_config &c = C->cfg;
     hipStream_t s = C->stream;
     const falm_domain *d = &C->dom;
@@ -297,7 +272,10 @@ static falm_status case_step(falm_case *C, falm_solve_info *si) {
     // boundary fill rewrites it
     std::swap(C->u, C->uprev);
     FALM_TRY(copy_shell(C, C->uprev, C->u, s));
-    if (C->alm) FALM_TRY(alm_step(C->alm, pr, C->uprev, C->ff, t, s, true));
+    if (C->alm) {
+        FALM_TRY(alm_step(C->alm, pr, C->uprev, C->ff, t, s, true));
+        if (power) FALM_TRY(alm_accumulate_power(C->alm, power, s));
+    }
     FALM_TRY(sep_pseudo_u(d, pr, c.adv_scheme, 1.0 / c.Re, c.dt, C->uprev, C->uprev, C->uu, C->u, C->nut, C->sm,
                           C->ff, C->alm ? alm_force_box(C->alm) : C->nobox, s));
     FALM_TRY(sep_u_to_uu(d, pr, C->u, C->uu, C->sm, s));
@@ -381,7 +359,7 @@ falm_status falm_case_create(const falm_case_config *cfg, falm_case **out) {
 falm_status falm_case_step(falm_case *C, int nsteps, falm_step_info *info) {
     if (!C || nsteps < 0) return FALM_ERR_INVALID;
     falm_solve_info si{0, 0.0, 0, FALM_OK};
-    for (int k = 0; k < nsteps; k++) FALM_TRY(case_step(C, &si));
+    for (int k = 0; k < nsteps; k++) FALM_TRY(case_step_power(C, &si, nullptr));
     FALM_HIP(hipMemcpyAsync(C->hnorm, C->dnorm, sizeof(double), hipMemcpyDeviceToHost, C->stream));
     FALM_HIP(hipStreamSynchronize(C->stream));
     if (info) {

@@ -1,0 +1,43 @@
+// case.hpp — the device-resident step driver's state, shared by the step
+// (case.hip) and the adjoint (adjoint.hip).
+#pragma once
+
+#include <vector>
+
+#include "cfd.hpp"
+
+struct falm_case {
+    falm_case_config cfg;
+    falm_domain dom;
+    size_t n;
+    int prec;                 // FALM_F64 or FALM_F32 (CFD fields)
+    size_t tb;                // bytes per CFD element
+    hipStream_t stream = nullptr;
+    // CFD fields (prec)
+    void *u = nullptr, *uprev = nullptr, *uu = nullptr, *nut = nullptr, *ff = nullptr, *div = nullptr;
+    void *xyz = nullptr, *kx = nullptr, *g = nullptr, *ja = nullptr;
+    // Poisson fields (double)
+    double *p = nullptr, *rhs = nullptr, *res = nullptr, *a = nullptr;
+    double maxdiag = 1.0;
+    falm_solver *solver = nullptr;
+    falm_alm *alm = nullptr;
+    std::vector<double> x, y, z, hx, hy, hz;
+    falm::SepMesh sm{};       // device 1-D widths and reciprocals
+    int64_t it = 0;
+    falm_face_bc ubc[6];
+    int pbc[6], uubc[6];
+    // device scalar for the divergence norm + its reduction scratch
+    double *dnorm = nullptr, *partials = nullptr, *hnorm = nullptr;
+    int *nobox = nullptr;     // empty cell box: no rotor, ff stays zero
+    std::vector<void *> bufs;
+};
+
+
+namespace falm {
+// one time step on the case's stream (no synchronisation); power, when
+// given, accumulates the step's rotor power on the device
+falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power);
+falm_status case_dalloc(falm_case *C, void **p, size_t bytes);
+// stopping tolerance of a solver's next solves (|r|/N), replacing its config's
+falm_status solver_set_tol(falm_solver *S, double tol);
+}  // namespace falm

@@ -864,6 +864,12 @@ static falm_status prepare_matrix(falm_solver *S, const void *a, hipStream_t s) 
     return FALM_OK;
 }
 
+falm_status solver_set_tol(falm_solver *S, double tol) {
+    if (!S || !(tol > 0.0)) return FALM_ERR_INVALID;
+    S->cfg.tol = tol;
+    return FALM_OK;
+}
+
 }  // namespace falm
 
 extern "C" {

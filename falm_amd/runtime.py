@@ -72,6 +72,9 @@ def _sigs(L):
         "falm_case_domain": [vp, dom],
         "falm_case_alm": [vp, C.POINTER(vp)],
         "falm_case_destroy": [vp],
+        "falm_case_window": [vp, ip, C.POINTER(dp)],
+        "falm_case_adjoint": [vp, ip, ip, dp, C.POINTER(dp), vp, vp, C.POINTER(C.c_size_t)],
+        "falm_alm_set_pitch": [vp, ip, vp],
         "falm_timing_enable": [ip],
         "falm_timing_read": [C.POINTER(ip), C.POINTER(dp), C.POINTER(dp)],
     }
@@ -210,10 +213,45 @@ class Case:
         v = np.ascontiguousarray(values, dtype=np.float64)
         call("falm_case_set_field", self.h, field, v.ctypes.data_as(C.c_void_p), v.size)
 
-    def alm_get(self):
-        import numpy as np
+    def _alm_handle(self):
         h = C.c_void_p()
         call("falm_case_alm", self.h, C.byref(h))
+        return h
+
+    def set_pitch(self, turbine, pitch_deg):
+        """Per-blade pitch of one turbine in degrees (up to 8 values)."""
+        import numpy as np
+        v = np.zeros(8)
+        v[:len(pitch_deg)] = pitch_deg
+        call("falm_alm_set_pitch", self._alm_handle(), turbine, v.ctypes.data_as(C.c_void_p))
+
+    def set_twist_offset(self, off):
+        """Per-section twist offset in degrees ([n_ap_per_blade])."""
+        import numpy as np
+        v = np.ascontiguousarray(off, dtype=np.float64)
+        call("falm_alm_set_twist_offset", self._alm_handle(), v.ctypes.data_as(C.c_void_p))
+
+    def window(self, nsteps):
+        """Mean rotor power over the next nsteps steps (state unchanged)."""
+        J = C.c_double()
+        call("falm_case_window", self.h, nsteps, C.byref(J))
+        return J.value
+
+    def adjoint(self, nsteps, ckpt, adj_rtol=1e-12):
+        """J and dJ/d(pitch [n_turbine x 8], twist offset [n_ap_per_blade]),
+        per degree, by the checkpointed discrete adjoint; also the device
+        bytes it allocated.  The state is unchanged."""
+        import numpy as np
+        a = self.setup.alm
+        gp, gt = np.zeros(a.n_turbine * 8), np.zeros(a.n_ap_per_blade)
+        J, nb = C.c_double(), C.c_size_t()
+        call("falm_case_adjoint", self.h, nsteps, ckpt, adj_rtol, C.byref(J), gp.ctypes.data_as(C.c_void_p),
+             gt.ctypes.data_as(C.c_void_p), C.byref(nb))
+        return J.value, gp.reshape(a.n_turbine, 8), gt, nb.value
+
+    def alm_get(self):
+        import numpy as np
+        h = self._alm_handle()
         a = self.setup.alm
         xyz, f = np.zeros(3 * a.n_ap), np.zeros(3 * a.n_ap)
         tq, th = np.zeros(a.n_turbine), np.zeros(a.n_turbine)

@@ -375,6 +375,29 @@ falm_status falm_case_domain(falm_case *c, falm_domain *dom);
 falm_status falm_case_alm(falm_case *c, falm_alm **alm);
 falm_status falm_case_destroy(falm_case *c);
 
+/* ------------------------------------------------- rotor-power adjoint
+ * No reference counterpart (the reference has no adjoint).  The objective is
+ * J = mean over the next nsteps steps of the rotor power
+ * P = sum_t Omega_t * torque_t, the quantity AlmHandler::writePowerThrust
+ * reports (src/alm/alm.h:126-138).  Both calls leave the case's state as it
+ * was (fields and step counter).
+ *
+ * falm_case_window: J by running the window forward (any case precision). */
+falm_status falm_case_window(falm_case *c, int nsteps, double *J);
+/* falm_case_adjoint: J and its gradient by the discrete adjoint of the step:
+ * g_pitch[n_turbine*8] = dJ/d(pitch_deg[b]) and g_twist[n_ap_per_blade] =
+ * dJ/d(twist_offset_deg[s]), both per degree.  The forward window keeps a
+ * checkpoint every ckpt steps and recomputes each segment in the reverse
+ * sweep; the transposed Poisson solves stop at |r|/N <= adj_rtol*|rhs|/N.
+ * bytes (may be NULL) receives the device memory the adjoint allocated.
+ * Wind-tunnel cases in FP64 with Upwind3-family advection and Smagorinsky or
+ * no SGS model; FALM_ERR_UNSUPPORTED otherwise. */
+falm_status falm_case_adjoint(falm_case *c, int nsteps, int ckpt, double adj_rtol, double *J,
+                              double *g_pitch, double *g_twist, size_t *bytes);
+/* Per-blade pitch of one turbine (8 values, degrees; NULL = 0), the
+ * falm_turbine::pitch_deg of falm_alm_create. */
+falm_status falm_alm_set_pitch(falm_alm *alm, int turbine, const double *pitch_deg);
+
 /* Device-time accounting of one kernel family (HIP events recorded around
  * each launch of that family on the stream it is launched on).  Used for the
  * roofline figure; off (FALM_TIME_NONE) by default.  bytes is the sum of the
