@@ -136,6 +136,73 @@ def cpu_baseline(n, steps=2, warmup=1):
     }
 
 
+def run_adjoint(n, spinup, window, ckpt, ndir, tol, adj_rtol, seed=7):
+    """The rotor-power gradient over a window of FP64 steps after a spin-up:
+    J = mean power, dJ/d(3 blade pitches, n_ap_per_blade twist offsets) by the
+    checkpointed device adjoint, checked in ndir seeded random unit
+    directions against central differences of the device forward with
+    h = 1e-4 rad; time and device memory against the forward window."""
+    import numpy as np
+
+    from falm_amd import cases, runtime
+
+    import torch
+
+    setup = cases.windtunnel(n=n, cells_per_d=24, tol=tol)
+    free0 = torch.cuda.mem_get_info(0)[0]
+    c = runtime.Case(setup)
+    case_bytes = free0 - torch.cuda.mem_get_info(0)[0]  # the forward case's device footprint
+    t0 = time.perf_counter()
+    c.step(spinup)
+    t_spin = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J0 = c.window(window)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, gp, gt, nbytes = c.adjoint(window, ckpt, adj_rtol)
+    t_adj = time.perf_counter() - t0
+    napb = setup.alm.n_ap_per_blade
+    g = np.concatenate([gp[0, :3], gt])
+    h = float(np.degrees(1e-4))
+    rng = np.random.default_rng(seed)
+    dirs = []
+    t0 = time.perf_counter()
+    for _ in range(ndir):
+        v = rng.standard_normal(3 + napb)
+        v /= np.linalg.norm(v)
+
+        def Jat(e):
+            c.set_pitch(0, e * v[:3])
+            c.set_twist_offset(e * v[3:])
+            return c.window(window)
+
+        fd = (Jat(h) - Jat(-h)) / (2 * h)
+        ad = float(g @ v)
+        dirs.append({"fd_per_deg": fd, "adjoint_per_deg": ad, "rel_err": abs(fd - ad) / abs(fd)})
+    Jat(0.0)
+    t_fd = time.perf_counter() - t0
+    c.close()
+    return {
+        "workload": f"windtunnel {n[0]}x{n[1]}x{n[2]} cells + 1 ALM rotor (3x{napb} points), dx=D/24, FP64, "
+                    f"Poisson tol {tol:g}, {spinup}-step spin-up, {window}-step window, checkpoints every {ckpt}",
+        "J_mean_power": J,
+        "J_forward_window": J0,
+        "grad_pitch_per_deg": [float(x) for x in gp[0, :3]],
+        "grad_twist_per_deg": [float(x) for x in gt],
+        "directions": dirs,
+        "max_rel_err_vs_fd": max(d["rel_err"] for d in dirs),
+        "fd_step_rad": 1e-4,
+        "time_spinup_s": t_spin,
+        "time_forward_window_s": t_fwd,
+        "time_adjoint_s": t_adj,
+        "adjoint_time_vs_forward": t_adj / t_fwd,
+        "time_fd_s": t_fd,
+        "adjoint_device_bytes": nbytes,
+        "forward_case_device_bytes": case_bytes,
+        "adjoint_bytes_vs_forward_case": nbytes / case_bytes if case_bytes > 0 else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +214,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--calibrate", action="store_true")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="rotor-power gradient report (256x96x96, D/24) instead of the step benchmark")
+    ap.add_argument("--adj-spinup", type=int, default=500)
+    ap.add_argument("--adj-window", type=int, default=200)
+    ap.add_argument("--adj-ckpt", type=int, default=20)
+    ap.add_argument("--adj-dirs", type=int, default=4)
+    ap.add_argument("--adj-tol", type=float, default=1e-10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -159,6 +233,12 @@ def main():
     from falm_amd import runtime
 
     runtime.init(0)
+    if args.adjoint:
+        rep = run_adjoint((256, 96, 96), args.adj_spinup, args.adj_window, args.adj_ckpt, args.adj_dirs,
+                          args.adj_tol, 1e-10)
+        print(json.dumps({"metric": "rotor-power adjoint gradient vs central differences",
+                          "unit": "relative error", "n_gpus": 1, "adjoint": rep}))
+        return
     n = tuple(args.n)
     calib = calibrate_copy() if args.calibrate else None
     r64 = rmx = u64 = umx = None
