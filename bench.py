@@ -10,13 +10,22 @@ beside it on the same workload.  A step is one full time step (ALM, pseudo
 velocity, face flux, Poisson RHS, Poisson solve, projection, boundary fill,
 SGS, divergence monitor) with all fields resident in HBM.
 
+The accuracy leg runs both precisions for 1000 steps from the same start
+and reports the mixed step's velocity and pressure error against the FP64
+step.  It uses CFL 0.1: the reference's pseudo-velocity step is explicit
+Euler with Upwind3 (ua = un + dt(-adv + vis + ff), src/FalmCFD.cpp:5-67),
+which amplifies resolved modes at CFL 0.3 (|u|max grows about 4x by step
+400 and overflows near step 450 at 256x96x96, in the GPU path and in the
+CPU restatement alike); the timed steps (the first ~30) are unaffected.
+
 Per-kernel HBM rooflines come from HIP events around each launch of one
 kernel family at a time (extra steps after the timed ones, so the timed
 steps carry no event overhead), with algorithmic bytes per launch from the
 operand-list model (every array a kernel touches counted once per cell).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--only mixed|fp64]
-                    [--no-cpu] [--calibrate]
+                    [--no-cpu] [--calibrate] [--accuracy-steps N]
+    python bench.py --adjoint     (rotor-power gradient report, 256x96x96)
 """
 import argparse
 import json
@@ -136,7 +145,64 @@ def cpu_baseline(n, steps=2, warmup=1):
     }
 
 
-def run_adjoint(n, spinup, window, ckpt, ndir, tol, adj_rtol, seed=7):
+def run_accuracy(n, steps, cfl):
+    """Mixed against FP64 on the GPU after `steps` steps of the same
+    workload from the same initial state: relative L2 errors of velocity
+    and pressure over the interior, and each run's divergence monitor."""
+    import numpy as np
+
+    from falm_amd import _abi as abi
+    from falm_amd import cases, runtime
+
+    out = {}
+    for key, prec in (("fp64", abi.CASE_F64), ("mixed", abi.CASE_MIXED)):
+        setup = cases.windtunnel(n=n, precision=prec, cfl=cfl)
+        c = runtime.Case(setup)
+        t0 = time.perf_counter()
+        info = c.step(steps)
+        dt = time.perf_counter() - t0
+        out[key] = {"u": c.get(abi.FIELD_U), "p": c.get(abi.FIELD_P), "div": info.div,
+                    "solver_err": info.solver_err, "status": info.status, "s": dt}
+        c.close()
+    nx, ny, nz = setup.shape
+    mask = np.zeros((nz, ny, nx), dtype=bool)
+    mask[2:-2, 2:-2, 2:-2] = True
+    m1 = mask.ravel()
+    m3 = np.tile(m1, 3)
+
+    def rel(a, b, m):
+        return float(np.linalg.norm(a[m] - b[m]) / np.linalg.norm(b[m]))
+
+    f, x = out["fp64"], out["mixed"]
+    return {
+        "workload": f"windtunnel {n[0]}x{n[1]}x{n[2]} + rotor, CFL {cfl}, {steps} steps from the same start",
+        "velocity_rel_l2": rel(x["u"], f["u"], m3),
+        "pressure_rel_l2": rel(x["p"], f["p"], m1),
+        "div_fp64": f["div"], "div_mixed": x["div"],
+        "u_max_fp64": float(np.abs(f["u"][m3]).max()), "u_max_mixed": float(np.abs(x["u"][m3]).max()),
+        "status": [f["status"], x["status"]],
+        "wall_s": [f["s"], x["s"]],
+    }
+
+
+def device_free_bytes():
+    """hipMemGetInfo through the HIP runtime the library already loaded."""
+    import ctypes as C
+
+    path = None
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64.so" in line:
+                path = line.split()[-1]
+                break
+    hip = C.CDLL(path)
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return free.value
+
+
+def run_adjoint(n, spinup, window, ckpt, ndir, tol, adj_rtol, cfl, seed=7):
     """The rotor-power gradient over a window of FP64 steps after a spin-up:
     J = mean power, dJ/d(3 blade pitches, n_ap_per_blade twist offsets) by the
     checkpointed device adjoint, checked in ndir seeded random unit
@@ -146,12 +212,10 @@ def run_adjoint(n, spinup, window, ckpt, ndir, tol, adj_rtol, seed=7):
 
     from falm_amd import cases, runtime
 
-    import torch
-
-    setup = cases.windtunnel(n=n, cells_per_d=24, tol=tol)
-    free0 = torch.cuda.mem_get_info(0)[0]
+    setup = cases.windtunnel(n=n, cells_per_d=24, tol=tol, cfl=cfl)
+    free0 = device_free_bytes()
     c = runtime.Case(setup)
-    case_bytes = free0 - torch.cuda.mem_get_info(0)[0]  # the forward case's device footprint
+    case_bytes = free0 - device_free_bytes()  # the forward case's device footprint
     t0 = time.perf_counter()
     c.step(spinup)
     t_spin = time.perf_counter() - t0
@@ -184,7 +248,7 @@ def run_adjoint(n, spinup, window, ckpt, ndir, tol, adj_rtol, seed=7):
     c.close()
     return {
         "workload": f"windtunnel {n[0]}x{n[1]}x{n[2]} cells + 1 ALM rotor (3x{napb} points), dx=D/24, FP64, "
-                    f"Poisson tol {tol:g}, {spinup}-step spin-up, {window}-step window, checkpoints every {ckpt}",
+                    f"CFL {cfl}, Poisson tol {tol:g}, {spinup}-step spin-up, {window}-step window, checkpoints every {ckpt}",
         "J_mean_power": J,
         "J_forward_window": J0,
         "grad_pitch_per_deg": [float(x) for x in gp[0, :3]],
@@ -220,7 +284,11 @@ def main():
     ap.add_argument("--adj-window", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
     ap.add_argument("--adj-dirs", type=int, default=4)
-    ap.add_argument("--adj-tol", type=float, default=1e-10)
+    ap.add_argument("--adj-tol", type=float, default=1e-12)
+    ap.add_argument("--adj-cfl", type=float, default=0.1)
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="mixed-vs-FP64 error after this many steps (0: skip)")
+    ap.add_argument("--accuracy-cfl", type=float, default=0.1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -235,7 +303,7 @@ def main():
     runtime.init(0)
     if args.adjoint:
         rep = run_adjoint((256, 96, 96), args.adj_spinup, args.adj_window, args.adj_ckpt, args.adj_dirs,
-                          args.adj_tol, 1e-10)
+                          args.adj_tol, 1e-10, args.adj_cfl)
         print(json.dumps({"metric": "rotor-power adjoint gradient vs central differences",
                           "unit": "relative error", "n_gpus": 1, "adjoint": rep}))
         return
@@ -277,6 +345,7 @@ def main():
             if key in tr:
                 roofline["traffic"] = tr[key].get("hbm_bytes_per_launch")
 
+    acc = run_accuracy(n, args.accuracy_steps, args.accuracy_cfl) if args.accuracy_steps > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(n, steps=args.cpu_steps)
 
     def strip(r):
@@ -307,6 +376,7 @@ def main():
         "speedup_mixed_vs_fp64": (r64["ms_per_step"] / rmx["ms_per_step"]) if (r64 and rmx) else None,
         "velocity_rel_l2_mixed_vs_fp64": rel_l2,
         "roofline": roofline,
+        "accuracy": acc,
         "cpu_baseline": cpu,
     }
     if calib:

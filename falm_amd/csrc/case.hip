@@ -211,6 +211,19 @@ static falm_status case_init(falm_case *C) {
         FALM_HIP(hipMemcpy(dr, r.data(), L * sizeof(double), hipMemcpyHostToDevice));
         C->sm.h[m] = dh;
         C->sm.rh[m] = dr;
+        std::vector<double> tab[SEP_NTAB];
+        sep_tables(wid[m]->data(), L, tab);
+        for (int q = 0; q < SEP_NTAB; q++) {
+            std::vector<float> tf(tab[q].begin(), tab[q].end());
+            double *td = nullptr;
+            float *tff = nullptr;
+            FALM_TRY(case_dalloc(C, (void **)&td, L * sizeof(double)));
+            FALM_TRY(case_dalloc(C, (void **)&tff, L * sizeof(float)));
+            FALM_HIP(hipMemcpy(td, tab[q].data(), L * sizeof(double), hipMemcpyHostToDevice));
+            FALM_HIP(hipMemcpy(tff, tf.data(), L * sizeof(float), hipMemcpyHostToDevice));
+            C->sm.td[q][m] = td;
+            C->sm.tf[q][m] = tff;
+        }
     }
     FALM_HIP(hipDeviceSynchronize());
     // Poisson matrix from double metrics

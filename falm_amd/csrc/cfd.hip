@@ -415,6 +415,265 @@ __global__ __launch_bounds__(KM_NT) void k_pseudo_u_km(KmTiling kt, Geom g, T Re
     }
 }
 
+// Device view of the coefficient tables in one precision.
+template <class T>
+struct SepTab {
+    const T *__restrict__ t[SEP_NTAB][3];
+};
+
+// ---------------------------------------------------------------------
+// Pseudo-velocity for the Upwind3 family on a tensor-product mesh: the same
+// k-marching structure as k_pseudo_u_km with the arithmetic reorganised
+// around face differences (the scheme is unchanged, its terms are summed
+// in another order):
+//   along axis e with v(m) = u_d(c + m e) and the face differences
+//     a+ = v(1) - v(0),  b+ = v(2) - v(-1),  a- = v(0) - v(-1),  b- = v(1) - v(-2)
+//   UDL = JP (27 a+ - b+)/24,  UDR = JM (27 a- - b-)/24   (src/FalmCFDScheme.h:68-165)
+//   4th difference = (b+ - 3 a+) - (b- - 3 a-)
+//   diffusion = K+ a+ - K- a-, K = (g_c + g_n)/(2J) (ReI + (nut_c + nut_n)/2)
+//              (src/FalmCFDScheme.h:167-191)
+// and the metric factors come from the per-axis tables (per column once,
+// per plane as workgroup-uniform loads).  The k-axis plus-face terms of
+// plane k are the minus-face terms of plane k+1 and are carried in
+// registers.  Compared with k_pseudo_u_km this removes the double-precision
+// metric products, the division by the cell volume and about half of the
+// scheme's operations; FP64 results agree with the reference expression
+// order to rounding.
+template <int SCHEME, class T>
+__global__ __launch_bounds__(KM_NT) void k_pseudo_u3_km(KmTiling kt, Geom g, T ReI, T dt, Vec3<T> u, Vec3<T> uu,
+                                                        Vec3W<T> ua, const T *__restrict__ nut, SepTab<T> tb,
+                                                        Vec3<T> ff, const int *__restrict__ ffbox) {
+    constexpr double ALPHA_D = SCHEME == FALM_ADV_UPWIND3 ? 1. / 24. : SCHEME == FALM_ADV_UTOPIA ? 1. / 12. : 1. / 4.;
+    const T ALPHA = T(ALPHA_D);
+    constexpr unsigned TS = sizeof(T);
+    __shared__ T su[2][3][KM_BY + 4][KM_BX + 4];
+    __shared__ T sn[2][KM_BY + 2][KM_BX + 2];
+    const int lane = threadIdx.x & (KM_BX - 1);
+    const int wy = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / KM_BX));
+    const int L = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x));
+    const int bx = L % kt.tx, by = (L / kt.tx) % kt.ty, kc = L / (kt.tx * kt.ty);
+    const int i0 = kt.box.lo[0] + bx * KM_BX, j0 = kt.box.lo[1] + by * KM_BY;
+    const int i = i0 + lane, j = j0 + wy;
+    const bool act = i < kt.box.lo[0] + kt.box.n[0] && j < kt.box.lo[1] + kt.box.n[1];
+    const int k0 = kt.box.lo[2] + kc * KM_KC;
+    const int k1 = min(k0 + KM_KC, kt.box.lo[2] + kt.box.n[2]);
+    const int sy = g.nx, sz = g.nxy;
+    const unsigned syb = (unsigned)sy * TS, szb = (unsigned)sz * TS;
+    // as in k_pseudo_u_km: threads past the box load their own cell when it
+    // lies in the array, threads past the array a valid cell
+    const bool ld = i < g.nx && j < g.ny;
+    const int col0 = (ld ? i : kt.box.lo[0]) + sy * (ld ? j : kt.box.lo[1]);
+    // per-column coefficients (cells of the box only)
+    const int ia = act ? i : kt.box.lo[0], ja = act ? j : kt.box.lo[1];
+    const T rx = tb.t[SEP_R][0][ia], ry = tb.t[SEP_R][1][ja];
+    const T cxe = tb.t[SEP_CP][0][ia], cxw = tb.t[SEP_CM][0][ia];
+    const T cyn = tb.t[SEP_CP][1][ja], cys = tb.t[SEP_CM][1][ja];
+    const T arx = ALPHA * rx, ary = ALPHA * ry;
+    const T rxy = rx * ry * T(0.5 / 24.0);
+    // body force: the column's part of the force box and its k range
+    bool fcol = act;
+    int fk0 = k0, fk1 = k1 - 1;
+    if (ffbox) {
+        fcol = fcol && i >= ffbox[0] && i <= ffbox[1] && j >= ffbox[2] && j <= ffbox[3];
+        fk0 = ffbox[4];
+        fk1 = ffbox[5];
+    }
+    // halo slots: a base pointer per slot (an in-array cell when the halo
+    // cell lies outside the array; its value is then replaced by zero)
+    const T *hp[KM_HPT];
+    int hl[KM_HPT];  // LDS element index of the slot in plane buffer 0
+    bool hok[KM_HPT], hu[KM_HPT];
+#pragma unroll
+    for (int q = 0; q < KM_HPT; q++) {
+        const int s = threadIdx.x + q * KM_NT;
+        int arr = 3, row = 0, col = 0;
+        hok[q] = false;
+        if (s < KM_HALO) km_halo_slot(s, arr, row, col);
+        const int gi = i0 - (arr < 3 ? 2 : 1) + col;
+        const int gj = j0 - (arr < 3 ? 2 : 1) + row;
+        hok[q] = s < KM_HALO && gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny;
+        hu[q] = s < KM_HALO;
+        const T *src = arr == 0 ? u.c[0] : arr == 1 ? u.c[1] : arr == 2 ? u.c[2] : nut;
+        hp[q] = src + (hok[q] ? gi + sy * gj : col0);
+        hl[q] = arr < 3 ? (&su[0][arr][row][col] - &su[0][0][0][0]) : (&sn[0][row][col] - &sn[0][0][0]);
+        if (arr == 3) hl[q] = -1 - hl[q];  // negative: the nut plane
+    }
+    auto halo_load = [&](int k, T (&hv)[KM_HPT]) {
+#pragma unroll
+        for (int q = 0; q < KM_HPT; q++) {
+            const T v = hp[q][(size_t)sz * k];
+            hv[q] = hok[q] ? v : T(0);
+        }
+    };
+    auto lds_fill = [&](int b, const T (&hv)[KM_HPT], T u0, T u1, T u2, T nv) {
+        su[b][0][wy + 2][lane + 2] = u0;
+        su[b][1][wy + 2][lane + 2] = u1;
+        su[b][2][wy + 2][lane + 2] = u2;
+        sn[b][wy + 1][lane + 1] = nv;
+#pragma unroll
+        for (int q = 0; q < KM_HPT; q++) {
+            if (!hu[q]) continue;
+            if (hl[q] >= 0) (&su[b][0][0][0])[hl[q]] = hv[q];
+            else (&sn[b][0][0])[-1 - hl[q]] = hv[q];
+        }
+    };
+    // column queues: u at k-1 .. k+2, nut at k-1 .. k+1, the face fluxes and
+    // body force of plane k.  Loads run PD planes ahead: while plane k is
+    // computed, the loads of planes k+1 .. k+PD-1 are in flight or in
+    // registers and plane k+PD's are issued.  Two planes ahead in float;
+    // one in double, where the second stage's registers would halve the
+    // waves per SIMD.
+    constexpr int PD = sizeof(T) == 4 ? 2 : 1;
+    unsigned cb = (unsigned)(col0 + sz * k0) * TS;  // byte offset of the column's plane-k cell
+    T qu[3][4], qn[3], cuu[5], cff[3];
+    T pzm[3], tzm[3], azm[3];  // k-axis minus-face terms
+    // pending loads of planes k+1 .. k+PD (stage 0 = plane k+1): u(kn+2),
+    // nut(kn+1), face fluxes, force and halo of plane kn
+    T pu[PD][3], pn[PD], puu[PD][5], pff[PD][3], phv[PD][KM_HPT];
+    auto issue = [&](int kn, unsigned nb, T (&xu)[3], T &xn, T (&xuu)[5], T (&xff)[3], T (&xhv)[KM_HPT]) {
+        // loads of plane kn (byte offset nb): its queue tail u(kn+2), nut(kn+1)
+        halo_load(kn, xhv);
+        xu[0] = ld_at(u.c[0], nb + 2 * szb);
+        xu[1] = ld_at(u.c[1], nb + 2 * szb);
+        xu[2] = ld_at(u.c[2], nb + 2 * szb);
+        xn = ld_at(nut, nb + szb);
+        xuu[0] = ld_at(uu.c[0], nb);
+        xuu[1] = ld_at(uu.c[0], nb - TS);
+        xuu[2] = ld_at(uu.c[1], nb);
+        xuu[3] = ld_at(uu.c[1], nb - syb);
+        xuu[4] = ld_at(uu.c[2], nb);
+        const bool f = fcol && kn >= fk0 && kn <= fk1;
+        for (int d = 0; d < 3; d++) xff[d] = f ? ld_at(ff.c[d], nb) : T(0);
+    };
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const T um2 = ld_at(u.c[c], cb - 2 * szb);
+#pragma unroll
+        for (int d = 0; d < 4; d++) qu[c][d] = ld_at(u.c[c], cb + (unsigned)(d - 1) * szb);
+        const T am = qu[c][1] - qu[c][0], bm = qu[c][2] - um2;
+        pzm[c] = T(27) * am - bm;
+        tzm[c] = bm - T(3) * am;
+        azm[c] = am;
+    }
+    qn[0] = ld_at(nut, cb - szb);
+    qn[1] = ld_at(nut, cb);
+    qn[2] = ld_at(nut, cb + szb);
+    T wb = ld_at(uu.c[2], cb - szb);
+    cuu[0] = ld_at(uu.c[0], cb);
+    cuu[1] = ld_at(uu.c[0], cb - TS);
+    cuu[2] = ld_at(uu.c[1], cb);
+    cuu[3] = ld_at(uu.c[1], cb - syb);
+    cuu[4] = ld_at(uu.c[2], cb);
+    {
+        const bool f = fcol && k0 >= fk0 && k0 <= fk1;
+        for (int d = 0; d < 3; d++) cff[d] = f ? ld_at(ff.c[d], cb) : T(0);
+    }
+    // planes past the chunk's end are loaded from its last plane (in the
+    // array, values unused) so that the unrolled loop has no branches
+    const int klast = k1 - 1;
+    {
+        T hv0[KM_HPT];
+        halo_load(k0, hv0);
+#pragma unroll
+        for (int st = 0; st + 1 < PD; st++) {
+            const int kn = min(k0 + 1 + st, klast);
+            issue(kn, (unsigned)(col0 + sz * kn) * TS, pu[st], pn[st], puu[st], pff[st], phv[st]);
+        }
+        lds_fill(0, hv0, qu[0][1], qu[1][1], qu[2][1], qn[1]);
+    }
+    __syncthreads();
+    // fully unrolled over the chunk's planes: the register queues and the LDS
+    // buffer index then rotate by renaming instead of moves
+#pragma unroll
+    for (int kk = 0; kk < KM_KC; kk++) {
+        const int k = k0 + kk;
+        if (k >= k1) break;
+        const int b = kk & 1;
+        {
+            const int kn = min(k + PD, klast);
+            issue(kn, (unsigned)(col0 + sz * kn) * TS, pu[PD - 1], pn[PD - 1], puu[PD - 1], pff[PD - 1],
+                  phv[PD - 1]);
+        }
+        // plane k
+        const T rz = tb.t[SEP_R][2][k], czt = tb.t[SEP_CP][2][k], czb = tb.t[SEP_CM][2][k];
+        const T rj = rxy * rz;  // (1/2)(1/24)/J
+        const T aux = fabs(qu[0][1] * arx), auy = fabs(qu[1][1] * ary), auz = fabs(qu[2][1] * (ALPHA * rz));
+        const T nc = qn[1];
+        const T ke = cxe * (ReI + T(0.5) * (nc + sn[b][wy + 1][lane + 2]));
+        const T kw = cxw * (ReI + T(0.5) * (nc + sn[b][wy + 1][lane]));
+        const T kn = cyn * (ReI + T(0.5) * (nc + sn[b][wy + 2][lane + 1]));
+        const T ks = cys * (ReI + T(0.5) * (nc + sn[b][wy][lane + 1]));
+        const T kt_ = czt * (ReI + T(0.5) * (nc + qn[2]));
+        const T kb = czb * (ReI + T(0.5) * (nc + qn[0]));
+        const T UE = cuu[0], UW = cuu[1], VN = cuu[2], VS = cuu[3], WT = cuu[4], WB = wb;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T ucc = qu[d][1];
+            T F, D, V;
+            {   // x
+                const T ue1 = su[b][d][wy + 2][lane + 3], ue2 = su[b][d][wy + 2][lane + 4];
+                const T uw1 = su[b][d][wy + 2][lane + 1], uw2 = su[b][d][wy + 2][lane];
+                const T ap = ue1 - ucc, bp = ue2 - uw1, am = ucc - uw1, bm = ue1 - uw2;
+                F = UE * (T(27) * ap - bp) + UW * (T(27) * am - bm);
+                D = aux * ((bp - T(3) * ap) - (bm - T(3) * am));
+                V = ke * ap - kw * am;
+            }
+            {   // y
+                const T un1 = su[b][d][wy + 3][lane + 2], un2 = su[b][d][wy + 4][lane + 2];
+                const T us1 = su[b][d][wy + 1][lane + 2], us2 = su[b][d][wy][lane + 2];
+                const T ap = un1 - ucc, bp = un2 - us1, am = ucc - us1, bm = un1 - us2;
+                F += VN * (T(27) * ap - bp) + VS * (T(27) * am - bm);
+                D += auy * ((bp - T(3) * ap) - (bm - T(3) * am));
+                V += kn * ap - ks * am;
+            }
+            {   // z: the minus face is the previous plane's plus face
+                const T ap = qu[d][2] - ucc, bp = qu[d][3] - qu[d][0];
+                const T pp = T(27) * ap - bp, tp = bp - T(3) * ap;
+                F += WT * pp + WB * pzm[d];
+                D += auz * (tp - tzm[d]);
+                V += kt_ * ap - kb * azm[d];
+                pzm[d] = pp;
+                tzm[d] = tp;
+                azm[d] = ap;
+            }
+            const T adv = F * rj + D;
+            if (act) st_at(ua.c[d], cb, ucc + dt * (V - adv + cff[d]));
+        }
+        {   // plane k+1 into the other LDS buffer; rotate the queues and stages
+            lds_fill(b ^ 1, phv[0], qu[0][2], qu[1][2], qu[2][2], qn[2]);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                qu[c][0] = qu[c][1];
+                qu[c][1] = qu[c][2];
+                qu[c][2] = qu[c][3];
+                qu[c][3] = pu[0][c];
+            }
+            qn[0] = qn[1];
+            qn[1] = qn[2];
+            qn[2] = pn[0];
+            wb = WT;
+#pragma unroll
+            for (int d = 0; d < 5; d++) cuu[d] = puu[0][d];
+#pragma unroll
+            for (int d = 0; d < 3; d++) cff[d] = pff[0][d];
+#pragma unroll
+            for (int st = 0; st + 1 < PD; st++) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) pu[st][c] = pu[st + 1][c];
+                pn[st] = pn[st + 1];
+#pragma unroll
+                for (int d = 0; d < 5; d++) puu[st][d] = puu[st + 1][d];
+#pragma unroll
+                for (int d = 0; d < 3; d++) pff[st][d] = pff[st + 1][d];
+#pragma unroll
+                for (int q = 0; q < KM_HPT; q++) phv[st][q] = phv[st + 1][q];
+            }
+        }
+        cb += szb;
+        __syncthreads();
+    }
+}
+
 // UtoCU + InterpolateCU fused (src/dev/FalmCFDDevCall.cu:303-349): the
 // face flux of face (i+1/2) is the mean of the two cell-centre fluxes
 // ja*kx*u, each evaluated exactly as the reference does.
@@ -778,6 +1037,38 @@ static MetField<T> met_field(const void *kx, const void *gg, const void *ja, siz
     return m;
 }
 template <class T>
+static SepTab<T> sep_tab(const SepMesh &sm);
+template <>
+SepTab<double> sep_tab<double>(const SepMesh &sm) {
+    SepTab<double> t;
+    for (int q = 0; q < SEP_NTAB; q++)
+        for (int d = 0; d < 3; d++) t.t[q][d] = sm.td[q][d];
+    return t;
+}
+template <>
+SepTab<float> sep_tab<float>(const SepMesh &sm) {
+    SepTab<float> t;
+    for (int q = 0; q < SEP_NTAB; q++)
+        for (int d = 0; d < 3; d++) t.t[q][d] = sm.tf[q][d];
+    return t;
+}
+
+void sep_tables(const double *h, int len, std::vector<double> (&out)[SEP_NTAB]) {
+    for (auto &v : out) v.assign(len, 0.0);
+    for (int i = 0; i < len; i++) {
+        const double r = 1.0 / h[i];
+        const double rp = 1.0 / h[i + 1 < len ? i + 1 : i], rm = 1.0 / h[i > 0 ? i - 1 : i];
+        const double c = cbrt(h[i]);
+        out[SEP_R][i] = r;
+        out[SEP_H][i] = h[i];
+        out[SEP_CP][i] = 0.5 * (r + rp) * r;
+        out[SEP_CM][i] = 0.5 * (r + rm) * r;
+        out[SEP_RF][i] = 0.5 * (r + rp);
+        out[SEP_H23][i] = c * c;
+    }
+}
+
+template <class T>
 static MetSep<T> met_sep(const SepMesh &sm) {
     MetSep<T> m;
     for (int d = 0; d < 3; d++) {
@@ -914,20 +1205,29 @@ static falm_status pseudo_u_km_t(const falm_domain *dom, int scheme, double ReI,
     size_t n = ncell_of(dom);
     KmTiling kt = make_km_tiling(interior_box(g));
     if (kt.blocks == 0) return FALM_OK;
+    if (n * sizeof(T) >= (size_t(1) << 32)) {
+        set_error("pseudo-velocity: a field component exceeds 4 GiB");
+        return FALM_ERR_UNSUPPORTED;
+    }
     TimedLaunch tm(FALM_TIME_PSEUDO_U, s);
 #define FALM_PUK(S)                                                                                        \
     hipLaunchKernelGGL((k_pseudo_u_km<S, T>), dim3(kt.blocks), dim3(KM_NT), 0, s, kt, g, (T)ReI, (T)dt,   \
                        v3<T>(u, n), v3<T>(uu, n), v3w<T>(ua, n), (const T *)nut, met_sep<T>(sm), v3<T>(ff, n), \
                        ffbox)
+#define FALM_PU3(S)                                                                                        \
+    hipLaunchKernelGGL((k_pseudo_u3_km<S, T>), dim3(kt.blocks), dim3(KM_NT), 0, s, kt, g, (T)ReI, (T)dt,  \
+                       v3<T>(u, n), v3<T>(uu, n), v3w<T>(ua, n), (const T *)nut, sep_tab<T>(sm), v3<T>(ff, n), \
+                       ffbox)
     switch (scheme) {
     case FALM_ADV_UPWIND1: FALM_PUK(FALM_ADV_UPWIND1); break;
-    case FALM_ADV_UPWIND3: FALM_PUK(FALM_ADV_UPWIND3); break;
+    case FALM_ADV_UPWIND3: FALM_PU3(FALM_ADV_UPWIND3); break;
     case FALM_ADV_QUICK: FALM_PUK(FALM_ADV_QUICK); break;
-    case FALM_ADV_UTOPIA: FALM_PUK(FALM_ADV_UTOPIA); break;
-    case FALM_ADV_KK: FALM_PUK(FALM_ADV_KK); break;
+    case FALM_ADV_UTOPIA: FALM_PU3(FALM_ADV_UTOPIA); break;
+    case FALM_ADV_KK: FALM_PU3(FALM_ADV_KK); break;
     default: FALM_PUK(FALM_ADV_UPWIND1); break;
     }
 #undef FALM_PUK
+#undef FALM_PU3
     FALM_CHECK_LAUNCH("k_pseudo_u_km");
     // u, uu, nut read, ua written (ff only inside the rotor's force box)
     tm.done((double)interior_box(g).size() * 10.0 * sizeof(T));

@@ -2,17 +2,38 @@
 // the step driver.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace falm {
 
-// Device copies of the 1-D cell widths of a tensor-product mesh and their
-// reciprocals (1.0/h computed on the host), one array per axis over the
-// padded local shape.
+// SepMesh: device copies of the 1-D cell widths of a tensor-product mesh and
+// their reciprocals (1.0/h computed on the host), one array per axis over the
+// padded local shape, and per-axis coefficient tables of the step driver's
+// stencils, each computed on the host in double from the widths and stored in
+// double (td) and float (tf).  On a tensor-product mesh every metric product
+// the schemes use
+// factors into one value per axis index:
+//   SEP_R   r = 1/h                            (kx along the axis)
+//   SEP_H   h
+//   SEP_CP  0.5 (r_i + r_{i+1}) r_i            ((g_c + g_{c+e}) / 2J)
+//   SEP_CM  0.5 (r_i + r_{i-1}) r_i            ((g_c + g_{c-e}) / 2J)
+//   SEP_RF  0.5 (r_i + r_{i+1})                (face mean of kx)
+//   SEP_H23 cbrt(h)^2                           (ja^(2/3) = product over axes)
+// (g = ja kx^2 and ja = hx hy hz, src/falm.h:383-404.)  Entries past the
+// array end use the last width.
+enum { SEP_R = 0, SEP_H, SEP_CP, SEP_CM, SEP_RF, SEP_H23, SEP_NTAB };
+
 struct SepMesh {
     const double *h[3];
     const double *rh[3];
+    const double *td[SEP_NTAB][3];
+    const float *tf[SEP_NTAB][3];
 };
+
+// host-side values of the tables for one axis: out[tab][i], tab < SEP_NTAB
+void sep_tables(const double *h, int len, std::vector<double> (&out)[SEP_NTAB]);
 
 // ffbox: device int[6] cell box outside which ff is zero and not read
 falm_status sep_pseudo_u(const falm_domain *dom, int prec, int scheme, double ReI, double dt, const void *un,

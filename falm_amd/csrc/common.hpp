@@ -178,6 +178,21 @@ inline dim3 tiling_grid(const Tiling &t) { return dim3((unsigned)t.tiles); }
 inline dim3 one_tile_grid(const Tiling &t) { return dim3((unsigned)t.tiles); }
 inline dim3 tiling_block() { return dim3(TX * TY); }
 
+// Access at an unsigned 32-bit byte offset from a wave-uniform base pointer.
+// The compiler then emits the SGPR-base form of the global load/store (one
+// VGPR offset shared by every array indexed at the same cell, constant
+// neighbour offsets folded into the instruction), where a signed element
+// index costs a 64-bit address computation per access.  Arrays addressed
+// this way must be below 4 GiB (checked by the launchers).
+template <class T>
+__device__ __forceinline__ T ld_at(const T *p, unsigned off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T *p, unsigned off, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + off) = v;
+}
+
 // 1-D element-wise launches
 constexpr int EW_BLOCK = 256;
 inline dim3 ew_grid(size_t n) {
