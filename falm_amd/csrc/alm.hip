@@ -8,10 +8,11 @@
 //   is separable on the tensor-product grid, exp(-r^2/eps^2) =
 //   ex(x_i) ey(y_j) ez(z_k), so each point's three 1-D factor rows are
 //   evaluated once per step and a cell-point pair costs two products
-//   instead of an exp.  Each 64x4 tile first culls, in LDS, the points whose
-//   kernel value is exactly zero in double precision over the whole tile
-//   (r^2/eps^2 > 746), keeping the survivors in ascending id order; a finite
-//   cutoff (radius in units of eps) culls harder.
+//   instead of an exp.  A fixed grid walks 32x8-cell tiles of the force box
+//   (of the whole interior on a full write); each tile first culls, in LDS,
+//   the points whose kernel value is exactly zero in double precision over
+//   the whole tile (r^2/eps^2 > 746), keeping the survivors in ascending id
+//   order; a finite cutoff (radius in units of eps) culls harder.
 // k_alm_totals: per-turbine torque/thrust in a fixed order (the reference
 //   uses thrust::reduce, src/alm/almDevCall.cu:362-370), kept on device.
 #include <cmath>
@@ -368,117 +369,118 @@ __device__ inline bool box_meets(const int *b, int ilo, int ihi, int jlo, int jh
 }
 
 constexpr int kCullChunk = 256;
+constexpr int PJ_X = 32, PJ_Y = 8;  // projection tile: 32 x 8 cells of one plane
 
-// union_only: the caller's ff is already zero outside the union box, so
-// tiles outside it are left alone; otherwise every interior cell is written.
+// The Gaussian projection, walking the tiles of the cells it writes with a
+// fixed grid: the union of this and the previous step's force box
+// (union_only; ff is zero elsewhere and stays untouched), or every interior
+// cell.  Cells outside this step's box get zero.  Per tile, the points whose
+// Gaussian reaches the tile within the culling radius are compacted into
+// LDS with their per-axis factors, then each cell sums over them.
 template <class T>
-__global__ __launch_bounds__(256) void k_alm_project(Tiling tl, AlmDev A, T *ff0, T *ff1, T *ff2, double cut2,
+__global__ __launch_bounds__(256) void k_alm_project(Box interior, AlmDev A, T *ff0, T *ff1, T *ff2, double cut2,
                                                      int union_only) {
     __shared__ double sf0[kCullChunk], sf1[kCullChunk], sf2[kCullChunk], sez[kCullChunk];
-    __shared__ double sey[TY][kCullChunk];
+    __shared__ double sey[PJ_Y][kCullChunk];
     __shared__ int sap[kCullChunk];
-    __shared__ int nkeep, tj0, tjn, tk;
-    __shared__ double box[6];
-    int i, j, k;
-    const bool valid = tile_cell(tl, i, j, k);
-    {
-        // tile extent in cells, against the force boxes
-        const int tile = xcd_tile(blockIdx.x, gridDim.x);
-        int bx, by, kk;
-        tile_coords(tl, tile, bx, by, kk);
-        const int ilo = tl.box.lo[0] + bx * TX, ihi = min(ilo + TX, tl.box.lo[0] + tl.box.n[0]) - 1;
-        const int jlo = tl.box.lo[1] + by * TY, jhi = min(jlo + TY, tl.box.lo[1] + tl.box.n[1]) - 1;
-        const int kc = tl.box.lo[2] + kk;
-        if (union_only && !box_meets(A.box + 6, ilo, ihi, jlo, jhi, kc, kc)) return;
-        if (!box_meets(A.box, ilo, ihi, jlo, jhi, kc, kc)) {
-            if (valid) {
-                const int cc = i + A.nx * j + A.nx * A.ny * k;
-                ff0[cc] = T(0);
-                ff1[cc] = T(0);
-                ff2[cc] = T(0);
-            }
-            return;
+    __shared__ int wcount[256 / 64];
+    int wlo[3], wn[3];
+    if (union_only) {
+        const int *U = A.box + 6;
+        if (U[0] > U[1]) return;
+        for (int m = 0; m < 3; m++) {
+            wlo[m] = U[2 * m];
+            wn[m] = U[2 * m + 1] - U[2 * m] + 1;
+        }
+    } else {
+        for (int m = 0; m < 3; m++) {
+            wlo[m] = interior.lo[m];
+            wn[m] = interior.n[m];
         }
     }
+    const int tx = (wn[0] + PJ_X - 1) / PJ_X, ty = (wn[1] + PJ_Y - 1) / PJ_Y;
+    const long long ntiles = (long long)tx * ty * wn[2];
     const int *B = A.box;
-    const bool inbox = valid && i >= B[0] && i <= B[1] && j >= B[2] && j <= B[3] && k >= B[4] && k <= B[5];
-    // tile bounding box in coordinates (cell centres of the tile)
-    if (threadIdx.x == 0) {
-        const int tile = xcd_tile(blockIdx.x, gridDim.x);
-        int bx, by, kk;
-        tile_coords(tl, tile, bx, by, kk);
-        const int ilo = tl.box.lo[0] + bx * TX, ihi = min(ilo + TX, tl.box.lo[0] + tl.box.n[0]) - 1;
-        const int jlo = tl.box.lo[1] + by * TY, jhi = min(jlo + TY, tl.box.lo[1] + tl.box.n[1]) - 1;
-        const int kc = tl.box.lo[2] + kk;
-        box[0] = fmin(A.x[ilo], A.x[ihi]); box[1] = fmax(A.x[ilo], A.x[ihi]);
-        box[2] = fmin(A.y[jlo], A.y[jhi]); box[3] = fmax(A.y[jlo], A.y[jhi]);
-        box[4] = A.z[kc]; box[5] = A.z[kc];
-        tj0 = jlo;
-        tjn = jhi - jlo + 1;
-        tk = kc;
-    }
-    __syncthreads();
     const double eps = A.eps;
     const double eps2 = eps * eps;
     const double sp = eps * sqrt(kPi);
     const double eta = 1. / (sp * sp * sp);
-    const int jr = (int)(threadIdx.x >> 6);  // row of this thread within the tile
-    __shared__ int wcount[256 / 64];
-    double fx = 0., fy = 0., fz = 0.;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int base = 0; base < A.nap; base += kCullChunk) {
-        // keep the point unless every cell of the tile is beyond the cutoff
-        const int ap = base + (int)threadIdx.x;
-        bool keep = false;
-        if (ap < A.nap) {
-            const double px = A.ap_xyz[3 * ap], py = A.ap_xyz[3 * ap + 1], pz = A.ap_xyz[3 * ap + 2];
-            const double dx = fmax(0.0, fmax(box[0] - px, px - box[1]));
-            const double dy = fmax(0.0, fmax(box[2] - py, py - box[3]));
-            const double dz = fmax(0.0, fmax(box[4] - pz, pz - box[5]));
-            const double d2 = (dx * dx + dy * dy + dz * dz) / eps2;
-            keep = !(d2 > cut2);
-        }
-        // order-preserving compaction: ballot prefix within the wavefront,
-        // wavefront totals across the block
-        const unsigned long long mask = __ballot(keep);
-        const int before = __popcll(mask & ((1ull << lane) - 1ull));
-        if (lane == 0) wcount[wid] = __popcll(mask);
-        __syncthreads();
-        int off = 0, tot = 0;
-        for (int w = 0; w < 256 / 64; w++) {
-            if (w < wid) off += wcount[w];
-            tot += wcount[w];
-        }
-        if (keep) {
-            const int slot = off + before;
-            sap[slot] = ap;
-            sf0[slot] = A.ap_force[3 * ap] * eta;
-            sf1[slot] = A.ap_force[3 * ap + 1] * eta;
-            sf2[slot] = A.ap_force[3 * ap + 2] * eta;
-            sez[slot] = A.fz[(size_t)ap * A.nz + tk];
-            for (int q = 0; q < tjn; q++) sey[q][slot] = A.fy[(size_t)ap * A.ny + tj0 + q];
-        }
-        if (threadIdx.x == 0) nkeep = tot;
-        __syncthreads();
-        const int nk = nkeep;
-        if (valid) {
-            for (int q = 0; q < nk; q++) {
-                // exp(-r^2/eps^2) = ex*ey*ez (per-axis factors of the point)
-                const double e = (A.fx[(size_t)sap[q] * A.nx + i] * sey[jr][q]) * sez[q];
-                fx += sf0[q] * e;
-                fy += sf1[q] * e;
-                fz += sf2[q] * e;
+    const int ci = (int)threadIdx.x % PJ_X, jr = (int)threadIdx.x / PJ_X;
+    for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int bx = (int)(t % tx), by = (int)((t / tx) % ty), kc = (int)(t / ((long long)tx * ty));
+        const int ilo = wlo[0] + bx * PJ_X, ihi = min(ilo + PJ_X, wlo[0] + wn[0]) - 1;
+        const int jlo = wlo[1] + by * PJ_Y, jhi = min(jlo + PJ_Y, wlo[1] + wn[1]) - 1;
+        const int k = wlo[2] + kc;
+        const int i = ilo + ci, j = jlo + jr;
+        const bool valid = i <= ihi && j <= jhi;
+        const size_t cc = (size_t)i + (size_t)A.nx * j + (size_t)A.nx * A.ny * k;
+        if (!box_meets(B, ilo, ihi, jlo, jhi, k, k)) {
+            if (valid) {
+                ff0[cc] = T(0);
+                ff1[cc] = T(0);
+                ff2[cc] = T(0);
             }
+            continue;  // uniform across the workgroup
         }
-        __syncthreads();
-    }
-    if (valid) {
-        // cells outside the box receive nothing (every term there is below
-        // the cutoff by more than a cell width)
-        const int cc = i + A.nx * j + A.nx * A.ny * k;
-        ff0[cc] = inbox ? (T)fx : T(0);
-        ff1[cc] = inbox ? (T)fy : T(0);
-        ff2[cc] = inbox ? (T)fz : T(0);
+        const bool inbox = valid && i >= B[0] && i <= B[1] && j >= B[2] && j <= B[3] && k >= B[4] && k <= B[5];
+        // tile bounding box in coordinates (cell centres of the tile)
+        const double bx0 = fmin(A.x[ilo], A.x[ihi]), bx1 = fmax(A.x[ilo], A.x[ihi]);
+        const double by0 = fmin(A.y[jlo], A.y[jhi]), by1 = fmax(A.y[jlo], A.y[jhi]);
+        const double bz = A.z[k];
+        const int tjn = jhi - jlo + 1;
+        double fx = 0., fy = 0., fz = 0.;
+        for (int base = 0; base < A.nap; base += kCullChunk) {
+            // keep the point unless every cell of the tile is beyond the cutoff
+            const int ap = base + (int)threadIdx.x;
+            bool keep = false;
+            if (ap < A.nap) {
+                const double px = A.ap_xyz[3 * ap], py = A.ap_xyz[3 * ap + 1], pz = A.ap_xyz[3 * ap + 2];
+                const double dx = fmax(0.0, fmax(bx0 - px, px - bx1));
+                const double dy = fmax(0.0, fmax(by0 - py, py - by1));
+                const double dz = fabs(bz - pz);
+                const double d2 = (dx * dx + dy * dy + dz * dz) / eps2;
+                keep = !(d2 > cut2);
+            }
+            // order-preserving compaction: ballot prefix within the
+            // wavefront, wavefront totals across the block
+            const unsigned long long mask = __ballot(keep);
+            const int before = __popcll(mask & ((1ull << lane) - 1ull));
+            if (lane == 0) wcount[wid] = __popcll(mask);
+            __syncthreads();
+            int off = 0, tot = 0;
+            for (int w = 0; w < 256 / 64; w++) {
+                if (w < wid) off += wcount[w];
+                tot += wcount[w];
+            }
+            if (keep) {
+                const int slot = off + before;
+                sap[slot] = ap;
+                sf0[slot] = A.ap_force[3 * ap] * eta;
+                sf1[slot] = A.ap_force[3 * ap + 1] * eta;
+                sf2[slot] = A.ap_force[3 * ap + 2] * eta;
+                sez[slot] = A.fz[(size_t)ap * A.nz + k];
+                for (int q = 0; q < tjn; q++) sey[q][slot] = A.fy[(size_t)ap * A.ny + jlo + q];
+            }
+            __syncthreads();
+            if (valid) {
+                for (int q = 0; q < tot; q++) {
+                    // exp(-r^2/eps^2) = ex*ey*ez (per-axis factors of the point)
+                    const double e = (A.fx[(size_t)sap[q] * A.nx + i] * sey[jr][q]) * sez[q];
+                    fx += sf0[q] * e;
+                    fy += sf1[q] * e;
+                    fz += sf2[q] * e;
+                }
+            }
+            __syncthreads();
+        }
+        if (valid) {
+            // cells outside the box receive nothing (every term there is
+            // below the cutoff by more than a cell width)
+            ff0[cc] = inbox ? (T)fx : T(0);
+            ff1[cc] = inbox ? (T)fy : T(0);
+            ff2[cc] = inbox ? (T)fz : T(0);
+        }
     }
 }
 
@@ -780,13 +782,16 @@ falm_status alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, h
     hipLaunchKernelGGL(k_alm_factors, dim3((nfac + 255) / 256), dim3(256), 0, s, A->d);
     FALM_CHECK_LAUNCH("k_alm_factors");
     if (tl.tiles > 0) {
+        // enough workgroups for a force box; a full write walks its tiles
+        const Box ib = interior_box(g);
+        const long long full = (long long)((ib.n[0] + PJ_X - 1) / PJ_X) * ((ib.n[1] + PJ_Y - 1) / PJ_Y) * ib.n[2];
+        const dim3 grid((unsigned)(full < 2048 ? full : 2048));
         if (prec == FALM_F32)
-            hipLaunchKernelGGL((k_alm_project<float>), one_tile_grid(tl), tiling_block(), 0, s, tl, A->d,
-                               col<float>(ff, n, 0), col<float>(ff, n, 1), col<float>(ff, n, 2), A->cut2, union_only);
+            hipLaunchKernelGGL((k_alm_project<float>), grid, dim3(256), 0, s, ib, A->d, col<float>(ff, n, 0),
+                               col<float>(ff, n, 1), col<float>(ff, n, 2), A->cut2, union_only);
         else
-            hipLaunchKernelGGL((k_alm_project<double>), one_tile_grid(tl), tiling_block(), 0, s, tl, A->d,
-                               col<double>(ff, n, 0), col<double>(ff, n, 1), col<double>(ff, n, 2), A->cut2,
-                               union_only);
+            hipLaunchKernelGGL((k_alm_project<double>), grid, dim3(256), 0, s, ib, A->d, col<double>(ff, n, 0),
+                               col<double>(ff, n, 1), col<double>(ff, n, 2), A->cut2, union_only);
         FALM_CHECK_LAUNCH("k_alm_project");
     }
     hipLaunchKernelGGL(k_alm_totals, dim3(A->nt), dim3(256), 0, s, A->d);

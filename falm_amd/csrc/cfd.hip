@@ -116,6 +116,8 @@ struct MetField {
     __device__ inline T KX(int d, int cc, int, int, int) const { return kx[d][cc]; }
     __device__ inline T G(int d, int cc, int, int, int) const { return g[d][cc]; }
     __device__ inline T JA(int cc, int, int, int) const { return ja[cc]; }
+    template <class V>
+    __device__ inline V div_ja(V x, int cc, int, int, int) const { return x / (V)ja[cc]; }
     static constexpr bool per_cell = true;  // metric arrays count as traffic
 };
 template <class T>
@@ -130,6 +132,8 @@ struct MetSep {
         return (T)(vol(i, j, k) * q * q);
     }
     __device__ inline T JA(int, int i, int j, int k) const { return (T)vol(i, j, k); }
+    template <class V>
+    __device__ inline V div_ja(V x, int cc, int i, int j, int k) const { return x / (V)JA(cc, i, j, k); }
     static constexpr bool per_cell = false;
 };
 
@@ -421,6 +425,18 @@ struct SepTab {
     const T *__restrict__ t[SEP_NTAB][3];
 };
 
+// Metric access from the tables (step driver): division by the cell
+// volume becomes a product of the three reciprocal widths.
+template <class T>
+struct MetTab {
+    SepTab<T> tb;
+    template <class V>
+    __device__ inline V div_ja(V x, int, int i, int j, int k) const {
+        return x * (V)(tb.t[SEP_R][0][i] * tb.t[SEP_R][1][j] * tb.t[SEP_R][2][k]);
+    }
+    static constexpr bool per_cell = false;
+};
+
 // ---------------------------------------------------------------------
 // Pseudo-velocity for the Upwind3 family on a tensor-product mesh: the same
 // k-marching structure as k_pseudo_u_km with the arithmetic reorganised
@@ -709,7 +725,7 @@ __global__ __launch_bounds__(256) void k_divergence(Tiling tl, Geom g, Vec3<T> u
             TO UE = uu.c[0][cc], UW = uu.c[0][cc - 1];
             TO VN = uu.c[1][cc], VS = uu.c[1][cc - g.nx];
             TO WT = uu.c[2][cc], WB = uu.c[2][cc - g.nxy];
-            TO d = (UE - UW + VN - VS + WT - WB) / (TO)met.JA(cc, i, j, k);
+            TO d = met.div_ja(UE - UW + VN - VS + WT - WB, cc, i, j, k);
             if (SCALE) d *= (TO)scale;
             div[cc] = d;
         } else if (SCALE) {
@@ -778,41 +794,50 @@ __global__ __launch_bounds__(256) void k_smagorinsky(Tiling tl, Geom g, T Cs, Ve
     });
 }
 
-// ------------------------------------------- k-marching forms (MetSep)
+// ------------------------------------------- k-marching forms (tables)
 // Projection on the interior cells: cell update and face update, the
-// pressure walked as a Plane7 (same expressions as k_project).
+// pressure walked as a Plane7, the metric factors from the per-axis tables
+// (in the pressure's precision P):
+//   u_d  -= dt kx_d (p(c+e) - p(c-e))/2                (ProjectPGrid)
+//   uu_d -= dt (g_d(c) + g_d(c+e))/2 (p(c+e) - p(c))   (ProjectPFace)
+// with kx_d = r_d and (g_d(c) + g_d(c+e))/2 = (product of the other two
+// widths) x (r_d(c) + r_d(c+e))/2 on the tensor-product mesh
+// (src/dev/FalmCFDDevCall.cu:351-407).
 template <class T, class P>
 __global__ __launch_bounds__(K7_NT) void k_project_km(K7Tiling kt, Geom g, P dt, Vec3W<T> u, Vec3W<T> uu,
-                                                      const P *__restrict__ p, MetSep<T> met) {
+                                                      const P *__restrict__ p, SepTab<P> tb) {
     __shared__ K7Lds<P> lp;
     const K7Pos Q = k7_pos(kt, g);
     const int sz = g.nxy;
+    const unsigned szt = (unsigned)sz * sizeof(T);
     Plane7<P> pp;
     pp.setup(p, &lp, Q, g);
     pp.prologue(Q, sz);
     __syncthreads();
     const int i = Q.act ? Q.i : kt.box.lo[0], j = Q.act ? Q.j : kt.box.lo[1];
+    const P cdx = dt * P(0.5) * tb.t[SEP_R][0][i], cdy = dt * P(0.5) * tb.t[SEP_R][1][j];
+    const P ffx = dt * tb.t[SEP_RF][0][i] * tb.t[SEP_H][1][j];
+    const P ffy = dt * tb.t[SEP_RF][1][j] * tb.t[SEP_H][0][i];
+    const P ffz = dt * tb.t[SEP_H][0][i] * tb.t[SEP_H][1][j];
+    unsigned cb = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szt;
     int b = 0;
-    for (int k = Q.k0; k < Q.k1; k++) {
+    for (int k = Q.k0; k < Q.k1; k++, cb += szt) {
         const bool more = k + 1 < Q.k1;
         if (more) pp.prefetch(Q, k, sz);
+        const P hz = tb.t[SEP_H][2][k];
+        const P cdz = dt * P(0.5) * tb.t[SEP_R][2][k], fz = ffz * tb.t[SEP_RF][2][k];
         if (Q.act) {
-            const int cc = Q.col0 + sz * k;
             const P pcc = pp.c();
-            P dpx = P(0.5) * (P)met.KX(0, cc, i, j, k) * (pp.xp(b) - pp.xm(b));
-            P dpy = P(0.5) * (P)met.KX(1, cc, i, j, k) * (pp.yp(b) - pp.ym(b));
-            P dpz = P(0.5) * (P)met.KX(2, cc, i, j, k) * (pp.zp() - pp.zm());
-            const T a0 = u.c[0][cc], a1 = u.c[1][cc], a2 = u.c[2][cc];
-            u.c[0][cc] = (T)((P)a0 - dt * dpx);
-            u.c[1][cc] = (T)((P)a1 - dt * dpy);
-            u.c[2][cc] = (T)((P)a2 - dt * dpz);
-            P dfx = P(0.5) * ((P)met.G(0, cc, i, j, k) + (P)met.G(0, cc + 1, i + 1, j, k)) * (pp.xp(b) - pcc);
-            P dfy = P(0.5) * ((P)met.G(1, cc, i, j, k) + (P)met.G(1, cc + g.nx, i, j + 1, k)) * (pp.yp(b) - pcc);
-            P dfz = P(0.5) * ((P)met.G(2, cc, i, j, k) + (P)met.G(2, cc + sz, i, j, k + 1)) * (pp.zp() - pcc);
-            const T b0 = uu.c[0][cc], b1 = uu.c[1][cc], b2 = uu.c[2][cc];
-            uu.c[0][cc] = (T)((P)b0 - dt * dfx);
-            uu.c[1][cc] = (T)((P)b1 - dt * dfy);
-            uu.c[2][cc] = (T)((P)b2 - dt * dfz);
+            const T a0 = ld_at((const T *)u.c[0], cb), a1 = ld_at((const T *)u.c[1], cb),
+                    a2 = ld_at((const T *)u.c[2], cb);
+            const T b0 = ld_at((const T *)uu.c[0], cb), b1 = ld_at((const T *)uu.c[1], cb),
+                    b2 = ld_at((const T *)uu.c[2], cb);
+            st_at(u.c[0], cb, (T)((P)a0 - cdx * (pp.xp(b) - pp.xm(b))));
+            st_at(u.c[1], cb, (T)((P)a1 - cdy * (pp.yp(b) - pp.ym(b))));
+            st_at(u.c[2], cb, (T)((P)a2 - cdz * (pp.zp() - pp.zm())));
+            st_at(uu.c[0], cb, (T)((P)b0 - (ffx * hz) * (pp.xp(b) - pcc)));
+            st_at(uu.c[1], cb, (T)((P)b1 - (ffy * hz) * (pp.yp(b) - pcc)));
+            st_at(uu.c[2], cb, (T)((P)b2 - fz * (pp.zp() - pcc)));
         }
         pp.advance(b, more);
         b ^= 1;
@@ -856,29 +881,34 @@ inline long long face_layer_count(const Geom &g) {
 
 template <class T, class P>
 __global__ __launch_bounds__(256) void k_project_layer(Geom g, P dt, Vec3W<T> uu, const P *__restrict__ p,
-                                                       MetSep<T> met) {
+                                                       SepTab<P> tb) {
     int i, j, k;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (!face_layer_cell(g, t, i, j, k)) return;
     const int sy = g.nx, sz = g.nxy;
     const int cc = g.id(i, j, k);
     const P pcc = p[cc];
-    P dfx = P(0.5) * ((P)met.G(0, cc, i, j, k) + (P)met.G(0, cc + 1, i + 1, j, k)) * (p[cc + 1] - pcc);
-    P dfy = P(0.5) * ((P)met.G(1, cc, i, j, k) + (P)met.G(1, cc + sy, i, j + 1, k)) * (p[cc + sy] - pcc);
-    P dfz = P(0.5) * ((P)met.G(2, cc, i, j, k) + (P)met.G(2, cc + sz, i, j, k + 1)) * (p[cc + sz] - pcc);
+    const P hx = tb.t[SEP_H][0][i], hy = tb.t[SEP_H][1][j], hz = tb.t[SEP_H][2][k];
+    const P fx = dt * tb.t[SEP_RF][0][i] * hy * hz;
+    const P fy = dt * tb.t[SEP_RF][1][j] * hx * hz;
+    const P fz = dt * hx * hy * tb.t[SEP_RF][2][k];
     const T b0 = uu.c[0][cc], b1 = uu.c[1][cc], b2 = uu.c[2][cc];
-    uu.c[0][cc] = (T)((P)b0 - dt * dfx);
-    uu.c[1][cc] = (T)((P)b1 - dt * dfy);
-    uu.c[2][cc] = (T)((P)b2 - dt * dfz);
+    uu.c[0][cc] = (T)((P)b0 - fx * (p[cc + 1] - pcc));
+    uu.c[1][cc] = (T)((P)b1 - fy * (p[cc + sy] - pcc));
+    uu.c[2][cc] = (T)((P)b2 - fz * (p[cc + sz] - pcc));
 }
 
-// Face flux on the interior cells (each needs u at c and c+e_d: the x and y
-// neighbours from LDS, the k neighbour from the column registers).
+// Face flux on the interior cells (UtoCU + InterpolateCU,
+// src/dev/FalmCFDDevCall.cu:303-349): uu_d = (ja kx_d u_d)(c)/2 +
+// (ja kx_d u_d)(c+e_d)/2, where ja kx_d is the product of the other two
+// widths and so the same at c and c+e_d.  The x and y neighbours come from
+// LDS, the k neighbour from the column registers.
 template <class T>
-__global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<T> u, Vec3W<T> uu, MetSep<T> met) {
+__global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<T> u, Vec3W<T> uu, SepTab<T> tb) {
     __shared__ K7Lds<T> l0, l1, l2;
     const K7Pos Q = k7_pos(kt, g);
     const int sz = g.nxy;
+    const unsigned szb = (unsigned)sz * sizeof(T);
     Plane7<T> px, py, pz;
     px.setup(u.c[0], &l0, Q, g);
     py.setup(u.c[1], &l1, Q, g);
@@ -888,27 +918,22 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<
     pz.prologue(Q, sz);
     __syncthreads();
     const int i = Q.act ? Q.i : kt.box.lo[0], j = Q.act ? Q.j : kt.box.lo[1];
+    const T hx2 = T(0.5) * tb.t[SEP_H][0][i], hy2 = T(0.5) * tb.t[SEP_H][1][j];
+    const T hxy2 = hx2 * tb.t[SEP_H][1][j];
+    unsigned cb = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szb;
     int b = 0;
-    for (int k = Q.k0; k < Q.k1; k++) {
+    for (int k = Q.k0; k < Q.k1; k++, cb += szb) {
         const bool more = k + 1 < Q.k1;
         if (more) {
             px.prefetch(Q, k, sz);
             py.prefetch(Q, k, sz);
             pz.prefetch(Q, k, sz);
         }
+        const T hz = tb.t[SEP_H][2][k];
         if (Q.act) {
-            const int cc = Q.col0 + sz * k;
-            const T jc = met.JA(cc, i, j, k);
-            const T v[3] = {px.c(), py.c(), pz.c()};
-            const T vn[3] = {px.xp(b), py.yp(b), pz.zp()};
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const int in = i + (d == 0), jn = j + (d == 1), kn = k + (d == 2);
-                const int cn = cc + (d == 0 ? 1 : d == 1 ? g.nx : sz);
-                const T ucc = jc * met.KX(d, cc, i, j, k) * v[d];
-                const T ucn = met.JA(cn, in, jn, kn) * met.KX(d, cn, in, jn, kn) * vn[d];
-                uu.c[d][cc] = T(0.5) * (ucc + ucn);
-            }
+            st_at(uu.c[0], cb, (hy2 * hz) * (px.c() + px.xp(b)));
+            st_at(uu.c[1], cb, (hx2 * hz) * (py.c() + py.yp(b)));
+            st_at(uu.c[2], cb, hxy2 * (pz.c() + pz.zp()));
         }
         px.advance(b, more);
         py.advance(b, more);
@@ -919,30 +944,27 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_u_to_uu_layer(Geom g, Vec3<T> u, Vec3W<T> uu, MetSep<T> met) {
+__global__ __launch_bounds__(256) void k_u_to_uu_layer(Geom g, Vec3<T> u, Vec3W<T> uu, SepTab<T> tb) {
     int i, j, k;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (!face_layer_cell(g, t, i, j, k)) return;
     const int cc = g.id(i, j, k);
-    const int nb[3] = {1, g.nx, g.nxy};
-    const T jc = met.JA(cc, i, j, k);
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const int cn = cc + nb[d];
-        const int in = i + (d == 0), jn = j + (d == 1), kn = k + (d == 2);
-        const T ucc = jc * met.KX(d, cc, i, j, k) * u.c[d][cc];
-        const T ucn = met.JA(cn, in, jn, kn) * met.KX(d, cn, in, jn, kn) * u.c[d][cn];
-        uu.c[d][cc] = T(0.5) * (ucc + ucn);
-    }
+    const T hx = tb.t[SEP_H][0][i], hy = tb.t[SEP_H][1][j], hz = tb.t[SEP_H][2][k];
+    uu.c[0][cc] = (T(0.5) * hy * hz) * (u.c[0][cc] + u.c[0][cc + 1]);
+    uu.c[1][cc] = (T(0.5) * hx * hz) * (u.c[1][cc] + u.c[1][cc + g.nx]);
+    uu.c[2][cc] = (T(0.5) * hx * hy) * (u.c[2][cc] + u.c[2][cc + g.nxy]);
 }
 
-// Smagorinsky on the interior (same expressions as k_smagorinsky).
+// Smagorinsky on the interior (kernel_Cartesian_Smagorinsky,
+// src/dev/FalmCFDDevCall.cu:409-476): nut = (Cs ja^(1/3))^2 |S| with
+// ja^(2/3) the product of the per-axis cbrt(h)^2 table entries.
 template <class T>
 __global__ __launch_bounds__(K7_NT) void k_smagorinsky_km(K7Tiling kt, Geom g, T Cs, Vec3<T> u, T *__restrict__ nut,
-                                                          MetSep<T> met) {
+                                                          SepTab<T> tb) {
     __shared__ K7Lds<T> l0, l1, l2;
     const K7Pos Q = k7_pos(kt, g);
     const int sz = g.nxy;
+    const unsigned szb = (unsigned)sz * sizeof(T);
     Plane7<T> pu[3];
     pu[0].setup(u.c[0], &l0, Q, g);
     pu[1].setup(u.c[1], &l1, Q, g);
@@ -951,31 +973,31 @@ __global__ __launch_bounds__(K7_NT) void k_smagorinsky_km(K7Tiling kt, Geom g, T
     for (int d = 0; d < 3; d++) pu[d].prologue(Q, sz);
     __syncthreads();
     const int i = Q.act ? Q.i : kt.box.lo[0], j = Q.act ? Q.j : kt.box.lo[1];
+    const T hrx = T(0.5) * tb.t[SEP_R][0][i], hry = T(0.5) * tb.t[SEP_R][1][j];
+    const T c2 = Cs * Cs * tb.t[SEP_H23][0][i] * tb.t[SEP_H23][1][j];
+    unsigned cb = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szb;
     int b = 0;
-    for (int k = Q.k0; k < Q.k1; k++) {
+    for (int k = Q.k0; k < Q.k1; k++, cb += szb) {
         const bool more = k + 1 < Q.k1;
         if (more)
 #pragma unroll
             for (int d = 0; d < 3; d++) pu[d].prefetch(Q, k, sz);
+        const T hrz = T(0.5) * tb.t[SEP_R][2][k], h23z = tb.t[SEP_H23][2][k];
         if (Q.act) {
-            const int cc = Q.col0 + sz * k;
-            const T kxx = met.KX(0, cc, i, j, k), kxy = met.KX(1, cc, i, j, k), kxz = met.KX(2, cc, i, j, k);
             T gr[3][3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                gr[d][0] = T(0.5) * kxx * (pu[d].xp(b) - pu[d].xm(b));
-                gr[d][1] = T(0.5) * kxy * (pu[d].yp(b) - pu[d].ym(b));
-                gr[d][2] = T(0.5) * kxz * (pu[d].zp() - pu[d].zm());
+                gr[d][0] = hrx * (pu[d].xp(b) - pu[d].xm(b));
+                gr[d][1] = hry * (pu[d].yp(b) - pu[d].ym(b));
+                gr[d][2] = hrz * (pu[d].zp() - pu[d].zm());
             }
             const T dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
             const T dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
             const T dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
             T d1 = T(2) * dux * dux, d2 = T(2) * dvy * dvy, d3 = T(2) * dwz * dwz;
             T d4 = (duy + dvx) * (duy + dvx), d5 = (dvz + dwy) * (dvz + dwy), d6 = (duz + dwx) * (duz + dwx);
-            T Du = sqrt(d1 + d2 + d3 + d4 + d5 + d6);
-            T De = cbrt(met.JA(cc, i, j, k));
-            T lc = Cs * De;
-            nut[cc] = lc * lc * Du;
+            const T Du = sqrt(d1 + d2 + d3 + d4 + d5 + d6);
+            st_at(nut, cb, (c2 * h23z) * Du);
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) pu[d].advance(b, more);
@@ -1000,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_div_norm(Tiling tl, Geom g, Vec3<T> uu,
         T UE = uu.c[0][cc], UW = uu.c[0][cc - 1];
         T VN = uu.c[1][cc], VS = uu.c[1][cc - g.nx];
         T WT = uu.c[2][cc], WB = uu.c[2][cc - g.nxy];
-        T d = (UE - UW + VN - VS + WT - WB) / met.JA(cc, i, j, k);
+        T d = met.div_ja(UE - UW + VN - VS + WT - WB, cc, i, j, k);
         div[cc] = d;
         v[0] += (double)d * (double)d;
     }
@@ -1066,6 +1088,11 @@ void sep_tables(const double *h, int len, std::vector<double> (&out)[SEP_NTAB]) 
         out[SEP_RF][i] = 0.5 * (r + rp);
         out[SEP_H23][i] = c * c;
     }
+}
+
+template <class T>
+static MetTab<T> met_tab(const SepMesh &sm) {
+    return MetTab<T>{sep_tab<T>(sm)};
 }
 
 template <class T>
@@ -1186,8 +1213,8 @@ static falm_status div_norm_t(const falm_domain *dom, const void *uu, void *div,
     Tiling tl = make_tiling(interior_box(g));
     if (tl.tiles == 0) return FALM_OK;
     TimedLaunch tm(FALM_TIME_DIVERGENCE, s);
-    hipLaunchKernelGGL((k_div_norm<T, MetSep<T>>), red_grid(tl), tiling_block(), 0, s, tl, g, v3<T>(uu, n),
-                       (T *)div, met_sep<T>(sm), partials);
+    hipLaunchKernelGGL((k_div_norm<T, MetTab<T>>), red_grid(tl), tiling_block(), 0, s, tl, g, v3<T>(uu, n),
+                       (T *)div, met_tab<T>(sm), partials);
     tm.done((double)tl.box.size() * 4.0 * sizeof(T));  // uu read, div written
     hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, red_blocks(tl), dev_out);
     FALM_CHECK_LAUNCH("k_div_norm");
@@ -1205,10 +1232,7 @@ static falm_status pseudo_u_km_t(const falm_domain *dom, int scheme, double ReI,
     size_t n = ncell_of(dom);
     KmTiling kt = make_km_tiling(interior_box(g));
     if (kt.blocks == 0) return FALM_OK;
-    if (n * sizeof(T) >= (size_t(1) << 32)) {
-        set_error("pseudo-velocity: a field component exceeds 4 GiB");
-        return FALM_ERR_UNSUPPORTED;
-    }
+    FALM_TRY(check_byte_offsets(n, sizeof(T)));
     TimedLaunch tm(FALM_TIME_PSEUDO_U, s);
 #define FALM_PUK(S)                                                                                        \
     hipLaunchKernelGGL((k_pseudo_u_km<S, T>), dim3(kt.blocks), dim3(KM_NT), 0, s, kt, g, (T)ReI, (T)dt,   \
@@ -1252,11 +1276,12 @@ static falm_status u_to_uu_km_t(const falm_domain *dom, const void *u, void *uu,
     K7Tiling kt = make_k7_tiling(interior_box(g));
     if (kt.blocks == 0) return FALM_OK;
     TimedLaunch tm(FALM_TIME_FACE_FLUX, s);
+    FALM_TRY(check_byte_offsets(n, sizeof(T)));
     hipLaunchKernelGGL((k_u_to_uu_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, v3<T>(u, n), v3w<T>(uu, n),
-                       met_sep<T>(sm));
+                       sep_tab<T>(sm));
     const long long nl = face_layer_count(g);
     hipLaunchKernelGGL((k_u_to_uu_layer<T>), ew_grid((size_t)nl), dim3(EW_BLOCK), 0, s, g, v3<T>(u, n),
-                       v3w<T>(uu, n), met_sep<T>(sm));
+                       v3w<T>(uu, n), sep_tab<T>(sm));
     FALM_CHECK_LAUNCH("k_u_to_uu_km");
     tm.done((double)make_tiling(face_box(g)).box.size() * 6.0 * sizeof(T));
     return FALM_OK;
@@ -1269,8 +1294,8 @@ falm_status sep_u_to_uu(const falm_domain *dom, int prec, const void *u, void *u
 }
 falm_status sep_poisson_rhs(const falm_domain *dom, int prec, const void *uu, double *rhs, const SepMesh &sm,
                             double scale, hipStream_t s) {
-    if (prec == FALM_F32) return divergence_t<float, double, true>(dom, uu, rhs, met_sep<float>(sm), scale, false, s);
-    return divergence_t<double, double, true>(dom, uu, rhs, met_sep<double>(sm), scale, false, s);
+    if (prec == FALM_F32) return divergence_t<float, double, true>(dom, uu, rhs, met_tab<float>(sm), scale, false, s);
+    return divergence_t<double, double, true>(dom, uu, rhs, met_tab<double>(sm), scale, false, s);
 }
 template <class T>
 static falm_status project_km_t(const falm_domain *dom, double dt, void *u, void *uu, const double *p,
@@ -1280,11 +1305,12 @@ static falm_status project_km_t(const falm_domain *dom, double dt, void *u, void
     K7Tiling kt = make_k7_tiling(interior_box(g));
     if (kt.blocks == 0) return FALM_OK;
     TimedLaunch tm(FALM_TIME_PROJECT, s);
+    FALM_TRY(check_byte_offsets(n, sizeof(double)));
     hipLaunchKernelGGL((k_project_km<T, double>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, dt, v3w<T>(u, n),
-                       v3w<T>(uu, n), p, met_sep<T>(sm));
+                       v3w<T>(uu, n), p, sep_tab<double>(sm));
     const long long nl = face_layer_count(g);
     hipLaunchKernelGGL((k_project_layer<T, double>), ew_grid((size_t)nl), dim3(EW_BLOCK), 0, s, g, dt, v3w<T>(uu, n),
-                       p, met_sep<T>(sm));
+                       p, sep_tab<double>(sm));
     FALM_CHECK_LAUNCH("k_project_km");
     // u, uu read and written, p read
     tm.done((double)make_tiling(face_box(g)).box.size() * (12.0 * sizeof(T) + sizeof(double)));
@@ -1304,8 +1330,9 @@ static falm_status smag_km_t(const falm_domain *dom, double Cs, const void *u, v
     K7Tiling kt = make_k7_tiling(interior_box(g));
     if (kt.blocks == 0) return FALM_OK;
     TimedLaunch tm(FALM_TIME_SGS, s);
+    FALM_TRY(check_byte_offsets(n, sizeof(T)));
     hipLaunchKernelGGL((k_smagorinsky_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, (T)Cs, v3<T>(u, n),
-                       (T *)nut, met_sep<T>(sm));
+                       (T *)nut, sep_tab<T>(sm));
     FALM_CHECK_LAUNCH("k_smagorinsky_km");
     tm.done((double)interior_box(g).size() * 4.0 * sizeof(T));  // u read, nut written
     return FALM_OK;

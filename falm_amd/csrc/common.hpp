@@ -193,6 +193,15 @@ __device__ __forceinline__ void st_at(T *p, unsigned off, T v) {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + off) = v;
 }
 
+// the ld_at/st_at kernels address one field column of ncell elements
+inline falm_status check_byte_offsets(size_t ncell, size_t elem) {
+    if (ncell * elem >= (size_t(1) << 32)) {
+        set_error("a field column exceeds 4 GiB (the stencil kernels use 32-bit byte offsets)");
+        return FALM_ERR_UNSUPPORTED;
+    }
+    return FALM_OK;
+}
+
 // 1-D element-wise launches
 constexpr int EW_BLOCK = 256;
 inline dim3 ew_grid(size_t n) {

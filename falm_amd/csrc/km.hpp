@@ -77,14 +77,17 @@ __device__ inline K7Pos k7_pos(const K7Tiling &kt, const Geom &g) {
 template <class T>
 using K7Lds = T[2][K7_BY + 2][K7_BX + 2];
 
-// One stencil array walked through the planes of a K7 column tile.
+// One stencil array walked through the planes of a K7 column tile.  Loads
+// use byte offsets from the array's base (ld_at), so arrays walked with the
+// same column share one address register.
 template <class T>
 struct Plane7 {
     const T *__restrict__ a;
     K7Lds<T> *lds;
     T q0, q1, q2;  // column values at k-1, k, k+1
     T nq, hv;      // prefetched: column at k+2, this thread's halo cell at k+1
-    int hoff, hrow, hcol, lane, wy;
+    unsigned cb, hb;  // byte offsets of the column's and the halo cell's plane-0 cells
+    int hrow, hcol, lane, wy;
     bool hslot, hok;
 
     __device__ inline void setup(const T *arr, K7Lds<T> *l, const K7Pos &P, const Geom &g) {
@@ -105,20 +108,31 @@ struct Plane7 {
         }
         const int gi = P.i0 - 1 + hcol, gj = P.j0 - 1 + hrow;
         hok = hslot && gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny;
-        hoff = hok ? gi + g.nx * gj : 0;
+        cb = (unsigned)P.col0 * sizeof(T);
+        // a halo slot outside the array reads the column's own cell (value unused)
+        hb = hok ? (unsigned)(gi + g.nx * gj) * sizeof(T) : cb;
     }
     // registers for plane k0 and LDS buffer 0 (the caller synchronises)
     __device__ inline void prologue(const K7Pos &P, int sz) {
-        q0 = a[P.col0 + sz * (P.k0 - 1)];
-        q1 = a[P.col0 + sz * P.k0];
-        q2 = a[P.col0 + sz * (P.k0 + 1)];
-        hv = hok ? a[hoff + sz * P.k0] : T(0);
+        const unsigned szb = (unsigned)sz * sizeof(T);
+        q0 = ld_at(a, cb + (unsigned)(P.k0 - 1) * szb);
+        q1 = ld_at(a, cb + (unsigned)P.k0 * szb);
+        q2 = ld_at(a, cb + (unsigned)(P.k0 + 1) * szb);
+        hv = T(0);
+        if (hslot) {  // the halo threads only (the first waves of the workgroup)
+            const T h = ld_at(a, hb + (unsigned)P.k0 * szb);
+            hv = hok ? h : T(0);
+        }
         fill(0, q1);
     }
     // loads for plane k+1 (call only when k+1 < k1)
     __device__ inline void prefetch(const K7Pos &P, int k, int sz) {
-        nq = a[P.col0 + sz * (k + 2)];
-        hv = hok ? a[hoff + sz * (k + 1)] : T(0);
+        const unsigned szb = (unsigned)sz * sizeof(T);
+        nq = ld_at(a, cb + (unsigned)(k + 2) * szb);
+        if (hslot) {
+            const T h = ld_at(a, hb + (unsigned)(k + 1) * szb);
+            hv = hok ? h : T(0);
+        }
     }
     __device__ inline void fill(int b, T centre) {
         lds[0][b][wy + 1][lane + 1] = centre;

@@ -46,6 +46,13 @@ struct OpCols {
         ac = c[0][cc]; aw = c[1][cc]; ae = c[2][cc]; as = c[3][cc]; an = c[4][cc]; ab = c[5][cc]; at = c[6][cc];
     }
     __device__ inline TA diag(int cc) const { return c[0][cc]; }
+    // the same at a byte offset ob of the cell (ld_at addressing)
+    __device__ inline void get_at(unsigned ob, int, int, int, TA &ac, TA &aw, TA &ae, TA &as, TA &an, TA &ab,
+                                  TA &at) const {
+        ac = ld_at(c[0], ob); aw = ld_at(c[1], ob); ae = ld_at(c[2], ob); as = ld_at(c[3], ob);
+        an = ld_at(c[4], ob); ab = ld_at(c[5], ob); at = ld_at(c[6], ob);
+    }
+    __device__ inline TA diag_at(unsigned ob) const { return ld_at(c[0], ob); }
 };
 
 template <class TA>
@@ -58,6 +65,11 @@ struct OpSep {
         c0 = ac[cc]; w = aw[i]; e = ae[i]; s = as[j]; n = an[j]; b = ab[k]; t = at[k];
     }
     __device__ inline TA diag(int cc) const { return ac[cc]; }
+    __device__ inline void get_at(unsigned ob, int i, int j, int k, TA &c0, TA &w, TA &e, TA &s, TA &n, TA &b,
+                                  TA &t) const {
+        c0 = ld_at(ac, ob); w = aw[i]; e = ae[i]; s = as[j]; n = an[j]; b = ab[k]; t = at[k];
+    }
+    __device__ inline TA diag_at(unsigned ob) const { return ld_at(ac, ob); }
 };
 
 template <class OP, class TA, class TX, class R>
@@ -74,6 +86,15 @@ template <class OP, class TA, class R>
 __device__ inline R apply_vals(const OP &op, int cc, int i, int j, int k, R xc, R xe, R xw, R xn, R xs, R xt, R xb) {
     TA ac, aw, ae, as, an, ab, at;
     op.get(cc, i, j, k, ac, aw, ae, as, an, ab, at);
+    return (R)ac * xc + (R)ae * xe + (R)aw * xw + (R)an * xn + (R)as * xs + (R)at * xt + (R)ab * xb;
+}
+
+// apply_vals with the operator's per-cell columns at byte offset ob
+template <class OP, class TA, class R>
+__device__ inline R apply_vals_at(const OP &op, unsigned ob, int i, int j, int k, R xc, R xe, R xw, R xn, R xs, R xt,
+                                  R xb) {
+    TA ac, aw, ae, as, an, ab, at;
+    op.get_at(ob, i, j, k, ac, aw, ae, as, an, ab, at);
     return (R)ac * xc + (R)ae * xe + (R)aw * xw + (R)an * xn + (R)as * xs + (R)at * xt + (R)ab * xb;
 }
 

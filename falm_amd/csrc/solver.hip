@@ -118,17 +118,19 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_init_km(K7Tiling kt, Geom g, OP 
     __syncthreads();
     double v[2] = {0.0, 0.0};
     int bf = 0;
-    for (int k = Q.k0; k < Q.k1; k++) {
+    const unsigned szd = (unsigned)sz * 8u, szt = (unsigned)sz * sizeof(T);
+    unsigned od = (unsigned)Q.col0 * 8u + (unsigned)Q.k0 * szd, ot = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szt;
+    for (int k = Q.k0; k < Q.k1; k++, od += szd, ot += szt) {
         const bool more = k + 1 < Q.k1;
         if (more) px.prefetch(Q, k, sz);
         if (Q.act) {
-            const int cc = Q.col0 + sz * k;
-            const double rv = b[cc] - apply_vals<OP, double, double>(a, cc, Q.i, Q.j, k, px.c(), px.xp(bf), px.xm(bf),
-                                                                       px.yp(bf), px.ym(bf), px.zp(), px.zm());
-            if (r64) r64[cc] = rv;
+            const double rv = ld_at(b, od) - apply_vals_at<OP, double, double>(a, od, Q.i, Q.j, k, px.c(), px.xp(bf),
+                                                                              px.xm(bf), px.yp(bf), px.ym(bf),
+                                                                              px.zp(), px.zm());
+            if (r64) st_at(r64, od, rv);
             const T rt = (T)rv;
-            if ((void *)r != (void *)r64) r[cc] = rt;
-            rr[cc] = rt;
+            if ((void *)r != (void *)r64) st_at(r, ot, rt);
+            st_at(rr, ot, rt);
             v[0] += rv * rv;
             v[1] += (double)rt * (double)rt;
         }
@@ -151,16 +153,19 @@ __global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, 
     px.prologue(Q, sz);
     __syncthreads();
     int bf = 0;
-    for (int k = Q.k0; k < Q.k1; k++) {
+    const unsigned szt = (unsigned)sz * sizeof(T);
+    unsigned ot = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szt;
+    for (int k = Q.k0; k < Q.k1; k++, ot += szt) {
         const bool more = k + 1 < Q.k1;
         if (more) px.prefetch(Q, k, sz);
         if (Q.act && ((Q.i + Q.j + k + g.osum) & 1) == color) {
-            const int cc = Q.col0 + sz * k;
             const T xc = px.c();
-            const T s = apply_vals<OP, T, T>(a, cc, Q.i, Q.j, k, xc, px.xp(bf), px.xm(bf), px.yp(bf), px.ym(bf),
-                                              px.zp(), px.zm());
-            const T cv = (b[cc] - s) / a.diag(cc);
-            x[cc] = xc + omega * cv;
+            T ac, aw, ae, as, an, ab, at;
+            a.get_at(ot, Q.i, Q.j, k, ac, aw, ae, as, an, ab, at);
+            const T s = ac * xc + ae * px.xp(bf) + aw * px.xm(bf) + an * px.yp(bf) + as * px.ym(bf) + at * px.zp() +
+                        ab * px.zm();
+            const T cv = (ld_at(b, ot) - s) / ac;
+            st_at(x, ot, xc + omega * cv);
         }
         px.advance(bf, more);
         bf ^= 1;
@@ -184,18 +189,20 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_mv_km(K7Tiling kt, Geom g, OP a,
     __syncthreads();
     double v[2] = {0.0, 0.0};
     int bf = 0;
-    for (int k = Q.k0; k < Q.k1; k++) {
+    const unsigned szt = (unsigned)sz * sizeof(T);
+    unsigned ot = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szt;
+    for (int k = Q.k0; k < Q.k1; k++, ot += szt) {
         const bool more = k + 1 < Q.k1;
         if (more) px.prefetch(Q, k, sz);
         if (Q.act) {
-            const int cc = Q.col0 + sz * k;
-            const T qv = apply_vals<OP, T, T>(a, cc, Q.i, Q.j, k, px.c(), px.xp(bf), px.xm(bf), px.yp(bf), px.ym(bf),
-                                               px.zp(), px.zm());
-            out[cc] = qv;
+            const T qv = apply_vals_at<OP, T, T>(a, ot, Q.i, Q.j, k, px.c(), px.xp(bf), px.xm(bf), px.yp(bf),
+                                                  px.ym(bf), px.zp(), px.zm());
+            st_at(out, ot, qv);
+            const T wv = ld_at(w, ot);
             if (MODE == 1) {
-                v[0] += (double)w[cc] * (double)qv;
+                v[0] += (double)wv * (double)qv;
             } else {
-                v[0] += (double)qv * (double)w[cc];
+                v[0] += (double)qv * (double)wv;
                 v[1] += (double)qv * (double)qv;
             }
         }
