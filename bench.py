@@ -292,7 +292,7 @@ def main():
     ap.add_argument("--adj-window", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
     ap.add_argument("--adj-dirs", type=int, default=4)
-    ap.add_argument("--adj-tol", type=float, default=1e-10)
+    ap.add_argument("--adj-tol", type=float, default=1e-12)
     ap.add_argument("--adj-cfl", type=float, default=0.1)
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="mixed-vs-FP64 error after this many steps (0: skip)")
