@@ -901,45 +901,29 @@ __global__ __launch_bounds__(256) void k_project_layer(Geom g, P dt, Vec3W<T> uu
 // Face flux on the interior cells (UtoCU + InterpolateCU,
 // src/dev/FalmCFDDevCall.cu:303-349): uu_d = (ja kx_d u_d)(c)/2 +
 // (ja kx_d u_d)(c+e_d)/2, where ja kx_d is the product of the other two
-// widths and so the same at c and c+e_d.  The x and y neighbours come from
-// LDS, the k neighbour from the column registers.
+// widths and so the same at c and c+e_d.  Each component needs one
+// neighbour only: u_x(c+1) and u_y(c+sy) are loaded directly (the same or
+// the next cache lines of the wavefront's row), u_z(c+sz) is the next
+// plane's value of the column, carried in a register.  No LDS.
 template <class T>
 __global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<T> u, Vec3W<T> uu, SepTab<T> tb) {
-    __shared__ K7Lds<T> l0, l1, l2;
     const K7Pos Q = k7_pos(kt, g);
+    if (!Q.act) return;  // no LDS, no barriers: inactive threads just leave
     const int sz = g.nxy;
-    const unsigned szb = (unsigned)sz * sizeof(T);
-    Plane7<T> px, py, pz;
-    px.setup(u.c[0], &l0, Q, g);
-    py.setup(u.c[1], &l1, Q, g);
-    pz.setup(u.c[2], &l2, Q, g);
-    px.prologue(Q, sz);
-    py.prologue(Q, sz);
-    pz.prologue(Q, sz);
-    __syncthreads();
-    const int i = Q.act ? Q.i : kt.box.lo[0], j = Q.act ? Q.j : kt.box.lo[1];
-    const T hx2 = T(0.5) * tb.t[SEP_H][0][i], hy2 = T(0.5) * tb.t[SEP_H][1][j];
-    const T hxy2 = hx2 * tb.t[SEP_H][1][j];
+    const unsigned szb = (unsigned)sz * sizeof(T), syb = (unsigned)g.nx * sizeof(T);
+    const T hx2 = T(0.5) * tb.t[SEP_H][0][Q.i], hy2 = T(0.5) * tb.t[SEP_H][1][Q.j];
+    const T hxy2 = hx2 * tb.t[SEP_H][1][Q.j];
     unsigned cb = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szb;
-    int b = 0;
+    T wz = ld_at(u.c[2], cb);
     for (int k = Q.k0; k < Q.k1; k++, cb += szb) {
-        const bool more = k + 1 < Q.k1;
-        if (more) {
-            px.prefetch(Q, k, sz);
-            py.prefetch(Q, k, sz);
-            pz.prefetch(Q, k, sz);
-        }
+        const T ux = ld_at(u.c[0], cb), uxe = ld_at(u.c[0], cb + (unsigned)sizeof(T));
+        const T vy = ld_at(u.c[1], cb), vyn = ld_at(u.c[1], cb + syb);
+        const T wzt = ld_at(u.c[2], cb + szb);
         const T hz = tb.t[SEP_H][2][k];
-        if (Q.act) {
-            st_at(uu.c[0], cb, (hy2 * hz) * (px.c() + px.xp(b)));
-            st_at(uu.c[1], cb, (hx2 * hz) * (py.c() + py.yp(b)));
-            st_at(uu.c[2], cb, hxy2 * (pz.c() + pz.zp()));
-        }
-        px.advance(b, more);
-        py.advance(b, more);
-        pz.advance(b, more);
-        b ^= 1;
-        __syncthreads();
+        st_at(uu.c[0], cb, (hy2 * hz) * (ux + uxe));
+        st_at(uu.c[1], cb, (hx2 * hz) * (vy + vyn));
+        st_at(uu.c[2], cb, hxy2 * (wz + wzt));
+        wz = wzt;
     }
 }
 
@@ -1027,6 +1011,34 @@ __global__ __launch_bounds__(256) void k_div_norm(Tiling tl, Geom g, Vec3<T> uu,
         v[0] += (double)d * (double)d;
     }
     block_partials<256, 1>(v, partials, lds);
+}
+
+// The same on K7 column tiles (step driver): uu_x(c-1) and uu_y(c-sy) are
+// loaded directly, uu_z(c-sz) is the previous plane's value of the column;
+// one partial per workgroup.
+template <class T>
+__global__ __launch_bounds__(K7_NT) void k_div_norm_km(K7Tiling kt, Geom g, Vec3<T> uu, T *__restrict__ div,
+                                                       SepTab<T> tb, double *partials) {
+    __shared__ double lds[K7_NT / 64];
+    const K7Pos Q = k7_pos(kt, g);
+    double v[1] = {0.0};
+    if (Q.act) {
+        const int sz = g.nxy;
+        const unsigned szb = (unsigned)sz * sizeof(T), syb = (unsigned)g.nx * sizeof(T);
+        const T rxy = tb.t[SEP_R][0][Q.i] * tb.t[SEP_R][1][Q.j];
+        unsigned cb = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szb;
+        T wb = ld_at(uu.c[2], cb - szb);
+        for (int k = Q.k0; k < Q.k1; k++, cb += szb) {
+            const T UE = ld_at(uu.c[0], cb), UW = ld_at(uu.c[0], cb - (unsigned)sizeof(T));
+            const T VN = ld_at(uu.c[1], cb), VS = ld_at(uu.c[1], cb - syb);
+            const T WT = ld_at(uu.c[2], cb);
+            const T d = (UE - UW + VN - VS + WT - wb) * (rxy * tb.t[SEP_R][2][k]);
+            st_at(div, cb, d);
+            v[0] += (double)d * (double)d;
+            wb = WT;
+        }
+    }
+    block_partials<K7_NT, 1>(v, partials, lds);
 }
 
 __global__ __launch_bounds__(FIN_NT) void k_fin_norm(const double *partials, int nb, double *out) {
@@ -1210,13 +1222,14 @@ static falm_status div_norm_t(const falm_domain *dom, const void *uu, void *div,
                               double *dev_out, double *partials, hipStream_t s) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
-    Tiling tl = make_tiling(interior_box(g));
-    if (tl.tiles == 0) return FALM_OK;
+    K7Tiling kt = make_k7_tiling(interior_box(g));
+    if (kt.blocks == 0) return FALM_OK;
+    FALM_TRY(check_byte_offsets(n, sizeof(T)));
     TimedLaunch tm(FALM_TIME_DIVERGENCE, s);
-    hipLaunchKernelGGL((k_div_norm<T, MetTab<T>>), red_grid(tl), tiling_block(), 0, s, tl, g, v3<T>(uu, n),
-                       (T *)div, met_tab<T>(sm), partials);
-    tm.done((double)tl.box.size() * 4.0 * sizeof(T));  // uu read, div written
-    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, red_blocks(tl), dev_out);
+    hipLaunchKernelGGL((k_div_norm_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, v3<T>(uu, n), (T *)div,
+                       sep_tab<T>(sm), partials);
+    tm.done((double)interior_box(g).size() * 4.0 * sizeof(T));  // uu read, div written
+    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out);
     FALM_CHECK_LAUNCH("k_div_norm");
     return FALM_OK;
 }
