@@ -284,7 +284,7 @@ def main():
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--fam-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=15)
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--adjoint", action="store_true",
                     help="rotor-power gradient report (256x96x96, D/24) instead of the step benchmark")
