@@ -119,24 +119,19 @@ struct Plane7 {
         q1 = ld_at(a, cb + (unsigned)P.k0 * szb);
         q2 = ld_at(a, cb + (unsigned)(P.k0 + 1) * szb);
         hv = T(0);
-        if (hslot) {  // the halo threads only (the first waves of the workgroup)
-            const T h = ld_at(a, hb + (unsigned)P.k0 * szb);
-            hv = hok ? h : T(0);
-        }
+        if (hslot) hv = ld_at(a, hb + (unsigned)P.k0 * szb);  // the halo threads only (the first waves)
         fill(0, q1);
     }
-    // loads for plane k+1 (call only when k+1 < k1)
+    // loads for plane k+1 (call only when k+1 < k1).  The raw values are kept
+    // and masked only when they go to LDS, so nothing here waits for a load.
     __device__ inline void prefetch(const K7Pos &P, int k, int sz) {
         const unsigned szb = (unsigned)sz * sizeof(T);
         nq = ld_at(a, cb + (unsigned)(k + 2) * szb);
-        if (hslot) {
-            const T h = ld_at(a, hb + (unsigned)(k + 1) * szb);
-            hv = hok ? h : T(0);
-        }
+        if (hslot) hv = ld_at(a, hb + (unsigned)(k + 1) * szb);
     }
     __device__ inline void fill(int b, T centre) {
         lds[0][b][wy + 1][lane + 1] = centre;
-        if (hslot) lds[0][b][hrow][hcol] = hv;
+        if (hslot) lds[0][b][hrow][hcol] = hok ? hv : T(0);
     }
     // after plane k: LDS for plane k+1 and the register shift
     __device__ inline void advance(int b, bool more) {

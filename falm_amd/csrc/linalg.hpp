@@ -53,6 +53,48 @@ struct OpCols {
         an = ld_at(c[4], ob); ab = ld_at(c[5], ob); at = ld_at(c[6], ob);
     }
     __device__ inline TA diag_at(unsigned ob) const { return ld_at(c[0], ob); }
+    // the view a k-marching kernel keeps for one column (nothing to hoist)
+    __device__ inline OpCols column(int, int) const { return *this; }
+    // a cell's coefficients, loaded ahead of use (k-marching prefetch)
+    struct Cell {
+        TA v[7];
+    };
+    __device__ inline Cell load(unsigned ob, int) const {
+        Cell q;
+#pragma unroll
+        for (int m = 0; m < 7; m++) q.v[m] = ld_at(c[m], ob);
+        return q;
+    }
+    template <class R>
+    __device__ inline R apply(const Cell &q, R xc, R xe, R xw, R xn, R xs, R xt, R xb) const {
+        return (R)q.v[0] * xc + (R)q.v[2] * xe + (R)q.v[1] * xw + (R)q.v[4] * xn + (R)q.v[3] * xs + (R)q.v[6] * xt +
+               (R)q.v[5] * xb;
+    }
+    __device__ inline static TA diag(const Cell &q) { return q.v[0]; }
+};
+
+// One column (i, j) of a separable operator: the four in-plane coefficients
+// in registers, loaded once before a kernel walks the column's planes.
+template <class TA>
+struct OpSepCol {
+    TA w, e, s, n;
+    const TA *__restrict__ ac;
+    const TA *__restrict__ ab, *__restrict__ at;
+    __device__ inline void get_at(unsigned ob, int, int, int k, TA &c0, TA &w_, TA &e_, TA &s_, TA &n_, TA &b_,
+                                  TA &t_) const {
+        c0 = ld_at(ac, ob); w_ = w; e_ = e; s_ = s; n_ = n; b_ = ab[k]; t_ = at[k];
+    }
+    __device__ inline TA diag_at(unsigned ob) const { return ld_at(ac, ob); }
+    struct Cell {
+        TA c0, b, t;
+    };
+    __device__ inline Cell load(unsigned ob, int k) const { return Cell{ld_at(ac, ob), ab[k], at[k]}; }
+    // the reference's summation order (src/dev/MVDevCall.cu:39)
+    template <class R>
+    __device__ inline R apply(const Cell &q, R xc, R xe, R xw, R xn, R xs, R xt, R xb) const {
+        return (R)q.c0 * xc + (R)e * xe + (R)w * xw + (R)n * xn + (R)s * xs + (R)q.t * xt + (R)q.b * xb;
+    }
+    __device__ inline static TA diag(const Cell &q) { return q.c0; }
 };
 
 template <class TA>
@@ -70,6 +112,13 @@ struct OpSep {
         c0 = ld_at(ac, ob); w = aw[i]; e = ae[i]; s = as[j]; n = an[j]; b = ab[k]; t = at[k];
     }
     __device__ inline TA diag_at(unsigned ob) const { return ld_at(ac, ob); }
+    // i, j must lie in the array (callers clamp inactive threads)
+    __device__ inline OpSepCol<TA> column(int i, int j) const {
+        OpSepCol<TA> c;
+        c.w = aw[i]; c.e = ae[i]; c.s = as[j]; c.n = an[j];
+        c.ac = ac; c.ab = ab; c.at = at;
+        return c;
+    }
 };
 
 template <class OP, class TA, class TX, class R>

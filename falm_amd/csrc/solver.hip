@@ -116,18 +116,26 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_init_km(K7Tiling kt, Geom g, OP 
     Plane7<double> px;
     px.setup(x, &lx, Q, g);
     px.prologue(Q, sz);
-    __syncthreads();
     double v[2] = {0.0, 0.0};
     int bf = 0;
+    const auto ca = a.column(Q.act ? Q.i : kt.box.lo[0], Q.act ? Q.j : kt.box.lo[1]);
     const unsigned szd = (unsigned)sz * 8u, szt = (unsigned)sz * sizeof(T);
     unsigned od = (unsigned)Q.col0 * 8u + (unsigned)Q.k0 * szd, ot = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szt;
+    // b and the operator's cell coefficients are loaded a plane ahead too
+    auto cq = ca.load(od, Q.k0);
+    double bq = ld_at(b, od);
+    __syncthreads();
     for (int k = Q.k0; k < Q.k1; k++, od += szd, ot += szt) {
         const bool more = k + 1 < Q.k1;
-        if (more) px.prefetch(Q, k, sz);
+        const auto cc = cq;
+        const double bc = bq;
+        if (more) {
+            px.prefetch(Q, k, sz);
+            cq = ca.load(od + szd, k + 1);
+            bq = ld_at(b, od + szd);
+        }
         if (Q.act) {
-            const double rv = ld_at(b, od) - apply_vals_at<OP, double, double>(a, od, Q.i, Q.j, k, px.c(), px.xp(bf),
-                                                                              px.xm(bf), px.yp(bf), px.ym(bf),
-                                                                              px.zp(), px.zm());
+            const double rv = bc - ca.apply(cc, px.c(), px.xp(bf), px.xm(bf), px.yp(bf), px.ym(bf), px.zp(), px.zm());
             if (r64) st_at(r64, od, rv);
             const T rt = (T)rv;
             if ((void *)r != (void *)r64) st_at(r, ot, rt);
@@ -152,20 +160,27 @@ __global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, 
     Plane7<T> px;
     px.setup(x, &lx, Q, g);
     px.prologue(Q, sz);
-    __syncthreads();
     int bf = 0;
+    const auto ca = a.column(Q.act ? Q.i : kt.box.lo[0], Q.act ? Q.j : kt.box.lo[1]);
     const unsigned szt = (unsigned)sz * sizeof(T);
     unsigned ot = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szt;
+    auto cq = ca.load(ot, Q.k0);
+    T bq = ld_at(b, ot);
+    __syncthreads();
     for (int k = Q.k0; k < Q.k1; k++, ot += szt) {
         const bool more = k + 1 < Q.k1;
-        if (more) px.prefetch(Q, k, sz);
+        const auto cc = cq;
+        const T bc = bq;
+        if (more) {
+            px.prefetch(Q, k, sz);
+            cq = ca.load(ot + szt, k + 1);
+            bq = ld_at(b, ot + szt);
+        }
         if (Q.act && ((Q.i + Q.j + k + g.osum) & 1) == color) {
             const T xc = px.c();
-            T ac, aw, ae, as, an, ab, at;
-            a.get_at(ot, Q.i, Q.j, k, ac, aw, ae, as, an, ab, at);
-            const T s = ac * xc + ae * px.xp(bf) + aw * px.xm(bf) + an * px.yp(bf) + as * px.ym(bf) + at * px.zp() +
-                        ab * px.zm();
-            const T cv = (ld_at(b, ot) - s) / ac;
+            // kernel_Struct3d7p_SOR's update (src/dev/FalmEqDevCall.cu:195-215)
+            const T s = ca.apply(cc, xc, px.xp(bf), px.xm(bf), px.yp(bf), px.ym(bf), px.zp(), px.zm());
+            const T cv = (bc - s) / decltype(ca)::diag(cc);
             st_at(x, ot, xc + omega * cv);
         }
         px.advance(bf, more);
@@ -401,19 +416,26 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_mv_km(K7Tiling kt, Geom g, OP a,
     Plane7<T> px;
     px.setup(xin, &lx, Q, g);
     px.prologue(Q, sz);
-    __syncthreads();
     double v[2] = {0.0, 0.0};
     int bf = 0;
+    const auto ca = a.column(Q.act ? Q.i : kt.box.lo[0], Q.act ? Q.j : kt.box.lo[1]);
     const unsigned szt = (unsigned)sz * sizeof(T);
     unsigned ot = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szt;
+    auto cq = ca.load(ot, Q.k0);
+    T wq = ld_at(w, ot);
+    __syncthreads();
     for (int k = Q.k0; k < Q.k1; k++, ot += szt) {
         const bool more = k + 1 < Q.k1;
-        if (more) px.prefetch(Q, k, sz);
+        const auto cc = cq;
+        const T wv = wq;
+        if (more) {
+            px.prefetch(Q, k, sz);
+            cq = ca.load(ot + szt, k + 1);
+            wq = ld_at(w, ot + szt);
+        }
         if (Q.act) {
-            const T qv = apply_vals_at<OP, T, T>(a, ot, Q.i, Q.j, k, px.c(), px.xp(bf), px.xm(bf), px.yp(bf),
-                                                  px.ym(bf), px.zp(), px.zm());
+            const T qv = ca.apply(cc, px.c(), px.xp(bf), px.xm(bf), px.yp(bf), px.ym(bf), px.zp(), px.zm());
             st_at(out, ot, qv);
-            const T wv = ld_at(w, ot);
             if (MODE == 1) {
                 v[0] += (double)wv * (double)qv;
             } else {
