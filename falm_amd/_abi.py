@@ -110,7 +110,7 @@ class CaseConfig(C.Structure):
                 ("u_inflow", C.c_double), ("lid_velocity", C.c_double),
                 ("inlet_intensity", C.c_double), ("seed", C.c_uint64), ("zminus_slip", C.c_int),
                 ("precision", C.c_int), ("solver", SolverConfig),
-                ("alm", C.POINTER(AlmConfig))]
+                ("alm", C.POINTER(AlmConfig)), ("cv", C.POINTER(C.c_double) * 6)]
 
 
 class StepInfo(C.Structure):

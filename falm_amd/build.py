@@ -18,7 +18,9 @@ LIB = os.path.join(LIBDIR, "libfalm_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FALM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["runtime.hip", "cfd.hip", "linalg.hip", "solver.hip", "bc.hip", "alm.hip", "case.hip", "adjoint.hip"]
+SOURCES = ["runtime.hip", "cfd.hip", "linalg.hip", "solver.hip", "bc.hip", "alm.hip", "case.hip", "adjoint.hip",
+           "mesher.cpp"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
@@ -27,7 +29,10 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno
 
 def _compile(src, verbose):
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".cpp"):  # host-only code
+        cmd = ["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -57,7 +62,7 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     return LIB
 
 
-EXAMPLES = {"falm_windtunnel": "windtunnel.cpp"}
+EXAMPLES = {"falm_windtunnel": "windtunnel.cpp", "falm_nedo": "nedo.cpp"}
 
 
 def build_examples(verbose: bool = False) -> None:

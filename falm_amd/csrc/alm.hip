@@ -15,6 +15,7 @@
 //   order; a finite cutoff (radius in units of eps) culls harder.
 // k_alm_totals: per-turbine torque/thrust in a fixed order (the reference
 //   uses thrust::reduce, src/alm/almDevCall.cu:362-370), kept on device.
+#include <cstddef>
 #include <cmath>
 #include <vector>
 
@@ -25,7 +26,8 @@ namespace falm {
 constexpr double kPi = 3.14159265358979323846;  // Falm::Pi (src/falmath.h:10)
 
 struct TurbDev {
-    double base[3], hub[3], tip_rate, motion[3], surge[3], pitch_deg[8];
+    // bvel: base velocity set with falm_alm_set_base (0 unless set)
+    double base[3], bvel[3], hub[3], tip_rate, motion[3], surge[3], pitch_deg[8];
     int angle_type;
 };
 
@@ -168,10 +170,10 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
         ang[T0.angle_type - 1] = T0.motion[0] * sin(T0.motion[1] * t + T0.motion[2]);
         om[T0.angle_type - 1] = T0.motion[0] * T0.motion[1] * cos(T0.motion[1] * t + T0.motion[2]);
     }
-    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {0, 0, 0};
+    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {T0.bvel[0], T0.bvel[1], T0.bvel[2]};
     if (T0.surge[0] != 0.0) {
         base[0] = T0.base[0] + T0.surge[0] * sin(T0.surge[1] * t + T0.surge[2]);
-        bvel[0] = T0.surge[0] * T0.surge[1] * cos(T0.surge[1] * t + T0.surge[2]);
+        bvel[0] = T0.bvel[0] + T0.surge[0] * T0.surge[1] * cos(T0.surge[1] * t + T0.surge[2]);
     }
     const double nang[3] = {-ang[0], -ang[1], -ang[2]};
     // kernel_UpdateAPX
@@ -618,10 +620,10 @@ __global__ void k_alm_adj_points(AlmDev A, const double *u0, const double *u1, c
         ang[T0.angle_type - 1] = T0.motion[0] * sin(T0.motion[1] * t + T0.motion[2]);
         om[T0.angle_type - 1] = T0.motion[0] * T0.motion[1] * cos(T0.motion[1] * t + T0.motion[2]);
     }
-    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {0, 0, 0};
+    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {T0.bvel[0], T0.bvel[1], T0.bvel[2]};
     if (T0.surge[0] != 0.0) {
         base[0] = T0.base[0] + T0.surge[0] * sin(T0.surge[1] * t + T0.surge[2]);
-        bvel[0] = T0.surge[0] * T0.surge[1] * cos(T0.surge[1] * t + T0.surge[2]);
+        bvel[0] = T0.bvel[0] + T0.surge[0] * T0.surge[1] * cos(T0.surge[1] * t + T0.surge[2]);
     }
     const double nang[3] = {-ang[0], -ang[1], -ang[2]};
     const double tip = T0.tip_rate;
@@ -883,6 +885,7 @@ falm_status falm_alm_create(const falm_domain *dom, const falm_alm_config *c, co
         const falm_turbine &T = c->turbines[t];
         for (int m = 0; m < 3; m++) {
             tb[t].base[m] = T.base[m];
+            tb[t].bvel[m] = 0.0;
             tb[t].hub[m] = T.hub[m];
             tb[t].motion[m] = T.motion[m];
             tb[t].surge[m] = T.surge[m];
@@ -960,6 +963,19 @@ falm_status falm_alm_set_pitch(falm_alm *A, int turbine, const double *pitch_deg
         for (int b = 0; b < 8; b++) v[b] = pitch_deg[b];
     TurbDev *dst = const_cast<TurbDev *>(A->d.tb) + turbine;
     FALM_HIP(hipMemcpy(dst->pitch_deg, v, sizeof(v), hipMemcpyHostToDevice));
+    return FALM_OK;
+}
+
+falm_status falm_alm_set_base(falm_alm *A, int turbine, const double *base, const double *velocity, void *stream) {
+    if (!A || turbine < 0 || turbine >= A->nt || !base) return FALM_ERR_INVALID;
+    double v[6] = {base[0], base[1], base[2], 0.0, 0.0, 0.0};
+    if (velocity)
+        for (int m = 0; m < 3; m++) v[3 + m] = velocity[m];
+    // the step kernels read the turbine record: let the stream drain first
+    FALM_HIP(hipStreamSynchronize((hipStream_t)stream));
+    TurbDev *dst = const_cast<TurbDev *>(A->d.tb) + turbine;
+    static_assert(offsetof(TurbDev, bvel) == offsetof(TurbDev, base) + 3 * sizeof(double), "base, bvel adjacent");
+    FALM_HIP(hipMemcpy(dst->base, v, sizeof(v), hipMemcpyHostToDevice));
     return FALM_OK;
 }
 

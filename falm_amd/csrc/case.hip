@@ -142,7 +142,7 @@ static falm_status case_init(falm_case *C) {
     hipStream_t s = C->stream;
     const int gc = 2;
     for (int m = 0; m < 3; m++) {
-        if (c.n[m] < 1 || !(c.h[m] > 0.0)) {
+        if (c.n[m] < 1 || (!c.cv[2 * m] && !(c.h[m] > 0.0)) || (!c.cv[2 * m]) != (!c.cv[2 * m + 1])) {
             set_error("falm_case_create: bad grid");
             return FALM_ERR_INVALID;
         }
@@ -194,8 +194,12 @@ static falm_status case_init(falm_case *C) {
         cen[m]->resize(C->dom.shape[m]);
         wid[m]->resize(C->dom.shape[m]);
         for (int i = 0; i < C->dom.shape[m]; i++) {
-            (*cen[m])[i] = c.origin[m] + (i - gc + 0.5) * c.h[m];
-            (*wid[m])[i] = c.h[m];
+            (*cen[m])[i] = c.cv[2 * m] ? c.cv[2 * m][i] : c.origin[m] + (i - gc + 0.5) * c.h[m];
+            (*wid[m])[i] = c.cv[2 * m] ? c.cv[2 * m + 1][i] : c.h[m];
+            if (!((*wid[m])[i] > 0.0)) {
+                set_error("falm_case_create: control-volume widths must be positive");
+                return FALM_ERR_INVALID;
+            }
         }
     }
     FALM_TRY(falm_metrics(&C->dom, C->prec, C->x.data(), C->y.data(), C->z.data(), C->hx.data(), C->hy.data(),
@@ -361,6 +365,7 @@ falm_status falm_case_create(const falm_case_config *cfg, falm_case **out) {
     }
     falm_status st = case_init(C);
     C->cfg.alm = nullptr;
+    for (int q = 0; q < 6; q++) C->cfg.cv[q] = nullptr;
     if (st != FALM_OK) {
         falm_case_destroy(C);
         return st;

@@ -75,6 +75,8 @@ def _sigs(L):
         "falm_case_window": [vp, ip, C.POINTER(dp)],
         "falm_case_adjoint": [vp, ip, ip, dp, C.POINTER(dp), vp, vp, C.POINTER(C.c_size_t)],
         "falm_alm_set_pitch": [vp, ip, vp],
+        "falm_alm_set_base": [vp, ip, vp, vp, vp],
+        "falm_mesher_build": [C.c_char_p, C.c_char_p, C.c_char_p, ip],
         "falm_timing_enable": [ip],
         "falm_timing_read": [C.POINTER(ip), C.POINTER(dp), C.POINTER(dp)],
     }
@@ -224,6 +226,14 @@ class Case:
         v = np.zeros(8)
         v[:len(pitch_deg)] = pitch_deg
         call("falm_alm_set_pitch", self._alm_handle(), turbine, v.ctypes.data_as(C.c_void_p))
+
+    def set_base(self, turbine, base, velocity=None):
+        """Base position and velocity of one turbine from the next step on."""
+        import numpy as np
+        b = np.ascontiguousarray(base, dtype=np.float64)
+        v = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float64)
+        call("falm_alm_set_base", self._alm_handle(), turbine, b.ctypes.data_as(C.c_void_p),
+             None if v is None else v.ctypes.data_as(C.c_void_p), None)
 
     def set_twist_offset(self, off):
         """Per-section twist offset in degrees ([n_ap_per_blade])."""

@@ -1,0 +1,380 @@
+// Falm C++ host API: FalmCore (src/falm.h:16-483), the set-up and output
+// driver of the reference's applications, over the C ABI.
+//
+//   parse_settings(setup.json)  runtime / solver / mesh parameters
+//   computation_init(...)       Mesher conversion, control-volume file,
+//                               metrics, field allocation, setup copies
+//   TAvg() / outputUVWP()       running time average and snapshot files
+//   env_finalize()              index file and end message
+//
+// One process drives one GPU and holds the whole domain (CPM single-rank).
+// Setup errors are reported as FalmErr::setUpFileErr (the reference throws
+// from its JSON library or asserts).
+#ifndef FALM_AMD_FALMCORE_H
+#define FALM_AMD_FALMCORE_H
+
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "FalmCFD.h"
+#include "FalmEq.h"
+#include "FalmIO.h"
+#include "FalmMeshBase.h"
+#include "MV.h"
+#include "alm.h"
+#include "mesher.h"
+
+namespace Falm {
+
+struct FalmBasicVar {
+    Matrix<Real> xyz, kx, g, ja;
+    Matrix<Real> u, uu, uc, p, nut;
+    Matrix<Real> poi_a, poi_rhs, poi_res;
+    Matrix<Real> ff;
+    Matrix<Real> divergence;
+    Matrix<Real> utavg, ptavg;
+    Matrix<Real> uvwp;
+
+    void release_all() {
+        for (Matrix<Real> *m : {&xyz, &kx, &g, &ja, &u, &uu, &uc, &p, &nut, &poi_a, &poi_rhs, &poi_res, &ff,
+                                &divergence, &utavg, &ptavg, &uvwp})
+            m->release();
+    }
+};
+
+struct FalmMeshInfo {
+    bool convert = false;
+    std::string convertSrc;
+    std::string cvFile;
+    std::string cvCenter;
+};
+
+class FalmCore {
+public:
+    Json params;
+    FalmCFD falmCfd;
+    FalmEq falmEq;
+    FalmMeshInfo falmMeshInfo;
+    CPM cpm;
+    FalmBasicVar fv;
+    FalmBaseMesh gBaseMesh;
+    FalmBaseMesh baseMesh;
+
+    Int it = 0;
+    Real dt = 0;
+    Real startTime = 0;
+    Int maxIt = 0;
+    Int timeAvgStartIt = 0;
+    Int timeAvgEndIt = -1;
+    Int timeAvgCount = 0;
+    Int outputStartIt = 0;
+    Int outputEndIt = -1;
+    Int outputIntervalIt = 1;
+    std::string workdir = ".";
+    std::string outputPrefix;
+    std::string setupFile;
+    std::string cvFile;
+    std::vector<FalmSnapshotInfo> timeSlices;
+    int ngpu = 1;
+    int gpuid = 0;
+
+    Real gettime() { return dt * it; }
+
+    bool is_time_averaging() { return it >= timeAvgStartIt && it <= timeAvgEndIt; }
+
+    // FalmCore::env_init (src/falm.h:87-94): device 0 of this process
+    int env_init(int & /*argc*/, char **& /*argv*/) {
+        timeSlices.clear();
+        gpuid = 0;
+        return falmErrCheckMacro(falm_init(gpuid));
+    }
+
+    // FalmCore::env_finalize (src/falm.h:96-113)
+    void env_finalize() {
+        if (cpm.rank == 0 && !outputPrefix.empty())
+            FalmIO::writeIndexFile(wpath(outputPrefix + ".json"), cpm, timeSlices);
+        fv.release_all();
+        falmEq.release();
+        falmCfd.release();
+        gBaseMesh.release();
+        baseMesh.release();
+        // written relative to the current directory, as the reference does
+        std::ofstream mfs(outputPrefix + "_endmsg.txt");
+        mfs << "Falm/Main ends normally.";
+    }
+
+    std::string wpath(const std::string &str) { return glue_path(workdir, str); }
+
+    // FalmCore::print_info (src/falm.h:125-220), single rank
+    void print_info(int output_rank = 0) {
+        if (cpm.rank != output_rank) return;
+        std::printf("SETUP INFO START\n");
+        std::printf("Working dir %s\n", workdir.c_str());
+        std::printf("Read setup file %s\n", wpath(setupFile).c_str());
+        std::printf("Control Params:\n");
+        std::printf("\tdt %e\n", dt);
+        std::printf("\tstart time %e\n", startTime);
+        std::printf("\titeration number %d\n", maxIt);
+        std::printf("\ttime avg start iteration %d\n", timeAvgStartIt);
+        std::printf("\ttime avg end iteration %d\n", timeAvgEndIt);
+        std::printf("\toutput start iteration %d\n", outputStartIt);
+        std::printf("\toutput end iteration %d\n", outputEndIt);
+        std::printf("\toutput every %d iterations\n", outputIntervalIt);
+        std::printf("\toutput to %s\n", wpath(outputPrefix).c_str());
+        std::printf("CFD Solver Params\n");
+        std::printf("\tRe = %e, 1/Re = %e\n", falmCfd.Re, falmCfd.ReI);
+        std::printf("\tAdvection scheme %s\n", FalmCFD::advscheme2str(falmCfd.AdvScheme).c_str());
+        std::printf("\tSGS model %s\n", FalmCFD::sgs2str(falmCfd.SGSModel).c_str());
+        if (falmCfd.SGSModel == SGSType::Smagorinsky) std::printf("\tCsmagorinsky %e\n", falmCfd.CSmagorinsky);
+        std::printf("Poisson Solver Params\n");
+        std::printf("\tsolver type %s\n", FalmEq::type2str(falmEq.type).c_str());
+        std::printf("\tmax iteration number %d\n", falmEq.maxit);
+        std::printf("\ttolerance %e\n", falmEq.tol);
+        if (falmEq.type == LSType::SOR) std::printf("\trelaxation factor %e\n", falmEq.relax_factor);
+        if (falmEq.type == LSType::PBiCGStab) {
+            std::printf("Preconditioner Params\n");
+            std::printf("\tsolver type %s\n", FalmEq::type2str(falmEq.pc_type).c_str());
+            std::printf("\titeration number %d\n", falmEq.pc_maxit);
+            if (falmEq.pc_type == LSType::SOR) std::printf("\trelaxation factor %e\n", falmEq.pc_relax_factor);
+        }
+        if (params.contains("inflow")) {
+            std::printf("Inflow Params\n");
+            std::printf("\tProfile %s\n", params["inflow"]["type"].get<std::string>().c_str());
+            std::printf("\tVelocity %e\n", params["inflow"]["velocity"].get<Real>());
+        }
+        std::printf("SETUP INFO END\n");
+        const Int gc = cpm.gc;
+        const Region &pdm = cpm.pdm_list[cpm.rank];
+        std::printf("MPI INFO START\n");
+        std::printf("\tGlobal voxel (%d %d %d)\n", cpm.global.shape[0], cpm.global.shape[1], cpm.global.shape[2]);
+        std::printf("\tGlobal inner voxel (%d %d %d)\n", cpm.global.shape[0] - 2 * gc, cpm.global.shape[1] - 2 * gc,
+                    cpm.global.shape[2] - 2 * gc);
+        std::printf("\tGuide voxel length %d\n", gc);
+        std::printf("\tMPI cluster shape (%d %d %d)\n", cpm.shape[0], cpm.shape[1], cpm.shape[2]);
+        std::printf("\tRank 0\n");
+        std::printf("\t\tVoxel (%d %d %d)\n", pdm.shape[0], pdm.shape[1], pdm.shape[2]);
+        std::printf("\t\tInner voxel (%d %d %d)\n", pdm.shape[0] - 2 * gc, pdm.shape[1] - 2 * gc,
+                    pdm.shape[2] - 2 * gc);
+        std::printf("\t\tOffset (%d %d %d)\n", pdm.offset[0], pdm.offset[1], pdm.offset[2]);
+        std::printf("MPI INFO END\n");
+        std::printf("GPU INFO START\n\t1 GPU in this process\nGPU INFO END\n");
+    }
+
+    // FalmCore::parse_settings (src/falm.h:222-340)
+    Int parse_settings(const std::string &setup_file_path) {
+        const size_t cut = setup_file_path.find_last_of('/');
+        if (cut == std::string::npos) {
+            workdir = ".";
+            setupFile = setup_file_path;
+        } else if (cut == 0) {
+            workdir = "/";
+            setupFile = setup_file_path.substr(1);
+        } else {
+            workdir = setup_file_path.substr(0, cut);
+            setupFile = setup_file_path.substr(cut + 1);
+        }
+        std::ifstream f(wpath(setupFile));
+        if (!f) return FalmErr::setUpFileErr;
+        try {
+            params = Json::parse(f);
+            const Json &rt = params["runtime"];
+            startTime = rt["time"]["start"].get<Real>();
+            dt = rt["time"]["dt"].get<Real>();
+            maxIt = Int((rt["time"]["end"].get<Real>() - startTime) / dt);
+            if (rt.contains("timeAvg")) {
+                const Json &ta = rt["timeAvg"];
+                if (ta.contains("start")) {
+                    Real s = ta["start"].get<Real>();
+                    if (s < startTime) s = startTime;
+                    timeAvgStartIt = Int((s - startTime) / dt);
+                } else {
+                    timeAvgStartIt = 0;
+                }
+                timeAvgEndIt = ta.contains("end") ? Int((ta["end"].get<Real>() - startTime) / dt) : maxIt;
+            } else {
+                timeAvgStartIt = 0;
+                timeAvgEndIt = -1;
+            }
+            if (rt.contains("output")) {
+                const Json &o = rt["output"];
+                if (o.contains("start")) {
+                    Real s = o["start"].get<Real>();
+                    if (s < startTime) s = startTime;
+                    outputStartIt = Int((s - startTime) / dt);
+                } else {
+                    outputStartIt = 0;
+                }
+                outputEndIt = o.contains("end") ? Int((o["end"].get<Real>() - startTime) / dt) : maxIt;
+                outputIntervalIt = Int(o["interval"].get<Real>() / dt);
+                outputPrefix = o["prefix"].get<std::string>();
+            } else {
+                outputStartIt = 0;
+                outputEndIt = -1;
+            }
+
+            const Json &ls = params["solver"]["linearSolver"];
+            FalmEq tmp;
+            tmp.type = FalmEq::str2type(ls["type"].get<std::string>());
+            tmp.maxit = ls["iteration"].get<Int>();
+            tmp.tol = ls["tolerance"].get<Real>();
+            if (tmp.type == LSType::SOR) tmp.relax_factor = ls["relaxationFactor"].get<Real>();
+            if (tmp.type == LSType::PBiCGStab) {
+                const Json &pc = ls["preconditioner"];
+                tmp.pc_type = FalmEq::str2type(pc["type"].get<std::string>());
+                tmp.pc_maxit = pc["iteration"].get<Int>();
+                if (tmp.pc_type == LSType::SOR) tmp.pc_relax_factor = pc["relaxationFactor"].get<Real>();
+            } else {
+                tmp.pc_type = LSType::Empty;
+            }
+            falmEq.init(tmp.type, tmp.maxit, tmp.tol, tmp.relax_factor, tmp.pc_type, tmp.pc_maxit,
+                        tmp.pc_relax_factor);
+
+            const Json &cfd = params["solver"]["cfd"];
+            const Real Re = cfd["Re"].get<Real>();
+            const Flag adv = FalmCFD::str2advscheme(cfd["advectionScheme"].get<std::string>());
+            const Flag sgs = cfd.contains("SGS") ? FalmCFD::str2sgs(cfd["SGS"].get<std::string>()) : SGSType::Empty;
+            const Real cs = sgs == SGSType::Smagorinsky ? cfd["Cs"].get<Real>() : 0.1;
+            falmCfd.init(Re, adv, sgs, cs);
+
+            const Json &mesh = params["mesh"];
+            falmMeshInfo.cvFile = mesh["controlVolumeFile"].get<std::string>();
+            falmMeshInfo.cvCenter = mesh["controlVolumeCenter"].get<std::string>();
+            if (mesh.contains("convert")) {
+                falmMeshInfo.convert = true;
+                falmMeshInfo.convertSrc = mesh["convert"].get<std::string>();
+            }
+        } catch (const std::exception &e) {
+            std::fprintf(stderr, "FalmCore::parse_settings: %s: %s\n", wpath(setupFile).c_str(), e.what());
+            return lastStatus() = FalmErr::setUpFileErr;
+        }
+        cvFile = falmMeshInfo.cvFile;
+        return FalmErr::success;
+    }
+
+    // FalmCore::computation_init (src/falm.h:342-431)
+    int computation_init(const Int3 &division, int gc) {
+        if (falmMeshInfo.convert && cpm.rank == 0) {
+            if (Mesher::build_mesh(falmMeshInfo.cvCenter, wpath(falmMeshInfo.convertSrc), wpath(falmMeshInfo.cvFile),
+                                   gc) != 0)
+                return lastStatus() = FalmErr::setUpFileErr;
+        }
+        Int3 idmax;
+        Int cgc = 0;
+        if (!FalmIO::readControlVolumeFile(wpath(falmMeshInfo.cvFile), gBaseMesh, idmax, cgc)) {
+            std::fprintf(stderr, "FalmCore: cannot read control-volume file %s\n", wpath(falmMeshInfo.cvFile).c_str());
+            return lastStatus() = FalmErr::setUpFileErr;
+        }
+        if (cgc != gc) {
+            std::fprintf(stderr, "FalmCore: control-volume file has %d guide cells, expected %d\n", cgc, gc);
+            return lastStatus() = FalmErr::setUpFileErr;
+        }
+        int st = cpm.initPartition(idmax, gc, division);
+        if (st != FALM_OK) return st;
+        const Int3 shape = cpm.pdm_list[cpm.rank].shape;
+        const Int3 offset = cpm.pdm_list[cpm.rank].offset;
+        if (cpm.rank == 0 && outputEndIt >= outputStartIt) {
+            FalmIO::writeIndexFile(wpath(outputPrefix + ".json"), cpm, timeSlices);
+            FalmIO::writeControlVolumeFile(wpath(outputPrefix + ".cv"), gBaseMesh, idmax, gc);
+            FalmIO::writeSetupFile(wpath(outputPrefix + "_setup.json"), params);
+        }
+        std::printf("(%d %d %d) (%d %d %d)\n", shape[0], shape[1], shape[2], offset[0], offset[1], offset[2]);
+
+        baseMesh.alloc(shape[0], shape[1], shape[2], HDC::HstDev);
+        for (Int i = 0; i < shape[0]; i++) {
+            baseMesh.x(i) = gBaseMesh.x(i + offset[0]);
+            baseMesh.hx(i) = gBaseMesh.hx(i + offset[0]);
+        }
+        for (Int j = 0; j < shape[1]; j++) {
+            baseMesh.y(j) = gBaseMesh.y(j + offset[1]);
+            baseMesh.hy(j) = gBaseMesh.hy(j + offset[1]);
+        }
+        for (Int k = 0; k < shape[2]; k++) {
+            baseMesh.z(k) = gBaseMesh.z(k + offset[2]);
+            baseMesh.hz(k) = gBaseMesh.hz(k + offset[2]);
+        }
+        baseMesh.sync(MCP::Hst2Dev);
+
+        // metrics on the device (falm_metrics: ja = hx hy hz, kx = 1/h,
+        // g = ja kx^2, xyz = centres), then host copies for the
+        // application's matrix assembly
+        fv.xyz.alloc(shape, 3, HDC::HstDev, "coordinate");
+        fv.kx.alloc(shape, 3, HDC::HstDev, "d kxi / d x");
+        fv.g.alloc(shape, 3, HDC::HstDev, "metric tensor");
+        fv.ja.alloc(shape, 1, HDC::HstDev, "jacobian");
+        falm_domain d = cpm.domain();
+        st = falmErrCheckMacro(falm_metrics(&d, FALM_F64, baseMesh.x.host.ptr, baseMesh.y.host.ptr,
+                                            baseMesh.z.host.ptr, baseMesh.hx.host.ptr, baseMesh.hy.host.ptr,
+                                            baseMesh.hz.host.ptr, fv.xyz.dev.ptr, fv.kx.dev.ptr, fv.g.dev.ptr,
+                                            fv.ja.dev.ptr, nullptr));
+        if (st != FALM_OK) return st;
+        falmErrCheckMacro(falm_stream_sync(nullptr));
+        for (Matrix<Real> *m : {&fv.xyz, &fv.kx, &fv.g, &fv.ja}) m->sync(MCP::Dev2Hst);
+
+        fv.u.alloc(shape, 3, HDC::HstDev, "velocity");
+        fv.uu.alloc(shape, 3, HDC::HstDev, "contravariant face velocity");
+        fv.p.alloc(shape, 1, HDC::HstDev, "pressure");
+        fv.nut.alloc(shape, 1, HDC::HstDev, "turbulence viscosity");
+        fv.poi_a.alloc(shape, 7, HDC::HstDev, "poisson coefficients", StencilMatrix::D3P7);
+        fv.poi_rhs.alloc(shape, 1, HDC::HstDev, "poisson rhs");
+        fv.poi_res.alloc(shape, 1, HDC::HstDev, "poisson residual");
+        fv.ff.alloc(shape, 3, HDC::HstDev, "ALM force");
+        fv.divergence.alloc(shape, 1, HDC::HstDev, "divergence");
+        // the reference allocates these only when end > start, yet averages
+        // at it == start == end too (src/falm.h:421, 433-444)
+        if (timeAvgEndIt >= timeAvgStartIt) {
+            fv.utavg.alloc(shape, 3, HDC::Device, "time-avg velocity");
+            fv.ptavg.alloc(shape, 1, HDC::Device, "time-avg pressure");
+            fv.utavg.clear(HDC::Device);
+            fv.ptavg.clear(HDC::Device);
+        }
+        fv.uvwp.alloc(shape, 4, HDC::HstDev, "uvwp buffer");
+        falmCfd.alloc(shape);
+        falmEq.alloc(shape);
+        return lastStatus() == FALM_OK ? FALM_OK : lastStatus();
+    }
+
+    // FalmCore::TAvg (src/falm.h:433-444): avg = (1 - 1/n) avg + (1/n) v
+    void TAvg(dim3 block = dim3(8, 8, 8)) {
+        if (!is_time_averaging()) return;
+        timeAvgCount = it - timeAvgStartIt + 1;
+        const Real b = 1. / timeAvgCount;
+        const Real a = 1. - b;
+        FalmMV::Vecaxby(a, fv.utavg, b, fv.u, fv.utavg, block);
+        FalmMV::Vecaxby(a, fv.ptavg, b, fv.p, fv.ptavg, block);
+    }
+
+    // FalmCore::outputUVWP (src/falm.h:446-479): file prefix_RRRRRR_IIIIIIIIII
+    // holding u, v, w, p, and prefix_tavg_... while averaging
+    void outputUVWP(dim3 /*block*/ = dim3(8, 8, 8)) {
+        if (it < outputStartIt || it > outputEndIt || outputIntervalIt <= 0 || it % outputIntervalIt != 0) return;
+        const size_t n = (size_t)fv.uvwp.shape[0];
+        falmErrCheckMacro(falm_memcpy(&fv.uvwp(0, 0), fv.u.dev.ptr, sizeof(Real) * n * 3, MCP::Dev2Hst, nullptr));
+        falmErrCheckMacro(falm_memcpy(&fv.uvwp(0, 3), fv.p.dev.ptr, sizeof(Real) * n, MCP::Dev2Hst, nullptr));
+        FalmIO::writeVectorFile(wpath(snapshot_name(outputPrefix)), cpm, fv.uvwp, (size_t)it, gettime());
+        FalmSnapshotInfo snap = {(size_t)it, gettime(), false};
+        if (is_time_averaging()) {
+            falmErrCheckMacro(falm_memcpy(&fv.uvwp.dev(0, 0), fv.utavg.dev.ptr, sizeof(Real) * n * 3, MCP::Dev2Dev,
+                                          nullptr));
+            falmErrCheckMacro(falm_memcpy(&fv.uvwp.dev(0, 3), fv.ptavg.dev.ptr, sizeof(Real) * n, MCP::Dev2Dev,
+                                          nullptr));
+            fv.uvwp.sync(MCP::Dev2Hst);
+            FalmIO::writeVectorFile(wpath(snapshot_name(outputPrefix + "_tavg")), cpm, fv.uvwp, (size_t)it,
+                                    gettime());
+            snap.tavg = true;
+        }
+        timeSlices.push_back(snap);
+    }
+
+private:
+    std::string snapshot_name(const std::string &prefix) {
+        std::vector<char> buf(prefix.size() + 32);
+        std::snprintf(buf.data(), buf.size(), "%s_%06d_%010d", prefix.c_str(), cpm.rank, it);
+        return std::string(buf.data());
+    }
+};
+
+}  // namespace Falm
+
+#endif

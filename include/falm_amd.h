@@ -330,6 +330,15 @@ falm_status falm_build_ap(int n_station, int n_attack, const double *rr,
                           double *ap_r, double *ap_chord, double *ap_twist,
                           double *ap_cl, double *ap_cd);
 
+/* ------------------------------------------------------- input formats */
+/* Mesher::build_mesh (src/mesher/mesher.hpp:12-260): convert the node list
+ * in meshpath (mesh.json with "coordinatePoint": "node", x.txt, y.txt,
+ * z.txt) into the control-volume file outputfile ("imax jmax kmax gc", then
+ * "\t%.15e\t%.15e" centre/width lines for x, y, z; src/FalmIO.h:118-133).
+ * cvcenter "innerNode": nodes are cell centres, the outermost ones the
+ * first ghost layer; "cell": nodes are cell faces.  Host-only. */
+falm_status falm_mesher_build(const char *cvcenter, const char *meshpath, const char *outputfile, int gc);
+
 /* ------------------------------------------------------------ step driver */
 /* The per-step orchestration of the reference apps (app/nedo/main.cpp:150-204,
  * app/windtunnel/main.cpp:135-165, app/lid3d/main.cpp:122-175,
@@ -351,6 +360,11 @@ typedef struct falm_case_config {
     int    precision;        /* FALM_CASE_F64 / FALM_CASE_MIXED          */
     falm_solver_config solver;
     const falm_alm_config *alm;  /* NULL = no rotor                      */
+    /* Optional non-uniform control volumes, the centres and widths of the
+     * padded cells of each axis as a control-volume file holds them
+     * (src/FalmIO.h:99-116): cv = {x, hx, y, hy, z, hz}, n[m]+4 values
+     * each.  NULL = uniform cells of width h[m] from origin[m]. */
+    const double *cv[6];
 } falm_case_config;
 typedef struct falm_step_info {
     int64_t step;            /* FalmCore::it after the step              */
@@ -397,6 +411,13 @@ falm_status falm_case_adjoint(falm_case *c, int nsteps, int ckpt, double adj_rto
 /* Per-blade pitch of one turbine (8 values, degrees; NULL = 0), the
  * falm_turbine::pitch_deg of falm_alm_create. */
 falm_status falm_alm_set_pitch(falm_alm *alm, int turbine, const double *pitch_deg);
+/* Base position and base velocity of one turbine from the next step on
+ * (velocity NULL = 0); the surge motion of falm_turbine adds to both.
+ * Drives base motion read from a file: the reference parses a file name
+ * given for a baseLocation coordinate but never reads it
+ * (src/alm/turbineHandler.h:144-150).  Synchronises the stream first. */
+falm_status falm_alm_set_base(falm_alm *alm, int turbine, const double *base, const double *velocity,
+                              void *stream);
 
 /* Device-time accounting of one kernel family (HIP events recorded around
  * each launch of that family on the stream it is launched on).  Used for the

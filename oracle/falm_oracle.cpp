@@ -1023,6 +1023,7 @@ struct OrcAlm {
     std::vector<double> x, y, z;
     /* state */
     std::vector<double> angle, angvel, base, base_vel; /* [nt*3] */
+    std::vector<double> set_vel; /* [nt*3] base velocity of orc_alm_set_base */
     std::vector<double> ap_xyz, ap_force, ap_torque, ap_thrust;
     std::vector<int> ap_ijk, ap_rank;
     std::vector<double> torque, thrust;
@@ -1048,7 +1049,7 @@ int orc_alm_create(const falm_domain *dom, const falm_alm_config *c, const doubl
     A->y.assign(y, y + dom->shape[1]);
     A->z.assign(z, z + dom->shape[2]);
     A->angle.assign(3 * A->nt, 0.0); A->angvel.assign(3 * A->nt, 0.0);
-    A->base.assign(3 * A->nt, 0.0); A->base_vel.assign(3 * A->nt, 0.0);
+    A->base.assign(3 * A->nt, 0.0); A->base_vel.assign(3 * A->nt, 0.0); A->set_vel.assign(3 * A->nt, 0.0);
     A->ap_xyz.assign(3 * A->nap, 0.0); A->ap_force.assign(3 * A->nap, 0.0);
     A->ap_torque.assign(A->nap, 0.0); A->ap_thrust.assign(A->nap, 0.0);
     A->ap_ijk.assign(3 * A->nap, 0); A->ap_rank.assign(A->nap, -1);
@@ -1089,11 +1090,11 @@ int orc_alm_step(void *h, const double *u_, double *ff_, double t) {
             A->angle[3 * tid + m] = ang[m];
             A->angvel[3 * tid + m] = om[m];
             A->base[3 * tid + m] = T.base[m];
-            A->base_vel[3 * tid + m] = 0.0;
+            A->base_vel[3 * tid + m] = A->set_vel[3 * tid + m];
         }
         if (T.surge[0] != 0.0) {
             A->base[3 * tid] = T.base[0] + T.surge[0] * sin(T.surge[1] * t + T.surge[2]);
-            A->base_vel[3 * tid] = T.surge[0] * T.surge[1] * cos(T.surge[1] * t + T.surge[2]);
+            A->base_vel[3 * tid] = A->set_vel[3 * tid] + T.surge[0] * T.surge[1] * cos(T.surge[1] * t + T.surge[2]);
         }
     }
     const int napt = A->napb * A->nb;
@@ -1303,10 +1304,11 @@ int orc_case_create(const falm_case_config *cfg, void **out) {
         cen[m]->resize(d.shape[m]);
         wid[m]->resize(d.shape[m]);
         for (int i = 0; i < d.shape[m]; i++) {
-            (*cen[m])[i] = cfg->origin[m] + (i - gc + 0.5) * cfg->h[m];
-            (*wid[m])[i] = cfg->h[m];
+            (*cen[m])[i] = cfg->cv[2 * m] ? cfg->cv[2 * m][i] : cfg->origin[m] + (i - gc + 0.5) * cfg->h[m];
+            (*wid[m])[i] = cfg->cv[2 * m] ? cfg->cv[2 * m + 1][i] : cfg->h[m];
         }
     }
+    for (int q = 0; q < 6; q++) C->cfg.cv[q] = nullptr; /* read during create only */
     orc_metrics(&d, C->x.data(), C->y.data(), C->z.data(), C->hx.data(), C->hy.data(), C->hz.data(),
                 C->xyz.data(), C->kx.data(), C->g.data(), C->ja.data());
     /* initial field (app/windtunnel/main.cpp:121-129) */
@@ -1715,10 +1717,11 @@ static void adj_alm(OrcAlm *A, const double *u_, double t, const double *ffbar_,
             ang[T.angle_type - 1] = T.motion[0] * sin(T.motion[1] * t + T.motion[2]);
             om[T.angle_type - 1] = T.motion[0] * T.motion[1] * cos(T.motion[1] * t + T.motion[2]);
         }
-        double base[3] = {T.base[0], T.base[1], T.base[2]}, bvel[3] = {0, 0, 0};
+        const double *sv = &A->set_vel[3 * tid];
+        double base[3] = {T.base[0], T.base[1], T.base[2]}, bvel[3] = {sv[0], sv[1], sv[2]};
         if (T.surge[0] != 0.0) {
             base[0] = T.base[0] + T.surge[0] * sin(T.surge[1] * t + T.surge[2]);
-            bvel[0] = T.surge[0] * T.surge[1] * cos(T.surge[1] * t + T.surge[2]);
+            bvel[0] = sv[0] + T.surge[0] * T.surge[1] * cos(T.surge[1] * t + T.surge[2]);
         }
         const double nang[3] = {-ang[0], -ang[1], -ang[2]};
         const double theta = T.tip_rate * t + (2 * PI / A->nb) * bid;
@@ -2019,6 +2022,19 @@ int orc_alm_set_pitch(void *h, int turbine, const double *pitch_deg) {
     OrcAlm *A = (OrcAlm *)h;
     if (turbine < 0 || turbine >= A->nt) return FALM_ERR_INVALID;
     for (int b = 0; b < 8; b++) A->tb[turbine].pitch_deg[b] = pitch_deg ? pitch_deg[b] : 0.0;
+    return FALM_OK;
+}
+
+/* base position and velocity of one turbine from now on (no reference
+ * counterpart: the reference parses a file name for a base coordinate but
+ * never reads the file, src/alm/turbineHandler.h:144-150) */
+int orc_alm_set_base(void *h, int turbine, const double *base, const double *velocity) {
+    OrcAlm *A = (OrcAlm *)h;
+    if (turbine < 0 || turbine >= A->nt || !base) return FALM_ERR_INVALID;
+    for (int m = 0; m < 3; m++) {
+        A->tb[turbine].base[m] = base[m];
+        A->set_vel[3 * turbine + m] = velocity ? velocity[m] : 0.0;
+    }
     return FALM_OK;
 }
 

@@ -69,6 +69,7 @@ def lib():
             "orc_case_window": [vp, ip, dp],
             "orc_case_adjoint": [vp, ip, ip, C.c_double, dp, dp, dp, dp],
             "orc_alm_set_pitch": [vp, ip, dp],
+            "orc_alm_set_base": [vp, ip, dp, dp],
             "orc_adj_sgs": [dom, C.c_double, vp, vp, vp, vp, vp],
             "orc_adj_pseudo_u": [dom, C.c_double, C.c_double, C.c_double] + [vp] * 11,
             "orc_adj_u_to_uu": [dom, vp, vp, vp, vp],
@@ -256,6 +257,12 @@ class Alm:
     def set_twist_offset(self, off):
         check(lib().orc_alm_set_twist_offset(self.h, ptr(np.ascontiguousarray(off, dtype=np.float64))))
 
+    def set_base(self, turbine, base, velocity=None):
+        b = np.ascontiguousarray(base, dtype=np.float64)
+        v = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float64)
+        check(lib().orc_alm_set_base(self.h, turbine, b.ctypes.data_as(C.POINTER(C.c_double)),
+                                     None if v is None else v.ctypes.data_as(C.POINTER(C.c_double))))
+
     def get(self):
         nap, nt = self.cfg.n_ap, self.cfg.n_turbine
         xyz, f = np.zeros(3 * nap), np.zeros(3 * nap)
@@ -315,6 +322,12 @@ class Case:
 
     def set_twist_offset(self, off):
         check(lib().orc_alm_set_twist_offset(self._alm_handle(), ptr(np.ascontiguousarray(off, dtype=np.float64))))
+
+    def set_base(self, turbine, base, velocity=None):
+        b = np.ascontiguousarray(base, dtype=np.float64)
+        v = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float64)
+        check(lib().orc_alm_set_base(self._alm_handle(), turbine, b.ctypes.data_as(C.POINTER(C.c_double)),
+                                     None if v is None else v.ctypes.data_as(C.POINTER(C.c_double))))
 
     def window(self, nsteps):
         """Mean rotor power over the next nsteps steps (state unchanged)."""
