@@ -104,7 +104,7 @@ def test_nedo_app_matches_oracle(gpu, tmp_path):
                 hd, data = falmio.read_vector_file(falmio.snapshot_path(str(case), "run", it, tavg=tavg))
                 assert (hd["step"], hd["n"], hd["gc"], hd["dsize"]) == (it, 4, 2, 8)
                 assert hd["time"] == it * dt
-                assert (hd["imax"], hd["jmax"], hd["kmax"]) == dom.shape
+                assert (hd["imax"], hd["jmax"], hd["kmax"]) == tuple(dom.shape)
                 flat = data.reshape(4, -1)
                 assert rel_err(flat[:3].ravel(), uu, mask3) < 1e-10, (it, tavg)
                 assert rel_err(flat[3], pp, interior_mask(dom)) < 1e-10, (it, tavg)
