@@ -46,7 +46,7 @@ def families():
     return {
         "pseudo_u": (abi.TIME_PSEUDO_U, "k_pseudo_u (Upwind3 + diffusion + force)"),
         "project": (abi.TIME_PROJECT, "k_project (pressure-gradient projection, cell + face)"),
-        "face_flux": (abi.TIME_FACE_FLUX, "k_u_to_uu (face flux)"),
+        "face_flux": (abi.TIME_FACE_FLUX, "k_u_to_uu_rhs (face flux + Poisson right-hand side)"),
         "rhs": (abi.TIME_RHS, "k_divergence (Poisson right-hand side)"),
         "sgs": (abi.TIME_SGS, "k_smagorinsky"),
         "divergence": (abi.TIME_DIVERGENCE, "k_div_norm (divergence + |div|^2)"),
@@ -374,7 +374,8 @@ def main():
         "data": "synthetic (uniform inflow + seeded inlet noise, synthetic blade polar)",
         "config": {
             "workload": f"windtunnel {n[0]}x{n[1]}x{n[2]} cells + 1 ALM rotor (3x32 points), dx=D/32, "
-                        "Re 2.5e5, Cs 0.1, dt 0.3dx/U, BiCGStab+RB-SOR(2, w=1.5) tol 1e-6",
+                        f"Re 2.5e5, Cs 0.1, dt 0.3dx/U, BiCGStab+RB-SOR({setup.struct.solver.pc_maxit} pass, "
+                        f"w={setup.struct.solver.pc_relax:g}) tol 1e-6",
             "cells": setup.n_interior,
             "precision": "mixed" if rmx is not None else "fp64",
         },

@@ -295,9 +295,13 @@ falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
     }
     FALM_TRY(sep_pseudo_u(d, pr, c.adv_scheme, 1.0 / c.Re, c.dt, C->uprev, C->uprev, C->uu, C->u, C->nut, C->sm,
                           C->ff, C->alm ? alm_force_box(C->alm) : C->nobox, s));
-    FALM_TRY(sep_u_to_uu(d, pr, C->u, C->uu, C->sm, s));
-    if (c.kind != FALM_CASE_WINDTUNNEL) FALM_TRY(launch_bc_face_flux(d, pr, C->uubc, C->uu, s));
-    FALM_TRY(sep_poisson_rhs(d, pr, C->uu, C->rhs, C->sm, 1.0 / (c.dt * C->maxdiag), s));
+    if (c.kind == FALM_CASE_WINDTUNNEL) {
+        FALM_TRY(sep_u_to_uu_rhs(d, pr, C->u, C->uu, C->rhs, C->sm, 1.0 / (c.dt * C->maxdiag), s));
+    } else {  // the cavities zero the wall fluxes before the right-hand side
+        FALM_TRY(sep_u_to_uu(d, pr, C->u, C->uu, C->sm, s));
+        FALM_TRY(launch_bc_face_flux(d, pr, C->uubc, C->uu, s));
+        FALM_TRY(sep_poisson_rhs(d, pr, C->uu, C->rhs, C->sm, 1.0 / (c.dt * C->maxdiag), s));
+    }
     FALM_TRY(falm_solver_solve(C->solver, C->a, C->p, C->rhs, C->prec == FALM_F64 ? C->res : nullptr, s, si));
     FALM_TRY(launch_bc_pressure(d, FALM_F64, C->pbc, C->p, s));
     if (is2d) FALM_TRY(launch_copy_z5(d, FALM_F64, C->p, 1, s));

@@ -927,6 +927,46 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<
     }
 }
 
+// k_u_to_uu_km followed by the Poisson right-hand side of the interior
+// (MACCalcPoissonRHS, src/FalmCFD.h:96-109) in one pass.  Besides its own
+// upper faces each cell recomputes its lower faces from the same operands,
+// in the same order, as the cell (or the ghost-face layer) that stores them,
+// so rhs equals the divergence of the stored uu bit for bit without reading
+// uu back.  Used where no wall-flux correction runs between the two.
+template <class T>
+__global__ __launch_bounds__(K7_NT) void k_u_to_uu_rhs_km(K7Tiling kt, Geom g, Vec3<T> u, Vec3W<T> uu, SepTab<T> tb,
+                                                          double *__restrict__ rhs, double scale) {
+    const K7Pos Q = k7_pos(kt, g);
+    if (!Q.act) return;
+    const int sz = g.nxy;
+    const unsigned szb = (unsigned)sz * sizeof(T), syb = (unsigned)g.nx * sizeof(T), xb = (unsigned)sizeof(T);
+    const T hx2 = T(0.5) * tb.t[SEP_H][0][Q.i], hy2 = T(0.5) * tb.t[SEP_H][1][Q.j];
+    const T hxy2 = hx2 * tb.t[SEP_H][1][Q.j];
+    const T rxy = tb.t[SEP_R][0][Q.i] * tb.t[SEP_R][1][Q.j];
+    unsigned cb = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szb;
+    unsigned rb = ((unsigned)Q.col0 + (unsigned)Q.k0 * (unsigned)sz) * (unsigned)sizeof(double);
+    const unsigned rzb = (unsigned)sz * (unsigned)sizeof(double);
+    T wz = ld_at(u.c[2], cb);
+    T fzb = hxy2 * (ld_at(u.c[2], cb - szb) + wz);  // the face below plane k0
+    for (int k = Q.k0; k < Q.k1; k++, cb += szb, rb += rzb) {
+        const T ux = ld_at(u.c[0], cb), uxe = ld_at(u.c[0], cb + xb), uxw = ld_at(u.c[0], cb - xb);
+        const T vy = ld_at(u.c[1], cb), vyn = ld_at(u.c[1], cb + syb), vys = ld_at(u.c[1], cb - syb);
+        const T wzt = ld_at(u.c[2], cb + szb);
+        const T hz = tb.t[SEP_H][2][k];
+        const T fxe = (hy2 * hz) * (ux + uxe), fxw = (hy2 * hz) * (uxw + ux);
+        const T fyn = (hx2 * hz) * (vy + vyn), fys = (hx2 * hz) * (vys + vy);
+        const T fzt = hxy2 * (wz + wzt);
+        st_at(uu.c[0], cb, fxe);
+        st_at(uu.c[1], cb, fyn);
+        st_at(uu.c[2], cb, fzt);
+        double d = (double)fxe - (double)fxw + (double)fyn - (double)fys + (double)fzt - (double)fzb;
+        d = d * (double)(rxy * tb.t[SEP_R][2][k]);
+        st_at(rhs, rb, d * scale);
+        fzb = fzt;
+        wz = wzt;
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_u_to_uu_layer(Geom g, Vec3<T> u, Vec3W<T> uu, SepTab<T> tb) {
     int i, j, k;
@@ -1298,6 +1338,33 @@ static falm_status u_to_uu_km_t(const falm_domain *dom, const void *u, void *uu,
     FALM_CHECK_LAUNCH("k_u_to_uu_km");
     tm.done((double)make_tiling(face_box(g)).box.size() * 6.0 * sizeof(T));
     return FALM_OK;
+}
+
+template <class T>
+static falm_status u_to_uu_rhs_km_t(const falm_domain *dom, const void *u, void *uu, double *rhs, const SepMesh &sm,
+                                    double scale, hipStream_t s) {
+    Geom g = make_geom(dom);
+    size_t n = ncell_of(dom);
+    K7Tiling kt = make_k7_tiling(interior_box(g));
+    if (kt.blocks == 0) return FALM_OK;
+    TimedLaunch tm(FALM_TIME_FACE_FLUX, s);
+    FALM_TRY(check_byte_offsets(n, sizeof(double)));
+    hipLaunchKernelGGL((k_u_to_uu_rhs_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, v3<T>(u, n),
+                       v3w<T>(uu, n), sep_tab<T>(sm), rhs, scale);
+    const long long nl = face_layer_count(g);
+    hipLaunchKernelGGL((k_u_to_uu_layer<T>), ew_grid((size_t)nl), dim3(EW_BLOCK), 0, s, g, v3<T>(u, n),
+                       v3w<T>(uu, n), sep_tab<T>(sm));
+    FALM_CHECK_LAUNCH("k_u_to_uu_rhs_km");
+    // u read, uu written, rhs written
+    tm.done((double)make_tiling(face_box(g)).box.size() * 6.0 * sizeof(T) +
+            (double)interior_box(g).size() * sizeof(double));
+    return FALM_OK;
+}
+
+falm_status sep_u_to_uu_rhs(const falm_domain *dom, int prec, const void *u, void *uu, double *rhs,
+                            const SepMesh &sm, double scale, hipStream_t s) {
+    if (prec == FALM_F32) return u_to_uu_rhs_km_t<float>(dom, u, uu, rhs, sm, scale, s);
+    return u_to_uu_rhs_km_t<double>(dom, u, uu, rhs, sm, scale, s);
 }
 
 falm_status sep_u_to_uu(const falm_domain *dom, int prec, const void *u, void *uu, const SepMesh &sm,

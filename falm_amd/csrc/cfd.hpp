@@ -41,6 +41,10 @@ falm_status sep_pseudo_u(const falm_domain *dom, int prec, int scheme, double Re
                          const int *ffbox, hipStream_t s);
 falm_status sep_u_to_uu(const falm_domain *dom, int prec, const void *u, void *uu, const SepMesh &sm,
                         hipStream_t s);
+// sep_u_to_uu then sep_poisson_rhs in one pass over u (no wall-flux
+// correction in between)
+falm_status sep_u_to_uu_rhs(const falm_domain *dom, int prec, const void *u, void *uu, double *rhs,
+                            const SepMesh &sm, double scale, hipStream_t s);
 // interior cells only: rhs = div(uu)/ja * scale (the ghost cells stay zero)
 falm_status sep_poisson_rhs(const falm_domain *dom, int prec, const void *uu, double *rhs, const SepMesh &sm,
                             double scale, hipStream_t s);

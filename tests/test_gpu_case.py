@@ -47,12 +47,17 @@ def test_lid3d_steps(gpu):
     _check(ref, dev, recs, 1e-9)
 
 
-def test_windtunnel_rotor_steps(gpu):
-    # a tight Poisson tolerance makes the pressure a property of the system
-    # rather than of the iteration at which BiCGStab stops (both sides stop
-    # at the same iteration, but reordered dot products drift by ~1e-8 over
-    # 30 iterations)
-    setup = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, tol=1e-12, rotor_kw=dict(n_ap_per_blade=8))
+@pytest.mark.parametrize("pc_maxit,pc_relax,tol,maxit", [(1, 1.0, 1e-30, 6), (2, 1.5, 1e-12, 1000)])
+def test_windtunnel_rotor_steps(gpu, pc_maxit, pc_relax, tol, maxit):
+    # The pressure must be a property of the system, not of the iteration
+    # at which BiCGStab stops: either a tight tolerance that two SOR passes
+    # reach at the same iteration on both sides (reordered dot products
+    # drift by ~1e-8 over 30 iterations), or a fixed number of iterations
+    # (tol 1e-30, maxit 6) for the one-pass preconditioner, which needs
+    # ~100 iterations to reach 1e-12 and stops a few iterations apart.
+    # pc_maxit 2 runs the fused three-sweep SOR kernel.
+    setup = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, tol=tol, maxit=maxit, rotor_kw=dict(n_ap_per_blade=8),
+                             pc_maxit=pc_maxit, pc_relax=pc_relax)
     ref, dev, recs = _run_pair(setup, 4)
     _check(ref, dev, recs, 1e-10)
     xyz, f, tq, th = dev.alm_get()
@@ -66,7 +71,7 @@ def test_windtunnel_moving_rotor(gpu):
     step to step (the step driver rewrites only the union of the old and new
     force boxes), and ff set from outside is fully rewritten by the next
     step."""
-    setup = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, tol=1e-12,
+    setup = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, tol=1e-12, pc_maxit=2, pc_relax=1.5,
                              rotor_kw=dict(n_ap_per_blade=8, angle_type=2, motion=(np.radians(5.0), 6.0, 0.2),
                                            surge=(0.25, 12.0, 0.0)))
     ref, dev, recs = _run_pair(setup, 5)
@@ -94,11 +99,13 @@ def test_deterministic_rerun(gpu):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("tol,bar", [(1e-6, 2e-3), (1e-10, 1e-5)])
+@pytest.mark.parametrize("tol,bar", [(1e-6, 5e-3), (1e-10, 1e-5)])
 def test_mixed_tracks_fp64(gpu, tol, bar):
     """Mixed precision against FP64 on the GPU after 10 steps.  At the
-    benchmark tolerance (1e-6) the difference is dominated by where each
-    solver stops; at 1e-10 it isolates the float stencils."""
+    benchmark tolerance (1e-6; |r|/N on this 36 864-cell grid) the difference
+    is dominated by where each solver stops (2.4e-3 with the one-pass
+    preconditioner, 1.5e-3 with two passes); at 1e-10 it isolates the float
+    stencils."""
     from falm_amd import runtime as rt
     s64 = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, rotor_kw=dict(n_ap_per_blade=8), tol=tol)
     smx = cases.windtunnel(n=(64, 24, 24), cells_per_d=8, rotor_kw=dict(n_ap_per_blade=8),
