@@ -284,11 +284,14 @@ falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
     const bool is2d = c.kind == FALM_CASE_LID2D;
     C->it++;
     const double t = c.dt * (double)C->it;
-    // u_previous <- u (app/windtunnel/main.cpp:136): swap the buffers, then
-    // restore the ghost shell of the new u, which u_to_uu reads before the
-    // boundary fill rewrites it
+    // u_previous <- u (app/windtunnel/main.cpp:136): swap the buffers.  The
+    // new u's ghost cells are stale until the boundary fill rewrites them;
+    // the wind tunnel's face-flux pass reads them from u_previous (equal to
+    // what the reference's in-place update leaves there), the cavities copy
+    // the ghost shell back first.  Edge and corner ghost cells are never
+    // written nor read after the initial fill, so both buffers keep them.
     std::swap(C->u, C->uprev);
-    FALM_TRY(copy_shell(C, C->uprev, C->u, s));
+    if (c.kind != FALM_CASE_WINDTUNNEL) FALM_TRY(copy_shell(C, C->uprev, C->u, s));
     if (C->alm) {
         FALM_TRY(alm_step(C->alm, pr, C->uprev, C->ff, t, s, true));
         if (power) FALM_TRY(alm_accumulate_power(C->alm, power, s));
@@ -296,7 +299,7 @@ falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
     FALM_TRY(sep_pseudo_u(d, pr, c.adv_scheme, 1.0 / c.Re, c.dt, C->uprev, C->uprev, C->uu, C->u, C->nut, C->sm,
                           C->ff, C->alm ? alm_force_box(C->alm) : C->nobox, s));
     if (c.kind == FALM_CASE_WINDTUNNEL) {
-        FALM_TRY(sep_u_to_uu_rhs(d, pr, C->u, C->uu, C->rhs, C->sm, 1.0 / (c.dt * C->maxdiag), s));
+        FALM_TRY(sep_u_to_uu_rhs(d, pr, C->u, C->uprev, C->uu, C->rhs, C->sm, 1.0 / (c.dt * C->maxdiag), s));
     } else {  // the cavities zero the wall fluxes before the right-hand side
         FALM_TRY(sep_u_to_uu(d, pr, C->u, C->uu, C->sm, s));
         FALM_TRY(launch_bc_face_flux(d, pr, C->uubc, C->uu, s));

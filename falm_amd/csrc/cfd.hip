@@ -933,9 +933,13 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<
 // in the same order, as the cell (or the ghost-face layer) that stores them,
 // so rhs equals the divergence of the stored uu bit for bit without reading
 // uu back.  Used where no wall-flux correction runs between the two.
+// Ghost cells are read from gu: the step driver passes the previous step's
+// velocity, whose ghost cells are the ones the in-place update of the
+// reference leaves in u until the boundary fill (its interior is never
+// read here).
 template <class T>
-__global__ __launch_bounds__(K7_NT) void k_u_to_uu_rhs_km(K7Tiling kt, Geom g, Vec3<T> u, Vec3W<T> uu, SepTab<T> tb,
-                                                          double *__restrict__ rhs, double scale) {
+__global__ __launch_bounds__(K7_NT) void k_u_to_uu_rhs_km(K7Tiling kt, Geom g, Vec3<T> u, Vec3<T> gu, Vec3W<T> uu,
+                                                          SepTab<T> tb, double *__restrict__ rhs, double scale) {
     const K7Pos Q = k7_pos(kt, g);
     if (!Q.act) return;
     const int sz = g.nxy;
@@ -943,15 +947,26 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_rhs_km(K7Tiling kt, Geom g, V
     const T hx2 = T(0.5) * tb.t[SEP_H][0][Q.i], hy2 = T(0.5) * tb.t[SEP_H][1][Q.j];
     const T hxy2 = hx2 * tb.t[SEP_H][1][Q.j];
     const T rxy = tb.t[SEP_R][0][Q.i] * tb.t[SEP_R][1][Q.j];
+    // neighbours in the ghost layers (the interior box is at least one cell
+    // from the array's edge)
+    const bool gw = Q.i - 1 < g.gc, ge = Q.i + 1 >= g.nx - g.gc;
+    const bool gs = Q.j - 1 < g.gc, gn = Q.j + 1 >= g.ny - g.gc;
+    const int ktop = g.nz - g.gc;  // first ghost plane above
     unsigned cb = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szb;
     unsigned rb = ((unsigned)Q.col0 + (unsigned)Q.k0 * (unsigned)sz) * (unsigned)sizeof(double);
     const unsigned rzb = (unsigned)sz * (unsigned)sizeof(double);
     T wz = ld_at(u.c[2], cb);
-    T fzb = hxy2 * (ld_at(u.c[2], cb - szb) + wz);  // the face below plane k0
+    const T wbelow = Q.k0 - 1 < g.gc ? ld_at(gu.c[2], cb - szb) : ld_at(u.c[2], cb - szb);
+    T fzb = hxy2 * (wbelow + wz);  // the face below plane k0
     for (int k = Q.k0; k < Q.k1; k++, cb += szb, rb += rzb) {
-        const T ux = ld_at(u.c[0], cb), uxe = ld_at(u.c[0], cb + xb), uxw = ld_at(u.c[0], cb - xb);
-        const T vy = ld_at(u.c[1], cb), vyn = ld_at(u.c[1], cb + syb), vys = ld_at(u.c[1], cb - syb);
-        const T wzt = ld_at(u.c[2], cb + szb);
+        const T ux = ld_at(u.c[0], cb), vy = ld_at(u.c[1], cb);
+        T uxe = ld_at(u.c[0], cb + xb), uxw = ld_at(u.c[0], cb - xb);
+        T vyn = ld_at(u.c[1], cb + syb), vys = ld_at(u.c[1], cb - syb);
+        if (ge) uxe = ld_at(gu.c[0], cb + xb);
+        if (gw) uxw = ld_at(gu.c[0], cb - xb);
+        if (gn) vyn = ld_at(gu.c[1], cb + syb);
+        if (gs) vys = ld_at(gu.c[1], cb - syb);
+        const T wzt = k + 1 >= ktop ? ld_at(gu.c[2], cb + szb) : ld_at(u.c[2], cb + szb);
         const T hz = tb.t[SEP_H][2][k];
         const T fxe = (hy2 * hz) * (ux + uxe), fxw = (hy2 * hz) * (uxw + ux);
         const T fyn = (hx2 * hz) * (vy + vyn), fys = (hx2 * hz) * (vys + vy);
@@ -967,16 +982,23 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_rhs_km(K7Tiling kt, Geom g, V
     }
 }
 
+// gu: where ghost cells are read (u itself, or the previous step's
+// velocity as in k_u_to_uu_rhs_km)
 template <class T>
-__global__ __launch_bounds__(256) void k_u_to_uu_layer(Geom g, Vec3<T> u, Vec3W<T> uu, SepTab<T> tb) {
+__global__ __launch_bounds__(256) void k_u_to_uu_layer(Geom g, Vec3<T> u, Vec3<T> gu, Vec3W<T> uu, SepTab<T> tb) {
     int i, j, k;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (!face_layer_cell(g, t, i, j, k)) return;
     const int cc = g.id(i, j, k);
     const T hx = tb.t[SEP_H][0][i], hy = tb.t[SEP_H][1][j], hz = tb.t[SEP_H][2][k];
-    uu.c[0][cc] = (T(0.5) * hy * hz) * (u.c[0][cc] + u.c[0][cc + 1]);
-    uu.c[1][cc] = (T(0.5) * hx * hz) * (u.c[1][cc] + u.c[1][cc + g.nx]);
-    uu.c[2][cc] = (T(0.5) * hx * hy) * (u.c[2][cc] + u.c[2][cc + g.nxy]);
+    auto at = [&](int d, int ii, int jj, int kk, int c) {
+        const bool ghost = ii < g.gc || ii >= g.nx - g.gc || jj < g.gc || jj >= g.ny - g.gc || kk < g.gc ||
+                           kk >= g.nz - g.gc;
+        return ghost ? gu.c[d][c] : u.c[d][c];
+    };
+    uu.c[0][cc] = (T(0.5) * hy * hz) * (at(0, i, j, k, cc) + at(0, i + 1, j, k, cc + 1));
+    uu.c[1][cc] = (T(0.5) * hx * hz) * (at(1, i, j, k, cc) + at(1, i, j + 1, k, cc + g.nx));
+    uu.c[2][cc] = (T(0.5) * hx * hy) * (at(2, i, j, k, cc) + at(2, i, j, k + 1, cc + g.nxy));
 }
 
 // Smagorinsky on the interior (kernel_Cartesian_Smagorinsky,
@@ -1334,15 +1356,15 @@ static falm_status u_to_uu_km_t(const falm_domain *dom, const void *u, void *uu,
                        sep_tab<T>(sm));
     const long long nl = face_layer_count(g);
     hipLaunchKernelGGL((k_u_to_uu_layer<T>), ew_grid((size_t)nl), dim3(EW_BLOCK), 0, s, g, v3<T>(u, n),
-                       v3w<T>(uu, n), sep_tab<T>(sm));
+                       v3<T>(u, n), v3w<T>(uu, n), sep_tab<T>(sm));
     FALM_CHECK_LAUNCH("k_u_to_uu_km");
     tm.done((double)make_tiling(face_box(g)).box.size() * 6.0 * sizeof(T));
     return FALM_OK;
 }
 
 template <class T>
-static falm_status u_to_uu_rhs_km_t(const falm_domain *dom, const void *u, void *uu, double *rhs, const SepMesh &sm,
-                                    double scale, hipStream_t s) {
+static falm_status u_to_uu_rhs_km_t(const falm_domain *dom, const void *u, const void *gu, void *uu, double *rhs,
+                                    const SepMesh &sm, double scale, hipStream_t s) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     K7Tiling kt = make_k7_tiling(interior_box(g));
@@ -1350,10 +1372,10 @@ static falm_status u_to_uu_rhs_km_t(const falm_domain *dom, const void *u, void 
     TimedLaunch tm(FALM_TIME_FACE_FLUX, s);
     FALM_TRY(check_byte_offsets(n, sizeof(double)));
     hipLaunchKernelGGL((k_u_to_uu_rhs_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, v3<T>(u, n),
-                       v3w<T>(uu, n), sep_tab<T>(sm), rhs, scale);
+                       v3<T>(gu, n), v3w<T>(uu, n), sep_tab<T>(sm), rhs, scale);
     const long long nl = face_layer_count(g);
     hipLaunchKernelGGL((k_u_to_uu_layer<T>), ew_grid((size_t)nl), dim3(EW_BLOCK), 0, s, g, v3<T>(u, n),
-                       v3w<T>(uu, n), sep_tab<T>(sm));
+                       v3<T>(gu, n), v3w<T>(uu, n), sep_tab<T>(sm));
     FALM_CHECK_LAUNCH("k_u_to_uu_rhs_km");
     // u read, uu written, rhs written
     tm.done((double)make_tiling(face_box(g)).box.size() * 6.0 * sizeof(T) +
@@ -1361,10 +1383,10 @@ static falm_status u_to_uu_rhs_km_t(const falm_domain *dom, const void *u, void 
     return FALM_OK;
 }
 
-falm_status sep_u_to_uu_rhs(const falm_domain *dom, int prec, const void *u, void *uu, double *rhs,
+falm_status sep_u_to_uu_rhs(const falm_domain *dom, int prec, const void *u, const void *gu, void *uu, double *rhs,
                             const SepMesh &sm, double scale, hipStream_t s) {
-    if (prec == FALM_F32) return u_to_uu_rhs_km_t<float>(dom, u, uu, rhs, sm, scale, s);
-    return u_to_uu_rhs_km_t<double>(dom, u, uu, rhs, sm, scale, s);
+    if (prec == FALM_F32) return u_to_uu_rhs_km_t<float>(dom, u, gu, uu, rhs, sm, scale, s);
+    return u_to_uu_rhs_km_t<double>(dom, u, gu, uu, rhs, sm, scale, s);
 }
 
 falm_status sep_u_to_uu(const falm_domain *dom, int prec, const void *u, void *uu, const SepMesh &sm,

@@ -42,8 +42,8 @@ falm_status sep_pseudo_u(const falm_domain *dom, int prec, int scheme, double Re
 falm_status sep_u_to_uu(const falm_domain *dom, int prec, const void *u, void *uu, const SepMesh &sm,
                         hipStream_t s);
 // sep_u_to_uu then sep_poisson_rhs in one pass over u (no wall-flux
-// correction in between)
-falm_status sep_u_to_uu_rhs(const falm_domain *dom, int prec, const void *u, void *uu, double *rhs,
+// correction in between); ghost cells of u are read from gu
+falm_status sep_u_to_uu_rhs(const falm_domain *dom, int prec, const void *u, const void *gu, void *uu, double *rhs,
                             const SepMesh &sm, double scale, hipStream_t s);
 // interior cells only: rhs = div(uu)/ja * scale (the ghost cells stay zero)
 falm_status sep_poisson_rhs(const falm_domain *dom, int prec, const void *uu, double *rhs, const SepMesh &sm,
