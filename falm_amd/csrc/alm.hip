@@ -139,9 +139,18 @@ __device__ inline double trilinear(double xd, double yd, double zd, const double
 // chains instead of dependent global loads.
 constexpr int kPointsLds = 48 * 1024;
 
+__device__ void alm_box_wg(const AlmDev &A, const double *pxyz, const double *X, const double *Y, const double *Z);
+
+// fused (one workgroup holds every point): the force box and the
+// per-turbine totals follow in the same launch, from LDS copies of the
+// points' positions, torque and thrust
+constexpr int kFusedPoints = 256;
+
 template <class T>
-__global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, double t, int staged) {
+__global__ __launch_bounds__(256) void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, double t,
+                                                    int staged, int fused) {
     extern __shared__ double stage[];
+    __shared__ double sxyz[3 * kFusedPoints], stq[kFusedPoints], sth[kFusedPoints];
     const double *X = A.x, *Y = A.y, *Z = A.z, *AT = A.attack;
     if (staged) {
         const int L = A.nx + A.ny + A.nz + A.nat;
@@ -160,6 +169,7 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
         AT = stage + A.nx + A.ny + A.nz;
     }
     const int ap = blockIdx.x * blockDim.x + threadIdx.x;
+    [&]() {  // one point (a return ends this point's work only)
     if (ap >= A.nap) return;
     const int napt = A.napb * A.nb;
     const int tid = ap / napt, bid = (ap % napt) / A.napb, sec = ap % A.napb;
@@ -188,6 +198,8 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
     rot(c1, nang, T0.angle_type, P);
     for (int m = 0; m < 3; m++) P[m] += base[m];
     for (int m = 0; m < 3; m++) A.ap_xyz[3 * ap + m] = P[m];
+    if (fused)
+        for (int m = 0; m < 3; m++) sxyz[3 * ap + m] = P[m];
     const int I = find_index(X, P[0], A.nx), J = find_index(Y, P[1], A.ny), K = find_index(Z, P[2], A.nz);
     A.ap_ijk[3 * ap] = I;
     A.ap_ijk[3 * ap + 1] = J;
@@ -201,6 +213,7 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
         for (int m = 0; m < 3; m++) A.ap_force[3 * ap + m] = 0.0;
         A.ap_torque[ap] = 0.0;
         A.ap_thrust[ap] = 0.0;
+        if (fused) stq[ap] = sth[ap] = 0.0;
         return;
     }
     const double dr = (A.radius - A.hub_radius) / A.napb;
@@ -269,11 +282,30 @@ __global__ void k_alm_points(AlmDev A, const T *u0, const T *u1, const T *u2, do
     for (int m = 0; m < 3; m++) A.ap_force[3 * ap + m] = F[m];
     A.ap_torque[ap] = fabs(ft) * apr;
     A.ap_thrust[ap] = fx;
+    if (fused) {
+        stq[ap] = fabs(ft) * apr;
+        sth[ap] = fx;
+    }
+    }();
+    if (!fused) return;
+    __syncthreads();
+    alm_box_wg(A, sxyz, X, Y, Z);
+    // k_alm_totals: per-turbine sums in ascending point order
+    if ((int)threadIdx.x < A.nt) {
+        const int napt = A.napb * A.nb, tid = threadIdx.x;
+        double tq = 0.0, th = 0.0;
+        for (int a = 0; a < napt; a++) {
+            tq += stq[tid * napt + a];
+            th += sth[tid * napt + a];
+        }
+        A.torque[tid] = tq;
+        A.thrust[tid] = th;
+    }
 }
 
 // The cells that can receive force: centres within cutr (plus one cell) of
 // the bounding box of the points along every axis.  One workgroup.
-__global__ __launch_bounds__(256) void k_alm_box(AlmDev A) {
+__device__ void alm_box_wg(const AlmDev &A, const double *pxyz, const double *X, const double *Y, const double *Z) {
     __shared__ double smn[3][4], smx[3][4];
     __shared__ int slo[3][4], shi[3][4];
     __shared__ double lim[6];
@@ -281,8 +313,8 @@ __global__ __launch_bounds__(256) void k_alm_box(AlmDev A) {
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int ap = threadIdx.x; ap < A.nap; ap += blockDim.x)
         for (int m = 0; m < 3; m++) {
-            mn[m] = fmin(mn[m], A.ap_xyz[3 * ap + m]);
-            mx[m] = fmax(mx[m], A.ap_xyz[3 * ap + m]);
+            mn[m] = fmin(mn[m], pxyz[3 * ap + m]);
+            mx[m] = fmax(mx[m], pxyz[3 * ap + m]);
         }
     for (int m = 0; m < 3; m++) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -306,7 +338,7 @@ __global__ __launch_bounds__(256) void k_alm_box(AlmDev A) {
         lim[2 * m + 1] = b + A.cutr;
     }
     __syncthreads();
-    const double *C[3] = {A.x, A.y, A.z};
+    const double *C[3] = {X, Y, Z};
     const int N[3] = {A.nx, A.ny, A.nz};
     int lo[3], hi[3];
     for (int m = 0; m < 3; m++) {
@@ -365,6 +397,8 @@ __global__ __launch_bounds__(256) void k_alm_box(AlmDev A) {
         }
     }
 }
+
+__global__ __launch_bounds__(256) void k_alm_box(AlmDev A) { alm_box_wg(A, A.ap_xyz, A.x, A.y, A.z); }
 
 __device__ inline bool box_meets(const int *b, int ilo, int ihi, int jlo, int jhi, int klo, int khi) {
     return b[0] <= b[1] && ilo <= b[1] && ihi >= b[0] && jlo <= b[3] && jhi >= b[2] && klo <= b[5] && khi >= b[4];
@@ -773,14 +807,16 @@ falm_status alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, h
     const size_t lds = staged ? stage_bytes : 0;
     const int union_only = (owned_ff && !A->full_next) ? 1 : 0;
     A->full_next = !owned_ff;
+    // one workgroup holds every point: box and totals in the same launch
+    const int fused = A->nap <= kFusedPoints && A->nt <= 256 ? 1 : 0;
     if (prec == FALM_F32)
         hipLaunchKernelGGL((k_alm_points<float>), dim3((A->nap + 255) / 256), dim3(256), lds, s, A->d,
-                           col<float>(u, n, 0), col<float>(u, n, 1), col<float>(u, n, 2), t, staged);
+                           col<float>(u, n, 0), col<float>(u, n, 1), col<float>(u, n, 2), t, staged, fused);
     else
         hipLaunchKernelGGL((k_alm_points<double>), dim3((A->nap + 255) / 256), dim3(256), lds, s, A->d,
-                           col<double>(u, n, 0), col<double>(u, n, 1), col<double>(u, n, 2), t, staged);
+                           col<double>(u, n, 0), col<double>(u, n, 1), col<double>(u, n, 2), t, staged, fused);
     FALM_CHECK_LAUNCH("k_alm_points");
-    hipLaunchKernelGGL(k_alm_box, dim3(1), dim3(256), 0, s, A->d);
+    if (!fused) hipLaunchKernelGGL(k_alm_box, dim3(1), dim3(256), 0, s, A->d);
     hipLaunchKernelGGL(k_alm_factors, dim3((nfac + 255) / 256), dim3(256), 0, s, A->d);
     FALM_CHECK_LAUNCH("k_alm_factors");
     if (tl.tiles > 0) {
@@ -796,7 +832,7 @@ falm_status alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, h
                                col<double>(ff, n, 1), col<double>(ff, n, 2), A->cut2, union_only);
         FALM_CHECK_LAUNCH("k_alm_project");
     }
-    hipLaunchKernelGGL(k_alm_totals, dim3(A->nt), dim3(256), 0, s, A->d);
+    if (!fused) hipLaunchKernelGGL(k_alm_totals, dim3(A->nt), dim3(256), 0, s, A->d);
     FALM_CHECK_LAUNCH("k_alm_totals");
     tm.done((double)tl.box.size() * 3.0 * prec_bytes(prec));
     return FALM_OK;
@@ -825,7 +861,7 @@ falm_status alm_adjoint(falm_alm *A, const double *u, double t, const double *ff
     const size_t stage_bytes = (size_t)(A->d.nx + A->d.ny + A->d.nz + A->d.nat) * sizeof(double);
     const int staged = stage_bytes <= (size_t)kPointsLds;
     hipLaunchKernelGGL((k_alm_points<double>), dim3((A->nap + 255) / 256), dim3(256), staged ? stage_bytes : 0, s,
-                       A->d, u, u + n, u + 2 * n, t, staged);
+                       A->d, u, u + n, u + 2 * n, t, staged, 0);  // no box or totals update
     hipLaunchKernelGGL(k_alm_adj_force, dim3(A->nap), dim3(256), 0, s, A->d, ffbar, ffbar + n, ffbar + 2 * n, A->Fb);
     hipLaunchKernelGGL(k_alm_adj_points, dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d, u, u + n, u + 2 * n, t,
                        (const double *)A->Fb, wP, ubar, ubar + n, ubar + 2 * n, gpitch, gtwist);
