@@ -1,9 +1,9 @@
-// nedo.cpp — the actuator-line wind-turbine application, written against
-// the Falm C++ class API the way the reference's app/nedo/main.cpp is
-// written against the reference's: FalmCore reads setup.json (runtime,
+// nedo.cpp — the reference's set-up-file applications, written against the
+// Falm C++ class API the way app/nedo/main.cpp and app/windtunnel/main.cpp
+// are written against the reference's: FalmCore reads setup.json (runtime,
 // solver, mesh, inflow, turbine), converts the node list into the
 // control-volume file, runs the step loop and writes snapshots, time
-// averages, the index file and the power/thrust CSV.
+// averages, the index file and (with a turbine) the power/thrust CSV.
 //
 //   nedo [SETUP_JSON] [info]
 //
@@ -11,10 +11,12 @@
 // it are relative to its directory, except the output prefix of the
 // power/thrust CSV and end message, which the reference resolves against
 // the current directory.  "info" stops after the set-up, as in the
-// reference.  Boundary conditions are those of app/nedo/bcdevcall.cu:
-// inlet (1, 0, 0) at x-, convective outflow at x+, slip at y+-, no-slip
-// wall at z-, slip at z+; pressure 0 beyond x+ and zero gradient elsewhere.
-// The "turbine" block is optional here (the reference requires it).
+// reference.  Boundary conditions: convective outflow at x+, slip at y+-,
+// no-slip wall at z-, slip at z+, pressure 0 beyond x+ and zero gradient
+// elsewhere, and at x- the inlet of the app the setup belongs to: with a
+// "turbine" block, app/nedo's (1, 0, 0) whatever the inflow velocity
+// (app/nedo/bcdevcall.cu:16); without one, app/windtunnel's
+// (inflow velocity, 0, 0) (app/windtunnel/bcdevcall.cu:8-22).
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -28,6 +30,7 @@ static Real maxdiag;
 static Matrix<Real> u_previous;
 static Alm::AlmHandler aalm;
 static bool with_turbine = false;
+static Real inlet_u = 1.0;
 static const dim3 block{8, 8, 8};
 
 // make_poisson_coefficient_matrix (app/nedo/main.cpp:25-100): the same
@@ -43,7 +46,7 @@ static int make_poisson_coefficient_matrix() {
 static void ubc(Matrix<Real> &u, Matrix<Real> &uprev, Matrix<Real> &x, Real dt) {
     falm_face_bc f[6] = {};
     f[CPM::XMINUS].type = FALM_BC_INLET;
-    f[CPM::XMINUS].value[0] = 1.0;
+    f[CPM::XMINUS].value[0] = inlet_u;
     f[CPM::XPLUS].type = FALM_BC_CONVECTIVE;
     f[CPM::YMINUS].type = f[CPM::YPLUS].type = f[CPM::ZPLUS].type = FALM_BC_SLIP;
     f[CPM::ZMINUS].type = FALM_BC_WALL0;
@@ -82,6 +85,7 @@ static int init(int &argc, char **&argv, const std::string &setup) {
     if (make_poisson_coefficient_matrix() != FALM_OK) return 1;
 
     with_turbine = falm.params.contains("turbine");
+    inlet_u = with_turbine ? 1.0 : u_inflow;
     if (with_turbine) {
         if (aalm.init(falm.workdir, falm.params["turbine"], falm.cpm, falm.outputPrefix + "_PT.csv") != FALM_OK)
             return 1;
