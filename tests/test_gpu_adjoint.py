@@ -18,7 +18,7 @@ def _setup(motion=False, sgs=abi.SGS_SMAGORINSKY, precision=abi.CASE_F64):
     if motion:
         kw.update(angle_type=2, motion=(np.radians(4.0), 3.0, 0.3), surge=(0.1, 5.0, 0.2))
     setup = cases.windtunnel(n=(24, 12, 12), cells_per_d=6, hub_x_d=1.5, re=1e3, tol=1e-13, rotor_kw=kw,
-                             precision=precision)
+                             precision=precision, pc_maxit=2, pc_relax=1.5)
     setup.struct.sgs_model = sgs
     return setup
 
