@@ -14,6 +14,7 @@
 // Scope: FALM_CASE_WINDTUNNEL, FALM_CASE_F64, the Upwind3 family of
 // advection schemes, Smagorinsky or no SGS model.
 #include <cmath>
+#include <initializer_list>
 #include <utility>
 #include <vector>
 
@@ -386,7 +387,132 @@ static F3 f3(void *p, size_t n) {
 
 using namespace falm;
 
+namespace {
+
+falm_status vjp_check(const falm_domain *dom, std::initializer_list<const void *> ptrs) {
+    if (!dom) return FALM_ERR_INVALID;
+    for (const void *p : ptrs)
+        if (!p) return FALM_ERR_INVALID;
+    return check_domain(dom);
+}
+
+FaceRange face_range(const Geom &g, const int type[6], long long &total) {
+    FaceRange fr;
+    const int shp[3] = {g.nx, g.ny, g.nz};
+    fr.start[0] = 0;
+    for (int f = 0; f < 6; f++) {
+        const int d = f / 2, ax = d == 0 ? 1 : 0, bx = d == 2 ? 1 : 2;
+        fr.type[f] = type[f];
+        fr.start[f + 1] = fr.start[f] + (shp[ax] - 2 * g.gc) * (shp[bx] - 2 * g.gc);
+    }
+    total = fr.start[6];
+    return fr;
+}
+
+}  // namespace
+
 extern "C" {
+
+// ------------------------------------------------ per-stage VJPs (FP64)
+falm_status falm_pseudo_u_vjp(const falm_domain *dom, int adv_scheme, double ReI, double dt, const void *u,
+                              const void *uu, const void *nut, const void *kx, const void *g, const void *ja,
+                              const void *uabar, void *ubar, void *uubar, void *nutbar, void *ffbar, void *stream) {
+    FALM_TRY(vjp_check(dom, {u, uu, nut, kx, g, ja, uabar, ubar, uubar, nutbar, ffbar}));
+    if (adv_scheme != FALM_ADV_UPWIND3 && adv_scheme != FALM_ADV_UTOPIA && adv_scheme != FALM_ADV_KK) {
+        set_error("falm_pseudo_u_vjp: Upwind3-family advection only");
+        return FALM_ERR_UNSUPPORTED;
+    }
+    const double ALPHA = adv_scheme == FALM_ADV_UPWIND3 ? 1. / 24. : adv_scheme == FALM_ADV_UTOPIA ? 1. / 12. : 1. / 4.;
+    const Geom gm = make_geom(dom);
+    const size_t n = ncell_of(dom);
+    const Box ib = interior_box(gm);
+    hipLaunchKernelGGL(k_adj_pseudo_u, box_grid(ib), dim3(EW_BLOCK), 0, (hipStream_t)stream, ib, gm, ALPHA, ReI, dt,
+                       c3(u, n), c3(uu, n), (const double *)nut, c3(kx, n), c3(g, n), (const double *)ja,
+                       c3(uabar, n), f3(ubar, n), f3(uubar, n), (double *)nutbar, f3(ffbar, n));
+    FALM_CHECK_LAUNCH("k_adj_pseudo_u");
+    return FALM_OK;
+}
+
+falm_status falm_u_to_uu_vjp(const falm_domain *dom, const void *kx, const void *ja, const void *uubar, void *ubar,
+                             void *stream) {
+    FALM_TRY(vjp_check(dom, {kx, ja, uubar, ubar}));
+    const Geom gm = make_geom(dom);
+    const size_t n = ncell_of(dom);
+    const Box fb = face_box(gm);
+    hipLaunchKernelGGL(k_adj_u_to_uu, box_grid(fb), dim3(EW_BLOCK), 0, (hipStream_t)stream, fb, gm, c3(kx, n),
+                       (const double *)ja, c3(uubar, n), f3(ubar, n));
+    FALM_CHECK_LAUNCH("k_adj_u_to_uu");
+    return FALM_OK;
+}
+
+falm_status falm_poisson_rhs_vjp(const falm_domain *dom, const void *ja, double dt, double maxdiag,
+                                 const void *rhsbar, void *uubar, void *stream) {
+    FALM_TRY(vjp_check(dom, {ja, rhsbar, uubar}));
+    if (!(dt * maxdiag != 0.0)) return FALM_ERR_INVALID;
+    const Geom gm = make_geom(dom);
+    const size_t n = ncell_of(dom);
+    const Box ib = interior_box(gm);
+    hipLaunchKernelGGL(k_adj_rhs, box_grid(ib), dim3(EW_BLOCK), 0, (hipStream_t)stream, ib, gm, (const double *)ja,
+                       1.0 / (dt * maxdiag), (const double *)rhsbar, f3(uubar, n));
+    FALM_CHECK_LAUNCH("k_adj_rhs");
+    return FALM_OK;
+}
+
+falm_status falm_project_p_vjp(const falm_domain *dom, double dt, const void *kx, const void *g, const void *ubar,
+                               const void *uubar, void *uabar, void *uuabar, void *pbar, void *stream) {
+    FALM_TRY(vjp_check(dom, {kx, g, ubar, uubar, uabar, uuabar, pbar}));
+    const Geom gm = make_geom(dom);
+    const size_t n = ncell_of(dom);
+    const Box fb = face_box(gm);
+    hipLaunchKernelGGL(k_adj_project, box_grid(fb), dim3(EW_BLOCK), 0, (hipStream_t)stream, fb, gm, dt, c3(kx, n),
+                       c3(g, n), c3(ubar, n), c3(uubar, n), f3(uabar, n), f3(uuabar, n), (double *)pbar);
+    FALM_CHECK_LAUNCH("k_adj_project");
+    return FALM_OK;
+}
+
+falm_status falm_sgs_vjp(const falm_domain *dom, double Cs, const void *u, const void *kx, const void *ja,
+                         const void *nutbar, void *ubar, void *stream) {
+    FALM_TRY(vjp_check(dom, {u, kx, ja, nutbar, ubar}));
+    const Geom gm = make_geom(dom);
+    const size_t n = ncell_of(dom);
+    const Box ib = interior_box(gm);
+    hipLaunchKernelGGL(k_adj_sgs, box_grid(ib), dim3(EW_BLOCK), 0, (hipStream_t)stream, ib, gm, Cs, c3(u, n),
+                       c3(kx, n), (const double *)ja, (const double *)nutbar, f3(ubar, n));
+    FALM_CHECK_LAUNCH("k_adj_sgs");
+    return FALM_OK;
+}
+
+falm_status falm_bc_velocity_vjp(const falm_domain *dom, const falm_face_bc faces[6], const void *xyz, double dt,
+                                 const void *unp1bar, void *usbar, void *unbar, void *stream) {
+    FALM_TRY(vjp_check(dom, {faces, xyz, unp1bar, usbar, unbar}));
+    const Geom gm = make_geom(dom);
+    const size_t n = ncell_of(dom);
+    const Box ib = interior_box(gm);
+    int type[6];
+    for (int f = 0; f < 6; f++) type[f] = faces[f].type;
+    long long nf = 0;
+    const FaceRange fr = face_range(gm, type, nf);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_adj_copy_interior, box_grid(ib), dim3(EW_BLOCK), 0, s, ib, gm, c3(unp1bar, n),
+                       f3(usbar, n));
+    if (nf > 0)
+        hipLaunchKernelGGL(k_adj_ubc, ew_grid((size_t)nf), dim3(EW_BLOCK), 0, s, gm, fr, (const double *)xyz, dt,
+                           c3(unp1bar, n), f3(usbar, n), f3(unbar, n));
+    FALM_CHECK_LAUNCH("k_adj_ubc");
+    return FALM_OK;
+}
+
+falm_status falm_bc_pressure_vjp(const falm_domain *dom, const int faces[6], void *pbar, void *stream) {
+    FALM_TRY(vjp_check(dom, {faces, pbar}));
+    const Geom gm = make_geom(dom);
+    long long nf = 0;
+    const FaceRange fr = face_range(gm, faces, nf);
+    if (nf > 0)
+        hipLaunchKernelGGL(k_adj_pbc, ew_grid((size_t)nf), dim3(EW_BLOCK), 0, (hipStream_t)stream, gm, fr,
+                           (double *)pbar);
+    FALM_CHECK_LAUNCH("k_adj_pbc");
+    return FALM_OK;
+}
 
 falm_status falm_case_window(falm_case *C, int nsteps, double *J) {
     if (!C || !J || nsteps < 1) return FALM_ERR_INVALID;

@@ -77,6 +77,13 @@ def _sigs(L):
         "falm_alm_set_pitch": [vp, ip, vp],
         "falm_alm_set_base": [vp, ip, vp, vp, vp],
         "falm_mesher_build": [C.c_char_p, C.c_char_p, C.c_char_p, ip],
+        "falm_pseudo_u_vjp": [dom, ip, dp, dp] + [vp] * 12,
+        "falm_u_to_uu_vjp": [dom, vp, vp, vp, vp, vp],
+        "falm_poisson_rhs_vjp": [dom, vp, dp, dp, vp, vp, vp],
+        "falm_project_p_vjp": [dom, dp] + [vp] * 8,
+        "falm_sgs_vjp": [dom, dp, vp, vp, vp, vp, vp, vp],
+        "falm_bc_velocity_vjp": [dom, C.POINTER(abi.FaceBC), vp, dp, vp, vp, vp, vp],
+        "falm_bc_pressure_vjp": [dom, C.POINTER(C.c_int), vp, vp],
         "falm_timing_enable": [ip],
         "falm_timing_read": [C.POINTER(ip), C.POINTER(dp), C.POINTER(dp)],
     }

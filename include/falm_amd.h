@@ -411,6 +411,46 @@ falm_status falm_case_adjoint(falm_case *c, int nsteps, int ckpt, double adj_rto
 /* Per-blade pitch of one turbine (8 values, degrees; NULL = 0), the
  * falm_turbine::pitch_deg of falm_alm_create. */
 falm_status falm_alm_set_pitch(falm_alm *alm, int turbine, const double *pitch_deg);
+
+/* ------------------------------------------- per-stage adjoints (VJPs)
+ * No reference counterpart.  Vector-Jacobian products of the FP64 forward
+ * stages above, the transposes the rotor-power adjoint chains: given the
+ * adjoint of a stage's output, each adds the adjoint of its inputs to the
+ * *bar outputs (+=, so several contributions accumulate), except where
+ * noted.  FP64 device arrays in the Matrix layout; atomics make the sums
+ * order-independent to rounding.
+ *
+ * falm_pseudo_u: adjoint w.r.t. u in both roles (un = u, as the step drivers
+ * call it), uu, nut and ff; Upwind3, UTOPIA and KK only (else
+ * FALM_ERR_UNSUPPORTED); ffbar is overwritten on interior cells (= dt uabar). */
+falm_status falm_pseudo_u_vjp(const falm_domain *dom, int adv_scheme, double ReI, double dt,
+                              const void *u, const void *uu, const void *nut, const void *kx,
+                              const void *g, const void *ja, const void *uabar, void *ubar,
+                              void *uubar, void *nutbar, void *ffbar, void *stream);
+/* falm_u_to_uu: uubar (face region) -> ubar */
+falm_status falm_u_to_uu_vjp(const falm_domain *dom, const void *kx, const void *ja,
+                             const void *uubar, void *ubar, void *stream);
+/* falm_poisson_rhs: rhsbar (interior) -> uubar */
+falm_status falm_poisson_rhs_vjp(const falm_domain *dom, const void *ja, double dt, double maxdiag,
+                                 const void *rhsbar, void *uubar, void *stream);
+/* falm_project_p: (ubar, uubar) of the outputs -> uabar, uuabar, pbar */
+falm_status falm_project_p_vjp(const falm_domain *dom, double dt, const void *kx, const void *g,
+                               const void *ubar, const void *uubar, void *uabar, void *uuabar,
+                               void *pbar, void *stream);
+/* falm_sgs (Smagorinsky): nutbar -> ubar; a zero subgradient where |S| = 0 */
+falm_status falm_sgs_vjp(const falm_domain *dom, double Cs, const void *u, const void *kx,
+                         const void *ja, const void *nutbar, void *ubar, void *stream);
+/* falm_bc_velocity (INLET, CONVECTIVE, SLIP, WALL0 faces; no noise): the
+ * adjoint unp1bar of the filled u -> usbar (the interior u it was filled
+ * from; its interior is overwritten with unp1bar's) and unbar (uprev, read
+ * by convective faces). */
+falm_status falm_bc_velocity_vjp(const falm_domain *dom, const falm_face_bc faces[6],
+                                 const void *xyz, double dt, const void *unp1bar, void *usbar,
+                                 void *unbar, void *stream);
+/* falm_bc_pressure: in place on pbar (ghost adjoints moved to the interior
+ * cells they were copied from, then zeroed) */
+falm_status falm_bc_pressure_vjp(const falm_domain *dom, const int faces[6], void *pbar,
+                                 void *stream);
 /* Base position and base velocity of one turbine from the next step on
  * (velocity NULL = 0); the surge motion of falm_turbine adds to both.
  * Drives base motion read from a file: the reference parses a file name

@@ -1842,6 +1842,14 @@ int orc_adj_ubc(const falm_domain *dom, const falm_face_bc faces[6], const doubl
     adj_ubc(Dom(dom), faces, xyz, dt, unp1bar, ustarbar, unbar);
     return FALM_OK;
 }
+int orc_adj_rhs(const falm_domain *dom, const double *ja, double scale, const double *rhsbar, double *uubar) {
+    adj_rhs(Dom(dom), ja, scale, rhsbar, uubar);
+    return FALM_OK;
+}
+int orc_adj_pbc(const falm_domain *dom, const int faces[6], double *pbar) {
+    adj_pbc(Dom(dom), faces, pbar);
+    return FALM_OK;
+}
 
 static double step_power(OrcCase *C) {
     if (!C->alm) return 0.0;

@@ -75,6 +75,8 @@ def lib():
             "orc_adj_u_to_uu": [dom, vp, vp, vp, vp],
             "orc_adj_project": [dom, C.c_double, vp, vp, vp, vp, vp, vp, vp],
             "orc_adj_ubc": [dom, C.POINTER(abi.FaceBC), vp, C.c_double, vp, vp, vp],
+            "orc_adj_rhs": [dom, vp, C.c_double, vp, vp],
+            "orc_adj_pbc": [dom, C.POINTER(C.c_int), vp],
             "orc_num_threads": [],
         }
         for name, args in sig.items():
@@ -395,6 +397,19 @@ def adj_project(dom, dt, kx, g, ustarbar, uunbar):
     check(lib().orc_adj_project(C.byref(dom), dt, ptr(kx), ptr(g), ptr(ustarbar), ptr(uunbar), ptr(uabar),
                                 ptr(uustarbar), ptr(pbar)))
     return uabar, uustarbar, pbar
+
+
+def adj_rhs(dom, ja, scale, rhsbar):
+    uubar = np.zeros(3 * dom.ncell)
+    check(lib().orc_adj_rhs(C.byref(dom), ptr(ja), scale, ptr(rhsbar), ptr(uubar)))
+    return uubar
+
+
+def adj_pbc(dom, faces, pbar):
+    out = np.array(pbar, dtype=np.float64, copy=True)
+    f = (C.c_int * 6)(*faces)
+    check(lib().orc_adj_pbc(C.byref(dom), f, ptr(out)))
+    return out
 
 
 def adj_ubc(dom, faces, xyz, dt, unp1bar):
