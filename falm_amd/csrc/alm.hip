@@ -1015,6 +1015,13 @@ falm_status falm_alm_set_base(falm_alm *A, int turbine, const double *base, cons
     return FALM_OK;
 }
 
+falm_status falm_alm_vjp(falm_alm *A, const void *u, double t, const void *ffbar, double wP, void *ubar,
+                         void *g_pitch, void *g_twist, void *stream) {
+    if (!A || !u || !ffbar || !ubar || !g_pitch || !g_twist) return FALM_ERR_INVALID;
+    return falm::alm_adjoint(A, (const double *)u, t, (const double *)ffbar, wP, (double *)ubar, (double *)g_pitch,
+                             (double *)g_twist, (hipStream_t)stream);
+}
+
 falm_status falm_alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, void *stream) {
     if (!A || !u || !ff) return FALM_ERR_INVALID;
     return falm::alm_step(A, prec, u, ff, t, (hipStream_t)stream, false);

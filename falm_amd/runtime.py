@@ -84,6 +84,7 @@ def _sigs(L):
         "falm_sgs_vjp": [dom, dp, vp, vp, vp, vp, vp, vp],
         "falm_bc_velocity_vjp": [dom, C.POINTER(abi.FaceBC), vp, dp, vp, vp, vp, vp],
         "falm_bc_pressure_vjp": [dom, C.POINTER(C.c_int), vp, vp],
+        "falm_alm_vjp": [vp, vp, dp, vp, dp, vp, vp, vp, vp],
         "falm_timing_enable": [ip],
         "falm_timing_read": [C.POINTER(ip), C.POINTER(dp), C.POINTER(dp)],
     }
@@ -178,6 +179,12 @@ class Alm:
         import numpy as np
         v = np.ascontiguousarray(off, dtype=np.float64)
         call("falm_alm_set_twist_offset", self.h, v.ctypes.data_as(C.c_void_p))
+
+    def set_pitch(self, turbine, pitch_deg):
+        import numpy as np
+        v = np.zeros(8)
+        v[:len(pitch_deg)] = pitch_deg
+        call("falm_alm_set_pitch", self.h, turbine, v.ctypes.data_as(C.c_void_p))
 
     def get(self, stream=None):
         import numpy as np

@@ -451,6 +451,14 @@ falm_status falm_bc_velocity_vjp(const falm_domain *dom, const falm_face_bc face
  * cells they were copied from, then zeroed) */
 falm_status falm_bc_pressure_vjp(const falm_domain *dom, const int faces[6], void *pbar,
                                  void *stream);
+/* falm_alm_step (FP64) at time t: the adjoint ffbar of the projected force
+ * field and the weight wP of the rotor power sum_t Omega_t torque_t ->
+ * ubar (3 columns) and the adjoints of falm_turbine::pitch_deg
+ * (g_pitch[n_turbine*8]) and of the twist offsets (g_twist[n_ap_per_blade]),
+ * per degree, all device arrays, accumulated.  Refreshes the handle's point
+ * positions and forces at t (as a step would), not ff or the force box. */
+falm_status falm_alm_vjp(falm_alm *alm, const void *u, double t, const void *ffbar, double wP,
+                         void *ubar, void *g_pitch, void *g_twist, void *stream);
 /* Base position and base velocity of one turbine from the next step on
  * (velocity NULL = 0); the surge motion of falm_turbine adds to both.
  * Drives base motion read from a file: the reference parses a file name

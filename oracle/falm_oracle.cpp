@@ -2026,6 +2026,13 @@ int orc_case_adjoint(void *h, int nsteps, int ckpt, double adj_rtol, double *J, 
     return FALM_OK;
 }
 
+/* adj_alm with a C signature (tests): accumulates ubar, gpitch, gtwist */
+int orc_alm_vjp(void *h, const double *u, double t, const double *ffbar, double wP, double *ubar, double *gpitch,
+                double *gtwist) {
+    adj_alm((OrcAlm *)h, u, t, ffbar, wP, ubar, gpitch, gtwist);
+    return FALM_OK;
+}
+
 int orc_alm_set_pitch(void *h, int turbine, const double *pitch_deg) {
     OrcAlm *A = (OrcAlm *)h;
     if (turbine < 0 || turbine >= A->nt) return FALM_ERR_INVALID;

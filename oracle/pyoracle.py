@@ -70,6 +70,7 @@ def lib():
             "orc_case_adjoint": [vp, ip, ip, C.c_double, dp, dp, dp, dp],
             "orc_alm_set_pitch": [vp, ip, dp],
             "orc_alm_set_base": [vp, ip, dp, dp],
+            "orc_alm_vjp": [vp, vp, C.c_double, vp, C.c_double, vp, vp, vp],
             "orc_adj_sgs": [dom, C.c_double, vp, vp, vp, vp, vp],
             "orc_adj_pseudo_u": [dom, C.c_double, C.c_double, C.c_double] + [vp] * 11,
             "orc_adj_u_to_uu": [dom, vp, vp, vp, vp],
@@ -264,6 +265,13 @@ class Alm:
         v = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float64)
         check(lib().orc_alm_set_base(self.h, turbine, b.ctypes.data_as(C.POINTER(C.c_double)),
                                      None if v is None else v.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def vjp(self, u, t, ffbar, wP):
+        """(ubar, gpitch[nt*8], gtwist[napb]) of adj_alm."""
+        ub = np.zeros(3 * self.dom.ncell)
+        gp, gt = np.zeros(8 * self.cfg.n_turbine), np.zeros(self.cfg.n_ap_per_blade)
+        check(lib().orc_alm_vjp(self.h, ptr(u), t, ptr(ffbar), wP, ptr(ub), ptr(gp), ptr(gt)))
+        return ub, gp, gt
 
     def get(self):
         nap, nt = self.cfg.n_ap, self.cfg.n_turbine

@@ -5,7 +5,7 @@ on seeded inputs (1e-12 relative; the device sums with atomics, so only the
 order of additions differs), and, independently of the oracle, against the
 library's own forward stage on the GPU: the transposition identity
 <J v, w> = <v, J^T w> for the linear stages and central differences for the
-nonlinear ones (pseudo-velocity, Smagorinsky)."""
+nonlinear ones (pseudo-velocity, Smagorinsky, actuator line)."""
 import ctypes as C
 
 import numpy as np
@@ -231,3 +231,51 @@ def test_bc_pressure_vjp(gpu):
     gpu.call("falm_bc_pressure", C.byref(dom), abi.F64, fa, gpu.ptr(p), None)
     lhs, rhs = from_dev(p) @ w, v @ got
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_alm_vjp(gpu):
+    """The actuator-line step at time t transposed: u, blade pitch and twist
+    offsets from the adjoint of the projected force plus a weight on the
+    rotor power; against the oracle, and against central differences of the
+    GPU's own ALM step."""
+    from falm_amd import cases
+    dom = domain((32, 20, 20))
+    h = 1.0 / 8.0
+    setup = cases.rotor(h, base=(1.6, 1.25, 1.25), radius=0.5, tsr=7.0, n_ap_per_blade=8, angle_type=2,
+                        motion=(np.radians(3.0), 0.8, 0.3), cutoff=0.0)
+    x, y, z = ((np.arange(dom.shape[m]) - 1.5) * h for m in range(3))
+    rng = np.random.default_rng(41)
+    u = smooth_field(dom, 3, rng, base=1.0, amp=0.2)
+    ffbar = rng.standard_normal(3 * dom.ncell) * interior_mask(dom, 3)
+    t, wP = 0.37, 0.25
+    nt, napb = setup.n_turbine, setup.n_ap_per_blade
+    ref = O.Alm(dom, setup, x, y, z)
+    r_u, r_gp, r_gt = ref.vjp(u, t, ffbar, wP)
+    A = gpu.Alm(dom, setup, x, y, z)
+    ud, fb = to_dev(u), to_dev(ffbar)
+    ub, gp, gt = to_dev(np.zeros(3 * dom.ncell)), to_dev(np.zeros(8 * nt)), to_dev(np.zeros(napb))
+    gpu.call("falm_alm_vjp", A.h, gpu.ptr(ud), t, gpu.ptr(fb), wP, *_ptrs(gpu, ub, gp, gt), None)
+    g_u, g_p, g_t = from_dev(ub), from_dev(gp), from_dev(gt)
+    assert rel_err(g_u, r_u) < 1e-10 and rel_err(g_p, r_gp) < 1e-10 and rel_err(g_t, r_gt) < 1e-10
+    assert np.any(g_p[:3] != 0) and np.all(g_p[3:8] == 0)  # three blades
+    tip = setup.turbines[0].tip_rate
+
+    def L(uh, pitch=(0.0, 0.0, 0.0), twist=None):
+        A.set_pitch(0, list(pitch))
+        A.set_twist_offset(np.zeros(napb) if twist is None else twist)
+        ff, uh_d = to_dev(np.zeros(3 * dom.ncell)), to_dev(uh)
+        A.step(abi.F64, uh_d, ff, t)
+        _, _, tq, _ = A.get()
+        return from_dev(ff) @ ffbar + wP * tip * tq[0]
+
+    v = rng.standard_normal(3 * dom.ncell) * interior_mask(dom, 3)
+    eps = 1e-6
+    fd = (L(u + eps * v) - L(u - eps * v)) / (2 * eps)
+    assert abs(fd - v @ g_u) <= 1e-6 * abs(fd)
+    e = 1e-5  # degrees
+    fd = (L(u, pitch=(0.0, e, 0.0)) - L(u, pitch=(0.0, -e, 0.0))) / (2 * e)
+    assert abs(fd - g_p[1]) <= 1e-5 * abs(fd) + 1e-12
+    dt_ = np.zeros(napb)
+    dt_[3] = e
+    fd = (L(u, twist=dt_) - L(u, twist=-dt_)) / (2 * e)
+    assert abs(fd - g_t[3]) <= 1e-5 * abs(fd) + 1e-12
