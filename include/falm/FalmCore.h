@@ -140,9 +140,12 @@ public:
             if (falmEq.pc_type == LSType::SOR) std::printf("\trelaxation factor %e\n", falmEq.pc_relax_factor);
         }
         if (params.contains("inflow")) {
+            const Json &in = params["inflow"];
             std::printf("Inflow Params\n");
-            std::printf("\tProfile %s\n", params["inflow"]["type"].get<std::string>().c_str());
-            std::printf("\tVelocity %e\n", params["inflow"]["velocity"].get<Real>());
+            if (in.contains("type") && in["type"].is_string())
+                std::printf("\tProfile %s\n", in["type"].get<std::string>().c_str());
+            if (in.contains("velocity") && in["velocity"].is_number())
+                std::printf("\tVelocity %e\n", in["velocity"].get<Real>());
         }
         std::printf("SETUP INFO END\n");
         const Int gc = cpm.gc;
