@@ -77,6 +77,32 @@ __device__ inline K7Pos k7_pos(const K7Tiling &kt, const Geom &g) {
 template <class T>
 using K7Lds = T[2][K7_BY + 2][K7_BX + 2];
 
+// The halo cell a thread fills in the LDS plane: the x halo (rows 1..BY,
+// columns 0 and BX+1) on the first 2*BY threads, the y halo (rows 0 and BY+1,
+// columns 1..BX) on the next 2*BX; (gi, gj) is its cell column.
+struct K7Halo {
+    int row, col, gi, gj;
+    bool slot;
+};
+
+__device__ inline K7Halo k7_halo(const K7Pos &P) {
+    K7Halo h;
+    const int s = (int)threadIdx.x;
+    h.slot = s < K7_HALO;
+    h.row = h.col = 0;
+    if (s < 2 * K7_BY) {
+        h.row = 1 + (s >> 1);
+        h.col = (s & 1) ? K7_BX + 1 : 0;
+    } else if (h.slot) {
+        const int r = s - 2 * K7_BY, q = r / K7_BX;
+        h.row = q ? K7_BY + 1 : 0;
+        h.col = 1 + (r - q * K7_BX);
+    }
+    h.gi = P.i0 - 1 + h.col;
+    h.gj = P.j0 - 1 + h.row;
+    return h;
+}
+
 // One stencil array walked through the planes of a K7 column tile.  Loads
 // use byte offsets from the array's base (ld_at), so arrays walked with the
 // same column share one address register.
@@ -95,18 +121,11 @@ struct Plane7 {
         lds = l;
         lane = P.lane;
         wy = P.wy;
-        const int s = (int)threadIdx.x;
-        hslot = s < K7_HALO;
-        hrow = hcol = 0;
-        if (s < 2 * K7_BY) {  // x halo: rows 1..BY, columns 0 and BX+1
-            hrow = 1 + (s >> 1);
-            hcol = (s & 1) ? K7_BX + 1 : 0;
-        } else if (hslot) {   // y halo: rows 0 and BY+1, columns 1..BX
-            const int r = s - 2 * K7_BY, q = r / K7_BX;
-            hrow = q ? K7_BY + 1 : 0;
-            hcol = 1 + (r - q * K7_BX);
-        }
-        const int gi = P.i0 - 1 + hcol, gj = P.j0 - 1 + hrow;
+        const K7Halo h = k7_halo(P);
+        hslot = h.slot;
+        hrow = h.row;
+        hcol = h.col;
+        const int gi = h.gi, gj = h.gj;
         hok = hslot && gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny;
         cb = (unsigned)P.col0 * sizeof(T);
         // a halo slot outside the array reads the column's own cell (value unused)

@@ -189,6 +189,139 @@ __global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, 
     }
 }
 
+// A Krylov vector and its one-pass SOR preconditioning (pc_maxit = 1: the
+// black half-sweep from zero, then the red one) in one k-marching pass:
+//   MODE 0: v = r on the first iteration, else r + beta(vin - omega q)
+//           (kernel_PBiCGStab_1, src/dev/FalmEqDevCall.cu)
+//   MODE 1: v = r - alpha q (kernel_PBiCGStab_2)
+//   black cells: vh = 0 + omega_pc (v / a_c)              (first half-sweep)
+//   red cells:   vh = 0 + omega_pc ((v - A'vh) / a_c)      (second half-sweep)
+// where A'vh sums the six neighbours, all black.  Their vh is recomputed from
+// r, q, vin and the diagonal at the neighbour with the expression the
+// neighbour's own thread uses (in LDS for x and y, along the column in
+// registers for k); cells outside the interior contribute zero, as the
+// preconditioned vector's ghost cells (zero from allocation, never written)
+// do in the separate sweeps.  MODE 0 reads p from vin and writes v, another
+// buffer: neighbouring workgroups read vin at this tile's cells.
+template <class T, class OP, int MODE>
+__global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
+                                                        const T *__restrict__ vin, const T *__restrict__ q,
+                                                        T *__restrict__ v, T *__restrict__ vh, const DevState *st,
+                                                        T omega_pc) {
+    if (st->done) return;
+    __shared__ K7Lds<T> lw;
+    const K7Pos Q = k7_pos(kt, g);
+    const int sz = g.nxy;
+    const bool upd = MODE == 1 || !st->first;  // q (and vin) are read
+    T c1 = T(0), c2 = T(0);
+    if (MODE == 0) {
+        c1 = (T)st->beta;
+        c2 = (T)st->omega;
+    } else {
+        c1 = (T)st->alpha;
+    }
+    const auto vec = [&](T rv, T pv, T qv) -> T {
+        if (MODE == 1) return rv - c1 * qv;
+        return upd ? rv + c1 * (pv - c2 * qv) : rv;
+    };
+    const auto ca = a.column(Q.act ? Q.i : kt.box.lo[0], Q.act ? Q.j : kt.box.lo[1]);
+    using Col = typename std::decay<decltype(ca)>::type;
+    using Cell = typename Col::Cell;
+    static_assert(sizeof(decltype(ca.diag_at(0u))) == sizeof(T), "operator and vector precision differ");
+    const int lo0 = kt.box.lo[0], hi0 = lo0 + kt.box.n[0], lo1 = kt.box.lo[1], hi1 = lo1 + kt.box.n[1];
+    const int lo2 = kt.box.lo[2], hi2 = lo2 + kt.box.n[2];
+    const K7Halo h = k7_halo(Q);
+    const bool hok = h.slot && h.gi >= 0 && h.gi < g.nx && h.gj >= 0 && h.gj < g.ny;
+    const bool hin = hok && h.gi >= lo0 && h.gi < hi0 && h.gj >= lo1 && h.gj < hi1;
+    const unsigned szt = (unsigned)sz * sizeof(T);
+    const unsigned cb = (unsigned)Q.col0 * sizeof(T);
+    const unsigned hb = hok ? (unsigned)(h.gi + g.nx * h.gj) * sizeof(T) : cb;
+    const int cpar = (Q.i + Q.j + g.osum) & 1, hpar = (h.gi + h.gj + g.osum) & 1;
+    // the black half-sweep's value (zero on red cells and outside the interior)
+    const auto wval = [&](T vv, T d, bool in, int par, int k) -> T {
+        return (in && ((par + k) & 1) == FALM_COLOR_BLACK) ? T(0) + omega_pc * (vv / d) : T(0);
+    };
+    const auto kin = [&](int k) { return k >= lo2 && k < hi2; };
+    const auto load = [&](unsigned ob, T &rv, T &pv, T &qv) {
+        rv = ld_at(r, ob);
+        if (upd) {
+            qv = ld_at(q, ob);
+            if (MODE == 0) pv = ld_at(vin, ob);
+        }
+    };
+    // registers: w at k-1 and k, v and the cell coefficients at k, the
+    // column's raw values at k+1 and the halo cell's at the next LDS plane
+    T wm, wc, vc;
+    Cell ec;
+    T nr, np = T(0), nq = T(0);
+    Cell ne;
+    T hr = T(0), hp = T(0), hq = T(0), hd = T(1);
+    {
+        unsigned ob = cb + (unsigned)(Q.k0 - 1) * szt;
+        T rm, pm = T(0), qm = T(0), rc, pc = T(0), qc = T(0);
+        load(ob, rm, pm, qm);
+        const T dm = ca.diag_at(ob);
+        load(ob + szt, rc, pc, qc);
+        ec = ca.load(ob + szt, Q.k0);
+        load(ob + 2 * szt, nr, np, nq);
+        ne = ca.load(ob + 2 * szt, Q.k0 + 1);
+        if (h.slot) {
+            const unsigned oh = hb + (unsigned)Q.k0 * szt;
+            load(oh, hr, hp, hq);
+            hd = ca.diag_at(oh);
+        }
+        wm = wval(vec(rm, pm, qm), dm, Q.act && kin(Q.k0 - 1), cpar, Q.k0 - 1);
+        vc = vec(rc, pc, qc);
+        wc = wval(vc, Col::diag(ec), Q.act, cpar, Q.k0);
+        lw[0][Q.wy + 1][Q.lane + 1] = wc;
+        if (h.slot) lw[0][h.row][h.col] = wval(vec(hr, hp, hq), hd, hin, hpar, Q.k0);
+    }
+    __syncthreads();
+    int bf = 0;
+    unsigned ob = cb + (unsigned)Q.k0 * szt;
+    for (int k = Q.k0; k < Q.k1; k++, ob += szt) {
+        const bool more = k + 1 < Q.k1;
+        T r2 = T(0), p2 = T(0), q2 = T(0);
+        Cell e2 = ne;
+        if (more) {
+            load(ob + 2 * szt, r2, p2, q2);
+            e2 = ca.load(ob + 2 * szt, k + 2);
+            if (h.slot) {
+                const unsigned oh = hb + (unsigned)(k + 1) * szt;
+                load(oh, hr, hp, hq);
+                hd = ca.diag_at(oh);
+            }
+        }
+        const T vn = vec(nr, np, nq);
+        const T wn = wval(vn, Col::diag(ne), Q.act && kin(k + 1), cpar, k + 1);
+        if (Q.act) {
+            st_at(v, ob, vc);
+            T out = wc;
+            if (((cpar + k) & 1) != FALM_COLOR_BLACK) {
+                // kernel_Struct3d7p_SOR's update from x = 0 (src/dev/FalmEqDevCall.cu:195-215)
+                const T s = ca.apply(ec, T(0), lw[bf][Q.wy + 1][Q.lane + 2], lw[bf][Q.wy + 1][Q.lane],
+                                     lw[bf][Q.wy + 2][Q.lane + 1], lw[bf][Q.wy][Q.lane + 1], wn, wm);
+                out = T(0) + omega_pc * ((vc - s) / Col::diag(ec));
+            }
+            st_at(vh, ob, out);
+        }
+        if (more) {
+            lw[bf ^ 1][Q.wy + 1][Q.lane + 1] = wn;
+            if (h.slot) lw[bf ^ 1][h.row][h.col] = wval(vec(hr, hp, hq), hd, hin && kin(k + 1), hpar, k + 1);
+        }
+        wm = wc;
+        wc = wn;
+        vc = vn;
+        ec = ne;
+        nr = r2;
+        np = p2;
+        nq = q2;
+        ne = e2;
+        bf ^= 1;
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------
 // Three red-black SOR half-sweeps (colours c0, 1-c0, c0) in one pass over
 // memory, out of place (x_in -> x_out).  A workgroup owns an S3_BX x S3_BY
@@ -806,6 +939,7 @@ struct falm_solver {
     float *a32 = nullptr;         // mixed: float columns (7 if not separable, else diagonal only)
     double *r64own = nullptr;     // residual when the caller passes r = NULL (double solvers)
     int last_it = 0;              // iterations of the previous solve
+    int pflip = 0;                // fused SOR path: p lives in w (1) or p (0)
 };
 
 namespace falm {
@@ -942,26 +1076,55 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     T *rr = (T *)S->rr, *p = (T *)S->p, *q = (T *)S->q, *sv = (T *)S->s, *pp = (T *)S->pp, *ss = (T *)S->ss,
       *t = (T *)S->t;
     const T wpc = (T)c.pc_relax;
-    hipLaunchKernelGGL((k_bicg_p<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, p, (const T *)q, pp,
-                       (const DevState *)S->st, c.pc_type, wpc);
-    FALM_CHECK_LAUNCH("k_bicg_p");
+    // one-pass SOR preconditioner: each vector and its two half-sweeps in
+    // one kernel (k_bicg_vsor_km); p alternates between p and w, since the
+    // kernel reads the old p at the neighbours of the cells it writes.  A
+    // pass's first iteration sets p = r, so the alternation may continue
+    // across iterations that a finished pass skips.
+    const bool vsor = c.pc_type == FALM_LS_SOR && c.pc_maxit <= 1 && S->kt.blocks > 0;
+    T *pcp = pp;
     falm_status pst;
-    // the preconditioned p lands in pp or, out of place, in w
-    T *pcp = pc_rest<T, OP>(S, a, pp, p, (T *)S->w, s, pst);
-    FALM_TRY(pst);
+    if (vsor) {
+        T *pin = (T *)(S->pflip ? S->w : S->p), *pout = (T *)(S->pflip ? S->p : S->w);
+        S->pflip ^= 1;
+        TimedLaunch tm(FALM_TIME_SOR, s);
+        hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 0>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)pin,
+                           (const T *)q, pout, pp, (const DevState *)S->st, wpc);
+        // r and the diagonal read, p and p^ written (a first iteration; later
+        // ones also read p and q)
+        tm.done((double)tl.box.size() * 4.0 * sizeof(T));
+        FALM_CHECK_LAUNCH("k_bicg_vsor_km");
+    } else {
+        hipLaunchKernelGGL((k_bicg_p<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, p, (const T *)q, pp,
+                           (const DevState *)S->st, c.pc_type, wpc);
+        FALM_CHECK_LAUNCH("k_bicg_p");
+        // the preconditioned p lands in pp or, out of place, in w
+        pcp = pc_rest<T, OP>(S, a, pp, p, (T *)S->w, s, pst);
+        FALM_TRY(pst);
+    }
     TimedLaunch tm1(FALM_TIME_MV, s);
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)pcp, q, (const T *)rr,
                        (const DevState *)S->st, S->partials);
     tm1.done((double)tl.box.size() * 4.0 * sizeof(T));
     hipLaunchKernelGGL(k_fin_alpha, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot");
-    hipLaunchKernelGGL((k_bicg_s<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, (const T *)q, sv, ss,
-                       (const DevState *)S->st, c.pc_type, wpc);
-    FALM_CHECK_LAUNCH("k_bicg_s");
-    // the preconditioned s lands in ss or, out of place, in pp (free once
-    // p's sweeps have read it)
-    T *pcs = pc_rest<T, OP>(S, a, ss, sv, pcp == pp ? ss : pp, s, pst);
-    FALM_TRY(pst);
+    T *pcs = ss;
+    if (vsor) {
+        TimedLaunch tm(FALM_TIME_SOR, s);
+        hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)nullptr,
+                           (const T *)q, sv, ss, (const DevState *)S->st, wpc);
+        // r, q and the diagonal read, s and s^ written
+        tm.done((double)tl.box.size() * 5.0 * sizeof(T));
+        FALM_CHECK_LAUNCH("k_bicg_vsor_km");
+    } else {
+        hipLaunchKernelGGL((k_bicg_s<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, (const T *)q, sv, ss,
+                           (const DevState *)S->st, c.pc_type, wpc);
+        FALM_CHECK_LAUNCH("k_bicg_s");
+        // the preconditioned s lands in ss or, out of place, in pp (free once
+        // p's sweeps have read it)
+        pcs = pc_rest<T, OP>(S, a, ss, sv, pcp == pp ? ss : pp, s, pst);
+        FALM_TRY(pst);
+    }
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 2>), FALM_KM(S), s, S->kt, g, a, (const T *)pcs, t, (const T *)sv,
                        (const DevState *)S->st, S->partials);
     hipLaunchKernelGGL(k_fin_omega, FALM_FIN, s, S->st, (const double *)S->partials, nb);

@@ -311,6 +311,44 @@ def test_declared_matrix_paths(gpu, stretch, precision):
     assert np.sqrt(np.sum(O.residual(dom, a, from_dev(dx), b)[m] ** 2)) / m.sum() <= 1.01e-12
 
 
+@pytest.mark.parametrize("stretch", [0.0, 0.02])
+@pytest.mark.parametrize("precision", [abi.SOLVE_F64, abi.SOLVE_MIXED])
+def test_one_pass_sor_preconditioner(gpu, stretch, precision):
+    """pc_maxit = 1 runs each Krylov vector and both SOR half-sweeps as one
+    k-marching kernel, which recomputes the black values at the neighbours of
+    a tile and alternates p between two buffers.  On a domain of several
+    column tiles and k chunks (70 x 10 x 34 interior: 2 x 2 x 3 workgroups)
+    the first four iterations follow the oracle's with the caller's seven
+    columns and with the declared (compressed, on the uniform mesh) operator,
+    and a tight solve reaches its tolerance in both precisions."""
+    dom = domain((74, 14, 38))
+    a, _, _, _ = _matrix(dom, abi.POISSON_OUTLET_XPLUS, stretch=stretch)
+    rng = np.random.default_rng(41)
+    b = rng.standard_normal(dom.ncell) * interior_mask(dom)
+    m = interior_mask(dom)
+    da, db = to_dev(a), to_dev(b)
+    if precision == abi.SOLVE_F64:
+        cfg = abi.solver_config(maxit=4, tol=1e-30, pc_type=abi.LS_SOR, pc_maxit=1, pc_relax=1.2)
+        x_ref, r_ref = np.zeros(dom.ncell), np.zeros(dom.ncell)
+        info_ref = O.Solver(dom, cfg).solve(a, x_ref, b, r_ref)
+        for declared in (False, True):
+            S = gpu.Solver(dom, cfg)
+            if declared:
+                S.set_matrix(da)
+            dx, dr = to_dev(np.zeros(dom.ncell)), to_dev(np.zeros(dom.ncell))
+            info = S.solve(da, dx, db, dr)
+            assert info.it == info_ref.it == 4
+            assert abs(info.err - info_ref.err) <= 1e-10 * info_ref.err, declared
+            assert rel_err(from_dev(dx), x_ref) < 1e-11, declared
+    cfg = abi.solver_config(maxit=3000, tol=1e-11, pc_type=abi.LS_SOR, pc_maxit=1, pc_relax=1.0, precision=precision)
+    S = gpu.Solver(dom, cfg)
+    S.set_matrix(da)
+    dx, dr = to_dev(np.zeros(dom.ncell)), to_dev(np.zeros(dom.ncell))
+    info = S.solve(da, dx, db, dr)
+    assert info.status == abi.FALM_OK
+    assert np.sqrt(np.sum(O.residual(dom, a, from_dev(dx), b)[m] ** 2)) / m.sum() <= 1.01e-11
+
+
 @pytest.mark.parametrize("precision", [abi.SOLVE_F64, abi.SOLVE_MIXED])
 def test_solve_without_residual_output(gpu, precision):
     """r = NULL: the solve is the same (same x, iterations and err) and only
