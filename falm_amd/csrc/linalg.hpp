@@ -53,6 +53,7 @@ struct OpCols {
         an = ld_at(c[4], ob); ab = ld_at(c[5], ob); at = ld_at(c[6], ob);
     }
     __device__ inline TA diag_at(unsigned ob) const { return ld_at(c[0], ob); }
+    __device__ inline TA diag_k(unsigned ob, int) const { return ld_at(c[0], ob); }
     // the view a k-marching kernel keeps for one column (nothing to hoist)
     __device__ inline OpCols column(int, int) const { return *this; }
     // a cell's coefficients, loaded ahead of use (k-marching prefetch)
@@ -73,22 +74,41 @@ struct OpCols {
     __device__ inline static TA diag(const Cell &q) { return q.v[0]; }
 };
 
+constexpr int DIAG_CLS = 4;  // diagonal classes per axis at most (OpSep)
+
 // One column (i, j) of a separable operator: the four in-plane coefficients
 // in registers, loaded once before a kernel walks the column's planes.
+//
+// With diagonal classes (see OpSep) the column also holds its diagonal on
+// each z run, and a plane's diagonal is picked by comparing the plane index
+// with the run starts (kernel arguments) instead of being loaded.
 template <class TA>
 struct OpSepCol {
     TA w, e, s, n;
     const TA *__restrict__ ac;
     const TA *__restrict__ ab, *__restrict__ at;
+    TA d0, d1, d2, d3;  // the diagonal on z runs 0..3
+    int z1, z2, z3;     // first plane of z runs 1..3
+    bool cls;
+    // selects of values read unconditionally (a select of member addresses
+    // would keep the column view in scratch memory)
+    __device__ inline TA pick(int k) const {
+        const TA a0 = d0, a1 = d1, a2 = d2, a3 = d3;
+        TA v = k >= z1 ? a1 : a0;
+        v = k >= z2 ? a2 : v;
+        return k >= z3 ? a3 : v;
+    }
     __device__ inline void get_at(unsigned ob, int, int, int k, TA &c0, TA &w_, TA &e_, TA &s_, TA &n_, TA &b_,
                                   TA &t_) const {
-        c0 = ld_at(ac, ob); w_ = w; e_ = e; s_ = s; n_ = n; b_ = ab[k]; t_ = at[k];
+        c0 = cls ? pick(k) : ld_at(ac, ob); w_ = w; e_ = e; s_ = s; n_ = n; b_ = ab[k]; t_ = at[k];
     }
     __device__ inline TA diag_at(unsigned ob) const { return ld_at(ac, ob); }
+    // the diagonal of the column's cell in plane k (byte offset ob)
+    __device__ inline TA diag_k(unsigned ob, int k) const { return cls ? pick(k) : ld_at(ac, ob); }
     struct Cell {
         TA c0, b, t;
     };
-    __device__ inline Cell load(unsigned ob, int k) const { return Cell{ld_at(ac, ob), ab[k], at[k]}; }
+    __device__ inline Cell load(unsigned ob, int k) const { return Cell{diag_k(ob, k), ab[k], at[k]}; }
     // the reference's summation order (src/dev/MVDevCall.cu:39)
     template <class R>
     __device__ inline R apply(const Cell &q, R xc, R xe, R xw, R xn, R xs, R xt, R xb) const {
@@ -97,11 +117,29 @@ struct OpSepCol {
     __device__ inline static TA diag(const Cell &q) { return q.c0; }
 };
 
+//
+// Diagonal classes: on a uniform mesh a cell's diagonal is set by which of
+// its six couplings the assembly kept, so it takes a few values.  Each axis's
+// indices are classed by their (minus, plus) coupling pair; the classes along
+// an axis form at most DIAG_CLS runs of consecutive indices.  When the solver
+// has checked that every interior cell's stored diagonal equals
+// dtab[(cx*DIAG_CLS + cy)*DIAG_CLS + cz] bit for bit (cx, cy, cz: the
+// classes of the cell's runs), dtab is set and the k-marching kernels take
+// the diagonal from it; ac stays the stored column.
 template <class TA>
 struct OpSep {
     const TA *__restrict__ ac;
     const TA *__restrict__ aw, *__restrict__ ae, *__restrict__ as, *__restrict__ an, *__restrict__ ab,
         *__restrict__ at;
+    const TA *__restrict__ dtab = nullptr;
+    // per axis: first index of runs 1..3 (INT_MAX when absent) and the class
+    // of run r in bits 8r..8r+7 (scalar members: the view is a kernel argument)
+    int x1 = 0, x2 = 0, x3 = 0, y1 = 0, y2 = 0, y3 = 0, z1 = 0, z2 = 0, z3 = 0;
+    int cxr = 0, cyr = 0, czr = 0;
+    __device__ static inline int run_class(int v, int s1, int s2, int s3, int cr) {
+        const int sh = v >= s2 ? (v >= s3 ? 24 : 16) : (v >= s1 ? 8 : 0);
+        return (cr >> sh) & 0xff;
+    }
     __device__ inline void get(int cc, int i, int j, int k, TA &c0, TA &w, TA &e, TA &s, TA &n, TA &b,
                                TA &t) const {
         c0 = ac[cc]; w = aw[i]; e = ae[i]; s = as[j]; n = an[j]; b = ab[k]; t = at[k];
@@ -117,6 +155,19 @@ struct OpSep {
         OpSepCol<TA> c;
         c.w = aw[i]; c.e = ae[i]; c.s = as[j]; c.n = an[j];
         c.ac = ac; c.ab = ab; c.at = at;
+        c.cls = dtab != nullptr;
+        c.z1 = z1;
+        c.z2 = z2;
+        c.z3 = z3;
+        c.d0 = c.d1 = c.d2 = c.d3 = TA(0);
+        if (c.cls) {
+            const TA *row =
+                dtab + (run_class(i, x1, x2, x3, cxr) * DIAG_CLS + run_class(j, y1, y2, y3, cyr)) * DIAG_CLS;
+            c.d0 = row[czr & 0xff];
+            c.d1 = row[(czr >> 8) & 0xff];
+            c.d2 = row[(czr >> 16) & 0xff];
+            c.d3 = row[(czr >> 24) & 0xff];
+        }
         return c;
     }
 };

@@ -29,6 +29,7 @@
 // starts from the true residual b - A x computed in double with the double
 // operator.
 #include <cfloat>
+#include <climits>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -104,8 +105,12 @@ __global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, OP a, const
 // once a pass has run, and otherwise sets the float pass's target.
 // k-marching forms of the solver's stencil kernels (x walked as a Plane7):
 // same arithmetic per cell, partial sums per workgroup in the K7 order.
+// Eight waves per SIMD (64 VGPRs at most): four workgroups per CU, so the
+// 1024 tiles of a 512 x 128 x 128 interior run in one round instead of
+// three workgroups per CU and a partly empty second round.
 template <class T, class OP>
-__global__ __launch_bounds__(K7_NT) void k_bicg_init_km(K7Tiling kt, Geom g, OP a, const double *__restrict__ x,
+__global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(8))) void k_bicg_init_km(
+    K7Tiling kt, Geom g, OP a, const double *__restrict__ x,
                                                         const double *__restrict__ b, double *r64, T *r, T *rr,
                                                         const DevState *st, double *partials) {
     if (st->final || !st->done) return;
@@ -237,6 +242,9 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
     const unsigned cb = (unsigned)Q.col0 * sizeof(T);
     const unsigned hb = hok ? (unsigned)(h.gi + g.nx * h.gj) * sizeof(T) : cb;
     const int cpar = (Q.i + Q.j + g.osum) & 1, hpar = (h.gi + h.gj + g.osum) & 1;
+    // the operator's view of the halo cell's column (a valid column when the
+    // halo cell lies outside the array; its values are then unused)
+    const auto ch = a.column(hok ? h.gi : (Q.act ? Q.i : kt.box.lo[0]), hok ? h.gj : (Q.act ? Q.j : kt.box.lo[1]));
     // the black half-sweep's value (zero on red cells and outside the interior)
     const auto wval = [&](T vv, T d, bool in, int par, int k) -> T {
         return (in && ((par + k) & 1) == FALM_COLOR_BLACK) ? T(0) + omega_pc * (vv / d) : T(0);
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         unsigned ob = cb + (unsigned)(Q.k0 - 1) * szt;
         T rm, pm = T(0), qm = T(0), rc, pc = T(0), qc = T(0);
         load(ob, rm, pm, qm);
-        const T dm = ca.diag_at(ob);
+        const T dm = ca.diag_k(ob, Q.k0 - 1);
         load(ob + szt, rc, pc, qc);
         ec = ca.load(ob + szt, Q.k0);
         load(ob + 2 * szt, nr, np, nq);
@@ -268,7 +276,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         if (h.slot) {
             const unsigned oh = hb + (unsigned)Q.k0 * szt;
             load(oh, hr, hp, hq);
-            hd = ca.diag_at(oh);
+            hd = ch.diag_k(oh, Q.k0);
         }
         wm = wval(vec(rm, pm, qm), dm, Q.act && kin(Q.k0 - 1), cpar, Q.k0 - 1);
         vc = vec(rc, pc, qc);
@@ -289,7 +297,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
             if (h.slot) {
                 const unsigned oh = hb + (unsigned)(k + 1) * szt;
                 load(oh, hr, hp, hq);
-                hd = ca.diag_at(oh);
+                hd = ch.diag_k(oh, k + 1);
             }
         }
         const T vn = vec(nr, np, nq);
@@ -912,6 +920,14 @@ __global__ __launch_bounds__(NT) void k_sep_check(Tiling tl, Geom g, Mat7<TA> a,
     });
 }
 
+// every interior cell's diagonal equals the one its runs' classes give, bit
+// for bit (through the same column view the k-marching kernels use)
+__global__ __launch_bounds__(NT) void k_diag_check(Tiling tl, Geom g, OpSep<double> o, int *mismatch) {
+    for_each_cell(tl, [&](int i, int j, int k) {
+        if (!same_bits(o.ac[g.id(i, j, k)], o.column(i, j).pick(k))) *mismatch = 1;  // benign race
+    });
+}
+
 }  // namespace falm
 
 using namespace falm;
@@ -936,6 +952,13 @@ struct falm_solver {
     bool sep = false;
     double *sep64 = nullptr;      // 1-D arrays: aw,ae [nx], as,an [ny], ab,at [nz]
     float *sep32 = nullptr;
+    // diagonal classes (OpSep): per-axis run starts and run classes, and the
+    // class-triple values in double and float
+    bool dclass = false;
+    int drs[3][DIAG_CLS - 1] = {};
+    int drc[3] = {};
+    double *dtab64 = nullptr;
+    float *dtab32 = nullptr;
     float *a32 = nullptr;         // mixed: float columns (7 if not separable, else diagonal only)
     double *r64own = nullptr;     // residual when the caller passes r = NULL (double solvers)
     int last_it = 0;              // iterations of the previous solve
@@ -965,6 +988,8 @@ static falm_status solver_alloc(falm_solver *S) {
     const size_t ns = 2 * ((size_t)S->g.nx + S->g.ny + S->g.nz);
     FALM_HIP(hipMalloc(&S->sep64, ns * sizeof(double)));
     FALM_HIP(hipMalloc(&S->sep32, ns * sizeof(float)));
+    FALM_HIP(hipMalloc(&S->dtab64, DIAG_CLS * DIAG_CLS * DIAG_CLS * sizeof(double)));
+    FALM_HIP(hipMalloc(&S->dtab32, DIAG_CLS * DIAG_CLS * DIAG_CLS * sizeof(float)));
     FALM_HIP(hipMalloc(&S->st, sizeof(DevState)));
     FALM_HIP(hipMemset(S->st, 0, sizeof(DevState)));
     FALM_HIP(hipHostMalloc(&S->hst, sizeof(DevState), hipHostMallocDefault));
@@ -1004,6 +1029,14 @@ static OpSep<TA> sep_view(const falm_solver *S, const TA *diag, const TA *arr) {
     o.an = arr + 2 * nx + ny;
     o.ab = arr + 2 * nx + 2 * ny;
     o.at = arr + 2 * nx + 2 * ny + nz;
+    if (S->dclass) {
+        if constexpr (std::is_same<TA, double>::value) o.dtab = S->dtab64;
+        else o.dtab = S->dtab32;
+        o.x1 = S->drs[0][0], o.x2 = S->drs[0][1], o.x3 = S->drs[0][2];
+        o.y1 = S->drs[1][0], o.y2 = S->drs[1][1], o.y3 = S->drs[1][2];
+        o.z1 = S->drs[2][0], o.z2 = S->drs[2][1], o.z3 = S->drs[2][2];
+        o.cxr = S->drc[0], o.cyr = S->drc[1], o.czr = S->drc[2];
+    }
     return o;
 }
 template <class TA>
@@ -1264,6 +1297,71 @@ static falm_status solve_stationary(falm_solver *S, const OP &a, double *x, cons
     return FALM_OK;
 }
 
+// Diagonal classes of a separable operator (OpSep): each interior index of
+// each axis is classed by its (minus, plus) coupling pair, the classes must
+// form at most DIAG_CLS runs per axis, one cell per class triple gives that
+// triple's diagonal, and the classes are kept only if every interior cell's
+// stored diagonal equals its triple's value bit for bit.
+static falm_status diag_classes(falm_solver *S, const double *a, hipStream_t s) {
+    const Geom &g = S->g;
+    const Box &b = S->tl.box;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const size_t ns = 2 * ((size_t)g.nx + g.ny + g.nz);
+    std::vector<double> h(ns);
+    FALM_HIP(hipMemcpyAsync(h.data(), S->sep64, ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    FALM_HIP(hipStreamSynchronize(s));
+    int rep[3][DIAG_CLS], ncls[3] = {0, 0, 0};
+    int rs[3][DIAG_CLS - 1], rc[3] = {0, 0, 0};
+    size_t base = 0;  // the axis's minus array in h (its plus array follows)
+    for (int m = 0; m < 3; m++) {
+        const double *mi = h.data() + base, *pl = mi + n[m];
+        int nrun = 0, prev = -1;
+        for (int q = 0; q < DIAG_CLS - 1; q++) rs[m][q] = INT_MAX;
+        for (int i = b.lo[m]; i < b.lo[m] + b.n[m]; i++) {
+            int c = 0;
+            while (c < ncls[m] && !(std::memcmp(&mi[i], &mi[rep[m][c]], sizeof(double)) == 0 &&
+                                    std::memcmp(&pl[i], &pl[rep[m][c]], sizeof(double)) == 0))
+                c++;
+            if (c == ncls[m]) {
+                if (c == DIAG_CLS) return FALM_OK;  // too many classes: the diagonal stays streamed
+                rep[m][c] = i;
+                ncls[m]++;
+            }
+            if (c != prev) {
+                if (nrun == DIAG_CLS) return FALM_OK;  // too many runs
+                if (nrun > 0) rs[m][nrun - 1] = i;
+                rc[m] |= c << (8 * nrun);
+                nrun++;
+                prev = c;
+            }
+        }
+        base += 2 * (size_t)n[m];
+    }
+    double t64[DIAG_CLS * DIAG_CLS * DIAG_CLS] = {};
+    float t32[DIAG_CLS * DIAG_CLS * DIAG_CLS] = {};
+    for (int cx = 0; cx < ncls[0]; cx++)
+        for (int cy = 0; cy < ncls[1]; cy++)
+            for (int cz = 0; cz < ncls[2]; cz++) {
+                const int q = (cx * DIAG_CLS + cy) * DIAG_CLS + cz;
+                const size_t cell = (size_t)g.id(rep[0][cx], rep[1][cy], rep[2][cz]);
+                FALM_HIP(hipMemcpyAsync(&t64[q], a + cell, sizeof(double), hipMemcpyDeviceToHost, s));
+            }
+    FALM_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < DIAG_CLS * DIAG_CLS * DIAG_CLS; q++) t32[q] = (float)t64[q];
+    FALM_HIP(hipMemcpyAsync(S->dtab64, t64, sizeof(t64), hipMemcpyHostToDevice, s));
+    FALM_HIP(hipMemcpyAsync(S->dtab32, t32, sizeof(t32), hipMemcpyHostToDevice, s));
+    std::memcpy(S->drs, rs, sizeof(rs));
+    std::memcpy(S->drc, rc, sizeof(rc));
+    S->dclass = true;  // for the view the check uses
+    FALM_HIP(hipMemsetAsync(S->flag, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_diag_check, FALM_GRID(S), s, S->tl, g, sep_view<double>(S, a, S->sep64), S->flag);
+    FALM_CHECK_LAUNCH("k_diag_check");
+    FALM_HIP(hipMemcpyAsync(S->hflag, S->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    FALM_HIP(hipStreamSynchronize(s));
+    S->dclass = (*S->hflag == 0);
+    return FALM_OK;
+}
+
 static falm_status prepare_matrix(falm_solver *S, const void *a, hipStream_t s) {
     const Geom &g = S->g;
     const int nmax = g.nx > g.ny ? (g.nx > g.nz ? g.nx : g.nz) : (g.ny > g.nz ? g.ny : g.nz);
@@ -1280,6 +1378,8 @@ static falm_status prepare_matrix(falm_solver *S, const void *a, hipStream_t s) 
     FALM_HIP(hipMemcpyAsync(S->hflag, S->flag, sizeof(int), hipMemcpyDeviceToHost, s));
     FALM_HIP(hipStreamSynchronize(s));
     S->sep = (*S->hflag == 0);
+    S->dclass = false;
+    if (S->sep) FALM_TRY(diag_classes(S, (const double *)a, s));
     if (S->cfg.precision == FALM_SOLVE_MIXED) {
         const size_t ncol = S->sep ? 1 : 7;
         hipLaunchKernelGGL((k_convert<double, float>), ew_grid(ncol * S->n), dim3(EW_BLOCK), 0, s, (const double *)a,
@@ -1391,7 +1491,7 @@ falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void
 falm_status falm_solver_destroy(falm_solver *S) {
     if (!S) return FALM_OK;
     void *bufs[] = {S->rr, S->p, S->q, S->s, S->pp, S->ss, S->t, S->w, S->r32, S->a32,
-                    S->sep64, S->sep32, S->st, S->partials, S->flag, S->r64own};
+                    S->sep64, S->sep32, S->dtab64, S->dtab32, S->st, S->partials, S->flag, S->r64own};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (S->hst) (void)hipHostFree(S->hst);
