@@ -51,7 +51,7 @@ def families():
         "sgs": (abi.TIME_SGS, "k_smagorinsky"),
         "divergence": (abi.TIME_DIVERGENCE, "k_div_norm (divergence + |div|^2)"),
         "solver_init": (abi.TIME_SOLVER_INIT, "k_bicg_init (b - A x in double + dots)"),
-        "sor": (abi.TIME_SOR, "k_sor_op (RB-SOR preconditioner half sweep)"),
+        "sor": (abi.TIME_SOR, "k_bicg_vsor (Krylov vector + one-pass RB-SOR preconditioner)"),
         "mv": (abi.TIME_MV, "k_bicg_mvdot (A p + dot)"),
         "solver_update": (abi.TIME_SOLVER_UPDATE, "k_bicg_update (x, r update + dots)"),
         "alm": (abi.TIME_ALM, "ALM step (points, box, factors, projection, totals)"),

@@ -32,6 +32,7 @@
 #include <climits>
 #include <cstring>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "km.hpp"
@@ -1051,6 +1052,15 @@ static OpCols<TA> cols_view(const falm_solver *S, const void *a) {
 #define FALM_KM(S) dim3((S)->kt.blocks), dim3(K7_NT), 0
 #define FALM_FIN dim3(1), dim3(FIN_NT), 0
 
+// bytes per cell of the operator's diagonal that a k-marching kernel
+// streams: none when a separable operator takes it from its classes
+template <class OP>
+static double diag_bytes(const falm_solver *S) {
+    using TA = decltype(std::declval<OP>().diag_at(0u));
+    if constexpr (std::is_same<OP, OpSep<TA>>::value) return S->dclass ? 0.0 : (double)sizeof(TA);
+    return (double)sizeof(TA);
+}
+
 // preconditioner sweeps after the fused first one (SORPC: pc_maxit x
 // {black, red}; JacobiPC: pc_maxit sweeps; both run at least once,
 // src/FalmEq.cpp:147-170,258-302).  Returns the vector holding the result:
@@ -1123,9 +1133,9 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         TimedLaunch tm(FALM_TIME_SOR, s);
         hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 0>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)pin,
                            (const T *)q, pout, pp, (const DevState *)S->st, wpc);
-        // r and the diagonal read, p and p^ written (a first iteration; later
-        // ones also read p and q)
-        tm.done((double)tl.box.size() * 4.0 * sizeof(T));
+        // r (and the diagonal) read, p and p^ written (a first iteration;
+        // later ones also read p and q)
+        tm.done((double)tl.box.size() * (3.0 * sizeof(T) + diag_bytes<OP>(S)));
         FALM_CHECK_LAUNCH("k_bicg_vsor_km");
     } else {
         hipLaunchKernelGGL((k_bicg_p<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, p, (const T *)q, pp,
@@ -1138,7 +1148,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     TimedLaunch tm1(FALM_TIME_MV, s);
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)pcp, q, (const T *)rr,
                        (const DevState *)S->st, S->partials);
-    tm1.done((double)tl.box.size() * 4.0 * sizeof(T));
+    tm1.done((double)tl.box.size() * (3.0 * sizeof(T) + diag_bytes<OP>(S)));
     hipLaunchKernelGGL(k_fin_alpha, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot");
     T *pcs = ss;
@@ -1146,8 +1156,8 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         TimedLaunch tm(FALM_TIME_SOR, s);
         hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)nullptr,
                            (const T *)q, sv, ss, (const DevState *)S->st, wpc);
-        // r, q and the diagonal read, s and s^ written
-        tm.done((double)tl.box.size() * 5.0 * sizeof(T));
+        // r, q (and the diagonal) read, s and s^ written
+        tm.done((double)tl.box.size() * (4.0 * sizeof(T) + diag_bytes<OP>(S)));
         FALM_CHECK_LAUNCH("k_bicg_vsor_km");
     } else {
         hipLaunchKernelGGL((k_bicg_s<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, (const T *)q, sv, ss,
@@ -1184,10 +1194,10 @@ static falm_status launch_init(falm_solver *S, const OP64 &a, const double *x, c
     TimedLaunch tm(FALM_TIME_SOLVER_INIT, s);
     hipLaunchKernelGGL((k_bicg_init_km<T, OP64>), FALM_KM(S), s, S->kt, S->g, a, x, b, r64, rT, (T *)S->rr,
                        (const DevState *)S->st, S->partials);
-    // x, b, diagonal read; r (double, if kept), r and rr (T) written
+    // x, b (and the diagonal) read; r (double, if kept), r and rr (T) written
     tm.done((double)S->tl.box.size() *
-            (3.0 * sizeof(double) + (r64 ? sizeof(double) : 0.0) + ((void *)rT != (void *)r64 ? sizeof(T) : 0.0) +
-             sizeof(T)));
+            (2.0 * sizeof(double) + diag_bytes<OP64>(S) + (r64 ? sizeof(double) : 0.0) +
+             ((void *)rT != (void *)r64 ? sizeof(T) : 0.0) + sizeof(T)));
     hipLaunchKernelGGL(k_fin_init, FALM_FIN, s, S->st, (const double *)S->partials, S->kt.blocks);
     FALM_CHECK_LAUNCH("k_bicg_init");
     return FALM_OK;
