@@ -405,21 +405,27 @@ __device__ inline bool box_meets(const int *b, int ilo, int ihi, int jlo, int jh
 }
 
 constexpr int kCullChunk = 256;
-constexpr int PJ_X = 32, PJ_Y = 8;  // projection tile: 32 x 8 cells of one plane
+constexpr int PJ_X = 32, PJ_Y = 2;  // projection tile: 32 x 2 cells of one plane, one per lane
+constexpr int PJ_W = 4;             // waves per workgroup, each summing a share of the tile's points
 
 // The Gaussian projection, walking the tiles of the cells it writes with a
 // fixed grid: the union of this and the previous step's force box
 // (union_only; ff is zero elsewhere and stays untouched), or every interior
 // cell.  Cells outside this step's box get zero.  Per tile, the points whose
 // Gaussian reaches the tile within the culling radius are compacted into
-// LDS with their per-axis factors, then each cell sums over them.
+// LDS with their per-axis factors; each of the four waves sums a contiguous
+// quarter of them for the tile's 64 cells, and the quarters are added in
+// wave order.  A box of a few tens of thousands of cells then keeps four
+// waves busy per tile instead of one.
 template <class T>
 __global__ __launch_bounds__(256) void k_alm_project(Box interior, AlmDev A, T *ff0, T *ff1, T *ff2, double cut2,
                                                      int union_only) {
+    static_assert(PJ_X * PJ_Y == 64 && PJ_W * 64 == 256, "one tile cell per lane, four waves");
     __shared__ double sf0[kCullChunk], sf1[kCullChunk], sf2[kCullChunk], sez[kCullChunk];
     __shared__ double sey[PJ_Y][kCullChunk];
     __shared__ int sap[kCullChunk];
-    __shared__ int wcount[256 / 64];
+    __shared__ double part[PJ_W - 1][3][64];
+    __shared__ int wcount[PJ_W];
     int wlo[3], wn[3];
     if (union_only) {
         const int *U = A.box + 6;
@@ -442,7 +448,7 @@ __global__ __launch_bounds__(256) void k_alm_project(Box interior, AlmDev A, T *
     const double sp = eps * sqrt(kPi);
     const double eta = 1. / (sp * sp * sp);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int ci = (int)threadIdx.x % PJ_X, jr = (int)threadIdx.x / PJ_X;
+    const int ci = lane % PJ_X, jr = lane / PJ_X;
     for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int bx = (int)(t % tx), by = (int)((t / tx) % ty), kc = (int)(t / ((long long)tx * ty));
         const int ilo = wlo[0] + bx * PJ_X, ihi = min(ilo + PJ_X, wlo[0] + wn[0]) - 1;
@@ -452,7 +458,7 @@ __global__ __launch_bounds__(256) void k_alm_project(Box interior, AlmDev A, T *
         const bool valid = i <= ihi && j <= jhi;
         const size_t cc = (size_t)i + (size_t)A.nx * j + (size_t)A.nx * A.ny * k;
         if (!box_meets(B, ilo, ihi, jlo, jhi, k, k)) {
-            if (valid) {
+            if (valid && wid == 0) {
                 ff0[cc] = T(0);
                 ff1[cc] = T(0);
                 ff2[cc] = T(0);
@@ -485,7 +491,7 @@ __global__ __launch_bounds__(256) void k_alm_project(Box interior, AlmDev A, T *
             if (lane == 0) wcount[wid] = __popcll(mask);
             __syncthreads();
             int off = 0, tot = 0;
-            for (int w = 0; w < 256 / 64; w++) {
+            for (int w = 0; w < PJ_W; w++) {
                 if (w < wid) off += wcount[w];
                 tot += wcount[w];
             }
@@ -499,8 +505,10 @@ __global__ __launch_bounds__(256) void k_alm_project(Box interior, AlmDev A, T *
                 for (int q = 0; q < tjn; q++) sey[q][slot] = A.fy[(size_t)ap * A.ny + jlo + q];
             }
             __syncthreads();
+            // this wave's quarter of the kept points
+            const int q0 = (tot * wid) / PJ_W, q1 = (tot * (wid + 1)) / PJ_W;
             if (valid) {
-                for (int q = 0; q < tot; q++) {
+                for (int q = q0; q < q1; q++) {
                     // exp(-r^2/eps^2) = ex*ey*ez (per-axis factors of the point)
                     const double e = (A.fx[(size_t)sap[q] * A.nx + i] * sey[jr][q]) * sez[q];
                     fx += sf0[q] * e;
@@ -510,13 +518,26 @@ __global__ __launch_bounds__(256) void k_alm_project(Box interior, AlmDev A, T *
             }
             __syncthreads();
         }
-        if (valid) {
+        // the quarters in wave order
+        if (wid > 0) {
+            part[wid - 1][0][lane] = fx;
+            part[wid - 1][1][lane] = fy;
+            part[wid - 1][2][lane] = fz;
+        }
+        __syncthreads();
+        if (wid == 0 && valid) {
+            for (int w = 0; w < PJ_W - 1; w++) {
+                fx += part[w][0][lane];
+                fy += part[w][1][lane];
+                fz += part[w][2][lane];
+            }
             // cells outside the box receive nothing (every term there is
             // below the cutoff by more than a cell width)
             ff0[cc] = inbox ? (T)fx : T(0);
             ff1[cc] = inbox ? (T)fy : T(0);
             ff2[cc] = inbox ? (T)fz : T(0);
         }
+        __syncthreads();
     }
 }
 
