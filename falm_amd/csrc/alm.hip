@@ -151,6 +151,27 @@ __global__ __launch_bounds__(256) void k_alm_points(AlmDev A, const T *u0, const
                                                     int staged, int fused) {
     extern __shared__ double stage[];
     __shared__ double sxyz[3 * kFusedPoints], stq[kFusedPoints], sth[kFusedPoints];
+    // The point's parameters are loaded first, so that they are in flight
+    // with the staging loads, and every result is stored at the end: the
+    // point's work is then three global round trips (parameters and tables,
+    // the eight velocity corners, the polar entries) instead of one per
+    // store/load alternation.
+    const int ap = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = ap < A.nap;
+    const int apc = live ? ap : 0;
+    const int napt = A.napb * A.nb;
+    const int tid = apc / napt, bid = (apc % napt) / A.napb, sec = apc % A.napb;
+    const TurbDev &T0 = A.tb[tid];
+    const int atype = T0.angle_type;
+    const double tip = T0.tip_rate;
+    const double mo0 = T0.motion[0], mo1 = T0.motion[1], mo2 = T0.motion[2];
+    const double su0 = T0.surge[0], su1 = T0.surge[1], su2 = T0.surge[2];
+    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {T0.bvel[0], T0.bvel[1], T0.bvel[2]};
+    const double hub[3] = {T0.hub[0], T0.hub[1], T0.hub[2]};
+    const double apr = A.r[apc], chord = A.chord[apc];
+    // APFrame::get_airfoil_params (src/alm/apHandler.h:121-141); blade pitch
+    // and per-section twist offset have no reference counterpart (0 default)
+    const double twist = A.twist[apc] + T0.pitch_deg[bid < 8 ? bid : 7] + A.twist_off[sec];
     const double *X = A.x, *Y = A.y, *Z = A.z, *AT = A.attack;
     if (staged) {
         const int L = A.nx + A.ny + A.nz + A.nat;
@@ -168,123 +189,114 @@ __global__ __launch_bounds__(256) void k_alm_points(AlmDev A, const T *u0, const
         Z = stage + A.nx + A.ny;
         AT = stage + A.nx + A.ny + A.nz;
     }
-    const int ap = blockIdx.x * blockDim.x + threadIdx.x;
     [&]() {  // one point (a return ends this point's work only)
-    if (ap >= A.nap) return;
-    const int napt = A.napb * A.nb;
-    const int tid = ap / napt, bid = (ap % napt) / A.napb, sec = ap % A.napb;
-    const TurbDev &T0 = A.tb[tid];
+    if (!live) return;
     // kernel_UpdateTurbineAngles (+ base surge, no reference counterpart)
     double ang[3] = {0, 0, 0}, om[3] = {0, 0, 0};
-    if (T0.angle_type) {
-        ang[T0.angle_type - 1] = T0.motion[0] * sin(T0.motion[1] * t + T0.motion[2]);
-        om[T0.angle_type - 1] = T0.motion[0] * T0.motion[1] * cos(T0.motion[1] * t + T0.motion[2]);
+    if (atype) {
+        const double sa = mo0 * sin(mo1 * t + mo2), oa = mo0 * mo1 * cos(mo1 * t + mo2);
+        ang[0] = atype == 1 ? sa : 0.0;
+        ang[1] = atype == 2 ? sa : 0.0;
+        ang[2] = atype == 3 ? sa : 0.0;
+        om[0] = atype == 1 ? oa : 0.0;
+        om[1] = atype == 2 ? oa : 0.0;
+        om[2] = atype == 3 ? oa : 0.0;
     }
-    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {T0.bvel[0], T0.bvel[1], T0.bvel[2]};
-    if (T0.surge[0] != 0.0) {
-        base[0] = T0.base[0] + T0.surge[0] * sin(T0.surge[1] * t + T0.surge[2]);
-        bvel[0] = T0.bvel[0] + T0.surge[0] * T0.surge[1] * cos(T0.surge[1] * t + T0.surge[2]);
+    if (su0 != 0.0) {
+        base[0] = base[0] + su0 * sin(su1 * t + su2);
+        bvel[0] = bvel[0] + su0 * su1 * cos(su1 * t + su2);
     }
     const double nang[3] = {-ang[0], -ang[1], -ang[2]};
     // kernel_UpdateAPX
-    const double tip = T0.tip_rate;
     const double theta0 = (2 * kPi / A.nb) * bid;
     const double theta = tip * t + theta0;
-    const double apr = A.r[ap];
-    double c1[3] = {T0.hub[0], T0.hub[1], T0.hub[2]};
+    double c1[3] = {hub[0], hub[1], hub[2]};
     c1[1] += apr * cos(theta);
     c1[2] += apr * sin(theta);
     double P[3];
-    rot(c1, nang, T0.angle_type, P);
+    rot(c1, nang, atype, P);
     for (int m = 0; m < 3; m++) P[m] += base[m];
-    for (int m = 0; m < 3; m++) A.ap_xyz[3 * ap + m] = P[m];
     if (fused)
         for (int m = 0; m < 3; m++) sxyz[3 * ap + m] = P[m];
     const int I = find_index(X, P[0], A.nx), J = find_index(Y, P[1], A.ny), K = find_index(Z, P[2], A.nz);
-    A.ap_ijk[3 * ap] = I;
-    A.ap_ijk[3 * ap + 1] = J;
-    A.ap_ijk[3 * ap + 2] = K;
     const int gc = A.gc;
     const bool in = I >= gc - 1 && I < A.nx - gc - 1 && J >= gc - 1 && J < A.ny - gc - 1 && K >= gc - 1 &&
                     K < A.nz - gc - 1;
-    A.ap_rank[ap] = in ? 0 : -1;
     // kernel_CalcAPForce
-    if (!in) {
-        for (int m = 0; m < 3; m++) A.ap_force[3 * ap + m] = 0.0;
-        A.ap_torque[ap] = 0.0;
-        A.ap_thrust[ap] = 0.0;
-        if (fused) stq[ap] = sth[ap] = 0.0;
-        return;
-    }
-    const double dr = (A.radius - A.hub_radius) / A.napb;
-    const int i0 = I, i1 = I + 1, j0 = J, j1 = J + 1, k0 = K, k1 = K + 1;
-    const double x0 = X[i0], x1 = X[i1], y0 = Y[j0], y1 = Y[j1], z0 = Z[k0], z1 = Z[k1];
-    const double xd = fabs(P[0] - x0) / fabs(x1 - x0);
-    const double yd = fabs(P[1] - y0) / fabs(y1 - y0);
-    const double zd = fabs(P[2] - z0) / fabs(z1 - z0);
-    const int nx = A.nx, nxy = A.nx * A.ny;
-    const int id[8] = {i0 + nx * j0 + nxy * k0, i1 + nx * j0 + nxy * k0, i0 + nx * j1 + nxy * k0,
-                       i1 + nx * j1 + nxy * k0, i0 + nx * j0 + nxy * k1, i1 + nx * j0 + nxy * k1,
-                       i0 + nx * j1 + nxy * k1, i1 + nx * j1 + nxy * k1};
-    const T *uc[3] = {u0, u1, u2};
-    double uap[3];
-    for (int m = 0; m < 3; m++) {
-        double c[8];
-        for (int q = 0; q < 8; q++) c[q] = (double)uc[m][id[q]];
-        uap[m] = trilinear(xd, yd, zd, c);
-    }
-    double rel[3], vrel[3], utt[3];
-    for (int m = 0; m < 3; m++) {
-        rel[m] = P[m] - base[m];
-        vrel[m] = uap[m] - bvel[m];
-    }
-    rot_dt(rel, vrel, ang, om, T0.angle_type, utt);
-    const double ux = utt[0];
-    const double ut = tip * apr + utt[1] * sin(theta) - utt[2] * cos(theta);
-    const double urel2 = ux * ux + ut * ut;
-    const double phi = atan(ux / ut);
-    // APFrame::get_airfoil_params (src/alm/apHandler.h:121-141); blade pitch
-    // and per-section twist offset have no reference counterpart (0 default)
-    const double chord = A.chord[ap];
-    const double twist = A.twist[ap] + T0.pitch_deg[bid < 8 ? bid : 7] + A.twist_off[sec];
-    const double att = phi * 180. / kPi - twist;
-    double cl = 0.0, cd = 0.0;
-    const int nat = A.nat;
-    if (att < AT[0]) {
-        cl = A.cl[ap];
-        cd = A.cd[ap];
-    } else if (att >= AT[nat - 1]) {
-        cl = A.cl[ap + (nat - 1) * A.nap];
-        cd = A.cd[ap + (nat - 1) * A.nap];
-    } else if (att == att) {
-        // the interval AT[a] <= att < AT[a+1]: the reference scans for it
-        // (src/alm/apHandler.h:121-141); on an ascending table a bisection
-        // finds the same one.  NaN matches no interval (cl = cd = 0).
-        int a = 0, hi = nat - 1;
-        while (hi - a > 1) {
-            const int mid = (a + hi) >> 1;
-            if (AT[mid] <= att) a = mid;
-            else hi = mid;
+    double F[3] = {0.0, 0.0, 0.0}, torque = 0.0, thrust = 0.0;
+    if (in) {
+        const double dr = (A.radius - A.hub_radius) / A.napb;
+        const int i0 = I, i1 = I + 1, j0 = J, j1 = J + 1, k0 = K, k1 = K + 1;
+        const double x0 = X[i0], x1 = X[i1], y0 = Y[j0], y1 = Y[j1], z0 = Z[k0], z1 = Z[k1];
+        const double xd = fabs(P[0] - x0) / fabs(x1 - x0);
+        const double yd = fabs(P[1] - y0) / fabs(y1 - y0);
+        const double zd = fabs(P[2] - z0) / fabs(z1 - z0);
+        const int nx = A.nx, nxy = A.nx * A.ny;
+        const int id[8] = {i0 + nx * j0 + nxy * k0, i1 + nx * j0 + nxy * k0, i0 + nx * j1 + nxy * k0,
+                           i1 + nx * j1 + nxy * k0, i0 + nx * j0 + nxy * k1, i1 + nx * j0 + nxy * k1,
+                           i0 + nx * j1 + nxy * k1, i1 + nx * j1 + nxy * k1};
+        const T *uc[3] = {u0, u1, u2};
+        double uap[3];
+        for (int m = 0; m < 3; m++) {
+            double c[8];
+            for (int q = 0; q < 8; q++) c[q] = (double)uc[m][id[q]];
+            uap[m] = trilinear(xd, yd, zd, c);
         }
-        const double p = (att - AT[a]) / (AT[a + 1] - AT[a]);
-        cl = (1. - p) * A.cl[ap + a * A.nap] + p * A.cl[ap + (a + 1) * A.nap];
-        cd = (1. - p) * A.cd[ap + a * A.nap] + p * A.cd[ap + (a + 1) * A.nap];
+        double rel[3], vrel[3], utt[3];
+        for (int m = 0; m < 3; m++) {
+            rel[m] = P[m] - base[m];
+            vrel[m] = uap[m] - bvel[m];
+        }
+        rot_dt(rel, vrel, ang, om, atype, utt);
+        const double ux = utt[0];
+        const double ut = tip * apr + utt[1] * sin(theta) - utt[2] * cos(theta);
+        const double urel2 = ux * ux + ut * ut;
+        const double phi = atan(ux / ut);
+        const double att = phi * 180. / kPi - twist;
+        double cl = 0.0, cd = 0.0;
+        const int nat = A.nat;
+        if (att < AT[0]) {
+            cl = A.cl[ap];
+            cd = A.cd[ap];
+        } else if (att >= AT[nat - 1]) {
+            cl = A.cl[ap + (nat - 1) * A.nap];
+            cd = A.cd[ap + (nat - 1) * A.nap];
+        } else if (att == att) {
+            // the interval AT[a] <= att < AT[a+1]: the reference scans for it
+            // (src/alm/apHandler.h:121-141); on an ascending table a bisection
+            // finds the same one.  NaN matches no interval (cl = cd = 0).
+            int a = 0, hi = nat - 1;
+            while (hi - a > 1) {
+                const int mid = (a + hi) >> 1;
+                if (AT[mid] <= att) a = mid;
+                else hi = mid;
+            }
+            const double p = (att - AT[a]) / (AT[a + 1] - AT[a]);
+            cl = (1. - p) * A.cl[ap + a * A.nap] + p * A.cl[ap + (a + 1) * A.nap];
+            cd = (1. - p) * A.cd[ap + a * A.nap] + p * A.cd[ap + (a + 1) * A.nap];
+        }
+        const double fl = .5 * cl * urel2 * chord * dr;
+        const double fd = .5 * cd * urel2 * chord * dr;
+        const double fx = fl * cos(phi) + fd * sin(phi);
+        double ft = (A.flags & FALM_ALM_FIX_DRAG_TANGENT) ? fl * sin(phi) - fd * cos(phi) : fl * sin(phi) - fd * sin(phi);
+        const int sg = (tip > 0) - (tip < 0);
+        ft *= sg;
+        const double ftt[3] = {-fx, ft * sin(theta), -ft * cos(theta)};
+        rot(ftt, nang, atype, F);
+        torque = fabs(ft) * apr;
+        thrust = fx;
     }
-    const double fl = .5 * cl * urel2 * chord * dr;
-    const double fd = .5 * cd * urel2 * chord * dr;
-    const double fx = fl * cos(phi) + fd * sin(phi);
-    double ft = (A.flags & FALM_ALM_FIX_DRAG_TANGENT) ? fl * sin(phi) - fd * cos(phi) : fl * sin(phi) - fd * sin(phi);
-    const int sg = (tip > 0) - (tip < 0);
-    ft *= sg;
-    const double ftt[3] = {-fx, ft * sin(theta), -ft * cos(theta)};
-    double F[3];
-    rot(ftt, nang, T0.angle_type, F);
+    for (int m = 0; m < 3; m++) A.ap_xyz[3 * ap + m] = P[m];
+    A.ap_ijk[3 * ap] = I;
+    A.ap_ijk[3 * ap + 1] = J;
+    A.ap_ijk[3 * ap + 2] = K;
+    A.ap_rank[ap] = in ? 0 : -1;
     for (int m = 0; m < 3; m++) A.ap_force[3 * ap + m] = F[m];
-    A.ap_torque[ap] = fabs(ft) * apr;
-    A.ap_thrust[ap] = fx;
+    A.ap_torque[ap] = torque;
+    A.ap_thrust[ap] = thrust;
     if (fused) {
-        stq[ap] = fabs(ft) * apr;
-        sth[ap] = fx;
+        stq[ap] = torque;
+        sth[ap] = thrust;
     }
     }();
     if (!fused) return;
