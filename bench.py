@@ -48,7 +48,7 @@ def families():
         "project": (abi.TIME_PROJECT, "k_project (pressure-gradient projection, cell + face)"),
         "face_flux": (abi.TIME_FACE_FLUX, "k_u_to_uu_rhs (face flux + Poisson right-hand side)"),
         "rhs": (abi.TIME_RHS, "k_divergence (Poisson right-hand side)"),
-        "sgs": (abi.TIME_SGS, "k_smagorinsky"),
+        "sgs": (abi.TIME_SGS, "k_sgs_div (Smagorinsky + divergence + |div|^2)"),
         "divergence": (abi.TIME_DIVERGENCE, "k_div_norm (divergence + |div|^2)"),
         "solver_init": (abi.TIME_SOLVER_INIT, "k_bicg_init (b - A x in double + dots)"),
         "sor": (abi.TIME_SOR, "k_bicg_vsor (Krylov vector + one-pass RB-SOR preconditioner)"),

@@ -312,11 +312,13 @@ falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
     falm_inlet_noise nz{c.inlet_intensity, c.seed, C->it};
     FALM_TRY(launch_bc_velocity(d, pr, C->ubc, C->u, C->uprev, C->xyz, c.dt, &nz, s));
     if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->u, 3, s));
+    // SGS and the divergence monitor in one pass (both read final fields)
     if (c.sgs_model == FALM_SGS_SMAGORINSKY) {
-        FALM_TRY(sep_sgs(d, pr, c.Cs, C->u, C->nut, C->sm, s));
+        FALM_TRY(sep_sgs_div_norm(d, pr, c.Cs, C->u, C->nut, C->uu, C->div, C->sm, C->dnorm, C->partials, s));
         if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->nut, 1, s));
+    } else {
+        FALM_TRY(sep_div_norm(d, pr, C->uu, C->div, C->sm, C->dnorm, C->partials, s));
     }
-    FALM_TRY(sep_div_norm(d, pr, C->uu, C->div, C->sm, C->dnorm, C->partials, s));
     return FALM_OK;
 }
 

@@ -1103,6 +1103,84 @@ __global__ __launch_bounds__(K7_NT) void k_div_norm_km(K7Tiling kt, Geom g, Vec3
     block_partials<K7_NT, 1>(v, partials, lds);
 }
 
+// k_smagorinsky_km and k_div_norm_km in one pass over the same column
+// tiles (the step's last two stencils read u and uu, both final by then):
+// the same arithmetic for nut and div, the same partial per workgroup, and
+// the face fluxes loaded a plane ahead like the velocity.
+template <class T>
+__global__ __launch_bounds__(K7_NT) void k_sgs_div_km(K7Tiling kt, Geom g, T Cs, Vec3<T> u, T *__restrict__ nut,
+                                                      Vec3<T> uu, T *__restrict__ div, SepTab<T> tb,
+                                                      double *partials) {
+    __shared__ K7Lds<T> l0, l1, l2;
+    __shared__ double lds[K7_NT / 64];
+    const K7Pos Q = k7_pos(kt, g);
+    const int sz = g.nxy;
+    const unsigned szb = (unsigned)sz * sizeof(T), syb = (unsigned)g.nx * sizeof(T);
+    Plane7<T> pu[3];
+    pu[0].setup(u.c[0], &l0, Q, g);
+    pu[1].setup(u.c[1], &l1, Q, g);
+    pu[2].setup(u.c[2], &l2, Q, g);
+#pragma unroll
+    for (int d = 0; d < 3; d++) pu[d].prologue(Q, sz);
+    const int i = Q.act ? Q.i : kt.box.lo[0], j = Q.act ? Q.j : kt.box.lo[1];
+    const T hrx = T(0.5) * tb.t[SEP_R][0][i], hry = T(0.5) * tb.t[SEP_R][1][j];
+    const T c2 = Cs * Cs * tb.t[SEP_H23][0][i] * tb.t[SEP_H23][1][j];
+    const T rxy = tb.t[SEP_R][0][i] * tb.t[SEP_R][1][j];
+    unsigned cb = (unsigned)Q.col0 * sizeof(T) + (unsigned)Q.k0 * szb;
+    // face fluxes of plane k0 (east, west, north, south, top) and the face below
+    T wb = ld_at(uu.c[2], cb - szb);
+    T fe = ld_at(uu.c[0], cb), fw = ld_at(uu.c[0], cb - (unsigned)sizeof(T));
+    T fn = ld_at(uu.c[1], cb), fs = ld_at(uu.c[1], cb - syb), ft = ld_at(uu.c[2], cb);
+    double v[1] = {0.0};
+    __syncthreads();
+    int b = 0;
+    for (int k = Q.k0; k < Q.k1; k++, cb += szb) {
+        const bool more = k + 1 < Q.k1;
+        T ne = T(0), nw = T(0), nn = T(0), ns = T(0), nt = T(0);
+        if (more) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) pu[d].prefetch(Q, k, sz);
+            const unsigned c1 = cb + szb;
+            ne = ld_at(uu.c[0], c1);
+            nw = ld_at(uu.c[0], c1 - (unsigned)sizeof(T));
+            nn = ld_at(uu.c[1], c1);
+            ns = ld_at(uu.c[1], c1 - syb);
+            nt = ld_at(uu.c[2], c1);
+        }
+        const T hrz = T(0.5) * tb.t[SEP_R][2][k], h23z = tb.t[SEP_H23][2][k];
+        if (Q.act) {
+            T gr[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                gr[d][0] = hrx * (pu[d].xp(b) - pu[d].xm(b));
+                gr[d][1] = hry * (pu[d].yp(b) - pu[d].ym(b));
+                gr[d][2] = hrz * (pu[d].zp() - pu[d].zm());
+            }
+            const T dux = gr[0][0], duy = gr[0][1], duz = gr[0][2];
+            const T dvx = gr[1][0], dvy = gr[1][1], dvz = gr[1][2];
+            const T dwx = gr[2][0], dwy = gr[2][1], dwz = gr[2][2];
+            T d1 = T(2) * dux * dux, d2 = T(2) * dvy * dvy, d3 = T(2) * dwz * dwz;
+            T d4 = (duy + dvx) * (duy + dvx), d5 = (dvz + dwy) * (dvz + dwy), d6 = (duz + dwx) * (duz + dwx);
+            const T Du = sqrt(d1 + d2 + d3 + d4 + d5 + d6);
+            st_at(nut, cb, (c2 * h23z) * Du);
+            const T dv = (fe - fw + fn - fs + ft - wb) * (rxy * tb.t[SEP_R][2][k]);
+            st_at(div, cb, dv);
+            v[0] += (double)dv * (double)dv;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) pu[d].advance(b, more);
+        wb = ft;
+        fe = ne;
+        fw = nw;
+        fn = nn;
+        fs = ns;
+        ft = nt;
+        b ^= 1;
+        __syncthreads();
+    }
+    block_partials<K7_NT, 1>(v, partials, lds);
+}
+
 __global__ __launch_bounds__(FIN_NT) void k_fin_norm(const double *partials, int nb, double *out) {
     double tot[1];
     sum_partials<1>(partials, nb, tot);
@@ -1445,6 +1523,29 @@ falm_status sep_sgs(const falm_domain *dom, int prec, double Cs, const void *u, 
     if (prec == FALM_F32) return smag_km_t<float>(dom, Cs, u, nut, sm, s);
     return smag_km_t<double>(dom, Cs, u, nut, sm, s);
 }
+template <class T>
+static falm_status sgs_div_norm_t(const falm_domain *dom, double Cs, const void *u, void *nut, const void *uu,
+                                  void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s) {
+    Geom g = make_geom(dom);
+    size_t n = ncell_of(dom);
+    K7Tiling kt = make_k7_tiling(interior_box(g));
+    if (kt.blocks == 0) return FALM_OK;
+    FALM_TRY(check_byte_offsets(n, sizeof(T)));
+    TimedLaunch tm(FALM_TIME_SGS, s);
+    hipLaunchKernelGGL((k_sgs_div_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, (T)Cs, v3<T>(u, n), (T *)nut,
+                       v3<T>(uu, n), (T *)div, sep_tab<T>(sm), partials);
+    tm.done((double)interior_box(g).size() * 8.0 * sizeof(T));  // u, uu read; nut, div written
+    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out);
+    FALM_CHECK_LAUNCH("k_sgs_div_km");
+    return FALM_OK;
+}
+
+falm_status sep_sgs_div_norm(const falm_domain *dom, int prec, double Cs, const void *u, void *nut, const void *uu,
+                             void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s) {
+    if (prec == FALM_F32) return sgs_div_norm_t<float>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s);
+    return sgs_div_norm_t<double>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s);
+}
+
 falm_status sep_div_norm(const falm_domain *dom, int prec, const void *uu, void *div, const SepMesh &sm,
                          double *dev_out, double *partials, hipStream_t s) {
     if (prec == FALM_F32) return div_norm_t<float>(dom, uu, div, sm, dev_out, partials, s);

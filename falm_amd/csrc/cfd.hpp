@@ -57,5 +57,9 @@ falm_status sep_sgs(const falm_domain *dom, int prec, double Cs, const void *u, 
 // red_blocks(interior tiling) doubles
 falm_status sep_div_norm(const falm_domain *dom, int prec, const void *uu, void *div, const SepMesh &sm,
                          double *dev_out, double *partials, hipStream_t s);
+// sep_sgs (Smagorinsky) and sep_div_norm in one pass: u and uu are read
+// once per tile, nut, div and the norm are as the two calls give them
+falm_status sep_sgs_div_norm(const falm_domain *dom, int prec, double Cs, const void *u, void *nut, const void *uu,
+                             void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s);
 
 }  // namespace falm
