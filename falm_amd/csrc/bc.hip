@@ -62,14 +62,19 @@ __device__ inline const FaceDesc *face_of(const FaceSet &fs, int t, int &ka, int
 // cell index of the cell at normal position `pos` (counted from the face:
 // 0 = first ghost o1, -1 = second ghost o2, 1 = first interior i1, 2 = i2)
 __device__ inline int face_cell(const Geom &g, const FaceDesc &F, int ka, int kb, int pos) {
-    int c[3];
-    c[F.ax] = ka + g.gc;
-    c[F.bx] = kb + g.gc;
-    const int n = F.d == 0 ? g.nx : F.d == 1 ? g.ny : g.nz;
+    // arithmetic rather than a select of the three shape fields, which the
+    // compiler turns into an indexed load from a scratch copy of g
+    const int n = (F.d == 0) * g.nx + (F.d == 1) * g.ny + (F.d == 2) * g.nz;
+    const int a = ka + g.gc, b = kb + g.gc;
     // - face: o1 = gc-1, o2 = gc-2, i1 = gc, i2 = gc+1
     // + face: o1 = n-gc, o2 = n-gc+1, i1 = n-gc-1, i2 = n-gc-2
-    c[F.d] = F.plus ? (n - g.gc - pos) : (g.gc - 1 + pos);
-    return g.id(c[0], c[1], c[2]);
+    const int cn = F.plus ? (n - g.gc - pos) : (g.gc - 1 + pos);
+    // axis d gets cn, ax gets a, bx gets b (selects, not a dynamically
+    // indexed array, which would live in scratch memory)
+    const int c0 = F.d == 0 ? cn : F.ax == 0 ? a : b;
+    const int c1 = F.d == 1 ? cn : F.ax == 1 ? a : b;
+    const int c2 = F.d == 2 ? cn : F.ax == 2 ? a : b;
+    return g.id(c0, c1, c2);
 }
 
 // Philox4x32-10 (Salmon et al., SC'11) keyed by the seed, counter
@@ -127,12 +132,12 @@ __global__ void k_bc_velocity(FaceSet fs, Geom g, T *u0, T *u1, T *u2, const T *
             u[m][o2] = T(2) * ub - u[m][i1];
         }
     } else if (F.type == FALM_BC_SLIP) {
-        T ub[3] = {u[0][i1], u[1][i1], u[2][i1]};
-        ub[F.d] = T(0.0);
+#pragma unroll
         for (int m = 0; m < 3; m++) {
             const T ui = u[m][i1];
-            u[m][o1] = ub[m];
-            u[m][o2] = T(2) * ub[m] - ui;
+            const T ub = m == F.d ? T(0.0) : ui;
+            u[m][o1] = ub;
+            u[m][o2] = T(2) * ub - ui;
         }
     } else if (F.type == FALM_BC_WALL0) {
         for (int m = 0; m < 3; m++) {
