@@ -199,8 +199,13 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
  * NULL when the caller does not need it. */
 falm_status falm_solver_solve(falm_solver *s, const void *a, void *x, const void *b,
                               void *r, void *stream, falm_solve_info *info);
-/* Mixed precision: declare the matrix once so its float copy is built once
- * (otherwise it is rebuilt on every solve). */
+/* Declare the matrix once: the solver derives its float copy (mixed
+ * precision) and, when every off-diagonal column depends on one index only,
+ * the one-index coefficient arrays and the diagonal's per-axis classes (each
+ * checked bit for bit against a).  Solves with the same pointer use them, so
+ * call this again after changing the matrix's values.  A solve with another
+ * pointer reads that matrix's seven columns as given (in mixed precision it
+ * first declares that matrix itself). */
 falm_status falm_solver_set_matrix(falm_solver *s, const void *a, void *stream);
 falm_status falm_solver_destroy(falm_solver *s);
 
