@@ -195,9 +195,11 @@ def windtunnel(n=(512, 128, 128), cells_per_d=32, re=2.5e5, cfl=0.3, u_inf=1.0, 
 def lid3d(n=128, re=1e4, dt=5e-4, cs=0.1, lid=1.0, precision=abi.CASE_F64, tol=1e-6, maxit=1000,
           pc_maxit=2, pc_relax=1.5, solver_kw=None):
     """3-D lid-driven cavity LES on the unit cube (app/lid3d): no-slip walls,
-    lid on y+ moving in +x, Smagorinsky, BiCGStab + RB-SOR."""
-    h = 1.0 / n
-    s = _base_struct(abi.CASE_LID3D, (n, n, n), (h, h, h))
+    lid on y+ moving in +x, Smagorinsky, BiCGStab + RB-SOR.  n may also be a
+    triple of interior cell counts: a box of cubic cells of width 1/n[0]."""
+    nn = (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
+    h = 1.0 / nn[0]
+    s = _base_struct(abi.CASE_LID3D, nn, (h, h, h))
     s.Re, s.dt, s.Cs = re, dt, cs
     s.adv_scheme, s.sgs_model = abi.ADV_UPWIND3, abi.SGS_SMAGORINSKY
     s.u_inflow, s.lid_velocity = 0.0, lid
@@ -207,7 +209,7 @@ def lid3d(n=128, re=1e4, dt=5e-4, cs=0.1, lid=1.0, precision=abi.CASE_F64, tol=1
                precision=abi.SOLVE_MIXED if precision == abi.CASE_MIXED else abi.SOLVE_F64)
     skw.update(solver_kw or {})
     s.solver = abi.solver_config(**skw)
-    return CaseSetup(s, None, name=f"lid3d {n}^3")
+    return CaseSetup(s, None, name=f"lid3d {nn[0]}x{nn[1]}x{nn[2]}")
 
 
 def lid2d(n=128, re=1000.0, dt=1e-3, lid=1.0, omega=1.2, tol=1e-8, maxit=100000):

@@ -441,8 +441,13 @@ double orc_norm2(const falm_domain *dom, const double *a, int ncol) {
 }
 
 /* MatColMax/Min/AbsMax/AbsMin, VecMax/Min (src/MV.h:67-136,
- * src/dev/MVDevCall.cu:216-598); the per-cell seed of the reference's block
- * caches is 0 for cells outside the map, which only affects empty maps. */
+ * src/dev/MVDevCall.cu:216-598): the extremum over the interior cells.  The
+ * reference seeds the block cache of threads outside the map with 0
+ * (src/dev/MVDevCall.cu:251-283), so when the interior does not fill its last
+ * thread blocks its MatColMax is max(true max, 0), MatColMin min(true min,
+ * 0) and MatColAbsMin 0; its result equals this one when the interior is a
+ * multiple of the caller's block shape (a launch hint this API does not
+ * take), and always for MatColAbsMax (MaxDiag), the one the step uses. */
 int orc_reduce(const falm_domain *dom, int op, const double *a, int ncol, int col, double *out) {
     Dom D(dom);
     const int gc = D.gc;

@@ -5,6 +5,7 @@ bench.py's cpu_baseline leg, never by the product.  Arrays are float64
 numpy arrays in the reference Matrix layout, flattened (col-major over
 [ncell, ncol]).
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -24,73 +25,91 @@ def lib():
         if not os.path.exists(LIB_PATH):
             from oracle import build as _b  # noqa: WPS433
             _b.build()
-        L = C.CDLL(LIB_PATH)
-        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
-        dom = C.POINTER(abi.Domain)
-        sig = {
-            "orc_pseudo_u": [dom, ip, C.c_double, C.c_double] + [vp] * 9,
-            "orc_u_to_uu": [dom, vp, vp, vp, vp],
-            "orc_divergence": [dom, vp, vp, vp],
-            "orc_scale": [vp, C.c_size_t, C.c_double],
-            "orc_poisson_rhs": [dom, vp, vp, vp, C.c_double, C.c_double],
-            "orc_project_p": [dom, C.c_double, vp, vp, vp, vp, vp, vp, vp],
-            "orc_sgs": [dom, ip, C.c_double, vp, vp, vp, vp],
-            "orc_mv": [dom, vp, vp, vp],
-            "orc_residual": [dom, vp, vp, vp, vp],
-            "orc_reduce": [dom, ip, vp, ip, ip, dp],
-            "orc_matrix_add": [vp, vp, C.c_size_t],
-            "orc_vecaxby": [C.c_double, vp, C.c_double, vp, vp, C.c_size_t],
-            "orc_sor_sweep": [dom, vp, vp, vp, C.c_double, ip],
-            "orc_jacobi_sweep": [dom, vp, vp, vp, vp],
-            "orc_solver_create": [dom, C.POINTER(abi.SolverConfig), C.POINTER(vp)],
-            "orc_solver_destroy": [vp],
-            "orc_solver_solve": [vp, vp, vp, vp, vp, C.POINTER(abi.SolveInfo)],
-            "orc_inlet_normals": [C.c_uint64, C.c_int64, ip, ip, dp],
-            "orc_bc_velocity": [dom, C.POINTER(abi.FaceBC), vp, vp, vp, C.c_double,
-                                C.POINTER(abi.InletNoise)],
-            "orc_bc_pressure": [dom, C.POINTER(C.c_int), vp],
-            "orc_bc_face_flux": [dom, C.POINTER(C.c_int), vp],
-            "orc_copy_z5": [dom, vp, ip],
-            "orc_metrics": [dom] + [vp] * 10,
-            "orc_poisson_matrix": [dom, ip, vp, vp, vp, dp],
-            "orc_alm_create": [dom, C.POINTER(abi.AlmConfig), vp, vp, vp, C.POINTER(vp)],
-            "orc_alm_destroy": [vp],
-            "orc_alm_set_twist_offset": [vp, vp],
-            "orc_alm_step": [vp, vp, vp, C.c_double],
-            "orc_alm_get": [vp, vp, vp, vp, vp],
-            "orc_build_ap": [ip, ip, vp, vp, vp, vp, vp, ip, ip, ip, C.c_double, C.c_double,
-                             vp, vp, vp, vp, vp],
-            "orc_case_create": [C.POINTER(abi.CaseConfig), C.POINTER(vp)],
-            "orc_case_destroy": [vp],
-            "orc_case_step": [vp, ip, C.POINTER(abi.StepInfo)],
-            "orc_case_get_field": [vp, ip, vp, C.c_size_t],
-            "orc_case_set_field": [vp, ip, vp, C.c_size_t],
-            "orc_case_alm": [vp, C.POINTER(vp)],
-            "orc_case_window": [vp, ip, dp],
-            "orc_case_adjoint": [vp, ip, ip, C.c_double, dp, dp, dp, dp],
-            "orc_alm_set_pitch": [vp, ip, dp],
-            "orc_alm_set_base": [vp, ip, dp, dp],
-            "orc_alm_vjp": [vp, vp, C.c_double, vp, C.c_double, vp, vp, vp],
-            "orc_adj_sgs": [dom, C.c_double, vp, vp, vp, vp, vp],
-            "orc_adj_pseudo_u": [dom, C.c_double, C.c_double, C.c_double] + [vp] * 11,
-            "orc_adj_u_to_uu": [dom, vp, vp, vp, vp],
-            "orc_adj_project": [dom, C.c_double, vp, vp, vp, vp, vp, vp, vp],
-            "orc_adj_ubc": [dom, C.POINTER(abi.FaceBC), vp, C.c_double, vp, vp, vp],
-            "orc_adj_rhs": [dom, vp, C.c_double, vp, vp],
-            "orc_adj_pbc": [dom, C.POINTER(C.c_int), vp],
-            "orc_num_threads": [],
-        }
-        for name, args in sig.items():
-            f = getattr(L, name)
-            f.argtypes = args
-            f.restype = C.c_int
-        L.orc_dot.argtypes = [dom, vp, vp]
-        L.orc_dot.restype = C.c_double
-        L.orc_norm2.argtypes = [dom, vp, ip]
-        L.orc_norm2.restype = C.c_double
-        L.orc_inlet_normals.restype = None
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Route every call of this module to another build of the oracle (the
+    planted-mistake builds of the pinning tests) for the duration of a
+    with-block."""
+    global _lib
+    saved = lib()
+    _lib = _load(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def _load(path):
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    dom = C.POINTER(abi.Domain)
+    sig = {
+        "orc_pseudo_u": [dom, ip, C.c_double, C.c_double] + [vp] * 9,
+        "orc_u_to_uu": [dom, vp, vp, vp, vp],
+        "orc_divergence": [dom, vp, vp, vp],
+        "orc_scale": [vp, C.c_size_t, C.c_double],
+        "orc_poisson_rhs": [dom, vp, vp, vp, C.c_double, C.c_double],
+        "orc_project_p": [dom, C.c_double, vp, vp, vp, vp, vp, vp, vp],
+        "orc_sgs": [dom, ip, C.c_double, vp, vp, vp, vp],
+        "orc_mv": [dom, vp, vp, vp],
+        "orc_residual": [dom, vp, vp, vp, vp],
+        "orc_reduce": [dom, ip, vp, ip, ip, dp],
+        "orc_matrix_add": [vp, vp, C.c_size_t],
+        "orc_vecaxby": [C.c_double, vp, C.c_double, vp, vp, C.c_size_t],
+        "orc_sor_sweep": [dom, vp, vp, vp, C.c_double, ip],
+        "orc_jacobi_sweep": [dom, vp, vp, vp, vp],
+        "orc_solver_create": [dom, C.POINTER(abi.SolverConfig), C.POINTER(vp)],
+        "orc_solver_destroy": [vp],
+        "orc_solver_solve": [vp, vp, vp, vp, vp, C.POINTER(abi.SolveInfo)],
+        "orc_inlet_normals": [C.c_uint64, C.c_int64, ip, ip, dp],
+        "orc_bc_velocity": [dom, C.POINTER(abi.FaceBC), vp, vp, vp, C.c_double,
+                            C.POINTER(abi.InletNoise)],
+        "orc_bc_pressure": [dom, C.POINTER(C.c_int), vp],
+        "orc_bc_face_flux": [dom, C.POINTER(C.c_int), vp],
+        "orc_copy_z5": [dom, vp, ip],
+        "orc_metrics": [dom] + [vp] * 10,
+        "orc_poisson_matrix": [dom, ip, vp, vp, vp, dp],
+        "orc_alm_create": [dom, C.POINTER(abi.AlmConfig), vp, vp, vp, C.POINTER(vp)],
+        "orc_alm_destroy": [vp],
+        "orc_alm_set_twist_offset": [vp, vp],
+        "orc_alm_step": [vp, vp, vp, C.c_double],
+        "orc_alm_get": [vp, vp, vp, vp, vp],
+        "orc_build_ap": [ip, ip, vp, vp, vp, vp, vp, ip, ip, ip, C.c_double, C.c_double,
+                         vp, vp, vp, vp, vp],
+        "orc_case_create": [C.POINTER(abi.CaseConfig), C.POINTER(vp)],
+        "orc_case_destroy": [vp],
+        "orc_case_step": [vp, ip, C.POINTER(abi.StepInfo)],
+        "orc_case_get_field": [vp, ip, vp, C.c_size_t],
+        "orc_case_set_field": [vp, ip, vp, C.c_size_t],
+        "orc_case_alm": [vp, C.POINTER(vp)],
+        "orc_case_window": [vp, ip, dp],
+        "orc_case_adjoint": [vp, ip, ip, C.c_double, dp, dp, dp, dp],
+        "orc_alm_set_pitch": [vp, ip, dp],
+        "orc_alm_set_base": [vp, ip, dp, dp],
+        "orc_alm_vjp": [vp, vp, C.c_double, vp, C.c_double, vp, vp, vp],
+        "orc_adj_sgs": [dom, C.c_double, vp, vp, vp, vp, vp],
+        "orc_adj_pseudo_u": [dom, C.c_double, C.c_double, C.c_double] + [vp] * 11,
+        "orc_adj_u_to_uu": [dom, vp, vp, vp, vp],
+        "orc_adj_project": [dom, C.c_double, vp, vp, vp, vp, vp, vp, vp],
+        "orc_adj_ubc": [dom, C.POINTER(abi.FaceBC), vp, C.c_double, vp, vp, vp],
+        "orc_adj_rhs": [dom, vp, C.c_double, vp, vp],
+        "orc_adj_pbc": [dom, C.POINTER(C.c_int), vp],
+        "orc_num_threads": [],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.orc_dot.argtypes = [dom, vp, vp]
+    L.orc_dot.restype = C.c_double
+    L.orc_norm2.argtypes = [dom, vp, ip]
+    L.orc_norm2.restype = C.c_double
+    L.orc_inlet_normals.restype = None
+    return L
 
 
 def ptr(a):

@@ -38,13 +38,13 @@ def _check(ref, dev, recs, tol):
 def test_lid2d_steps(gpu):
     setup = cases.lid2d(n=32)
     ref, dev, recs = _run_pair(setup, 6)
-    _check(ref, dev, recs, 1e-9)
+    _check(ref, dev, recs, 1e-10)
 
 
 def test_lid3d_steps(gpu):
     setup = cases.lid3d(n=20, dt=2e-3, tol=1e-8)
     ref, dev, recs = _run_pair(setup, 5)
-    _check(ref, dev, recs, 1e-9)
+    _check(ref, dev, recs, 1e-10)
 
 
 @pytest.mark.parametrize("pc_maxit,pc_relax,tol,maxit", [(1, 1.0, 1e-30, 6), (2, 1.5, 1e-12, 1000)])

@@ -1,0 +1,152 @@
+"""Whole steps on domains that span several partial column tiles along every
+axis, against the CPU oracle.
+
+The step driver's k-marching kernels (pseudo-velocity, face flux +
+right-hand side, projection, Smagorinsky + divergence, and the solver's
+operator kernels) tile the interior in 64 x 8 columns walked in chunks of
+16 planes; a 150 x 20 x 37 interior gives 3 x tiles (the last 22 wide),
+3 y tiles (the last 4 high) and 3 k chunks (the last 5 deep), so the LDS
+halo exchange across every seam and every partial-tile guard runs.  The
+state is seeded with a non-trivial field everywhere (not the uniform
+initial flow, whose stencil differences vanish away from the inlet and the
+rotor), the same on both sides.  The solver runs a fixed number of
+iterations (tol 1e-30) so both sides stop at the same iteration.
+
+Reference step order: app/nedo/main.cpp:150-204 (wind tunnel),
+app/lid3d/main.cpp:122-175 (cavity)."""
+import numpy as np
+import pytest
+
+from falm_amd import _abi as abi
+from falm_amd import cases
+from oracle import pyoracle as O
+from tests.helpers import interior_mask, rel_err
+
+pytestmark = pytest.mark.gpu
+
+N_SEAM = (150, 20, 37)
+
+
+def _seed_state(setup, seed=11):
+    """u, uu, nut, p over the padded domain: smooth large-scale parts plus
+    cell-scale noise, so that every stencil difference is O(1)."""
+    n = setup.n
+    nx, ny, nz = n[0] + 4, n[1] + 4, n[2] + 4
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0, 1, nx)[None, None, :]
+    y = np.linspace(0, 1, ny)[None, :, None]
+    z = np.linspace(0, 1, nz)[:, None, None]
+    h = setup.struct.h[0]
+
+    def smooth(a, b, c, ph):
+        return np.sin(a * x + ph) * np.cos(b * y) * np.sin(c * z + 0.5 * ph)
+
+    u = []
+    for d, base in enumerate((setup.struct.u_inflow or 0.3, 0.0, 0.0)):
+        f = base + 0.2 * smooth(5.1 + d, 3.3, 2.7 - 0.4 * d, 0.3 * d) + 0.03 * rng.standard_normal((nz, ny, nx))
+        u.append(f.ravel())
+    uu = [h * h * (ud + 0.01 * rng.standard_normal(ud.size)) for ud in u]
+    nut = 1e-4 * (1.0 + 0.5 * rng.uniform(size=nx * ny * nz))
+    p = 0.1 * smooth(2.0, 1.5, 1.1, 0.7).ravel() + 0.01 * rng.standard_normal(nx * ny * nz)
+    return {abi.FIELD_U: np.concatenate(u), abi.FIELD_UU: np.concatenate(uu), abi.FIELD_NUT: nut, abi.FIELD_P: p}
+
+
+def _pair(setup, state):
+    from falm_amd import runtime as rt
+    ref, dev = O.Case(setup), rt.Case(setup)
+    for f, v in state.items():
+        ref.set(f, v)
+        dev.set(f, v)
+    return ref, dev
+
+
+def _steps(ref, dev, steps):
+    recs = []
+    for _ in range(steps):
+        ir, ig = ref.step(1), dev.step(1)
+        # a fixed iteration count ends as "not converged" on both sides
+        assert ig.status == ir.status and ig.status in (abi.FALM_OK, abi.FALM_ERR_NOT_CONVERGED), ig.status
+        recs.append((ir.solver_it, ig.solver_it, ir.div, ig.div))
+        assert ir.solver_it == ig.solver_it, recs
+        assert abs(ir.div - ig.div) <= 1e-10 * abs(ir.div), recs
+    return recs
+
+
+def _check_fields(ref, dev, tol, fields=(abi.FIELD_U, abi.FIELD_P, abi.FIELD_NUT, abi.FIELD_UU, abi.FIELD_DIV)):
+    out = {}
+    for field in fields:
+        ncol = abi.FIELD_NCOL[field]
+        mask = interior_mask(ref.dom, ncol) if field in (abi.FIELD_U, abi.FIELD_P, abi.FIELD_DIV) else None
+        err = rel_err(dev.get(field), ref.get(field), mask)
+        out[field] = err
+        assert err < tol, (field, err)
+    return out
+
+
+@pytest.mark.parametrize("pc_maxit", [1, 2])
+def test_windtunnel_rotor_seams(gpu, pc_maxit):
+    """FP64 wind tunnel + rotor + inlet noise: u, p, nu_t, uu, div and ff at
+    1e-10 after 4 steps; the one-pass (pc_maxit 1) and the fused three-sweep
+    (pc_maxit 2) preconditioners."""
+    setup = cases.windtunnel(n=N_SEAM, cells_per_d=8, tol=1e-30, maxit=6, pc_maxit=pc_maxit,
+                             pc_relax=1.0 if pc_maxit == 1 else 1.5, rotor_kw=dict(n_ap_per_blade=8))
+    ref, dev = _pair(setup, _seed_state(setup))
+    _steps(ref, dev, 4)
+    _check_fields(ref, dev, 1e-10)
+    assert rel_err(dev.get(abi.FIELD_FF), ref.get(abi.FIELD_FF)) < 1e-10
+    _, f, tq, th = dev.alm_get()
+    _, f_r, tq_r, th_r = ref.alm_get()
+    assert rel_err(f, f_r) < 1e-10 and rel_err(tq, tq_r) < 1e-10 and rel_err(th, th_r) < 1e-10
+
+
+@pytest.mark.parametrize("scheme", [abi.ADV_QUICK, abi.ADV_UPWIND1, abi.ADV_UTOPIA, abi.ADV_KK])
+def test_windtunnel_schemes_seams(gpu, scheme):
+    """The other advection schemes through the step driver: QUICK and
+    Upwind1 run the k-marching kernel with per-cell scheme arithmetic
+    (src/FalmCFDScheme.h:12-66), UTOPIA and KK the face-difference one."""
+    setup = cases.windtunnel(n=N_SEAM, cells_per_d=8, tol=1e-30, maxit=4, rotor_kw=dict(n_ap_per_blade=8))
+    setup.struct.adv_scheme = scheme
+    ref, dev = _pair(setup, _seed_state(setup, seed=12 + scheme))
+    _steps(ref, dev, 3)
+    _check_fields(ref, dev, 1e-10)
+
+
+def test_lid3d_seams(gpu):
+    """FP64 cavity LES on a 150 x 20 x 37 box (reflection walls, lid on y+,
+    zero wall-normal face flux, Neumann pressure)."""
+    setup = cases.lid3d(n=N_SEAM, dt=2e-3, tol=1e-30, maxit=6)
+    ref, dev = _pair(setup, _seed_state(setup, seed=21))
+    _steps(ref, dev, 4)
+    _check_fields(ref, dev, 1e-10)
+
+
+@pytest.mark.parametrize("kind", ["windtunnel", "lid3d"])
+def test_mixed_seams_track_fp64(gpu, kind):
+    """Mixed precision (float stencils and Krylov iterations, double defect
+    correction) against the FP64 GPU step on the seam domain, from the same
+    seeded state at Poisson tolerance 1e-10: the velocity differs by the
+    float stencils' rounding only."""
+    from falm_amd import runtime as rt
+    if kind == "windtunnel":
+        mk = lambda prec: cases.windtunnel(n=N_SEAM, cells_per_d=8, tol=1e-10, precision=prec,  # noqa: E731
+                                           rotor_kw=dict(n_ap_per_blade=8))
+    else:
+        mk = lambda prec: cases.lid3d(n=N_SEAM, dt=2e-3, tol=1e-10, precision=prec)  # noqa: E731
+    s64, smx = mk(abi.CASE_F64), mk(abi.CASE_MIXED)
+    state = _seed_state(s64, seed=31)
+    a, b = rt.Case(s64), rt.Case(smx)
+    for f, v in state.items():
+        a.set(f, v)
+        b.set(f, v)
+    for _ in range(5):
+        ia, ib = a.step(1), b.step(1)
+        assert ia.status == abi.FALM_OK and ib.status == abi.FALM_OK
+        assert ib.solver_err <= 1e-10
+    m3, m1 = interior_mask(a.dom, 3), interior_mask(a.dom)
+    ua, ub = a.get(abi.FIELD_U)[m3], b.get(abi.FIELD_U)[m3]
+    pa, pb = a.get(abi.FIELD_P)[m1], b.get(abi.FIELD_P)[m1]
+    ru = np.linalg.norm(ua - ub) / np.linalg.norm(ua)
+    rp = np.linalg.norm(pa - pb) / np.linalg.norm(pa)
+    print(f"{kind} mixed vs fp64 after 5 steps: u rel L2 {ru:.3e}, p rel L2 {rp:.3e}, "
+          f"div {ia.div:.3e} / {ib.div:.3e}")
+    assert ru < 1e-5 and rp < 1e-4

@@ -1,8 +1,9 @@
 """Pins the CPU oracle (the checker of the HIP library) against what the
 reference's own tests hold and what the mathematics fixes.  Every check is
 chosen so that a plausible restatement mistake — a dropped term, a wrong
-sign, index or coefficient order, a transposed operand — fails it; the
-planted-mistake test at the end shows that for a set of such mutations."""
+sign, index or coefficient order, a transposed operand — fails it;
+tests/test_oracle_mutants.py plants such mistakes in a copy of the oracle
+and shows that these checks catch every one of them."""
 import math
 
 import numpy as np
@@ -74,7 +75,7 @@ def test_heat1d_bicgstab(pc):
     assert np.max(np.abs(x[interior_mask(dom)] - T)) < 1e-7
 
 
-def _textbook_laplacian(n, outlet):
+def _textbook_laplacian(n, outlet, zcouple=True):
     """7-point Laplacian of a uniform grid (spacing h): Neumann walls by
     dropping the coupling, an x+ Dirichlet wall (ghost p = 0) keeping it,
     divided by max|a_c|.  Built with scipy, independent of the oracle."""
@@ -84,7 +85,8 @@ def _textbook_laplacian(n, outlet):
     idx = np.arange(N).reshape(nz, ny, nx)
     rows, cols, vals = [], [], []
     diag = np.zeros(N)
-    for (di, dj, dk) in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+    dirs = ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0)) + (((0, 0, 1), (0, 0, -1)) if zcouple else ())
+    for (di, dj, dk) in dirs:
         for k in range(nz):
             for j in range(ny):
                 for i in range(nx):
@@ -201,11 +203,15 @@ def _grid(dom, cen):
     return X, Y, Z
 
 
-@pytest.mark.parametrize("scheme", [abi.ADV_UPWIND3, abi.ADV_UTOPIA, abi.ADV_KK])
+@pytest.mark.parametrize("scheme", [abi.ADV_UPWIND3, abi.ADV_UTOPIA, abi.ADV_KK, abi.ADV_QUICK])
 def test_upwind3_family_exact_for_quadratics(scheme):
     """With uniform face flux U (uu = U*J*kx) every Upwind3-family scheme
     gives adv = U . grad(phi) exactly for quadratic phi (the fourth difference
-    vanishes), so ua = un - dt*U.grad(phi) + dt*ff."""
+    vanishes), so ua = un - dt*U.grad(phi) + dt*ff.  QUICK too: its face
+    value (3 phi_d + 6 phi_u - phi_uu)/8 from the upwind side
+    (src/FalmCFDScheme.h:33-66) is the quadratic through three cells, exact
+    at the face, and the face difference of exact face values of a quadratic
+    is its exact derivative.  U has both signs across the axes."""
     dom = domain((9, 8, 7))
     h = 0.125
     cen, xyz, kx, g, ja = _uniform(dom, h)
@@ -275,6 +281,106 @@ def test_diffusion_exact_for_cubics():
     for d in range(3):
         got = ua[d * dom.ncell:(d + 1) * dom.ncell][m]
         assert np.max(np.abs(got - (ReI + 0.01) * lap[m])) < 1e-11
+
+
+def test_upwind1_closed_form_for_quadratics():
+    """Upwind1 (src/FalmCFDScheme.h:8-31) with uniform U: the flux through a
+    face is U times the upwind cell value, so for quadratic phi
+    adv = U . grad(phi) - (h/2) sum_e |U_e| d2phi/dx_e2 exactly: linear fields
+    are advected exactly and the error of a quadratic has the upwind sign
+    (a downwind flux would flip it)."""
+    dom = domain((9, 8, 7))
+    h = 0.125
+    cen, xyz, kx, g, ja = _uniform(dom, h)
+    X, Y, Z = _grid(dom, cen)
+    U = np.array([0.7, -0.4, 0.25])
+    phis = [0.3 + X * X - 0.5 * X * Y + 0.2 * Z * Z + X,
+            1.0 - Y * Y + 0.3 * X * Z + 0.1 * Y,
+            0.5 * Z * Z - X * Y + 2.0 * Z + 0.4 * X]
+    grads = [(2 * X - 0.5 * Y + 1, -0.5 * X, 0.4 * Z),
+             (0.3 * Z, -2 * Y + 0.1, 0.3 * X),
+             (-Y + 0.4, -X, Z + 2.0)]
+    curv = [(2.0, 0.0, 0.4), (0.0, -2.0, 0.0), (0.0, 0.0, 1.0)]  # d2phi/dx_e2
+    u = np.concatenate(phis)
+    uu = np.concatenate([np.full(dom.ncell, U[d] * h * h) for d in range(3)])
+    ff = np.concatenate([np.full(dom.ncell, 0.1 * d) for d in range(3)])
+    un = u - 0.5
+    dt = 0.01
+    ua = O.pseudo_u(dom, abi.ADV_UPWIND1, 0.0, dt, un, u, uu, np.zeros(dom.ncell), kx, g, ja, ff)
+    m = interior_mask(dom)
+    for d in range(3):
+        adv = sum(U[e] * grads[d][e] - 0.5 * h * abs(U[e]) * curv[d][e] for e in range(3))
+        expect = un[d * dom.ncell:(d + 1) * dom.ncell] + dt * (-adv + 0.1 * d)
+        assert np.max(np.abs(ua[d * dom.ncell:(d + 1) * dom.ncell][m] - expect[m])) < 1e-12
+
+
+def test_quick_closed_form_for_cubics():
+    """QUICK's upwind correction is a third difference, so quadratics cannot
+    see it.  For a cubic the upwind-side quadratic interpolant misses the
+    face value by sign(U) (d3phi/dx3) h^3/16 on every face; with d3phi/dx3
+    constant these cancel between the two faces of a cell and
+    adv = sum_e U_e (dphi/dx_e + (h^2/24) d3phi/dx_e3) exactly.  A wrong sign
+    or weight in the |U| terms leaves a multiple of the third difference."""
+    dom = domain((9, 8, 7))
+    h = 0.125
+    cen, xyz, kx, g, ja = _uniform(dom, h)
+    X, Y, Z = _grid(dom, cen)
+    U = np.array([0.7, -0.4, 0.25])
+    phis = [X ** 3 - 0.5 * X * Y * Y + 0.3 * Z ** 3 + X * Y * Z, 1 - Y ** 3 + 0.2 * X * X * Z,
+            0.5 * Z ** 3 - X * Y + 0.4 * X ** 3]
+    grads = [(3 * X * X - 0.5 * Y * Y + Y * Z, -X * Y + X * Z, 0.9 * Z * Z + X * Y),
+             (0.4 * X * Z, -3 * Y * Y, 0.2 * X * X),
+             (-Y + 1.2 * X * X, -X, 1.5 * Z * Z)]
+    d3 = [(6.0, 0.0, 1.8), (0.0, -6.0, 0.0), (2.4, 0.0, 3.0)]
+    u = np.concatenate(phis)
+    uu = np.concatenate([np.full(dom.ncell, U[d] * h * h) for d in range(3)])
+    zero = np.zeros(3 * dom.ncell)
+    ua = O.pseudo_u(dom, abi.ADV_QUICK, 0.0, 1.0, zero, u, uu, np.zeros(dom.ncell), kx, g, ja, zero)
+    m = interior_mask(dom)
+    for d in range(3):
+        adv = sum(U[e] * (grads[d][e] + h * h / 24 * d3[d][e]) for e in range(3))
+        assert np.max(np.abs(-ua[d * dom.ncell:(d + 1) * dom.ncell][m] - adv[m])) < 1e-12
+
+
+@pytest.mark.parametrize("cfl", [0.1, 0.3])
+def test_upwind3_explicit_euler_amplification(cfl):
+    """One pseudo-velocity step (explicit Euler, src/FalmCFD.cpp:5-67) of a
+    transverse velocity mode v = cos(j theta) carried by a uniform stream U
+    along x multiplies the mode by the von Neumann factor of Upwind3
+    (src/FalmCFDScheme.h:134-165):
+        G = 1 - nu [ i (26 sin theta - sin 2 theta)/24 + (2 - 2 cos theta)^2/24 ],
+    nu = U dt / h.  |G| > 1 for long waves at any CFL (the fourth-difference
+    damping is O(theta^4), the Euler error O(nu^2 theta^2)): the worst mode
+    grows by 2.4 % per step at CFL 0.3 (e-folding in 42 steps) and by 0.12 %
+    at CFL 0.1 (808 steps), which is why long inviscid runs at CFL 0.3
+    overflow and runs at CFL 0.1 do not within 1000 steps."""
+    n = 48
+    dom = domain((n, 3, 3))
+    h = 1.0
+    cen, xyz, kx, g, ja = _uniform(dom, h)
+    nx, ny, nz = dom.shape
+    I = np.broadcast_to(np.arange(nx)[None, None, :], (nz, ny, nx)).ravel().astype(float)
+    U = 1.0
+    dt = cfl * h / U
+    m = interior_mask(dom)
+    for theta in (2 * np.pi / 24, 2 * np.pi / 12, 1.08, 2 * np.pi / 5, 2 * np.pi / 3):
+        v = np.cos(I * theta)
+        u = np.concatenate([np.full(dom.ncell, U), v, np.zeros(dom.ncell)])
+        uu = np.concatenate([np.full(dom.ncell, U * h * h), np.zeros(dom.ncell), np.zeros(dom.ncell)])
+        zero = np.zeros(3 * dom.ncell)
+        ua = O.pseudo_u(dom, abi.ADV_UPWIND3, 0.0, dt, u, u, uu, np.zeros(dom.ncell), kx, g, ja, zero)
+        G = 1.0 - cfl * (1j * (26 * np.sin(theta) - np.sin(2 * theta)) / 24 + (2 - 2 * np.cos(theta)) ** 2 / 24)
+        expect = G.real * np.cos(I * theta) - G.imag * np.sin(I * theta)
+        got = ua[dom.ncell:2 * dom.ncell]
+        assert np.max(np.abs(got[m] - expect[m])) < 1e-13, theta
+        assert np.max(np.abs(ua[:dom.ncell][m] - U)) < 1e-14  # the carrier is untouched
+    th = np.linspace(1e-3, np.pi, 20001)
+    gmax = np.max(np.abs(1.0 - cfl * (1j * (26 * np.sin(th) - np.sin(2 * th)) / 24 + (2 - 2 * np.cos(th)) ** 2 / 24)))
+    efold = 1.0 / np.log(gmax)
+    if cfl == 0.3:
+        assert 1.02 < gmax < 1.03 and efold < 50
+    else:
+        assert 1.0 < gmax < 1.002 and efold > 700
 
 
 def test_quick_and_upwind1_constant_field():
@@ -567,3 +673,139 @@ def test_alm_pitch_rotation_keeps_radius():
     assert abs(d[1] - 0.25 * math.cos(theta)) < 1e-14
     # rotation about y by -psi of (0, r cos, r sin): x = sin(psi)*r sin(theta)
     assert abs(d[0] - math.sin(psi) * 0.25 * math.sin(theta)) < 1e-14
+
+
+def test_alm_velocity_sampling_exact_for_linear_field():
+    """The actuator point samples u by trilinear interpolation between the
+    eight surrounding cell centres (src/alm/almDevCall.cu:28-41,131-253),
+    which is exact for a linear field: the point's forces then follow the
+    blade-element closed form with u(P) in place of the uniform inflow
+    (axial u_x(P), tangential Omega r + u_y(P) sin(theta) - u_z(P) cos(theta)).
+    A wrong corner weight, a swapped corner or an index off by one cell
+    changes u(P) for a field with gradients along all three axes."""
+    h = 1.0 / 16
+    dom = domain((64, 64, 64))
+    cen = [(np.arange(dom.shape[m]) - 1.5) * h for m in range(3)]
+    base = (2.013, 1.987, 2.041)  # off the cell centres and the midpoints along every axis
+    setup = cases.rotor(h, base=base, radius=0.5, tsr=4.0, n_blade=1, n_ap_per_blade=1, twist=(20.0, 20.0),
+                        chord=(0.1, 0.1))
+    A = O.Alm(dom, setup, *cen)
+    X, Y, Z = _grid(dom, cen)
+    G = np.array([[0.15, -0.2, 0.1], [0.3, 0.05, -0.25], [-0.1, 0.2, 0.12]])
+    c0 = np.array([1.0, 0.1, -0.05])
+    u = np.concatenate([c0[d] + G[d, 0] * (X - 2) + G[d, 1] * (Y - 2) + G[d, 2] * (Z - 2) for d in range(3)])
+    ff = np.zeros(3 * dom.ncell)
+    t = 0.3
+    A.step(u, ff, t)
+    xyz, f, tq, th = A.get()
+    P = xyz[0]
+    up = c0 + G @ (P - 2.0)
+    Om, r = 8.0, 0.25
+    theta = Om * t
+    ux = up[0]
+    ut = Om * r + up[1] * math.sin(theta) - up[2] * math.cos(theta)
+    phi = math.atan(ux / ut)
+    alpha = math.degrees(phi) - 20.0
+    assert -12.0 < alpha < 12.0  # inside the linear part of the synthetic polar
+    table = np.arange(-45.0, 46.0, 1.0)
+    cl = 2 * math.pi * math.radians(alpha)
+    cd = np.interp(alpha, table, 0.01 + 0.05 * np.radians(table) ** 2)
+    urel2 = ux * ux + ut * ut
+    fl, fd = 0.5 * cl * urel2 * 0.1 * 0.5, 0.5 * cd * urel2 * 0.1 * 0.5
+    fx = fl * math.cos(phi) + fd * math.sin(phi)
+    ft = fl * math.sin(phi) - fd * math.sin(phi)
+    assert abs(th[0] - fx) < 1e-12 and abs(tq[0] - abs(ft) * r) < 1e-12
+    assert np.allclose(f[0], [-fx, ft * math.sin(theta), -ft * math.cos(theta)], atol=1e-12, rtol=0)
+
+
+def test_poisson_matrix_2d_is_textbook_five_point():
+    """app/ld2d/main.cpp:14-68: the 2-D cavity's matrix is the 5-point
+    Laplacian with Neumann walls (couplings across walls dropped), its z
+    columns zero, divided by max|a_c| = 4/h^2.  On a domain of several
+    planes each plane is its own 5-point problem (a z-thin domain alone
+    cannot tell z couplings from Neumann walls)."""
+    n = (7, 5, 3)
+    dom = domain(n)
+    h = 0.1
+    cen = [(np.arange(dom.shape[m]) - 1.5) * h for m in range(3)]
+    _, _, g, ja = O.metrics(dom, *cen, *[np.full(dom.shape[m], h) for m in range(3)])
+    a, md = O.poisson_matrix(dom, abi.POISSON_2D, g, ja)
+    assert abs(md - 4.0 / h ** 2) < 1e-9 * md
+    m = interior_mask(dom)
+    cols = a.reshape(7, -1)
+    assert np.all(cols[5] == 0.0) and np.all(cols[6] == 0.0)
+    assert np.max(np.abs(cols[:, m].sum(axis=0))) < 1e-15  # zero row sums
+    A = _textbook_laplacian(n, False, zcouple=False)
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        v = rng.standard_normal(m.sum())
+        x = np.zeros(dom.ncell)
+        x[m] = v
+        got = O.mv(dom, a, x)[m]
+        assert np.max(np.abs(got - A @ v)) < 1e-13
+
+
+def test_copy_z5_closed_form():
+    """app/ld2d/bcdevcall.cu:240-256: planes k = gc-2 .. gc+2 of every column
+    equal plane k = gc over the whole padded x-y extent; other planes keep
+    their values."""
+    for n in ((5, 4, 1), (4, 3, 3)):
+        dom = domain(n)
+        nx, ny, nz = dom.shape
+        v = np.random.default_rng(4).standard_normal(2 * dom.ncell)
+        out = v.copy()
+        O.copy_z5(dom, out, 2)
+        V, W = v.reshape(2, nz, ny, nx), out.reshape(2, nz, ny, nx)
+        for k in range(nz):
+            if k <= 4:
+                assert np.array_equal(W[:, k], V[:, 2])
+            else:
+                assert np.array_equal(W[:, k], V[:, k])
+
+
+def test_pressure_fill_closed_form():
+    """pbc (app/windtunnel/bcdevcall.cu:153-211): a Neumann face copies the
+    first interior layer into the first ghost layer, the Dirichlet outlet
+    sets that ghost layer to 0, over the face's interior extent only;
+    second ghost layers, edges and corners keep their values."""
+    dom = domain((6, 5, 4))
+    nx, ny, nz = dom.shape
+    p = np.random.default_rng(5).standard_normal(dom.ncell)
+    faces = [abi.PBC_NEUMANN, abi.PBC_ZERO, abi.PBC_NEUMANN, abi.PBC_NEUMANN, abi.PBC_NONE, abi.PBC_NEUMANN]
+    out = p.copy()
+    O.bc_pressure(dom, faces, out)
+    P, Q = p.reshape(nz, ny, nx), out.reshape(nz, ny, nx)
+    E = P.copy()
+    s = slice(2, -2)
+    E[s, s, 1] = P[s, s, 2]
+    E[s, s, nx - 2] = 0.0
+    E[s, 1, s] = P[s, 2, s]
+    E[s, ny - 2, s] = P[s, ny - 3, s]
+    E[nz - 2, s, s] = P[nz - 3, s, s]
+    assert np.array_equal(Q, E)
+
+
+def test_min_max_reductions_match_numpy():
+    """MatColMax/Min/AbsMax/AbsMin of one column and VecMax/Min of the
+    per-cell Euclidean norm over all columns (src/MV.h:67-136,
+    src/dev/MVDevCall.cu:216-598), interior cells only.  Values are chosen
+    so that each extremum sits in the interior and the ghosts hold larger
+    and smaller values that a wrong region would pick up."""
+    dom = domain((5, 6, 7))
+    rng = np.random.default_rng(9)
+    a = rng.uniform(0.5, 2.0, 3 * dom.ncell) * rng.choice([-1.0, 1.0], 3 * dom.ncell)
+    m = interior_mask(dom)
+    ghost = np.nonzero(~m)[0]
+    for col in range(3):
+        a[col * dom.ncell + ghost] = 50.0 * rng.choice([-1.0, 1.0], ghost.size)
+        a[col * dom.ncell + ghost[::7]] = 1e-3
+    cols = a.reshape(3, -1)[:, m]
+    for col in range(3):
+        c = cols[col]
+        assert O.reduce(dom, abi.RED_COL_MAX, a, 3, col) == c.max()
+        assert O.reduce(dom, abi.RED_COL_MIN, a, 3, col) == c.min()
+        assert O.reduce(dom, abi.RED_COL_ABSMAX, a, 3, col) == np.abs(c).max()
+        assert O.reduce(dom, abi.RED_COL_ABSMIN, a, 3, col) == np.abs(c).min()
+    norm = np.sqrt(cols[0] * cols[0] + cols[1] * cols[1] + cols[2] * cols[2])
+    assert O.reduce(dom, abi.RED_VEC_MAX, a, 3, 0) == norm.max()
+    assert O.reduce(dom, abi.RED_VEC_MIN, a, 3, 0) == norm.min()
