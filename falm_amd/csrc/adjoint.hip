@@ -396,6 +396,32 @@ falm_status vjp_check(const falm_domain *dom, std::initializer_list<const void *
     return check_domain(dom);
 }
 
+}  // namespace
+
+// The VJP kernels accumulate into their outputs with hardware FP64 atomic
+// adds, which are exact only on coarse-grained device memory (hipMalloc):
+// on host-pinned, managed or fine-grained memory they can drop updates
+// silently.  Host and managed memory are refused here (fine-grained device
+// memory from hipExtMallocWithFlags cannot be told apart by its attributes).
+falm_status falm::vjp_outputs_on_device(std::initializer_list<const void *> outs) {
+    for (const void *p : outs) {
+        hipPointerAttribute_t at{};
+        if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("VJP output %p is not device memory", p);
+            return FALM_ERR_INVALID;
+        }
+        if (at.type != hipMemoryTypeDevice || at.isManaged) {
+            set_error("VJP output %p must be device memory from hipMalloc (the adjoint accumulates with FP64 "
+                      "atomics)", p);
+            return FALM_ERR_INVALID;
+        }
+    }
+    return FALM_OK;
+}
+
+namespace {
+
 FaceRange face_range(const Geom &g, const int type[6], long long &total) {
     FaceRange fr;
     const int shp[3] = {g.nx, g.ny, g.nz};
@@ -418,6 +444,7 @@ falm_status falm_pseudo_u_vjp(const falm_domain *dom, int adv_scheme, double ReI
                               const void *uu, const void *nut, const void *kx, const void *g, const void *ja,
                               const void *uabar, void *ubar, void *uubar, void *nutbar, void *ffbar, void *stream) {
     FALM_TRY(vjp_check(dom, {u, uu, nut, kx, g, ja, uabar, ubar, uubar, nutbar, ffbar}));
+    FALM_TRY(vjp_outputs_on_device({ubar, uubar, nutbar, ffbar}));
     if (adv_scheme != FALM_ADV_UPWIND3 && adv_scheme != FALM_ADV_UTOPIA && adv_scheme != FALM_ADV_KK) {
         set_error("falm_pseudo_u_vjp: Upwind3-family advection only");
         return FALM_ERR_UNSUPPORTED;
@@ -436,6 +463,7 @@ falm_status falm_pseudo_u_vjp(const falm_domain *dom, int adv_scheme, double ReI
 falm_status falm_u_to_uu_vjp(const falm_domain *dom, const void *kx, const void *ja, const void *uubar, void *ubar,
                              void *stream) {
     FALM_TRY(vjp_check(dom, {kx, ja, uubar, ubar}));
+    FALM_TRY(vjp_outputs_on_device({ubar}));
     const Geom gm = make_geom(dom);
     const size_t n = ncell_of(dom);
     const Box fb = face_box(gm);
@@ -448,6 +476,7 @@ falm_status falm_u_to_uu_vjp(const falm_domain *dom, const void *kx, const void 
 falm_status falm_poisson_rhs_vjp(const falm_domain *dom, const void *ja, double dt, double maxdiag,
                                  const void *rhsbar, void *uubar, void *stream) {
     FALM_TRY(vjp_check(dom, {ja, rhsbar, uubar}));
+    FALM_TRY(vjp_outputs_on_device({uubar}));
     if (!(dt * maxdiag != 0.0)) return FALM_ERR_INVALID;
     const Geom gm = make_geom(dom);
     const size_t n = ncell_of(dom);
@@ -461,6 +490,7 @@ falm_status falm_poisson_rhs_vjp(const falm_domain *dom, const void *ja, double 
 falm_status falm_project_p_vjp(const falm_domain *dom, double dt, const void *kx, const void *g, const void *ubar,
                                const void *uubar, void *uabar, void *uuabar, void *pbar, void *stream) {
     FALM_TRY(vjp_check(dom, {kx, g, ubar, uubar, uabar, uuabar, pbar}));
+    FALM_TRY(vjp_outputs_on_device({uabar, uuabar, pbar}));
     const Geom gm = make_geom(dom);
     const size_t n = ncell_of(dom);
     const Box fb = face_box(gm);
@@ -473,6 +503,7 @@ falm_status falm_project_p_vjp(const falm_domain *dom, double dt, const void *kx
 falm_status falm_sgs_vjp(const falm_domain *dom, double Cs, const void *u, const void *kx, const void *ja,
                          const void *nutbar, void *ubar, void *stream) {
     FALM_TRY(vjp_check(dom, {u, kx, ja, nutbar, ubar}));
+    FALM_TRY(vjp_outputs_on_device({ubar}));
     const Geom gm = make_geom(dom);
     const size_t n = ncell_of(dom);
     const Box ib = interior_box(gm);
@@ -485,6 +516,7 @@ falm_status falm_sgs_vjp(const falm_domain *dom, double Cs, const void *u, const
 falm_status falm_bc_velocity_vjp(const falm_domain *dom, const falm_face_bc faces[6], const void *xyz, double dt,
                                  const void *unp1bar, void *usbar, void *unbar, void *stream) {
     FALM_TRY(vjp_check(dom, {faces, xyz, unp1bar, usbar, unbar}));
+    FALM_TRY(vjp_outputs_on_device({usbar, unbar}));
     const Geom gm = make_geom(dom);
     const size_t n = ncell_of(dom);
     const Box ib = interior_box(gm);
@@ -504,6 +536,7 @@ falm_status falm_bc_velocity_vjp(const falm_domain *dom, const falm_face_bc face
 
 falm_status falm_bc_pressure_vjp(const falm_domain *dom, const int faces[6], void *pbar, void *stream) {
     FALM_TRY(vjp_check(dom, {faces, pbar}));
+    FALM_TRY(vjp_outputs_on_device({pbar}));
     const Geom gm = make_geom(dom);
     long long nf = 0;
     const FaceRange fr = face_range(gm, faces, nf);

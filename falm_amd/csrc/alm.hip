@@ -1051,6 +1051,7 @@ falm_status falm_alm_set_base(falm_alm *A, int turbine, const double *base, cons
 falm_status falm_alm_vjp(falm_alm *A, const void *u, double t, const void *ffbar, double wP, void *ubar,
                          void *g_pitch, void *g_twist, void *stream) {
     if (!A || !u || !ffbar || !ubar || !g_pitch || !g_twist) return FALM_ERR_INVALID;
+    FALM_TRY(falm::vjp_outputs_on_device({ubar, g_pitch, g_twist}));
     return falm::alm_adjoint(A, (const double *)u, t, (const double *)ffbar, wP, (double *)ubar, (double *)g_pitch,
                              (double *)g_twist, (hipStream_t)stream);
 }

@@ -363,6 +363,12 @@ falm_status falm_case_create(const falm_case_config *cfg, falm_case **out) {
     *out = nullptr;
     if (cfg->kind < FALM_CASE_WINDTUNNEL || cfg->kind > FALM_CASE_LID2D) return FALM_ERR_INVALID;
     if (cfg->precision != FALM_CASE_F64 && cfg->precision != FALM_CASE_MIXED) return FALM_ERR_UNSUPPORTED;
+    {   // the pressure, Poisson right-hand side and the solver's vectors are
+        // double in both precisions: their columns must stay below 4 GiB
+        size_t cells = 1;
+        for (int m = 0; m < 3; m++) cells *= (size_t)(cfg->n[m] > 0 ? cfg->n[m] + 4 : 1);
+        FALM_TRY(check_byte_offsets(cells, sizeof(double)));
+    }
     falm_case *C = new falm_case();
     C->cfg = *cfg;
     C->cfg.alm = cfg->alm;  // read during create only

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <initializer_list>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -60,6 +61,9 @@ inline size_t ncell_of(const falm_domain *d) {
 }
 
 falm_status check_domain(const falm_domain *d);
+// FALM_ERR_INVALID unless every pointer is hipMalloc'd device memory (the
+// adjoint kernels accumulate into such outputs with FP64 atomics)
+falm_status vjp_outputs_on_device(std::initializer_list<const void *> outs);
 
 // A box of cells [lo, lo+n) in padded local indices.
 struct Box {
@@ -183,7 +187,8 @@ inline dim3 tiling_block() { return dim3(TX * TY); }
 // VGPR offset shared by every array indexed at the same cell, constant
 // neighbour offsets folded into the instruction), where a signed element
 // index costs a 64-bit address computation per access.  Arrays addressed
-// this way must be below 4 GiB (checked by the launchers).
+// this way must be below 4 GiB (checked by the launchers, by
+// falm_solver_create and by falm_case_create).
 template <class T>
 __device__ __forceinline__ T ld_at(const T *p, unsigned off) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + off);

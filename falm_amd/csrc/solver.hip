@@ -1417,6 +1417,9 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
     FALM_TRY(check_domain(dom));
     if (!cfg || !out) return FALM_ERR_INVALID;
     *out = nullptr;
+    // the k-marching kernels address the double vectors (x, b, the true
+    // residual) with 32-bit byte offsets
+    FALM_TRY(check_byte_offsets(ncell_of(dom), sizeof(double)));
     if (cfg->type != FALM_LS_PBICGSTAB && cfg->type != FALM_LS_SOR && cfg->type != FALM_LS_JACOBI) {
         set_error("falm_solver_create: solver type %d not supported", cfg->type);
         return FALM_ERR_UNSUPPORTED;

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <fstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "FalmCFD.h"
@@ -256,121 +257,149 @@ public:
         return FalmErr::success;
     }
 
-    // FalmCore::computation_init (src/falm.h:342-431)
+    // FalmCore::computation_init (src/falm.h:342-431): control volumes ->
+    // partition -> case description files -> local 1-D mesh -> metrics ->
+    // fields.
     int computation_init(const Int3 &division, int gc) {
-        if (falmMeshInfo.convert && cpm.rank == 0) {
-            if (Mesher::build_mesh(falmMeshInfo.cvCenter, wpath(falmMeshInfo.convertSrc), wpath(falmMeshInfo.cvFile),
-                                   gc) != 0)
-                return lastStatus() = FalmErr::setUpFileErr;
-        }
         Int3 idmax;
-        Int cgc = 0;
-        if (!FalmIO::readControlVolumeFile(wpath(falmMeshInfo.cvFile), gBaseMesh, idmax, cgc)) {
-            std::fprintf(stderr, "FalmCore: cannot read control-volume file %s\n", wpath(falmMeshInfo.cvFile).c_str());
-            return lastStatus() = FalmErr::setUpFileErr;
-        }
-        if (cgc != gc) {
-            std::fprintf(stderr, "FalmCore: control-volume file has %d guide cells, expected %d\n", cgc, gc);
-            return lastStatus() = FalmErr::setUpFileErr;
-        }
-        int st = cpm.initPartition(idmax, gc, division);
+        int st = load_control_volumes(gc, idmax);
         if (st != FALM_OK) return st;
-        const Int3 shape = cpm.pdm_list[cpm.rank].shape;
-        const Int3 offset = cpm.pdm_list[cpm.rank].offset;
-        if (cpm.rank == 0 && outputEndIt >= outputStartIt) {
-            FalmIO::writeIndexFile(wpath(outputPrefix + ".json"), cpm, timeSlices);
-            FalmIO::writeControlVolumeFile(wpath(outputPrefix + ".cv"), gBaseMesh, idmax, gc);
-            FalmIO::writeSetupFile(wpath(outputPrefix + "_setup.json"), params);
-        }
-        std::printf("(%d %d %d) (%d %d %d)\n", shape[0], shape[1], shape[2], offset[0], offset[1], offset[2]);
-
-        baseMesh.alloc(shape[0], shape[1], shape[2], HDC::HstDev);
-        for (Int i = 0; i < shape[0]; i++) {
-            baseMesh.x(i) = gBaseMesh.x(i + offset[0]);
-            baseMesh.hx(i) = gBaseMesh.hx(i + offset[0]);
-        }
-        for (Int j = 0; j < shape[1]; j++) {
-            baseMesh.y(j) = gBaseMesh.y(j + offset[1]);
-            baseMesh.hy(j) = gBaseMesh.hy(j + offset[1]);
-        }
-        for (Int k = 0; k < shape[2]; k++) {
-            baseMesh.z(k) = gBaseMesh.z(k + offset[2]);
-            baseMesh.hz(k) = gBaseMesh.hz(k + offset[2]);
-        }
-        baseMesh.sync(MCP::Hst2Dev);
-
-        // metrics on the device (falm_metrics: ja = hx hy hz, kx = 1/h,
-        // g = ja kx^2, xyz = centres), then host copies for the
-        // application's matrix assembly
-        fv.xyz.alloc(shape, 3, HDC::HstDev, "coordinate");
-        fv.kx.alloc(shape, 3, HDC::HstDev, "d kxi / d x");
-        fv.g.alloc(shape, 3, HDC::HstDev, "metric tensor");
-        fv.ja.alloc(shape, 1, HDC::HstDev, "jacobian");
-        falm_domain d = cpm.domain();
-        st = falmErrCheckMacro(falm_metrics(&d, FALM_F64, baseMesh.x.host.ptr, baseMesh.y.host.ptr,
-                                            baseMesh.z.host.ptr, baseMesh.hx.host.ptr, baseMesh.hy.host.ptr,
-                                            baseMesh.hz.host.ptr, fv.xyz.dev.ptr, fv.kx.dev.ptr, fv.g.dev.ptr,
-                                            fv.ja.dev.ptr, nullptr));
+        st = cpm.initPartition(idmax, gc, division);
         if (st != FALM_OK) return st;
-        falmErrCheckMacro(falm_stream_sync(nullptr));
-        for (Matrix<Real> *m : {&fv.xyz, &fv.kx, &fv.g, &fv.ja}) m->sync(MCP::Dev2Hst);
-
-        fv.u.alloc(shape, 3, HDC::HstDev, "velocity");
-        fv.uu.alloc(shape, 3, HDC::HstDev, "contravariant face velocity");
-        fv.p.alloc(shape, 1, HDC::HstDev, "pressure");
-        fv.nut.alloc(shape, 1, HDC::HstDev, "turbulence viscosity");
-        fv.poi_a.alloc(shape, 7, HDC::HstDev, "poisson coefficients", StencilMatrix::D3P7);
-        fv.poi_rhs.alloc(shape, 1, HDC::HstDev, "poisson rhs");
-        fv.poi_res.alloc(shape, 1, HDC::HstDev, "poisson residual");
-        fv.ff.alloc(shape, 3, HDC::HstDev, "ALM force");
-        fv.divergence.alloc(shape, 1, HDC::HstDev, "divergence");
-        // the reference allocates these only when end > start, yet averages
-        // at it == start == end too (src/falm.h:421, 433-444)
-        if (timeAvgEndIt >= timeAvgStartIt) {
-            fv.utavg.alloc(shape, 3, HDC::Device, "time-avg velocity");
-            fv.ptavg.alloc(shape, 1, HDC::Device, "time-avg pressure");
-            fv.utavg.clear(HDC::Device);
-            fv.ptavg.clear(HDC::Device);
-        }
-        fv.uvwp.alloc(shape, 4, HDC::HstDev, "uvwp buffer");
-        falmCfd.alloc(shape);
-        falmEq.alloc(shape);
+        const Region &local = cpm.pdm_list[cpm.rank];
+        if (cpm.rank == 0 && outputEndIt >= outputStartIt) write_case_description(idmax, gc);
+        std::printf("(%d %d %d) (%d %d %d)\n", local.shape[0], local.shape[1], local.shape[2], local.offset[0],
+                    local.offset[1], local.offset[2]);
+        slice_local_mesh(local);
+        st = derive_metrics(local.shape);
+        if (st != FALM_OK) return st;
+        allocate_fields(local.shape);
+        falmCfd.alloc(local.shape);
+        falmEq.alloc(local.shape);
         return lastStatus() == FALM_OK ? FALM_OK : lastStatus();
     }
 
-    // FalmCore::TAvg (src/falm.h:433-444): avg = (1 - 1/n) avg + (1/n) v
+    // FalmCore::TAvg (src/falm.h:433-444): the running mean over the
+    // averaging window, mean_n = mean_{n-1} + (v - mean_{n-1})/n written as
+    // (1 - 1/n) mean_{n-1} + (1/n) v with one Vecaxby per field
     void TAvg(dim3 block = dim3(8, 8, 8)) {
         if (!is_time_averaging()) return;
         timeAvgCount = it - timeAvgStartIt + 1;
-        const Real b = 1. / timeAvgCount;
-        const Real a = 1. - b;
-        FalmMV::Vecaxby(a, fv.utavg, b, fv.u, fv.utavg, block);
-        FalmMV::Vecaxby(a, fv.ptavg, b, fv.p, fv.ptavg, block);
+        const Real w_new = 1. / timeAvgCount, w_old = 1. - w_new;
+        const std::pair<Matrix<Real> *, Matrix<Real> *> avg_of[2] = {{&fv.utavg, &fv.u}, {&fv.ptavg, &fv.p}};
+        for (const auto &pr : avg_of) FalmMV::Vecaxby(w_old, *pr.first, w_new, *pr.second, *pr.first, block);
     }
 
-    // FalmCore::outputUVWP (src/falm.h:446-479): file prefix_RRRRRR_IIIIIIIIII
-    // holding u, v, w, p, and prefix_tavg_... while averaging
+    // FalmCore::outputUVWP (src/falm.h:446-479): at output steps, the
+    // snapshot PREFIX_RRRRRR_IIIIIIIIII of (u, v, w, p) and, inside the
+    // averaging window, PREFIX_tavg_... of the running means; the step is
+    // recorded for the index file.
     void outputUVWP(dim3 /*block*/ = dim3(8, 8, 8)) {
-        if (it < outputStartIt || it > outputEndIt || outputIntervalIt <= 0 || it % outputIntervalIt != 0) return;
-        const size_t n = (size_t)fv.uvwp.shape[0];
-        falmErrCheckMacro(falm_memcpy(&fv.uvwp(0, 0), fv.u.dev.ptr, sizeof(Real) * n * 3, MCP::Dev2Hst, nullptr));
-        falmErrCheckMacro(falm_memcpy(&fv.uvwp(0, 3), fv.p.dev.ptr, sizeof(Real) * n, MCP::Dev2Hst, nullptr));
-        FalmIO::writeVectorFile(wpath(snapshot_name(outputPrefix)), cpm, fv.uvwp, (size_t)it, gettime());
-        FalmSnapshotInfo snap = {(size_t)it, gettime(), false};
-        if (is_time_averaging()) {
-            falmErrCheckMacro(falm_memcpy(&fv.uvwp.dev(0, 0), fv.utavg.dev.ptr, sizeof(Real) * n * 3, MCP::Dev2Dev,
-                                          nullptr));
-            falmErrCheckMacro(falm_memcpy(&fv.uvwp.dev(0, 3), fv.ptavg.dev.ptr, sizeof(Real) * n, MCP::Dev2Dev,
-                                          nullptr));
-            fv.uvwp.sync(MCP::Dev2Hst);
-            FalmIO::writeVectorFile(wpath(snapshot_name(outputPrefix + "_tavg")), cpm, fv.uvwp, (size_t)it,
-                                    gettime());
-            snap.tavg = true;
-        }
-        timeSlices.push_back(snap);
+        const bool due = it >= outputStartIt && it <= outputEndIt && outputIntervalIt > 0 && it % outputIntervalIt == 0;
+        if (!due) return;
+        write_uvwp(outputPrefix, fv.u, fv.p);
+        const bool averaged = is_time_averaging();
+        if (averaged) write_uvwp(outputPrefix + "_tavg", fv.utavg, fv.ptavg);
+        timeSlices.push_back(FalmSnapshotInfo{(size_t)it, gettime(), averaged});
     }
 
 private:
+    // Mesher conversion (rank 0, when the setup asks for it) and the
+    // control-volume file; the file's guide-cell count must match gc
+    int load_control_volumes(int gc, Int3 &idmax) {
+        const std::string cv = wpath(falmMeshInfo.cvFile);
+        if (falmMeshInfo.convert && cpm.rank == 0 &&
+            Mesher::build_mesh(falmMeshInfo.cvCenter, wpath(falmMeshInfo.convertSrc), cv, gc) != 0)
+            return lastStatus() = FalmErr::setUpFileErr;
+        Int file_gc = 0;
+        if (!FalmIO::readControlVolumeFile(cv, gBaseMesh, idmax, file_gc)) {
+            std::fprintf(stderr, "FalmCore: cannot read control-volume file %s\n", cv.c_str());
+            return lastStatus() = FalmErr::setUpFileErr;
+        }
+        if (file_gc != gc) {
+            std::fprintf(stderr, "FalmCore: control-volume file has %d guide cells, expected %d\n", file_gc, gc);
+            return lastStatus() = FalmErr::setUpFileErr;
+        }
+        return FALM_OK;
+    }
+
+    // index file, copy of the control volumes and of the setup next to the
+    // snapshots
+    void write_case_description(const Int3 &idmax, int gc) {
+        FalmIO::writeIndexFile(wpath(outputPrefix + ".json"), cpm, timeSlices);
+        FalmIO::writeControlVolumeFile(wpath(outputPrefix + ".cv"), gBaseMesh, idmax, gc);
+        FalmIO::writeSetupFile(wpath(outputPrefix + "_setup.json"), params);
+    }
+
+    // this sub-domain's window of the global 1-D centres and widths
+    void slice_local_mesh(const Region &local) {
+        baseMesh.alloc(local.shape[0], local.shape[1], local.shape[2], HDC::HstDev);
+        Matrix<Real> *lc[3] = {&baseMesh.x, &baseMesh.y, &baseMesh.z}, *lw[3] = {&baseMesh.hx, &baseMesh.hy,
+                                                                                   &baseMesh.hz};
+        Matrix<Real> *gcen[3] = {&gBaseMesh.x, &gBaseMesh.y, &gBaseMesh.z},
+                     *gwid[3] = {&gBaseMesh.hx, &gBaseMesh.hy, &gBaseMesh.hz};
+        for (int m = 0; m < 3; m++)
+            for (Int q = 0; q < local.shape[m]; q++) {
+                (*lc[m])(q) = (*gcen[m])(q + local.offset[m]);
+                (*lw[m])(q) = (*gwid[m])(q + local.offset[m]);
+            }
+        baseMesh.sync(MCP::Hst2Dev);
+    }
+
+    // metric fields from the 1-D mesh on the device (falm_metrics: volume
+    // hx hy hz, kx = 1/h, g = volume kx^2, cell centres), with host copies
+    // for the application's host-side matrix assembly
+    int derive_metrics(const Int3 &shape) {
+        fv.xyz.alloc(shape, 3, HDC::HstDev, "xyz");
+        fv.kx.alloc(shape, 3, HDC::HstDev, "kx");
+        fv.g.alloc(shape, 3, HDC::HstDev, "g");
+        fv.ja.alloc(shape, 1, HDC::HstDev, "ja");
+        const falm_domain d = cpm.domain();
+        const int st = falmErrCheckMacro(falm_metrics(
+            &d, FALM_F64, baseMesh.x.host.ptr, baseMesh.y.host.ptr, baseMesh.z.host.ptr, baseMesh.hx.host.ptr,
+            baseMesh.hy.host.ptr, baseMesh.hz.host.ptr, fv.xyz.dev.ptr, fv.kx.dev.ptr, fv.g.dev.ptr, fv.ja.dev.ptr,
+            nullptr));
+        if (st != FALM_OK) return st;
+        falmErrCheckMacro(falm_stream_sync(nullptr));
+        for (Matrix<Real> *m : {&fv.xyz, &fv.kx, &fv.g, &fv.ja}) m->sync(MCP::Dev2Hst);
+        return FALM_OK;
+    }
+
+    // the solution, solver and output fields (zero-filled).  The running
+    // means exist whenever the averaging window is not empty: the reference
+    // allocates them only for end > start (src/falm.h:421) yet averages at
+    // it == start == end too
+    void allocate_fields(const Int3 &shape) {
+        struct Spec {
+            Matrix<Real> *m;
+            Int ncol;
+            Flag where;
+            const char *label;
+        };
+        const Spec specs[] = {{&fv.u, 3, HDC::HstDev, "u"},          {&fv.uu, 3, HDC::HstDev, "uu"},
+                              {&fv.p, 1, HDC::HstDev, "p"},          {&fv.nut, 1, HDC::HstDev, "nut"},
+                              {&fv.poi_rhs, 1, HDC::HstDev, "poi_rhs"}, {&fv.poi_res, 1, HDC::HstDev, "poi_res"},
+                              {&fv.ff, 3, HDC::HstDev, "ff"},        {&fv.divergence, 1, HDC::HstDev, "divergence"},
+                              {&fv.uvwp, 4, HDC::HstDev, "uvwp"}};
+        for (const Spec &sp : specs) sp.m->alloc(shape, sp.ncol, sp.where, sp.label);
+        fv.poi_a.alloc(shape, 7, HDC::HstDev, "poi_a", StencilMatrix::D3P7);
+        if (timeAvgEndIt >= timeAvgStartIt) {
+            fv.utavg.alloc(shape, 3, HDC::Device, "utavg");
+            fv.ptavg.alloc(shape, 1, HDC::Device, "ptavg");
+            fv.utavg.clear(HDC::Device);
+            fv.ptavg.clear(HDC::Device);
+        }
+    }
+
+    // stage (velocity, pressure) of the device fields in the host columns of
+    // uvwp and write them as one vector file named for this rank and step
+    void write_uvwp(const std::string &prefix, Matrix<Real> &vel, Matrix<Real> &pres) {
+        const size_t n = (size_t)fv.uvwp.shape[0];
+        falmErrCheckMacro(falm_memcpy(&fv.uvwp(0, 0), vel.dev.ptr, sizeof(Real) * n * 3, MCP::Dev2Hst, nullptr));
+        falmErrCheckMacro(falm_memcpy(&fv.uvwp(0, 3), pres.dev.ptr, sizeof(Real) * n, MCP::Dev2Hst, nullptr));
+        FalmIO::writeVectorFile(wpath(snapshot_name(prefix)), cpm, fv.uvwp, (size_t)it, gettime());
+    }
+
     std::string snapshot_name(const std::string &prefix) {
         std::vector<char> buf(prefix.size() + 32);
         std::snprintf(buf.data(), buf.size(), "%s_%06d_%010d", prefix.c_str(), cpm.rank, it);

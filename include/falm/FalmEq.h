@@ -1,14 +1,22 @@
 // Falm C++ host API: FalmEq (src/FalmEq.h:14-104, src/FalmEqDevCall.h:25-95)
 // over the C ABI's device-resident solver.
 //
-// The library owns the work vectors (created on the first Solve with the
-// sub-domain of the CPM passed in).  The first Solve with a given matrix
-// declares it to the solver (falm_solver_set_matrix), which may keep a
-// compressed copy; call resetMatrix() after changing the matrix in place.
+// The library owns the work vectors.  The reference reads type, maxit, tol,
+// relax and the preconditioner settings on every Solve (src/FalmEq.h:56-64),
+// so they may change between calls, through init() or the public members:
+// each Solve builds the configuration from the current members and uses the
+// library solver created for that configuration and sub-domain (created on
+// first use, kept until release(), so alternating PBiCGStab() and SOR()
+// allocates nothing after the first call of each; at most 8 are kept, the
+// oldest is dropped first).  The first Solve of a
+// solver with a given matrix declares it (falm_solver_set_matrix), which may
+// keep a compressed copy; call resetMatrix() after changing the matrix in
+// place.
 #ifndef FALM_AMD_FALMEQ_H
 #define FALM_AMD_FALMEQ_H
 
 #include <string>
+#include <vector>
 
 #include "MV.h"
 
@@ -58,9 +66,15 @@ typedef FalmEqDevCall::SolverType LSType;
 
 class FalmEq : public FalmEqDevCall {
 public:
-    falm_solver *solver = nullptr;
-    const void *declared = nullptr;
-    falm_domain dom_cached{};
+    // one library solver per (configuration, sub-domain) used so far
+    struct Entry {
+        falm_solver_config cfg;
+        falm_domain dom;
+        falm_solver *solver;
+        const void *declared;
+    };
+    std::vector<Entry> solvers;
+    falm_solver *solver = nullptr;  // the one the last Solve used
 
     using FalmEqDevCall::FalmEqDevCall;
     FalmEq(const FalmEq &) = delete;
@@ -71,33 +85,61 @@ public:
     // the first Solve, which knows the sub-domain
     void alloc(Int3) {}
     void release() {
-        if (solver) falm_solver_destroy(solver);
+        for (Entry &e : solvers) falm_solver_destroy(e.solver);
+        solvers.clear();
         solver = nullptr;
-        declared = nullptr;
     }
-    void resetMatrix() { declared = nullptr; }
+    void resetMatrix() {
+        for (Entry &e : solvers) e.declared = nullptr;
+    }
+
+    // the configuration the reference's Solve would run with now
+    falm_solver_config current_config() const {
+        falm_solver_config c{};
+        c.type = (int)type;
+        c.maxit = maxit;
+        c.tol = tol;
+        c.relax = relax_factor;
+        c.pc_type = (int)pc_type;
+        c.pc_maxit = pc_maxit;
+        c.pc_relax = pc_relax_factor;
+        c.precision = precision;
+        return c;
+    }
+    static bool same(const falm_solver_config &a, const falm_solver_config &b) {
+        return a.type == b.type && a.maxit == b.maxit && a.tol == b.tol && a.relax == b.relax &&
+               a.pc_type == b.pc_type && a.pc_maxit == b.pc_maxit && a.pc_relax == b.pc_relax &&
+               a.precision == b.precision && a.inner_maxit == b.inner_maxit && a.inner_rtol == b.inner_rtol &&
+               a.check_every == b.check_every;
+    }
+    static bool same(const falm_domain &a, const falm_domain &b) {
+        for (int m = 0; m < 3; m++)
+            if (a.shape[m] != b.shape[m] || a.offset[m] != b.offset[m] || a.gshape[m] != b.gshape[m]) return false;
+        return a.gc == b.gc;
+    }
 
     void Solve(Matrix<Real> &a, Matrix<Real> &x, Matrix<Real> &b, Matrix<Real> &r, CPM &cpm, dim3 = dim3(8, 8, 8),
                Stream *stream = nullptr) {
-        falm_domain d = cpm.domain();
-        if (!solver || memcmp(&d, &dom_cached, sizeof(d)) != 0) {
-            release();
-            falm_solver_config c{};
-            c.type = (int)type;
-            c.maxit = maxit;
-            c.tol = tol;
-            c.relax = relax_factor;
-            c.pc_type = (int)pc_type;
-            c.pc_maxit = pc_maxit;
-            c.pc_relax = pc_relax_factor;
-            c.precision = precision;
-            if (falmErrCheckMacro(falm_solver_create(&d, &c, &solver)) != FALM_OK) return;
-            dom_cached = d;
+        const falm_domain d = cpm.domain();
+        const falm_solver_config c = current_config();
+        Entry *e = nullptr;
+        for (Entry &q : solvers)
+            if (same(q.cfg, c) && same(q.dom, d)) e = &q;
+        if (!e) {
+            if (solvers.size() >= 8) {  // bound the cache: drop the oldest
+                falm_solver_destroy(solvers.front().solver);
+                solvers.erase(solvers.begin());
+            }
+            falm_solver *sv = nullptr;
+            if (falmErrCheckMacro(falm_solver_create(&d, &c, &sv)) != FALM_OK) return;
+            solvers.push_back(Entry{c, d, sv, nullptr});
+            e = &solvers.back();
         }
+        solver = e->solver;
         Stream s = stream ? stream[0] : nullptr;
-        if (declared != a.dev.ptr) {
+        if (e->declared != a.dev.ptr) {
             if (falmErrCheckMacro(falm_solver_set_matrix(solver, a.dev.ptr, s)) != FALM_OK) return;
-            declared = a.dev.ptr;
+            e->declared = a.dev.ptr;
         }
         falm_solve_info info{};
         falmErrCheckMacro(falm_solver_solve(solver, a.dev.ptr, x.dev.ptr, b.dev.ptr, r.dev.ptr, s, &info));

@@ -423,7 +423,12 @@ falm_status falm_alm_set_pitch(falm_alm *alm, int turbine, const double *pitch_d
  * adjoint of a stage's output, each adds the adjoint of its inputs to the
  * *bar outputs (+=, so several contributions accumulate), except where
  * noted.  FP64 device arrays in the Matrix layout; atomics make the sums
- * order-independent to rounding.
+ * order-independent to rounding (not bitwise reproducible from run to run).
+ * The *bar outputs (and falm_alm_vjp's g_pitch, g_twist) must be device
+ * memory from hipMalloc: the kernels accumulate into them with hardware FP64
+ * atomic adds, which can drop updates on host-pinned, managed or
+ * fine-grained memory.  Host and managed pointers are refused with
+ * FALM_ERR_INVALID.
  *
  * falm_pseudo_u: adjoint w.r.t. u in both roles (un = u, as the step drivers
  * call it), uu, nut and ff; Upwind3, UTOPIA and KK only (else

@@ -102,10 +102,28 @@ def test_cpp_class_api_compiles_and_links():
     against the library."""
     from falm_amd import runtime
     lib_dir = os.path.dirname(runtime.LIB_PATH)
-    src = os.path.join(ROOT, "falm_amd", "examples", "windtunnel.cpp")
-    with tempfile.TemporaryDirectory() as td:
-        exe = os.path.join(td, "wt")
-        r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I",
-                            os.path.join(ROOT, "include"), src, "-L", lib_dir, "-lfalm_amd", "-o", exe],
-                           capture_output=True, text=True)
-        assert r.returncode == 0, r.stderr
+    for src in (os.path.join(ROOT, "falm_amd", "examples", "windtunnel.cpp"),
+                os.path.join(ROOT, "tests", "native", "eq_config.cpp")):
+        with tempfile.TemporaryDirectory() as td:
+            exe = os.path.join(td, "wt")
+            r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I",
+                                os.path.join(ROOT, "include"), src, "-L", lib_dir, "-lfalm_amd", "-o", exe],
+                               capture_output=True, text=True)
+            assert r.returncode == 0, r.stderr
+
+
+def test_oversized_domains_are_refused_before_any_device_call():
+    """The k-marching kernels address a field column with 32-bit byte
+    offsets: a double column of 2^29 cells or more is refused with
+    FALM_ERR_UNSUPPORTED by the solver and by the step driver, before either
+    touches the device."""
+    from falm_amd import cases, runtime
+    L = runtime.lib()
+    dom = abi.Domain.make((1028, 1028, 520), gc=2)  # 5.5e8 cells: 4.4 GB of double
+    h = C.c_void_p()
+    cfg = abi.solver_config()
+    assert L.falm_solver_create(C.byref(dom), C.byref(cfg), C.byref(h)) == abi.FALM_ERR_UNSUPPORTED
+    assert not h.value
+    setup = cases.windtunnel(n=(1024, 1024, 516), with_rotor=False)
+    assert L.falm_case_create(C.byref(setup.struct), C.byref(h)) == abi.FALM_ERR_UNSUPPORTED
+    assert b"4 GiB" in L.falm_last_error()

@@ -279,3 +279,21 @@ def test_alm_vjp(gpu):
     dt_[3] = e
     fd = (L(u, twist=dt_) - L(u, twist=-dt_)) / (2 * e)
     assert abs(fd - g_t[3]) <= 1e-5 * abs(fd) + 1e-12
+
+
+def test_vjp_outputs_must_be_device_memory(gpu):
+    """The VJP kernels accumulate with hardware FP64 atomics: an output in
+    host-pinned memory is refused with FALM_ERR_INVALID (and nothing is
+    written); the same call with device memory succeeds."""
+    import torch
+    dom = domain(N)
+    rng, F = _fields(dom, 9)
+    kx, ja = to_dev(F["kx"]), to_dev(F["ja"])
+    wd = to_dev(rng.standard_normal(3 * dom.ncell))
+    pinned = torch.zeros(3 * dom.ncell, dtype=torch.float64, pin_memory=True)
+    st = gpu.lib().falm_u_to_uu_vjp(C.byref(dom), *_ptrs(gpu, kx, ja, wd), C.c_void_p(pinned.data_ptr()), None)
+    assert st == abi.FALM_ERR_INVALID
+    assert b"device memory" in gpu.lib().falm_last_error()
+    assert float(pinned.abs().max()) == 0.0
+    ub = to_dev(np.zeros(3 * dom.ncell))
+    gpu.call("falm_u_to_uu_vjp", C.byref(dom), *_ptrs(gpu, kx, ja, wd, ub), None)
