@@ -367,8 +367,8 @@ __global__ __launch_bounds__(KM_NT) void k_pseudo_u_km(KmTiling kt, Geom g, T Re
         }
         // plane k
         const T UE = cuu[0], UW = cuu[1], VN = cuu[2], VS = cuu[3], WT = cuu[4], WB = wb;
-        const double hzc = met.h[2][k], hzt = met.h[2][k + 1], hzb = met.h[2][k - 1];
-        const double rzc = met.rh[2][k], rzt = met.rh[2][k + 1], rzb = met.rh[2][k - 1];
+        const double hzc = ld_uniform(met.h[2], k), hzt = ld_uniform(met.h[2], k + 1), hzb = ld_uniform(met.h[2], k - 1);
+        const double rzc = ld_uniform(met.rh[2], k), rzt = ld_uniform(met.rh[2], k + 1), rzb = ld_uniform(met.rh[2], k - 1);
         const double volc = hxc * hyc * hzc;
         const T gxc = (T)(volc * rxc * rxc), gxe = (T)(hxe * hyc * hzc * rxe * rxe), gxw = (T)(hxw * hyc * hzc * rxw * rxw);
         const T gyc = (T)(volc * ryc * ryc), gyn = (T)(hxc * hyn * hzc * ryn * ryn), gys = (T)(hxc * hys * hzc * rys * rys);
@@ -611,7 +611,8 @@ __global__ __launch_bounds__(KM_NT) void k_pseudo_u3_km(KmTiling kt, Geom g, T R
                   phv[PD - 1]);
         }
         // plane k
-        const T rz = tb.t[SEP_R][2][k], czt = tb.t[SEP_CP][2][k], czb = tb.t[SEP_CM][2][k];
+        const T rz = ld_uniform(tb.t[SEP_R][2], k), czt = ld_uniform(tb.t[SEP_CP][2], k),
+                czb = ld_uniform(tb.t[SEP_CM][2], k);
         const T rj = rxy * rz;  // (1/2)(1/24)/J
         const T aux = fabs(qu[0][1] * arx), auy = fabs(qu[1][1] * ary), auz = fabs(qu[2][1] * (ALPHA * rz));
         const T nc = qn[1];
@@ -824,8 +825,8 @@ __global__ __launch_bounds__(K7_NT) void k_project_km(K7Tiling kt, Geom g, P dt,
     for (int k = Q.k0; k < Q.k1; k++, cb += szt) {
         const bool more = k + 1 < Q.k1;
         if (more) pp.prefetch(Q, k, sz);
-        const P hz = tb.t[SEP_H][2][k];
-        const P cdz = dt * P(0.5) * tb.t[SEP_R][2][k], fz = ffz * tb.t[SEP_RF][2][k];
+        const P hz = ld_uniform(tb.t[SEP_H][2], k);
+        const P cdz = dt * P(0.5) * ld_uniform(tb.t[SEP_R][2], k), fz = ffz * ld_uniform(tb.t[SEP_RF][2], k);
         if (Q.act) {
             const P pcc = pp.c();
             const T a0 = ld_at((const T *)u.c[0], cb), a1 = ld_at((const T *)u.c[1], cb),
@@ -919,7 +920,7 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_km(K7Tiling kt, Geom g, Vec3<
         const T ux = ld_at(u.c[0], cb), uxe = ld_at(u.c[0], cb + (unsigned)sizeof(T));
         const T vy = ld_at(u.c[1], cb), vyn = ld_at(u.c[1], cb + syb);
         const T wzt = ld_at(u.c[2], cb + szb);
-        const T hz = tb.t[SEP_H][2][k];
+        const T hz = ld_uniform(tb.t[SEP_H][2], k);
         st_at(uu.c[0], cb, (hy2 * hz) * (ux + uxe));
         st_at(uu.c[1], cb, (hx2 * hz) * (vy + vyn));
         st_at(uu.c[2], cb, hxy2 * (wz + wzt));
@@ -967,7 +968,7 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_rhs_km(K7Tiling kt, Geom g, V
         if (gn) vyn = ld_at(gu.c[1], cb + syb);
         if (gs) vys = ld_at(gu.c[1], cb - syb);
         const T wzt = k + 1 >= ktop ? ld_at(gu.c[2], cb + szb) : ld_at(u.c[2], cb + szb);
-        const T hz = tb.t[SEP_H][2][k];
+        const T hz = ld_uniform(tb.t[SEP_H][2], k);
         const T fxe = (hy2 * hz) * (ux + uxe), fxw = (hy2 * hz) * (uxw + ux);
         const T fyn = (hx2 * hz) * (vy + vyn), fys = (hx2 * hz) * (vys + vy);
         const T fzt = hxy2 * (wz + wzt);
@@ -975,7 +976,7 @@ __global__ __launch_bounds__(K7_NT) void k_u_to_uu_rhs_km(K7Tiling kt, Geom g, V
         st_at(uu.c[1], cb, fyn);
         st_at(uu.c[2], cb, fzt);
         double d = (double)fxe - (double)fxw + (double)fyn - (double)fys + (double)fzt - (double)fzb;
-        d = d * (double)(rxy * tb.t[SEP_R][2][k]);
+        d = d * (double)(rxy * ld_uniform(tb.t[SEP_R][2], k));
         st_at(rhs, rb, d * scale);
         fzb = fzt;
         wz = wzt;
@@ -1028,7 +1029,7 @@ __global__ __launch_bounds__(K7_NT) void k_smagorinsky_km(K7Tiling kt, Geom g, T
         if (more)
 #pragma unroll
             for (int d = 0; d < 3; d++) pu[d].prefetch(Q, k, sz);
-        const T hrz = T(0.5) * tb.t[SEP_R][2][k], h23z = tb.t[SEP_H23][2][k];
+        const T hrz = T(0.5) * ld_uniform(tb.t[SEP_R][2], k), h23z = ld_uniform(tb.t[SEP_H23][2], k);
         if (Q.act) {
             T gr[3][3];
 #pragma unroll
@@ -1094,7 +1095,7 @@ __global__ __launch_bounds__(K7_NT) void k_div_norm_km(K7Tiling kt, Geom g, Vec3
             const T UE = ld_at(uu.c[0], cb), UW = ld_at(uu.c[0], cb - (unsigned)sizeof(T));
             const T VN = ld_at(uu.c[1], cb), VS = ld_at(uu.c[1], cb - syb);
             const T WT = ld_at(uu.c[2], cb);
-            const T d = (UE - UW + VN - VS + WT - wb) * (rxy * tb.t[SEP_R][2][k]);
+            const T d = (UE - UW + VN - VS + WT - wb) * (rxy * ld_uniform(tb.t[SEP_R][2], k));
             st_at(div, cb, d);
             v[0] += (double)d * (double)d;
             wb = WT;
@@ -1147,7 +1148,7 @@ __global__ __launch_bounds__(K7_NT) void k_sgs_div_km(K7Tiling kt, Geom g, T Cs,
             ns = ld_at(uu.c[1], c1 - syb);
             nt = ld_at(uu.c[2], c1);
         }
-        const T hrz = T(0.5) * tb.t[SEP_R][2][k], h23z = tb.t[SEP_H23][2][k];
+        const T hrz = T(0.5) * ld_uniform(tb.t[SEP_R][2], k), h23z = ld_uniform(tb.t[SEP_H23][2], k);
         if (Q.act) {
             T gr[3][3];
 #pragma unroll
@@ -1163,7 +1164,7 @@ __global__ __launch_bounds__(K7_NT) void k_sgs_div_km(K7Tiling kt, Geom g, T Cs,
             T d4 = (duy + dvx) * (duy + dvx), d5 = (dvz + dwy) * (dvz + dwy), d6 = (duz + dwx) * (duz + dwx);
             const T Du = sqrt(d1 + d2 + d3 + d4 + d5 + d6);
             st_at(nut, cb, (c2 * h23z) * Du);
-            const T dv = (fe - fw + fn - fs + ft - wb) * (rxy * tb.t[SEP_R][2][k]);
+            const T dv = (fe - fw + fn - fs + ft - wb) * (rxy * ld_uniform(tb.t[SEP_R][2], k));
             st_at(div, cb, dv);
             v[0] += (double)dv * (double)dv;
         }

@@ -198,6 +198,18 @@ __device__ __forceinline__ void st_at(T *p, unsigned off, T v) {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + off) = v;
 }
 
+// Read of a wave-uniform element of a table the kernel never writes (the
+// per-axis coefficient tables), through the constant address space: the
+// compiler then issues a scalar load, counted by lgkmcnt.  A plain load of
+// such a value is a vector load when the kernel also stores to global
+// memory, and waiting for it (vmcnt) also waits for every vector load issued
+// before it: in a k-marching loop, for the prefetches of the next planes.
+// p and i must be the same in every lane of the wave.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p, int i) {
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
 // the ld_at/st_at kernels address one field column of ncell elements
 inline falm_status check_byte_offsets(size_t ncell, size_t elem) {
     if (ncell * elem >= (size_t(1) << 32)) {

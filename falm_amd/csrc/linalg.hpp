@@ -100,7 +100,8 @@ struct OpSepCol {
     }
     __device__ inline void get_at(unsigned ob, int, int, int k, TA &c0, TA &w_, TA &e_, TA &s_, TA &n_, TA &b_,
                                   TA &t_) const {
-        c0 = cls ? pick(k) : ld_at(ac, ob); w_ = w; e_ = e; s_ = s; n_ = n; b_ = ab[k]; t_ = at[k];
+        c0 = cls ? pick(k) : ld_at(ac, ob); w_ = w; e_ = e; s_ = s; n_ = n; b_ = ld_uniform(ab, k);
+        t_ = ld_uniform(at, k);
     }
     __device__ inline TA diag_at(unsigned ob) const { return ld_at(ac, ob); }
     // the diagonal of the column's cell in plane k (byte offset ob)
@@ -108,7 +109,10 @@ struct OpSepCol {
     struct Cell {
         TA c0, b, t;
     };
-    __device__ inline Cell load(unsigned ob, int k) const { return Cell{diag_k(ob, k), ab[k], at[k]}; }
+    // k is the plane of a k-marching kernel: the same in every lane
+    __device__ inline Cell load(unsigned ob, int k) const {
+        return Cell{diag_k(ob, k), ld_uniform(ab, k), ld_uniform(at, k)};
+    }
     // the reference's summation order (src/dev/MVDevCall.cu:39)
     template <class R>
     __device__ inline R apply(const Cell &q, R xc, R xe, R xw, R xn, R xs, R xt, R xb) const {
