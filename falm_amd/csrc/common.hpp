@@ -335,6 +335,9 @@ struct ReduceScratch {
     size_t max_blocks = 0;
 };
 falm_status reduce_scratch(size_t nblocks, ReduceScratch **out);
+// device address of falm_dot_noalloc's workspace on the current device,
+// taken on first use (falm_init calls it for its device)
+falm_status noalloc_setup(double **out);
 
 // ------------------------------------------------------------ timing
 struct TimedLaunch {

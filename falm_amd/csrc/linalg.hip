@@ -3,6 +3,8 @@
 // reduction), max/min reductions, scale/add/axpby, red-black SOR and Jacobi
 // sweeps.  Stencil coefficients are the 7 columns [c,w,e,s,n,b,t] of the
 // reference's Poisson matrix (src/dev/MVDevCall.cu:25-31).
+#include <mutex>
+
 #include "linalg.hpp"
 
 namespace falm {
@@ -218,8 +220,9 @@ falm_status launch_dot(const falm_domain *dom, int prec, const void *a, const vo
                        double *dev_out, double *partials, hipStream_t s) {
     Geom g = make_geom(dom);
     Tiling tl = make_tiling(interior_box(g));
-    if (tl.tiles == 0) {
-        FALM_HIP(hipMemsetAsync(dev_out, 0, sizeof(double), s));
+    if (tl.tiles == 0) {  // an empty interior sums to zero (a kernel, not a memset node under capture)
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, 0, dev_out);
+        FALM_CHECK_LAUNCH("k_fin_sum");
         return FALM_OK;
     }
     size_t n = ncell_of(dom);
@@ -285,6 +288,40 @@ falm_status host_minmax(const falm_domain *dom, int prec, int op, const void *a,
     return FALM_OK;
 }
 
+// ------------------------------------------------- allocation-free dot
+// Workspace of falm_dot_noalloc: the first pass's per-workgroup partial sums.
+// The reduction grid never exceeds RED_BLOCKS workgroups, so this fixed array
+// (32 KiB per device, held until the process ends) serves every domain.
+constexpr int NOALLOC_PARTIALS = RED_BLOCKS;
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+
+// Device address of `scratch` per device.  In host code the array's name is
+// only a stand-in; kernels get this address, taken once per device with that
+// device current.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_noalloc_mx;
+
+falm_status noalloc_setup(double **out) {
+    int dev = 0;
+    FALM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        set_error("device %d beyond the %d devices the allocation-free workspace serves", dev,
+                  NOALLOC_MAX_DEVICES);
+        return FALM_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mx);
+    if (!g_noalloc_ptr[dev]) {
+        double *p = nullptr;
+        hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_status(e == hipSuccess ? hipErrorInvalidSymbol : e,
+                                                     "hipGetSymbolAddress(scratch)");
+        g_noalloc_ptr[dev] = p;
+    }
+    if (out) *out = g_noalloc_ptr[dev];
+    return FALM_OK;
+}
+
 }  // namespace falm
 
 using namespace falm;
@@ -327,6 +364,24 @@ falm_status falm_dot(const falm_domain *dom, int prec, const void *a, const void
         return FALM_ERR_INVALID;
     }
     return host_scalar(dom, prec, a, b, 1, false, out, (hipStream_t)stream);
+}
+
+falm_status falm_dot_noalloc(const falm_domain *dom, int prec, const void *a, const void *b, double *out,
+                             void *stream) {
+    FALM_TRY(check_domain(dom));
+    if (!a || !b || !out) {
+        set_error("falm_dot_noalloc: null pointer");
+        return FALM_ERR_INVALID;
+    }
+    Geom g = make_geom(dom);
+    Tiling tl = make_tiling(interior_box(g));
+    if (red_blocks(tl) > NOALLOC_PARTIALS) {  // cannot happen while the grid is capped at RED_BLOCKS
+        set_error("falm_dot_noalloc: %d partial sums exceed the workspace of %d", red_blocks(tl), NOALLOC_PARTIALS);
+        return FALM_ERR_UNSUPPORTED;
+    }
+    double *partials = nullptr;
+    FALM_TRY(noalloc_setup(&partials));
+    return launch_dot(dom, prec, a, b, 1, false, out, partials, (hipStream_t)stream);
 }
 
 falm_status falm_norm2(const falm_domain *dom, int prec, const void *a, int ncol, double *out, void *stream) {

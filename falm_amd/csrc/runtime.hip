@@ -160,7 +160,7 @@ falm_status falm_init(int device) {
         return FALM_ERR_INVALID;
     }
     FALM_HIP(hipSetDevice(device));
-    return FALM_OK;
+    return noalloc_setup(nullptr);  // falm_dot_noalloc's workspace address on this device
 }
 
 falm_status falm_malloc(void **ptr, size_t bytes) {

@@ -41,6 +41,7 @@ def _sigs(L):
         "falm_divergence": [dom, ip, vp, vp, vp, vp],
         "falm_mv": [dom, ip, vp, vp, vp, vp],
         "falm_dot": [dom, ip, vp, vp, C.POINTER(dp), vp],
+        "falm_dot_noalloc": [dom, ip, vp, vp, vp, vp],
         "falm_norm2": [dom, ip, vp, ip, C.POINTER(dp), vp],
         "falm_reduce": [dom, ip, ip, vp, ip, ip, C.POINTER(dp), vp],
         "falm_scale": [ip, vp, C.c_size_t, dp, vp],

@@ -135,6 +135,16 @@ falm_status falm_dot(const falm_domain *dom, int prec, const void *a, const void
                      double *out, void *stream);
 falm_status falm_norm2(const falm_domain *dom, int prec, const void *a, int ncol,
                        double *out, void *stream);
+/* falm_dot without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * the same result bits as falm_dot, but `out` is DEVICE memory (one double)
+ * and the call only enqueues work on `stream` (no synchronisation).  The
+ * per-workgroup partial sums live in a fixed 32 KiB device array (4096
+ * doubles) per GPU, enough for every domain falm_dot accepts; the array is
+ * shared by all calls on one GPU, so calls that overlap on one GPU race. */
+falm_status falm_dot_noalloc(const falm_domain *dom, int prec, const void *a, const void *b,
+                             double *out, void *stream);
 /* FalmMV::MatColMax/MatColMin/MatColAbsMax/MatColAbsMin/VecMax/VecMin
  * (src/MV.h:67-136; kernels src/dev/MVDevCall.cu:216-598).  op selects one of
  * FALM_RED_*.  MaxDiag == MatColAbsMax(a, 0). */
