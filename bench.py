@@ -3,16 +3,24 @@
 
 Workload: wind-tunnel channel of 512x128x128 cells (dx = D/32) with one
 actuator-line rotor (3 blades x 32 points, TSR 7), Upwind3 + Smagorinsky,
-BiCGStab with a red-black SOR preconditioner at tolerance 1e-6 on
-|r|/N.  The headline figure is MLUPS of the mixed-precision step (float
-stencils and Krylov passes, double defect correction); the FP64 step runs
-beside it on the same workload.  A step is one full time step (ALM, pseudo
-velocity, face flux, Poisson RHS, Poisson solve, projection, boundary fill,
-SGS, divergence monitor) with all fields resident in HBM.
+BiCGStab with a red-black SOR preconditioner.  The headline figure is MLUPS
+of the mixed-precision step (float stencils and Krylov passes, double defect
+correction) with the Poisson solve stopped at |r|/N <= 1e-8 of the scaled
+system, the tolerance at which the divergence monitor sqrt(sum div^2)/N
+(app/windtunnel/main.cpp:185) stays below 1e-6 in both precisions; the FP64
+step runs beside it on the same workload.  A second pair of legs stops the
+solve at |r|/N <= 1e-6, the reference's criterion (src/FalmEq.cpp:330),
+where one BiCGStab iteration per step suffices and the monitor reads ~1e-5.
+A step is one full time step (ALM, pseudo velocity, face flux, Poisson RHS,
+Poisson solve, projection, boundary fill, SGS, divergence monitor) with all
+fields resident in HBM.
 
 The accuracy leg runs both precisions for 1000 steps from the same start
 and reports the mixed step's velocity and pressure error against the FP64
-step.  It uses CFL 0.1: the reference's pseudo-velocity step is explicit
+step, beside the FP64 step's own difference from an FP64 run at a ten times
+tighter Poisson tolerance (the error the tolerance itself allows; the
+channel's pressure has a smooth near-null mode that |r|/N barely
+constrains).  It uses CFL 0.1: the reference's pseudo-velocity step is explicit
 Euler with Upwind3 (ua = un + dt(-adv + vis + ff), src/FalmCFD.cpp:5-67),
 which amplifies resolved modes at CFL 0.3 (|u|max grows about 4x by step
 400 and overflows near step 450 at 256x96x96, in the GPU path and in the
@@ -24,7 +32,8 @@ steps carry no event overhead), with algorithmic bytes per launch from the
 operand-list model (every array a kernel touches counted once per cell).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--only mixed|fp64]
-                    [--no-cpu] [--calibrate] [--accuracy-steps N]
+                    [--tol T] [--solver-tol-leg T2] [--no-cpu] [--calibrate]
+                    [--accuracy-steps N]
     python bench.py --adjoint     (rotor-power gradient report, 256x96x96)
 """
 import argparse
@@ -54,17 +63,23 @@ def families():
         "sor": (abi.TIME_SOR, "k_bicg_vsor (Krylov vector + one-pass RB-SOR preconditioner)"),
         "mv": (abi.TIME_MV, "k_bicg_mvdot (A p + dot)"),
         "solver_update": (abi.TIME_SOLVER_UPDATE, "k_bicg_update (x, r update + dots)"),
-        "alm": (abi.TIME_ALM, "ALM step (points, box, factors, projection, totals)"),
+        "alm": (abi.TIME_ALM, "ALM stage (points, box, factors, projection, totals): a latency chain, "
+                              "bytes = point tables + force-box cells written"),
     }
 
 
-def run_gpu(precision, steps, warmup, n, fam_steps):
+# stages whose time is launch latency rather than streaming: reported in us,
+# with no bandwidth fraction
+LATENCY_FAMILIES = ("alm",)
+
+
+def run_gpu(precision, steps, warmup, n, fam_steps, tol):
     """Every falm_case_step call returns after synchronising the case's
     stream, so the wall clock brackets completed device work."""
     from falm_amd import _abi as abi
     from falm_amd import cases, runtime
 
-    setup = cases.windtunnel(n=n, precision=precision)
+    setup = cases.windtunnel(n=n, precision=precision, tol=tol)
     c = runtime.Case(setup)
     for _ in range(warmup):
         c.step(1)
@@ -90,9 +105,12 @@ def run_gpu(precision, steps, warmup, n, fam_steps):
         kern[name] = {"kernel": desc, "launches_per_step": launches / fam_steps,
                       "avg_launch_us": 1e3 * kms / launches, "bytes_per_launch": kbytes / launches,
                       "gbs": gbs, "frac": gbs / PEAK_HBM_GBS}
+        if name in LATENCY_FAMILIES:
+            kern[name].update(bound="latency", frac=None)
     c.close()
     cells = setup.n_interior
     return {
+        "tol": tol,
         "ms_per_step": 1e3 * dt / steps,
         "steps_per_s": steps / dt,
         "mlups": cells * steps / dt / 1e6,
@@ -139,13 +157,26 @@ def cpu_baseline(n, steps=2, warmup=1):
         "value": setup.n_interior * steps / dt / 1e6,
         "unit": "MLUPS",
         "cores": O.num_threads(),
+        "cpu_model": cpu_model(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
         "kind": "port",
         "sample": f"{steps} steps after {warmup} warm-up step(s) of the same 512x128x128 + rotor workload, FP64, "
                   f"{dt / steps:.2f} s/step",
     }
 
 
-def run_accuracy(n, steps, cfl):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def run_accuracy(n, steps, cfl, tol):
     """Mixed against FP64 on the GPU after `steps` steps of the same
     workload from the same initial state: relative L2 errors of velocity
     and pressure over the interior, and each run's divergence monitor."""
@@ -155,8 +186,9 @@ def run_accuracy(n, steps, cfl):
     from falm_amd import cases, runtime
 
     out = {}
-    for key, prec in (("fp64", abi.CASE_F64), ("mixed", abi.CASE_MIXED)):
-        setup = cases.windtunnel(n=n, precision=prec, cfl=cfl)
+    for key, prec, t in (("fp64", abi.CASE_F64, tol), ("mixed", abi.CASE_MIXED, tol),
+                         ("fp64_tight", abi.CASE_F64, 0.1 * tol)):
+        setup = cases.windtunnel(n=n, precision=prec, cfl=cfl, tol=t)
         c = runtime.Case(setup)
         t0 = time.perf_counter()
         info = c.step(steps)
@@ -173,15 +205,63 @@ def run_accuracy(n, steps, cfl):
     def rel(a, b, m):
         return float(np.linalg.norm(a[m] - b[m]) / np.linalg.norm(b[m]))
 
-    f, x = out["fp64"], out["mixed"]
+    f, x, ft = out["fp64"], out["mixed"], out["fp64_tight"]
     return {
-        "workload": f"windtunnel {n[0]}x{n[1]}x{n[2]} + rotor, CFL {cfl}, {steps} steps from the same start",
+        "workload": f"windtunnel {n[0]}x{n[1]}x{n[2]} + rotor, CFL {cfl}, Poisson tol {tol:g}, {steps} steps "
+                    "from the same start",
         "velocity_rel_l2": rel(x["u"], f["u"], m3),
         "pressure_rel_l2": rel(x["p"], f["p"], m1),
+        # the solver tolerance's own effect: FP64 at tol against FP64 at tol/10
+        "fp64_tol_vs_tol_over_10": {"velocity_rel_l2": rel(f["u"], ft["u"], m3),
+                                    "pressure_rel_l2": rel(f["p"], ft["p"], m1)},
         "div_fp64": f["div"], "div_mixed": x["div"],
         "u_max_fp64": float(np.abs(f["u"][m3]).max()), "u_max_mixed": float(np.abs(x["u"][m3]).max()),
         "status": [f["status"], x["status"]],
         "wall_s": [f["s"], x["s"]],
+    }
+
+
+def run_lid3d(n, steps, tol):
+    """The 3-D lid-driven cavity LES (128^3, Re 1e4, Cs 0.1, dt 5e-4,
+    BiCGStab + RB-SOR at |r|/N <= tol, app/lid3d/main.cpp:122-175) in FP64 and
+    in mixed precision for `steps` steps from rest: the mixed run's velocity,
+    pressure and divergence against the FP64 run, and each run's time."""
+    import numpy as np
+
+    from falm_amd import _abi as abi
+    from falm_amd import cases, runtime
+
+    out = {}
+    for key, prec in (("fp64", abi.CASE_F64), ("mixed", abi.CASE_MIXED)):
+        setup = cases.lid3d(n=n, precision=prec, tol=tol)
+        c = runtime.Case(setup)
+        t0 = time.perf_counter()
+        for q in range(0, steps, 100):
+            info = c.step(min(100, steps - q))
+        dt = time.perf_counter() - t0
+        out[key] = {"u": c.get(abi.FIELD_U), "p": c.get(abi.FIELD_P), "div_field": c.get(abi.FIELD_DIV),
+                    "div": info.div, "solver_err": info.solver_err, "solver_it_last": info.solver_it,
+                    "status": info.status, "s": dt}
+        c.close()
+    nx, ny, nz = setup.shape
+    mask = np.zeros((nz, ny, nx), dtype=bool)
+    mask[2:-2, 2:-2, 2:-2] = True
+    m1 = mask.ravel()
+    m3 = np.tile(m1, 3)
+    f, x = out["fp64"], out["mixed"]
+    # Neumann walls: the pressure is defined up to a constant
+    pf, px = f["p"][m1] - f["p"][m1].mean(), x["p"][m1] - x["p"][m1].mean()
+    return {
+        "workload": f"lid-driven cavity LES {n}^3, Re 1e4, Cs 0.1, dt 5e-4, BiCGStab+RB-SOR(2 passes, w=1.5) "
+                    f"tol {tol:g} on |r|/N, {steps} steps from rest",
+        "velocity_rel_l2": float(np.linalg.norm(x["u"][m3] - f["u"][m3]) / np.linalg.norm(f["u"][m3])),
+        "pressure_rel_l2_mean_free": float(np.linalg.norm(px - pf) / np.linalg.norm(pf)),
+        "divergence_field_max_abs_diff": float(np.abs(x["div_field"][m1] - f["div_field"][m1]).max()),
+        "div_fp64": f["div"], "div_mixed": x["div"],
+        "status": [f["status"], x["status"]],
+        "solver_it_last_step": [f["solver_it_last"], x["solver_it_last"]],
+        "wall_s": [f["s"], x["s"]],
+        "ms_per_step": [1e3 * f["s"] / steps, 1e3 * x["s"] / steps],
     }
 
 
@@ -296,9 +376,16 @@ def main():
     ap.add_argument("--adj-dirs", type=int, default=4)
     ap.add_argument("--adj-tol", type=float, default=1e-12)
     ap.add_argument("--adj-cfl", type=float, default=0.1)
+    ap.add_argument("--tol", type=float, default=1e-8,
+                    help="Poisson tolerance on |r|/N of the headline legs (1e-8 keeps sqrt(sum div^2)/N < 1e-6)")
+    ap.add_argument("--solver-tol-leg", type=float, default=1e-6,
+                    help="second pair of legs stopped at this |r|/N (0: skip)")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="mixed-vs-FP64 error after this many steps (0: skip)")
     ap.add_argument("--accuracy-cfl", type=float, default=0.1)
+    ap.add_argument("--lid3d-steps", type=int, default=1000,
+                    help="cavity LES mixed-vs-FP64 leg: steps (0: skip)")
+    ap.add_argument("--lid3d-n", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -319,21 +406,45 @@ def main():
         return
     n = tuple(args.n)
     calib = calibrate_copy() if args.calibrate else None
-    r64 = rmx = u64 = umx = None
-    if args.only in ("both", "fp64"):
-        r64, u64, setup = run_gpu(abi.CASE_F64, args.steps, args.warmup, n, args.fam_steps)
-    if args.only in ("both", "mixed"):
-        rmx, umx, setup = run_gpu(abi.CASE_MIXED, args.steps, args.warmup, n, args.fam_steps)
-    head = rmx if rmx is not None else r64
-    rel_l2 = None
-    if r64 is not None and rmx is not None:
-        nx, ny, nz = setup.shape
-        mask = np.zeros((nz, ny, nx), dtype=bool)
-        mask[2:-2, 2:-2, 2:-2] = True
-        m3 = np.tile(mask.ravel(), 3)
-        rel_l2 = float(np.linalg.norm(umx[m3] - u64[m3]) / np.linalg.norm(u64[m3]))
 
-    dom = head["kernels"].get(DOMINANT)
+    def leg(tol):
+        """Both precisions at one Poisson tolerance; velocity difference of
+        the two after the timed and timing steps."""
+        r64 = rmx = u64 = umx = setup = None
+        if args.only in ("both", "fp64"):
+            r64, u64, setup = run_gpu(abi.CASE_F64, args.steps, args.warmup, n, args.fam_steps, tol)
+        if args.only in ("both", "mixed"):
+            rmx, umx, setup = run_gpu(abi.CASE_MIXED, args.steps, args.warmup, n, args.fam_steps, tol)
+        rel_l2 = None
+        if r64 is not None and rmx is not None:
+            nx, ny, nz = setup.shape
+            mask = np.zeros((nz, ny, nx), dtype=bool)
+            mask[2:-2, 2:-2, 2:-2] = True
+            m3 = np.tile(mask.ravel(), 3)
+            rel_l2 = float(np.linalg.norm(umx[m3] - u64[m3]) / np.linalg.norm(u64[m3]))
+        return r64, rmx, rel_l2, setup
+
+    # headline: the tolerance at which the divergence monitor
+    # sqrt(sum div^2)/N stays below 1e-6 in both precisions
+    r64, rmx, rel_l2, setup = leg(args.tol)
+    head = rmx if rmx is not None else r64
+    divs = [r["div"] for r in (r64, rmx) if r is not None]
+    tol_leg = None
+    if args.solver_tol_leg > 0:
+        s64, smx, srel, _ = leg(args.solver_tol_leg)
+        tol_leg = {
+            "what": f"same workload with the solver stopped at |r|/N <= {args.solver_tol_leg:g} (the reference's "
+                    "criterion, src/FalmEq.cpp:330) instead of at the divergence condition",
+            "mixed": smx, "fp64": s64,
+            "speedup_mixed_vs_fp64": (s64["ms_per_step"] / smx["ms_per_step"]) if (s64 and smx) else None,
+            "velocity_rel_l2_mixed_vs_fp64": srel,
+        }
+
+    # dominant kernel: the streaming family with the most time per step
+    stream_fams = {k_: v for k_, v in head["kernels"].items() if k_ not in LATENCY_FAMILIES}
+    dom_name = max(stream_fams, key=lambda k_: stream_fams[k_]["avg_launch_us"] * stream_fams[k_]["launches_per_step"]) \
+        if stream_fams else None
+    dom = stream_fams.get(dom_name)
     roofline = None
     if dom:
         roofline = {
@@ -347,7 +458,7 @@ def main():
             "avg_launch_us": dom["avg_launch_us"],
             "algorithmic_bytes_per_launch": dom["bytes_per_launch"],
         }
-        tr_path = os.path.join(ROOT, "profiles", f"traffic_{DOMINANT}.json")
+        tr_path = os.path.join(ROOT, "profiles", f"traffic_{dom_name}.json")
         if os.path.exists(tr_path):
             with open(tr_path) as f:
                 tr = json.load(f)
@@ -355,11 +466,9 @@ def main():
             if key in tr:
                 roofline["traffic"] = tr[key].get("hbm_bytes_per_launch")
 
-    acc = run_accuracy(n, args.accuracy_steps, args.accuracy_cfl) if args.accuracy_steps > 0 else None
+    acc = run_accuracy(n, args.accuracy_steps, args.accuracy_cfl, args.tol) if args.accuracy_steps > 0 else None
+    lid = run_lid3d(args.lid3d_n, args.lid3d_steps, 1e-6) if args.lid3d_steps > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(n, steps=args.cpu_steps)
-
-    def strip(r):
-        return None if r is None else {k_: v for k_, v in r.items()}
 
     out = {
         "metric": METRIC,
@@ -377,17 +486,21 @@ def main():
         "config": {
             "workload": f"windtunnel {n[0]}x{n[1]}x{n[2]} cells + 1 ALM rotor (3x32 points), dx=D/32, "
                         f"Re 2.5e5, Cs 0.1, dt 0.3dx/U, BiCGStab+RB-SOR({setup.struct.solver.pc_maxit} pass, "
-                        f"w={setup.struct.solver.pc_relax:g}) tol 1e-6",
+                        f"w={setup.struct.solver.pc_relax:g}) tol {args.tol:g} on |r|/N",
             "cells": setup.n_interior,
             "precision": "mixed" if rmx is not None else "fp64",
         },
         "steps_per_s": head["steps_per_s"],
-        "mixed": strip(rmx),
-        "fp64": strip(r64),
+        "div_condition": {"bound": 1e-6, "monitor": "sqrt(sum div^2)/N after the last timed step",
+                          "met": bool(divs) and max(divs) < 1e-6, "poisson_tol": args.tol},
+        "mixed": rmx,
+        "fp64": r64,
         "speedup_mixed_vs_fp64": (r64["ms_per_step"] / rmx["ms_per_step"]) if (r64 and rmx) else None,
         "velocity_rel_l2_mixed_vs_fp64": rel_l2,
+        "solver_tol_leg": tol_leg,
         "roofline": roofline,
         "accuracy": acc,
+        "lid3d_mixed_vs_fp64": lid,
         "cpu_baseline": cpu,
     }
     if calib:

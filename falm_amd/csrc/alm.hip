@@ -867,7 +867,24 @@ falm_status alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, h
     }
     if (!fused) hipLaunchKernelGGL(k_alm_totals, dim3(A->nt), dim3(256), 0, s, A->d);
     FALM_CHECK_LAUNCH("k_alm_totals");
-    tm.done((double)tl.box.size() * 3.0 * prec_bytes(prec));
+    // bytes of the point tables: per point the 8 velocity samples, the two
+    // polar columns, its state; the axis factors written once and read once
+    const double pb = prec_bytes(prec);
+    tm.done((double)A->nap * (24.0 * pb + 16.0 * A->d.nat + 64.0) + 16.0 * (double)nfac);
+    if (tm.on) {
+        // the force cells written: the interior on a full write, otherwise
+        // the union box the point kernel chose (read back only when timed)
+        int box[18];
+        FALM_HIP(hipMemcpyAsync(box, A->d.box, sizeof(box), hipMemcpyDeviceToHost, s));
+        FALM_HIP(hipStreamSynchronize(s));
+        double cells = (double)tl.box.size();
+        if (union_only) {
+            const int *u6 = box + 6;
+            cells = 1.0;
+            for (int m = 0; m < 3; m++) cells *= u6[2 * m + 1] >= u6[2 * m] ? (double)(u6[2 * m + 1] - u6[2 * m] + 1) : 0.0;
+        }
+        tm.add_bytes(cells * 3.0 * pb);
+    }
     return FALM_OK;
 }
 

@@ -346,6 +346,9 @@ struct TimedLaunch {
     bool on;
     TimedLaunch(int fam, hipStream_t s);
     void done(double bytes);
+    // bytes known only after the launch (e.g. a box the kernels chose);
+    // counts toward the family's total when the launch was timed
+    void add_bytes(double bytes);
 };
 
 }  // namespace falm

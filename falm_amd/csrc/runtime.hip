@@ -124,6 +124,10 @@ void TimedLaunch::done(double bytes) {
     }
 }
 
+void TimedLaunch::add_bytes(double bytes) {
+    if (on) g_tm.bytes += bytes;
+}
+
 }  // namespace falm
 
 using namespace falm;

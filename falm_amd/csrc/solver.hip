@@ -62,6 +62,16 @@ struct DevState {
 
 constexpr int NT = 256;
 
+// A mixed-precision float pass stops once its recursive residual is below
+// this fraction of the tolerance (or below inner_rtol times the pass's
+// starting residual); the true residual in double then decides, and a
+// further pass runs only if it is still above the tolerance.  At the full
+// tolerance the float pass takes the FP64 solve's iteration count (7 per
+// step at 512 x 128 x 128, tol 1e-8); a margin of 0.5 cost 12.3 iterations.
+#ifndef FALM_MIXED_TOL_MARGIN
+#define FALM_MIXED_TOL_MARGIN 1.0
+#endif
+
 template <class T>
 __device__ inline T pc_first(int pc_type, T omega_pc, T b, T ac, bool black) {
     if (pc_type == FALM_LS_SOR) {
@@ -628,7 +638,8 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_init(DevState *st, const double 
         }
         if (st->mixed) {
             const double target = st->inner_rtol * err;
-            st->tol_inner = target > 0.5 * st->tol ? target : 0.5 * st->tol;
+            const double floor = FALM_MIXED_TOL_MARGIN * st->tol;
+            st->tol_inner = target > floor ? target : floor;
             const int left = st->maxit - st->total;
             st->maxit_pass = left < st->inner_maxit ? left : st->inner_maxit;
         }
@@ -1110,7 +1121,7 @@ static T *pc_rest(falm_solver *S, const OP &a, T *x, const T *b, T *spare, hipSt
 
 // one BiCGStab iteration (src/FalmEq.cpp:345-407)
 template <class T, class OP>
-static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, hipStream_t s) {
+static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, hipStream_t s, bool first) {
     const falm_solver_config &c = S->cfg;
     const Tiling &tl = S->tl;
     const Geom &g = S->g;
@@ -1133,9 +1144,9 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         TimedLaunch tm(FALM_TIME_SOR, s);
         hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 0>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)pin,
                            (const T *)q, pout, pp, (const DevState *)S->st, wpc);
-        // r (and the diagonal) read, p and p^ written (a first iteration;
-        // later ones also read p and q)
-        tm.done((double)tl.box.size() * (3.0 * sizeof(T) + diag_bytes<OP>(S)));
+        // r (and the diagonal) read, p and p^ written; after a solve's first
+        // iteration also p and q read
+        tm.done((double)tl.box.size() * ((first ? 3.0 : 5.0) * sizeof(T) + diag_bytes<OP>(S)));
         FALM_CHECK_LAUNCH("k_bicg_vsor_km");
     } else {
         hipLaunchKernelGGL((k_bicg_p<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, p, (const T *)q, pp,
@@ -1168,8 +1179,10 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         pcs = pc_rest<T, OP>(S, a, ss, sv, pcp == pp ? ss : pp, s, pst);
         FALM_TRY(pst);
     }
+    TimedLaunch tm3(FALM_TIME_MV, s);
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 2>), FALM_KM(S), s, S->kt, g, a, (const T *)pcs, t, (const T *)sv,
                        (const DevState *)S->st, S->partials);
+    tm3.done((double)tl.box.size() * (3.0 * sizeof(T) + diag_bytes<OP>(S)));
     hipLaunchKernelGGL(k_fin_omega, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot2");
     TimedLaunch tm2(FALM_TIME_SOLVER_UPDATE, s);
@@ -1214,7 +1227,7 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
     int batch = first_batch(S), enqueued = 0;
     while (true) {
         for (int k = 0; k < batch && enqueued < c.maxit; k++, enqueued++)
-            FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s)));
+            FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s, enqueued == 0)));
         FALM_TRY(read_state(S, s));
         if (S->hst->done || enqueued >= c.maxit) break;
         batch = check_period(S);
@@ -1247,7 +1260,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
     // iteration or ends/starts a pass; the device ends the solve at maxit
     int batch = first_batch(S), polls = 0;
     while (true) {
-        for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s)));
+        for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, polls == 0 && k == 0)));
         FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
         FALM_TRY(read_state(S, s));
         polls++;
