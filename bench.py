@@ -35,6 +35,7 @@ operand-list model (every array a kernel touches counted once per cell).
                     [--tol T] [--solver-tol-leg T2] [--no-cpu] [--calibrate]
                     [--accuracy-steps N]
     python bench.py --adjoint     (rotor-power gradient report, 256x96x96)
+    python bench.py --farm        (floating tandem-rotor segment, 1024x256x256)
 """
 import argparse
 import json
@@ -265,6 +266,101 @@ def run_lid3d(n, steps, tol):
     }
 
 
+def run_farm(n, steps, window, ckpt, fam_steps=2):
+    """The floating tandem-rotor segment (two rotors 7 D apart, the upstream
+    one in prescribed surge and pitch, 8 % inlet turbulence, CFL 0.1) in
+    mixed precision and in FP64 for `steps` steps from the same start: MLUPS,
+    per-kernel HBM rooflines of the mixed run, the mixed run's error against
+    the FP64 run; then, on the FP64 run's state, the adjoint gradient of the
+    summed power of both rotors over `window` steps with respect to the
+    upstream rotor's yaw and its platform's pitch (the static part of the
+    pitch angle), each against a central difference (h = 1e-4 rad)."""
+    import numpy as np
+
+    from falm_amd import _abi as abi
+    from falm_amd import cases, runtime
+
+    def note(msg):
+        print(f"[farm] {msg}", file=sys.stderr, flush=True)
+
+    out = {}
+    fields = {}
+    for key, prec in (("mixed", abi.CASE_MIXED), ("fp64", abi.CASE_F64)):
+        setup = cases.farm_segment(n=n, precision=prec)
+        c = runtime.Case(setup)
+        its = []
+        t0 = time.perf_counter()
+        for q in range(0, steps, 100):
+            info = c.step(min(100, steps - q))
+            its.append(info.solver_it)
+            note(f"{key}: {min(q + 100, steps)}/{steps} steps, {time.perf_counter() - t0:.1f} s, div {info.div:.3e}")
+        dt = time.perf_counter() - t0
+        rec = {"ms_per_step": 1e3 * dt / steps, "mlups": setup.n_interior * steps / dt / 1e6,
+               "steps_per_s": steps / dt, "div": info.div, "solver_err": info.solver_err,
+               "solver_it_last_step": info.solver_it, "status": info.status}
+        _, _, tq, th = c.alm_get()
+        rec["torque"], rec["thrust"] = [float(x) for x in tq], [float(x) for x in th]
+        fields[key] = (c.get(abi.FIELD_U), c.get(abi.FIELD_P))
+        if key == "mixed":
+            kern = {}
+            for name, (fam, desc) in families().items():
+                runtime.timing_enable(fam)
+                c.step(fam_steps)
+                launches, kms, kbytes = runtime.timing_read()
+                runtime.timing_enable(abi.TIME_NONE)
+                if launches == 0 or kms <= 0:
+                    continue
+                gbs = kbytes / (kms * 1e-3) / 1e9
+                kern[name] = {"launches_per_step": launches / fam_steps, "avg_launch_us": 1e3 * kms / launches,
+                              "gbs": gbs, "frac": None if name in LATENCY_FAMILIES else gbs / PEAK_HBM_GBS}
+            rec["kernels"] = kern
+            c.close()
+        out[key] = rec
+    nx, ny, nz = setup.shape
+    mask = np.zeros((nz, ny, nx), dtype=bool)
+    mask[2:-2, 2:-2, 2:-2] = True
+    m1 = mask.ravel()
+    m3 = np.tile(m1, 3)
+    (um, pm), (uf, pf) = fields["mixed"], fields["fp64"]
+    out["velocity_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(um[m3] - uf[m3]) / np.linalg.norm(uf[m3]))
+    out["pressure_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(pm[m1] - pf[m1]) / np.linalg.norm(pf[m1]))
+    del fields
+    # adjoint on the FP64 state
+    t0 = time.perf_counter()
+    J0 = c.window(window)
+    t_fwd = time.perf_counter() - t0
+    note(f"forward window J = {J0:.8g}, {t_fwd:.2f} s")
+    t0 = time.perf_counter()
+    J, gp, gt, nbytes = c.adjoint(window, ckpt, 1e-10)
+    t_adj = time.perf_counter() - t0
+    gy, go = c.adjoint_attitude()
+    note(f"adjoint {t_adj:.2f} s, {nbytes / 2**30:.2f} GiB: dJ/dyaw {gy}, dJ/doffset {go}")
+    h = float(np.degrees(1e-4))
+    checks = []
+    for which, ad in (("yaw", gy[0]), ("pitch_offset", go[0])):
+        def Jat(e):
+            c.set_attitude(0, e if which == "yaw" else 0.0, e if which != "yaw" else 0.0)
+            return c.window(window)
+        fd = (Jat(h) - Jat(-h)) / (2 * h)
+        Jat(0.0)
+        checks.append({"parameter": which, "adjoint_per_deg": float(ad), "fd_per_deg": fd,
+                       "rel_err": abs(fd - ad) / abs(fd) if fd != 0 else None})
+        note(f"{which}: adjoint {ad:.8e} central difference {fd:.8e}")
+    c.close()
+    out["adjoint"] = {
+        "objective": f"mean summed power of both rotors over {window} FP64 steps after the {steps}-step run",
+        "J": J, "J_forward_window": J0,
+        "grad_yaw_per_deg": [float(x) for x in gy], "grad_pitch_offset_per_deg": [float(x) for x in go],
+        "grad_blade_pitch_per_deg": [float(x) for x in gp[0, :3]],
+        "checks": checks, "fd_step_rad": 1e-4,
+        "time_forward_window_s": t_fwd, "time_adjoint_s": t_adj, "adjoint_device_bytes": nbytes,
+    }
+    out["workload"] = (f"farm segment {n[0]}x{n[1]}x{n[2]} cells (dx=D/32), rotors at 4D and 11D, upstream surge "
+                       f"0.1D sin(2 pi t/T) and pitch 3 deg sin(2 pi t/T + pi/2), T = 20 revolutions, 8% inlet "
+                       f"turbulence, CFL 0.1, Poisson tol 1e-6, {steps} steps")
+    return out
+
+
 def device_free_bytes():
     """hipMemGetInfo through the HIP runtime the library already loaded."""
     import ctypes as C
@@ -370,6 +466,12 @@ def main():
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--adjoint", action="store_true",
                     help="rotor-power gradient report (256x96x96, D/24) instead of the step benchmark")
+    ap.add_argument("--farm", action="store_true",
+                    help="floating tandem-rotor segment report (1024x256x256, two rotors) instead of the step benchmark")
+    ap.add_argument("--farm-n", type=int, nargs=3, default=[1024, 256, 256])
+    ap.add_argument("--farm-steps", type=int, default=2000)
+    ap.add_argument("--farm-window", type=int, default=10)
+    ap.add_argument("--farm-ckpt", type=int, default=5)
     ap.add_argument("--adj-spinup", type=int, default=500)
     ap.add_argument("--adj-window", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
@@ -398,6 +500,11 @@ def main():
     from falm_amd import runtime
 
     runtime.init(0)
+    if args.farm:
+        rep = run_farm(tuple(args.farm_n), args.farm_steps, args.farm_window, args.farm_ckpt)
+        print(json.dumps({"metric": "MLUPS + mixed vs FP64 error + attitude gradient: floating tandem-rotor segment",
+                          "unit": "MLUPS", "value": rep["mixed"]["mlups"], "n_gpus": 1, "farm": rep}))
+        return
     if args.adjoint:
         rep = run_adjoint((256, 96, 96), args.adj_spinup, args.adj_window, args.adj_ckpt, args.adj_dirs,
                           args.adj_tol, 1e-10, args.adj_cfl)

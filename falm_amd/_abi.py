@@ -89,7 +89,7 @@ class SolveInfo(C.Structure):
 class Turbine(C.Structure):
     _fields_ = [("base", C.c_double * 3), ("hub", C.c_double * 3), ("tip_rate", C.c_double),
                 ("angle_type", C.c_int), ("motion", C.c_double * 3), ("surge", C.c_double * 3),
-                ("pitch_deg", C.c_double * 8)]
+                ("pitch_deg", C.c_double * 8), ("yaw_deg", C.c_double), ("angle_offset_deg", C.c_double)]
 
 
 DP = C.POINTER(C.c_double)

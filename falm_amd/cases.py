@@ -8,6 +8,8 @@ Workloads (interior cells):
   lid2d      128x128x1  2-D lid-driven cavity, Re 1000, RB-SOR (omega 1.2, tol 1e-8)
   lid3d      128^3      3-D lid-driven cavity LES, Re 1e4, Cs 0.1, BiCGStab+RB-SOR
   windtunnel 512x128x128 channel with one actuator-line rotor (D = 1, dx = D/32)
+  farm_segment 1024x256x256 channel with two rotors 7D apart, the upstream one
+             floating (surge + pitch), 8 % inlet turbulence
 """
 import ctypes as C
 import math
@@ -69,13 +71,14 @@ class CaseSetup:
 
 
 def make_turbine(base, hub=(0.0, 0.0, 0.0), tip_rate=0.0, angle_type=0, motion=(0.0, 0.0, 0.0),
-                 surge=(0.0, 0.0, 0.0), pitch_deg=()):
+                 surge=(0.0, 0.0, 0.0), pitch_deg=(), yaw_deg=0.0, angle_offset_deg=0.0):
     t = abi.Turbine()
     for m in range(3):
         t.base[m], t.hub[m] = base[m], hub[m]
         t.motion[m], t.surge[m] = motion[m], surge[m]
     t.tip_rate = tip_rate
     t.angle_type = angle_type
+    t.yaw_deg, t.angle_offset_deg = yaw_deg, angle_offset_deg
     for b, v in enumerate(pitch_deg):
         t.pitch_deg[b] = v
     return t
@@ -136,16 +139,21 @@ def build_ap_host(rr, chord, twist, cl, cd, n_turbine, n_blade, n_ap_per_blade, 
 def rotor(h, base, radius=0.5, tsr=7.0, u_inf=1.0, n_blade=3, n_ap_per_blade=32, eps_cells=2.0,
           chord=(0.08, 0.02), twist=(20.0, 0.0), hub_radius=0.0, cutoff=abi.ALM_CUTOFF_ROUNDING, flags=0,
           angle_type=0, motion=(0.0, 0.0, 0.0), surge=(0.0, 0.0, 0.0), pitch_deg=(),
-          twist_offset=None, extra_bases=()):
+          twist_offset=None, extra_bases=(), yaw_deg=0.0, angle_offset_deg=0.0):
     """One (or more, via extra_bases) three-bladed rotor of diameter 2*radius:
     chord and twist linear in r, synthetic polar, Gaussian width eps_cells*h,
     tip rate Omega = tsr*u_inf/radius.  cutoff: culling radius in units of
     eps (default: terms below 2^-53 of a point's peak are dropped; 0 keeps
-    every term, the reference's all-pairs sum)."""
+    every term, the reference's all-pairs sum).  The platform motion
+    (angle_type, motion, angle_offset_deg, surge) and the yaw belong to the
+    first rotor; the rotors at extra_bases stand still."""
     rr, ch, tw, attack, cl, cd = blade_table(radius, chord[0], chord[1], twist[0], twist[1])
     bases = [tuple(base)] + [tuple(b) for b in extra_bases]
-    turbines = [make_turbine(b, tip_rate=tsr * u_inf / radius, angle_type=angle_type, motion=motion,
-                             surge=surge if i == 0 else (0.0, 0.0, 0.0), pitch_deg=pitch_deg)
+    zero = (0.0, 0.0, 0.0)
+    turbines = [make_turbine(b, tip_rate=tsr * u_inf / radius, angle_type=angle_type if i == 0 else 0,
+                             motion=motion if i == 0 else zero, surge=surge if i == 0 else zero,
+                             pitch_deg=pitch_deg, yaw_deg=yaw_deg if i == 0 else 0.0,
+                             angle_offset_deg=angle_offset_deg if i == 0 else 0.0)
                 for i, b in enumerate(bases)]
     ap = build_ap_host(rr, ch, tw, cl, cd, len(turbines), n_blade, n_ap_per_blade, radius, hub_radius)
     return AlmSetup(turbines, n_blade, n_ap_per_blade, radius, hub_radius, eps_cells * h,
@@ -190,6 +198,35 @@ def windtunnel(n=(512, 128, 128), cells_per_d=32, re=2.5e5, cfl=0.3, u_inf=1.0, 
         kw = dict(rotor_kw or {})
         alm = rotor(h, (hub_x_d, 0.5 * ly, 0.5 * lz), u_inf=u_inf, **kw)
     return CaseSetup(s, alm, name=f"windtunnel {n[0]}x{n[1]}x{n[2]}" + (" + ALM" if alm else ""))
+
+
+def farm_segment(n=(1024, 256, 256), cells_per_d=32, spacing_d=7.0, hub_x_d=4.0, re=2.5e5, cfl=0.1, u_inf=1.0,
+                 cs=0.1, inlet_intensity=0.08, seed=20241203, precision=abi.CASE_MIXED, tol=1e-6, maxit=1000,
+                 surge_d=0.1, pitch_amp_deg=3.0, period_revs=20.0, yaw_deg=0.0, angle_offset_deg=0.0,
+                 pc_maxit=1, pc_relax=1.0, rotor_kw=None):
+    """Floating tandem-rotor segment: two rotors spacing_d diameters apart in
+    line on the channel's centre line, the upstream one on a platform in
+    prescribed surge surge_d*D*sin(2 pi t/T) and pitch
+    pitch_amp_deg*sin(2 pi t/T + pi/2), T = period_revs rotor revolutions,
+    with 8 % inlet turbulence.  CFL 0.1: the explicit Upwind3 step amplifies
+    the longest resolved waves by 2.4 % per step at CFL 0.3, which a
+    2000-step run does not survive."""
+    h = 1.0 / cells_per_d
+    radius, tsr = 0.5, 7.0
+    omega = tsr * u_inf / radius
+    w = omega / period_revs  # 2 pi / T with T = period_revs * 2 pi / Omega
+    ly, lz = n[1] * h, n[2] * h
+    kw = dict(rotor_kw or {})
+    kw.setdefault("angle_type", 2)
+    kw.setdefault("motion", (math.radians(pitch_amp_deg), w, 0.5 * math.pi))
+    kw.setdefault("surge", (surge_d * 2 * radius, w, 0.0))
+    setup = windtunnel(n=n, cells_per_d=cells_per_d, re=re, cfl=cfl, u_inf=u_inf, cs=cs,
+                       inlet_intensity=inlet_intensity, seed=seed, precision=precision, with_rotor=False,
+                       tol=tol, maxit=maxit, pc_maxit=pc_maxit, pc_relax=pc_relax)
+    alm = rotor(h, (hub_x_d, 0.5 * ly, 0.5 * lz), radius=radius, tsr=tsr, u_inf=u_inf,
+                extra_bases=[(hub_x_d + spacing_d * 2 * radius, 0.5 * ly, 0.5 * lz)], yaw_deg=yaw_deg,
+                angle_offset_deg=angle_offset_deg, **kw)
+    return CaseSetup(setup.struct, alm, name=f"farm segment {n[0]}x{n[1]}x{n[2]} + 2 rotors {spacing_d:g}D apart")
 
 
 def lid3d(n=128, re=1e4, dt=5e-4, cs=0.1, lid=1.0, precision=abi.CASE_F64, tol=1e-6, maxit=1000,

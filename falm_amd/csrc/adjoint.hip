@@ -624,7 +624,7 @@ falm_status falm_case_adjoint(falm_case *C, int nsteps, int ckpt, double adj_rto
     double hJ = 0.0;
     FALM_HIP(hipMemcpyAsync(&hJ, acc, sizeof(double), hipMemcpyDeviceToHost, s));
     // adjoint work fields
-    double *ub, *uub, *unb, *uunb, *usb, *uab, *uusb, *pb, *lam, *nutb, *ffb, *nutn, *at, *gp, *gt;
+    double *ub, *uub, *unb, *uunb, *usb, *uab, *uusb, *pb, *lam, *nutb, *ffb, *nutn, *at, *gp, *gt, *ga;
     FALM_TRY(W.alloc(&ub, 3 * n));
     FALM_TRY(W.alloc(&uub, 3 * n));
     FALM_TRY(W.alloc(&unb, 3 * n));
@@ -640,11 +640,13 @@ falm_status falm_case_adjoint(falm_case *C, int nsteps, int ckpt, double adj_rto
     FALM_TRY(W.alloc(&at, 7 * n));
     FALM_TRY(W.alloc(&gp, (size_t)nt * 8));
     FALM_TRY(W.alloc(&gt, (size_t)napb));
+    FALM_TRY(W.alloc(&ga, 2 * (size_t)nt));
     FALM_HIP(hipMemsetAsync(ub, 0, 3 * n * sizeof(double), s));
     FALM_HIP(hipMemsetAsync(uub, 0, 3 * n * sizeof(double), s));
     FALM_HIP(hipMemsetAsync(at, 0, 7 * n * sizeof(double), s));
     FALM_HIP(hipMemsetAsync(gp, 0, (size_t)nt * 8 * sizeof(double), s));
     FALM_HIP(hipMemsetAsync(gt, 0, (size_t)napb * sizeof(double), s));
+    FALM_HIP(hipMemsetAsync(ga, 0, 2 * (size_t)nt * sizeof(double), s));
     hipLaunchKernelGGL(k_transpose7, box_grid(ib), dim3(EW_BLOCK), 0, s, ib, g, mat7<double>(C->a, n), at, n);
     FALM_CHECK_LAUNCH("k_transpose7");
     // the transposed solve: BiCGStab with the case's preconditioner, double
@@ -712,7 +714,7 @@ falm_status falm_case_adjoint(falm_case *C, int nsteps, int ckpt, double adj_rto
                                f3(uunb, n), nutb, f3(ffb, n));
             FALM_CHECK_LAUNCH("k_adj_pseudo_u");
             // rotor: this step's power and force
-            FALM_TRY(alm_adjoint(C->alm, Sn.u, t, ffb, wP, unb, gp, gt, s));
+            FALM_TRY(alm_adjoint(C->alm, Sn.u, t, ffb, wP, unb, gp, gt, ga, s));
             // nut^n = SGS(u^n)
             if (c.sgs_model == FALM_SGS_SMAGORINSKY) {
                 hipLaunchKernelGGL(k_adj_sgs, box_grid(ib), dim3(EW_BLOCK), 0, s, ib, g, c.Cs, c3(Sn.u, n), kx, ja,
@@ -723,15 +725,33 @@ falm_status falm_case_adjoint(falm_case *C, int nsteps, int ckpt, double adj_rto
             std::swap(uub, uunb);
         }
     }
-    std::vector<double> hgp((size_t)nt * 8), hgt((size_t)napb);
+    std::vector<double> hgp((size_t)nt * 8), hgt((size_t)napb), hga(2 * (size_t)nt);
     FALM_HIP(hipMemcpyAsync(hgp.data(), gp, hgp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     FALM_HIP(hipMemcpyAsync(hgt.data(), gt, hgt.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    FALM_HIP(hipMemcpyAsync(hga.data(), ga, hga.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     FALM_TRY(snap_restore(C, S0));
     FALM_HIP(hipStreamSynchronize(s));
     *J = hJ / nsteps;
+    // the attitude gradient, per degree
+    C->g_att.assign(hga.size(), 0.0);
+    for (size_t q = 0; q < hga.size(); q++) C->g_att[q] = hga[q] * (3.14159265358979323846 / 180.0);
     for (size_t q = 0; q < hgp.size(); q++) g_pitch[q] = hgp[q];
     for (size_t q = 0; q < hgt.size(); q++) g_twist[q] = hgt[q];
     if (bytes) *bytes = W.bytes + 7 * n * sizeof(double) * 2;  // + the transposed solver's operator copies
+    return FALM_OK;
+}
+
+falm_status falm_case_adjoint_attitude(falm_case *C, double *g_yaw, double *g_offset) {
+    if (!C || !g_yaw || !g_offset) return FALM_ERR_INVALID;
+    if (C->g_att.empty()) {
+        set_error("falm_case_adjoint_attitude: no falm_case_adjoint call on this case yet");
+        return FALM_ERR_SETUP;
+    }
+    const size_t nt = C->g_att.size() / 2;
+    for (size_t t = 0; t < nt; t++) {
+        g_yaw[t] = C->g_att[2 * t];
+        g_offset[t] = C->g_att[2 * t + 1];
+    }
     return FALM_OK;
 }
 

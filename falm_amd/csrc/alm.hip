@@ -26,8 +26,11 @@ namespace falm {
 constexpr double kPi = 3.14159265358979323846;  // Falm::Pi (src/falmath.h:10)
 
 struct TurbDev {
-    // bvel: base velocity set with falm_alm_set_base (0 unless set)
+    // bvel: base velocity set with falm_alm_set_base (0 unless set);
+    // yaw, aoff: the rotor's static yaw and the platform angle's static part
+    // (radians)
     double base[3], bvel[3], hub[3], tip_rate, motion[3], surge[3], pitch_deg[8];
+    double yaw, aoff;
     int angle_type;
 };
 
@@ -105,6 +108,30 @@ __device__ inline void rot_dt(const double x[3], const double v[3], const double
         out[0] = O * (-S * X + C * Y) + (C * U + S * V); out[1] = O * (-C * X - S * Y) + (-S * U + C * V); out[2] = 0. + W;
     } else {
         out[0] = U; out[1] = V; out[2] = W;
+    }
+}
+
+// Rotor frame -> inertial frame for a yawed rotor on a moving platform: x
+// turned by -yaw about z (the rotor's own yaw, no reference counterpart),
+// then by -ang about the platform's axis (one_angle_frame_rotation with the
+// negated angles, src/alm/almDevCall.cu:88-104).  yaw = 0 is the reference.
+__device__ inline void to_inertial(const double x[3], double yaw, const double nang[3], int type, double out[3]) {
+    if (yaw != 0.0) {
+        const double ny[3] = {0.0, 0.0, -yaw};
+        double y[3];
+        rot(x, ny, 3, y);
+        rot(y, nang, type, out);
+    } else {
+        rot(x, nang, type, out);
+    }
+}
+
+// The platform frame's relative velocity turned into the yawed rotor frame.
+__device__ inline void to_rotor(double v[3], double yaw) {
+    if (yaw != 0.0) {
+        const double py[3] = {0.0, 0.0, yaw};
+        const double w[3] = {v[0], v[1], v[2]};
+        rot(w, py, 3, v);
     }
 }
 
@@ -195,9 +222,10 @@ __global__ __launch_bounds__(256) void k_alm_points(AlmDev A, const T *u0, const
     double ang[3] = {0, 0, 0}, om[3] = {0, 0, 0};
     if (atype) {
         const double sa = mo0 * sin(mo1 * t + mo2), oa = mo0 * mo1 * cos(mo1 * t + mo2);
-        ang[0] = atype == 1 ? sa : 0.0;
-        ang[1] = atype == 2 ? sa : 0.0;
-        ang[2] = atype == 3 ? sa : 0.0;
+        const double sb = T0.aoff != 0.0 ? T0.aoff + sa : sa;
+        ang[0] = atype == 1 ? sb : 0.0;
+        ang[1] = atype == 2 ? sb : 0.0;
+        ang[2] = atype == 3 ? sb : 0.0;
         om[0] = atype == 1 ? oa : 0.0;
         om[1] = atype == 2 ? oa : 0.0;
         om[2] = atype == 3 ? oa : 0.0;
@@ -214,7 +242,7 @@ __global__ __launch_bounds__(256) void k_alm_points(AlmDev A, const T *u0, const
     c1[1] += apr * cos(theta);
     c1[2] += apr * sin(theta);
     double P[3];
-    rot(c1, nang, atype, P);
+    to_inertial(c1, T0.yaw, nang, atype, P);
     for (int m = 0; m < 3; m++) P[m] += base[m];
     if (fused)
         for (int m = 0; m < 3; m++) sxyz[3 * ap + m] = P[m];
@@ -248,6 +276,7 @@ __global__ __launch_bounds__(256) void k_alm_points(AlmDev A, const T *u0, const
             vrel[m] = uap[m] - bvel[m];
         }
         rot_dt(rel, vrel, ang, om, atype, utt);
+        to_rotor(utt, T0.yaw);
         const double ux = utt[0];
         const double ut = tip * apr + utt[1] * sin(theta) - utt[2] * cos(theta);
         const double urel2 = ux * ux + ut * ut;
@@ -282,7 +311,7 @@ __global__ __launch_bounds__(256) void k_alm_points(AlmDev A, const T *u0, const
         const int sg = (tip > 0) - (tip < 0);
         ft *= sg;
         const double ftt[3] = {-fx, ft * sin(theta), -ft * cos(theta)};
-        rot(ftt, nang, atype, F);
+        to_inertial(ftt, T0.yaw, nang, atype, F);
         torque = fabs(ft) * apr;
         thrust = fx;
     }
@@ -621,12 +650,15 @@ __device__ inline int lower_index(const double *seq, double v, int n) {
 // Adjoint of the projection with respect to the point forces: one
 // workgroup per point sums ffbar(c) eta exp(-r^2/eps^2) over the cells within
 // the culling radius (the cells the forward projection gives its force).
+// With Mb, also the moments Mb[9 ap + 3 m + n] = sum_c ffbar_m(c) dG/dX_n of
+// the same kernel G = eta exp(-|x - X|^2/eps^2) with respect to the point's
+// position X (dG/dX_n = G 2 (x_n - X_n)/eps^2).
 __global__ __launch_bounds__(256) void k_alm_adj_force(AlmDev A, const double *ffb0, const double *ffb1,
-                                                       const double *ffb2, double *Fb) {
+                                                       const double *ffb2, double *Fb, double *Mb) {
     const int ap = blockIdx.x;
     if (ap >= A.nap) return;
     __shared__ int rng[6];
-    __shared__ double lds[3 * 4];
+    __shared__ double lds[12 * 4];
     const double X = A.ap_xyz[3 * ap], Y = A.ap_xyz[3 * ap + 1], Z = A.ap_xyz[3 * ap + 2];
     const double R = A.cutr;
     if (threadIdx.x == 0) {
@@ -644,7 +676,9 @@ __global__ __launch_bounds__(256) void k_alm_adj_force(AlmDev A, const double *f
     const double eps = A.eps, eps2 = eps * eps;
     const double sp = eps * sqrt(kPi);
     const double eta = 1. / (sp * sp * sp);
-    double v[3] = {0.0, 0.0, 0.0};
+    const int nv = Mb ? 12 : 3;
+    double v[12];
+    for (int q = 0; q < 12; q++) v[q] = 0.0;
     for (long long q = threadIdx.x; q < tot; q += blockDim.x) {
         const int i = rng[0] + (int)(q % ni);
         const long long r = q / ni;
@@ -652,28 +686,219 @@ __global__ __launch_bounds__(256) void k_alm_adj_force(AlmDev A, const double *f
         const double dx = A.x[i] - X, dy = A.y[j] - Y, dz = A.z[k] - Z;
         const double e = eta * exp(-(dx * dx + dy * dy + dz * dz) / eps2);
         const size_t c = (size_t)i + (size_t)A.nx * ((size_t)j + (size_t)A.ny * k);
-        v[0] += ffb0[c] * e;
-        v[1] += ffb1[c] * e;
-        v[2] += ffb2[c] * e;
+        const double f[3] = {ffb0[c], ffb1[c], ffb2[c]};
+        v[0] += f[0] * e;
+        v[1] += f[1] * e;
+        v[2] += f[2] * e;
+        if (Mb) {
+            const double w = 2.0 * e / eps2;
+            const double d[3] = {dx * w, dy * w, dz * w};
+            for (int m = 0; m < 3; m++)
+                for (int n = 0; n < 3; n++) v[3 + 3 * m + n] += f[m] * d[n];
+        }
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int m = 0; m < 3; m++) {
+    for (int m = 0; m < nv; m++) {
         double x = v[m];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
         if (lane == 0) lds[m * 4 + wid] = x;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if ((int)threadIdx.x < nv) {
         double x = 0.0;
         for (int w = 0; w < 4; w++) x += lds[threadIdx.x * 4 + w];
-        Fb[3 * ap + threadIdx.x] = x;
+        if (threadIdx.x < 3) Fb[3 * ap + threadIdx.x] = x;
+        else Mb[9 * ap + threadIdx.x - 3] = x;
     }
 }
 
-// Adjoint of the point pipeline (kernel_UpdateTurbineAngles, UpdateAPX,
-// CalcAPForce, src/alm/almDevCall.cu:43-253) at (u, t): given the force
-// adjoints Fb and the power weight wP, accumulate ubar at the interpolation
-// corners and the pitch / twist-offset adjoints (per degree).
+// ---------------------------------------------- attitude tangents
+// A value with its derivatives with respect to the rotor's yaw (d[0]) and
+// the platform angle's static part (d[1]), both in radians.
+struct Tan {
+    double v, d[2];
+};
+__device__ inline Tan tc(double v) { return {v, {0.0, 0.0}}; }
+__device__ inline Tan operator+(Tan a, Tan b) { return {a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1]}}; }
+__device__ inline Tan operator-(Tan a, Tan b) { return {a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1]}}; }
+__device__ inline Tan operator-(Tan a) { return {-a.v, {-a.d[0], -a.d[1]}}; }
+__device__ inline Tan operator*(Tan a, Tan b) {
+    return {a.v * b.v, {a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1]}};
+}
+__device__ inline Tan operator*(double a, Tan b) { return {a * b.v, {a * b.d[0], a * b.d[1]}}; }
+__device__ inline Tan operator/(Tan a, Tan b) {
+    const double q = a.v / b.v;
+    return {q, {(a.d[0] - q * b.d[0]) / b.v, (a.d[1] - q * b.d[1]) / b.v}};
+}
+__device__ inline Tan tsin(Tan a) {
+    const double c = cos(a.v);
+    return {sin(a.v), {c * a.d[0], c * a.d[1]}};
+}
+__device__ inline Tan tcos(Tan a) {
+    const double s = -sin(a.v);
+    return {cos(a.v), {s * a.d[0], s * a.d[1]}};
+}
+__device__ inline Tan tatan(Tan a) {
+    const double w = 1.0 / (1.0 + a.v * a.v);
+    return {atan(a.v), {w * a.d[0], w * a.d[1]}};
+}
+
+// rot / rot_dt of the forward kernels on tangent values (the same
+// arithmetic; the angle and its rate may carry derivatives)
+__device__ inline void rot_t(const Tan x[3], const Tan ang[3], int type, Tan out[3]) {
+    if (type == 1) {
+        const Tan C = tcos(ang[0]), S = tsin(ang[0]);
+        out[0] = x[0]; out[1] = C * x[1] + S * x[2]; out[2] = -S * x[1] + C * x[2];
+    } else if (type == 2) {
+        const Tan C = tcos(ang[1]), S = tsin(ang[1]);
+        out[0] = C * x[0] - S * x[2]; out[1] = x[1]; out[2] = S * x[0] + C * x[2];
+    } else if (type == 3) {
+        const Tan C = tcos(ang[2]), S = tsin(ang[2]);
+        out[0] = C * x[0] + S * x[1]; out[1] = -S * x[0] + C * x[1]; out[2] = x[2];
+    } else {
+        out[0] = x[0]; out[1] = x[1]; out[2] = x[2];
+    }
+}
+__device__ inline void rot_dt_t(const Tan x[3], const Tan v[3], const Tan ang[3], const Tan om[3], int type,
+                                Tan out[3]) {
+    if (type == 1) {
+        const Tan C = tcos(ang[0]), S = tsin(ang[0]), O = om[0];
+        out[0] = v[0];
+        out[1] = O * (-S * x[1] + C * x[2]) + (C * v[1] + S * v[2]);
+        out[2] = O * (-C * x[1] - S * x[2]) + (-S * v[1] + C * v[2]);
+    } else if (type == 2) {
+        const Tan C = tcos(ang[1]), S = tsin(ang[1]), O = om[1];
+        out[0] = O * (-S * x[0] - C * x[2]) + (C * v[0] - S * v[2]);
+        out[1] = v[1];
+        out[2] = O * (C * x[0] - S * x[2]) + (S * v[0] + C * v[2]);
+    } else if (type == 3) {
+        const Tan C = tcos(ang[2]), S = tsin(ang[2]), O = om[2];
+        out[0] = O * (-S * x[0] + C * x[1]) + (C * v[0] + S * v[1]);
+        out[1] = O * (-C * x[0] - S * x[1]) + (-S * v[0] + C * v[1]);
+        out[2] = v[2];
+    } else {
+        out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
+    }
+}
+
+// Per point: the derivatives of its position X, its force on the fluid F
+// and its power Omega |ft| r with respect to the turbine's yaw and platform
+// angle offset, the velocity field held (the sampling point moves within
+// its cell, the polar segment and the cell are kept), contracted with this
+// step's adjoints: the force adjoint Fb, the position moments Mb of
+// k_alm_adj_force and the power weight wP.  Accumulates per turbine, per
+// radian, into g[2 t] (yaw) and g[2 t + 1] (offset).
+__global__ void k_alm_adj_attitude(AlmDev A, const double *u0, const double *u1, const double *u2, double t,
+                                   const double *Fb, const double *Mb, double wP, double *g) {
+    const int ap = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ap >= A.nap) return;
+    const int napt = A.napb * A.nb;
+    const int tid = ap / napt, bid = (ap % napt) / A.napb, sec = ap % A.napb;
+    const TurbDev &T0 = A.tb[tid];
+    const int atype = T0.angle_type;
+    const Tan yaw = {T0.yaw, {1.0, 0.0}};
+    Tan ang[3] = {tc(0), tc(0), tc(0)}, om[3] = {tc(0), tc(0), tc(0)};
+    if (atype) {
+        const double sa = T0.motion[0] * sin(T0.motion[1] * t + T0.motion[2]);
+        ang[atype - 1] = {T0.aoff + sa, {0.0, 1.0}};
+        om[atype - 1] = tc(T0.motion[0] * T0.motion[1] * cos(T0.motion[1] * t + T0.motion[2]));
+    }
+    double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {T0.bvel[0], T0.bvel[1], T0.bvel[2]};
+    if (T0.surge[0] != 0.0) {
+        base[0] = T0.base[0] + T0.surge[0] * sin(T0.surge[1] * t + T0.surge[2]);
+        bvel[0] = T0.bvel[0] + T0.surge[0] * T0.surge[1] * cos(T0.surge[1] * t + T0.surge[2]);
+    }
+    const Tan nang[3] = {-ang[0], -ang[1], -ang[2]};
+    const double tip = T0.tip_rate;
+    const double theta = tip * t + (2 * kPi / A.nb) * bid;
+    const double apr = A.r[ap];
+    const Tan c1[3] = {tc(T0.hub[0]), tc(T0.hub[1] + apr * cos(theta)), tc(T0.hub[2] + apr * sin(theta))};
+    // to_inertial: turned by -yaw about z, then by -ang about the platform axis
+    const Tan nyaw[3] = {tc(0), tc(0), -yaw};
+    Tan cy[3], P[3];
+    rot_t(c1, nyaw, 3, cy);
+    rot_t(cy, nang, atype, P);
+    for (int m = 0; m < 3; m++) P[m] = P[m] + tc(base[m]);
+    const int I = find_index(A.x, P[0].v, A.nx), J = find_index(A.y, P[1].v, A.ny), K = find_index(A.z, P[2].v, A.nz);
+    const int gc = A.gc;
+    const bool in = I >= gc - 1 && I < A.nx - gc - 1 && J >= gc - 1 && J < A.ny - gc - 1 && K >= gc - 1 &&
+                    K < A.nz - gc - 1;
+    if (!in) return;  // no force, no power
+    const double dr = (A.radius - A.hub_radius) / A.napb;
+    const int i0 = I, j0 = J, k0 = K;
+    // P lies at or above the lower cell centre, so |P - x0| = P - x0
+    const Tan xd = (P[0] - tc(A.x[i0])) / tc(A.x[i0 + 1] - A.x[i0]);
+    const Tan yd = (P[1] - tc(A.y[j0])) / tc(A.y[j0 + 1] - A.y[j0]);
+    const Tan zd = (P[2] - tc(A.z[k0])) / tc(A.z[k0 + 1] - A.z[k0]);
+    const int nx = A.nx, nxy = A.nx * A.ny;
+    const int cb = i0 + nx * j0 + nxy * k0;
+    const int id[8] = {cb, cb + 1, cb + nx, cb + nx + 1, cb + nxy, cb + nxy + 1, cb + nxy + nx, cb + nxy + nx + 1};
+    const double *uc[3] = {u0, u1, u2};
+    const Tan one = tc(1.0);
+    Tan vrel[3], rel[3];
+    for (int m = 0; m < 3; m++) {
+        Tan c[8];
+        for (int q = 0; q < 8; q++) c[q] = tc(uc[m][id[q]]);
+        // trilinear_interpolation (src/alm/almDevCall.cu:28-41)
+        const Tan c01 = c[0] * (one - xd) + c[1] * xd, c23 = c[2] * (one - xd) + c[3] * xd;
+        const Tan c45 = c[4] * (one - xd) + c[5] * xd, c67 = c[6] * (one - xd) + c[7] * xd;
+        const Tan c0123 = c01 * (one - yd) + c23 * yd, c4567 = c45 * (one - yd) + c67 * yd;
+        vrel[m] = c0123 * (one - zd) + c4567 * zd - tc(bvel[m]);
+        rel[m] = P[m] - tc(base[m]);
+    }
+    Tan up[3], utt[3];
+    rot_dt_t(rel, vrel, ang, om, atype, up);
+    const Tan pyaw[3] = {tc(0), tc(0), yaw};
+    rot_t(up, pyaw, 3, utt);  // to_rotor
+    const Tan ux = utt[0];
+    const Tan ut = tc(tip * apr) + sin(theta) * utt[1] - cos(theta) * utt[2];
+    const Tan urel2 = ux * ux + ut * ut;
+    const Tan phi = tatan(ux / ut);
+    const double twist = A.twist[ap] + T0.pitch_deg[bid < 8 ? bid : 7] + A.twist_off[sec];
+    const Tan att = (180. / kPi) * phi - tc(twist);
+    const int nat = A.nat;
+    Tan cl = tc(0.0), cd = tc(0.0);
+    if (att.v < A.attack[0]) {
+        cl = tc(A.cl[ap]);
+        cd = tc(A.cd[ap]);
+    } else if (att.v >= A.attack[nat - 1]) {
+        cl = tc(A.cl[ap + (nat - 1) * A.nap]);
+        cd = tc(A.cd[ap + (nat - 1) * A.nap]);
+    } else if (att.v == att.v) {
+        int a = 0, hi = nat - 1;
+        while (hi - a > 1) {
+            const int mid = (a + hi) >> 1;
+            if (A.attack[mid] <= att.v) a = mid;
+            else hi = mid;
+        }
+        const Tan p = (att - tc(A.attack[a])) / tc(A.attack[a + 1] - A.attack[a]);
+        cl = (one - p) * tc(A.cl[ap + a * A.nap]) + p * tc(A.cl[ap + (a + 1) * A.nap]);
+        cd = (one - p) * tc(A.cd[ap + a * A.nap]) + p * tc(A.cd[ap + (a + 1) * A.nap]);
+    }
+    const double chord = A.chord[ap];
+    const Tan fl = (.5 * chord * dr) * (cl * urel2), fd = (.5 * chord * dr) * (cd * urel2);
+    const Tan sphi = tsin(phi), cphi = tcos(phi);
+    const Tan fx = fl * cphi + fd * sphi;
+    const bool fix = (A.flags & FALM_ALM_FIX_DRAG_TANGENT) != 0;
+    const int sg = (tip > 0) - (tip < 0);
+    const Tan ft = (double)sg * (fix ? fl * sphi - fd * cphi : fl * sphi - fd * sphi);
+    const Tan ftt[3] = {-fx, sin(theta) * ft, (-cos(theta)) * ft};
+    Tan fy[3], F[3];
+    rot_t(ftt, nyaw, 3, fy);
+    rot_t(fy, nang, atype, F);
+    // contraction with the step's adjoints
+    const double *M = Mb + 9 * ap;
+    double Q[3];
+    for (int n = 0; n < 3; n++) Q[n] = F[0].v * M[n] + F[1].v * M[3 + n] + F[2].v * M[6 + n];
+    const double ps = wP * tip * apr * (double)((ft.v > 0) - (ft.v < 0));  // d(Omega |ft| r)/d ft
+    for (int q = 0; q < 2; q++) {
+        if (q == 1 && !atype) break;
+        double gq = ps * ft.d[q];
+        for (int m = 0; m < 3; m++) gq += Fb[3 * ap + m] * F[m].d[q] + Q[m] * P[m].d[q];
+        unsafeAtomicAdd(&g[2 * tid + q], gq);
+    }
+}
+
 __global__ void k_alm_adj_points(AlmDev A, const double *u0, const double *u1, const double *u2, double t,
                                  const double *Fb, double wP, double *ub0, double *ub1, double *ub2,
                                  double *gpitch, double *gtwist) {
@@ -684,7 +909,8 @@ __global__ void k_alm_adj_points(AlmDev A, const double *u0, const double *u1, c
     const TurbDev &T0 = A.tb[tid];
     double ang[3] = {0, 0, 0}, om[3] = {0, 0, 0};
     if (T0.angle_type) {
-        ang[T0.angle_type - 1] = T0.motion[0] * sin(T0.motion[1] * t + T0.motion[2]);
+        const double sa = T0.motion[0] * sin(T0.motion[1] * t + T0.motion[2]);
+        ang[T0.angle_type - 1] = T0.aoff != 0.0 ? T0.aoff + sa : sa;
         om[T0.angle_type - 1] = T0.motion[0] * T0.motion[1] * cos(T0.motion[1] * t + T0.motion[2]);
     }
     double base[3] = {T0.base[0], T0.base[1], T0.base[2]}, bvel[3] = {T0.bvel[0], T0.bvel[1], T0.bvel[2]};
@@ -700,7 +926,7 @@ __global__ void k_alm_adj_points(AlmDev A, const double *u0, const double *u1, c
     c1[1] += apr * cos(theta);
     c1[2] += apr * sin(theta);
     double P[3];
-    rot(c1, nang, T0.angle_type, P);
+    to_inertial(c1, T0.yaw, nang, T0.angle_type, P);
     for (int m = 0; m < 3; m++) P[m] += base[m];
     const int I = find_index(A.x, P[0], A.nx), J = find_index(A.y, P[1], A.ny), K = find_index(A.z, P[2], A.nz);
     const int gc = A.gc;
@@ -732,6 +958,7 @@ __global__ void k_alm_adj_points(AlmDev A, const double *u0, const double *u1, c
         vrel[m] = uap[m] - bvel[m];
     }
     rot_dt(rel, vrel, ang, om, T0.angle_type, utt);
+    to_rotor(utt, T0.yaw);
     const double ux = utt[0];
     const double ut = tip * apr + utt[1] * sin(theta) - utt[2] * cos(theta);
     const double urel2 = ux * ux + ut * ut;
@@ -773,6 +1000,11 @@ __global__ void k_alm_adj_points(AlmDev A, const double *u0, const double *u1, c
     const double Fbv[3] = {Fb[3 * ap], Fb[3 * ap + 1], Fb[3 * ap + 2]};
     double fttb[3];
     rot_T(Fbv, nang, T0.angle_type, fttb);
+    if (T0.yaw != 0.0) {  // transpose of the yaw turn in to_inertial
+        const double ny[3] = {0.0, 0.0, -T0.yaw};
+        const double w[3] = {fttb[0], fttb[1], fttb[2]};
+        rot_T(w, ny, 3, fttb);
+    }
     const double fxb = -fttb[0];
     double ftb = fttb[1] * sin(theta) - fttb[2] * cos(theta);
     ftb += wP * tip * apr * (double)((ft > 0) - (ft < 0));  // power: Omega |ft| r
@@ -790,7 +1022,12 @@ __global__ void k_alm_adj_points(AlmDev A, const double *u0, const double *u1, c
     const double den = ux * ux + ut * ut;
     const double uxb = phib * ut / den + 2 * ux * urel2b;
     const double utb = -phib * ux / den + 2 * ut * urel2b;
-    const double uttb[3] = {uxb, sin(theta) * utb, -cos(theta) * utb};
+    double uttb[3] = {uxb, sin(theta) * utb, -cos(theta) * utb};
+    if (T0.yaw != 0.0) {  // transpose of to_rotor
+        const double py[3] = {0.0, 0.0, T0.yaw};
+        const double w[3] = {uttb[0], uttb[1], uttb[2]};
+        rot_T(w, py, 3, uttb);
+    }
     double vrelb[3];
     rot_T(uttb, ang, T0.angle_type, vrelb);
     double *ubc[3] = {ub0, ub1, ub2};
@@ -810,6 +1047,7 @@ struct falm_alm {
     int nt, nap, napb;
     bool full_next = true;  // the next owned-ff step writes every cell
     double *Fb = nullptr;   // adjoint: force adjoint per point
+    double *Mb = nullptr;   // adjoint: position moments per point (9 each)
 };
 
 namespace falm {
@@ -900,11 +1138,15 @@ falm_status alm_accumulate_power(falm_alm *A, double *acc, hipStream_t s) {
 }
 
 falm_status alm_adjoint(falm_alm *A, const double *u, double t, const double *ffbar, double wP, double *ubar,
-                        double *gpitch, double *gtwist, hipStream_t s) {
+                        double *gpitch, double *gtwist, double *gatt, hipStream_t s) {
     const size_t n = ncell_of(&A->dom);
     if (!A->Fb) {
         FALM_HIP(hipMalloc(&A->Fb, 3 * (size_t)A->nap * sizeof(double)));
         A->bufs.push_back(A->Fb);
+    }
+    if (gatt && !A->Mb) {
+        FALM_HIP(hipMalloc(&A->Mb, 9 * (size_t)A->nap * sizeof(double)));
+        A->bufs.push_back(A->Mb);
     }
     // point positions at time t (the forward's k_alm_points, which also
     // refreshes the point state)
@@ -912,10 +1154,16 @@ falm_status alm_adjoint(falm_alm *A, const double *u, double t, const double *ff
     const int staged = stage_bytes <= (size_t)kPointsLds;
     hipLaunchKernelGGL((k_alm_points<double>), dim3((A->nap + 255) / 256), dim3(256), staged ? stage_bytes : 0, s,
                        A->d, u, u + n, u + 2 * n, t, staged, 0);  // no box or totals update
-    hipLaunchKernelGGL(k_alm_adj_force, dim3(A->nap), dim3(256), 0, s, A->d, ffbar, ffbar + n, ffbar + 2 * n, A->Fb);
+    hipLaunchKernelGGL(k_alm_adj_force, dim3(A->nap), dim3(256), 0, s, A->d, ffbar, ffbar + n, ffbar + 2 * n, A->Fb,
+                       gatt ? A->Mb : nullptr);
     hipLaunchKernelGGL(k_alm_adj_points, dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d, u, u + n, u + 2 * n, t,
                        (const double *)A->Fb, wP, ubar, ubar + n, ubar + 2 * n, gpitch, gtwist);
     FALM_CHECK_LAUNCH("k_alm_adj_points");
+    if (gatt) {
+        hipLaunchKernelGGL(k_alm_adj_attitude, dim3((A->nap + 63) / 64), dim3(64), 0, s, A->d, u, u + n, u + 2 * n, t,
+                           (const double *)A->Fb, (const double *)A->Mb, wP, gatt);
+        FALM_CHECK_LAUNCH("k_alm_adj_attitude");
+    }
     return FALM_OK;
 }
 
@@ -977,6 +1225,8 @@ falm_status falm_alm_create(const falm_domain *dom, const falm_alm_config *c, co
             tb[t].surge[m] = T.surge[m];
         }
         for (int b = 0; b < 8; b++) tb[t].pitch_deg[b] = T.pitch_deg[b];
+        tb[t].yaw = T.yaw_deg * kPi / 180.0;
+        tb[t].aoff = T.angle_offset_deg * kPi / 180.0;
         tb[t].tip_rate = T.tip_rate;
         tb[t].angle_type = T.angle_type;
         if (T.angle_type < 0 || T.angle_type > 3) {
@@ -1052,6 +1302,16 @@ falm_status falm_alm_set_pitch(falm_alm *A, int turbine, const double *pitch_deg
     return FALM_OK;
 }
 
+falm_status falm_alm_set_attitude(falm_alm *A, int turbine, double yaw_deg, double angle_offset_deg) {
+    if (!A || turbine < 0 || turbine >= A->nt || !std::isfinite(yaw_deg) || !std::isfinite(angle_offset_deg))
+        return FALM_ERR_INVALID;
+    const double v[2] = {yaw_deg * kPi / 180.0, angle_offset_deg * kPi / 180.0};
+    TurbDev *dst = const_cast<TurbDev *>(A->d.tb) + turbine;
+    static_assert(offsetof(TurbDev, aoff) == offsetof(TurbDev, yaw) + sizeof(double), "yaw, aoff adjacent");
+    FALM_HIP(hipMemcpy(&dst->yaw, v, sizeof(v), hipMemcpyHostToDevice));
+    return FALM_OK;
+}
+
 falm_status falm_alm_set_base(falm_alm *A, int turbine, const double *base, const double *velocity, void *stream) {
     if (!A || turbine < 0 || turbine >= A->nt || !base) return FALM_ERR_INVALID;
     double v[6] = {base[0], base[1], base[2], 0.0, 0.0, 0.0};
@@ -1070,7 +1330,7 @@ falm_status falm_alm_vjp(falm_alm *A, const void *u, double t, const void *ffbar
     if (!A || !u || !ffbar || !ubar || !g_pitch || !g_twist) return FALM_ERR_INVALID;
     FALM_TRY(falm::vjp_outputs_on_device({ubar, g_pitch, g_twist}));
     return falm::alm_adjoint(A, (const double *)u, t, (const double *)ffbar, wP, (double *)ubar, (double *)g_pitch,
-                             (double *)g_twist, (hipStream_t)stream);
+                             (double *)g_twist, nullptr, (hipStream_t)stream);
 }
 
 falm_status falm_alm_step(falm_alm *A, int prec, const void *u, void *ff, double t, void *stream) {

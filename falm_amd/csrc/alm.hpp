@@ -20,9 +20,11 @@ falm_status alm_accumulate_power(falm_alm *A, double *acc, hipStream_t s);
 // adjoint of alm_step at (u, t) in double: ffbar is the adjoint of the
 // projected force, wP the weight of this step's power in the objective;
 // accumulates ubar (3 columns) and the parameter adjoints gpitch
-// [n_turbine*8] and gtwist [n_ap_per_blade] (per degree), device arrays
+// [n_turbine*8] and gtwist [n_ap_per_blade] (per degree), device arrays;
+// gatt (may be NULL) [2*n_turbine]: per turbine the yaw and platform angle
+// offset adjoints (per radian)
 falm_status alm_adjoint(falm_alm *A, const double *u, double t, const double *ffbar, double wP, double *ubar,
-                        double *gpitch, double *gtwist, hipStream_t s);
+                        double *gpitch, double *gtwist, double *gatt, hipStream_t s);
 int alm_n_turbine(const falm_alm *A);
 int alm_n_ap_per_blade(const falm_alm *A);
 

@@ -30,6 +30,9 @@ struct falm_case {
     double *dnorm = nullptr, *partials = nullptr, *hnorm = nullptr;
     int *nobox = nullptr;     // empty cell box: no rotor, ff stays zero
     std::vector<void *> bufs;
+    // the last falm_case_adjoint's attitude gradient: per turbine
+    // (d/d yaw_deg, d/d angle_offset_deg)
+    std::vector<double> g_att;
 };
 
 

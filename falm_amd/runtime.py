@@ -76,6 +76,8 @@ def _sigs(L):
         "falm_case_window": [vp, ip, C.POINTER(dp)],
         "falm_case_adjoint": [vp, ip, ip, dp, C.POINTER(dp), vp, vp, C.POINTER(C.c_size_t)],
         "falm_alm_set_pitch": [vp, ip, vp],
+        "falm_alm_set_attitude": [vp, ip, dp, dp],
+        "falm_case_adjoint_attitude": [vp, vp, vp],
         "falm_alm_set_base": [vp, ip, vp, vp, vp],
         "falm_mesher_build": [C.c_char_p, C.c_char_p, C.c_char_p, ip],
         "falm_pseudo_u_vjp": [dom, ip, dp, dp] + [vp] * 12,
@@ -241,6 +243,19 @@ class Case:
         v = np.zeros(8)
         v[:len(pitch_deg)] = pitch_deg
         call("falm_alm_set_pitch", self._alm_handle(), turbine, v.ctypes.data_as(C.c_void_p))
+
+    def set_attitude(self, turbine, yaw_deg, angle_offset_deg):
+        """Static yaw and platform angle offset of one turbine, degrees."""
+        call("falm_alm_set_attitude", self._alm_handle(), turbine, float(yaw_deg), float(angle_offset_deg))
+
+    def adjoint_attitude(self):
+        """dJ/d(yaw_deg) and dJ/d(angle_offset_deg) per turbine (per degree)
+        from the last adjoint() call."""
+        import numpy as np
+        nt = self.setup.alm.n_turbine
+        gy, go = np.zeros(nt), np.zeros(nt)
+        call("falm_case_adjoint_attitude", self.h, gy.ctypes.data_as(C.c_void_p), go.ctypes.data_as(C.c_void_p))
+        return gy, go
 
     def set_base(self, turbine, base, velocity=None):
         """Base position and velocity of one turbine from the next step on."""

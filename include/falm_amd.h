@@ -280,9 +280,14 @@ falm_status falm_poisson_matrix(const falm_domain *dom, int prec, int kind, cons
 /* One turbine (src/alm/turbineHandler.h:13-205).  angle_type: 0 none,
  * 1 roll, 2 pitch, 3 yaw (src/falmath.h:33); motion = {amplitude [rad],
  * angular frequency, phase} of that one Euler angle
- * (src/alm/almDevCall.cu:43-61).  surge and pitch_deg have no reference
- * counterpart: surge = {amplitude, angular frequency, phase} of a base
- * translation along x; pitch_deg adds a per-blade pitch to the AP twist. */
+ * (src/alm/almDevCall.cu:43-61).  surge, pitch_deg, yaw_deg and
+ * angle_offset_deg have no reference counterpart: surge = {amplitude,
+ * angular frequency, phase} of a base translation along x; pitch_deg adds a
+ * per-blade pitch to the AP twist; yaw_deg turns the rotor about the vertical
+ * axis through the base inside the platform motion (rotor frame = platform
+ * frame turned by yaw_deg about z); angle_offset_deg is a static part of the
+ * platform's Euler angle (angle = offset + A sin(w t + phase); used only when
+ * angle_type is not 0). */
 typedef struct falm_turbine {
     double base[3];
     double hub[3];        /* (-overhang, 0, tower)                     */
@@ -291,6 +296,8 @@ typedef struct falm_turbine {
     double motion[3];
     double surge[3];
     double pitch_deg[8];
+    double yaw_deg;
+    double angle_offset_deg;
 } falm_turbine;
 enum { FALM_ALM_FIX_DRAG_TANGENT = 1 };  /* use cos(phi) for the drag share of
                                             the tangential force instead of the
@@ -426,6 +433,19 @@ falm_status falm_case_adjoint(falm_case *c, int nsteps, int ckpt, double adj_rto
 /* Per-blade pitch of one turbine (8 values, degrees; NULL = 0), the
  * falm_turbine::pitch_deg of falm_alm_create. */
 falm_status falm_alm_set_pitch(falm_alm *alm, int turbine, const double *pitch_deg);
+/* The rotor's static yaw and the static part of the platform angle of one
+ * turbine (degrees), the falm_turbine::yaw_deg / angle_offset_deg of
+ * falm_alm_create. */
+falm_status falm_alm_set_attitude(falm_alm *alm, int turbine, double yaw_deg, double angle_offset_deg);
+/* The attitude gradient of the last falm_case_adjoint call on this case:
+ * g_yaw[n_turbine] = dJ/d(yaw_deg) and g_offset[n_turbine] =
+ * dJ/d(angle_offset_deg), per degree (the offset's entry is 0 for a turbine
+ * with angle_type 0).  Each step's point kinematics are differentiated in
+ * forward mode with the velocity held (positions, sampling weights, frame
+ * turns, polar segment) and contracted with that step's adjoints of the force
+ * field and of the power; the flow's response comes through those adjoints.
+ * FALM_ERR_SETUP before any falm_case_adjoint call. */
+falm_status falm_case_adjoint_attitude(falm_case *c, double *g_yaw, double *g_offset);
 
 /* ------------------------------------------- per-stage adjoints (VJPs)
  * No reference counterpart.  Vector-Jacobian products of the FP64 forward

@@ -946,6 +946,28 @@ static void rot_dt(const double x[3], const double v[3], const double ang[3], co
     }
 }
 
+/* A yawed rotor on a moving platform (no reference counterpart; yaw 0 is
+ * the reference): rotor frame -> inertial frame turns x by -yaw about z and
+ * then by -ang about the platform axis; the platform frame's relative
+ * velocity is turned by +yaw about z into the rotor frame. */
+static void yawed_to_inertial(const double x[3], double yaw_deg, const double nang[3], int type, double out[3]) {
+    if (yaw_deg != 0.0) {
+        const double ny[3] = {0.0, 0.0, -deg2rad(yaw_deg)};
+        double y[3];
+        rot(x, ny, 3, y);
+        rot(y, nang, type, out);
+    } else {
+        rot(x, nang, type, out);
+    }
+}
+static void platform_to_rotor(double v[3], double yaw_deg) {
+    if (yaw_deg != 0.0) {
+        const double py[3] = {0.0, 0.0, deg2rad(yaw_deg)};
+        const double w[3] = {v[0], v[1], v[2]};
+        rot(w, py, 3, v);
+    }
+}
+
 /* find_index, src/alm/almDevCall.cu:12-26 */
 static int find_index(const double *seq, double v, int size) {
     if (v < seq[0]) return -1;
@@ -1088,7 +1110,10 @@ int orc_alm_step(void *h, const double *u_, double *ff_, double t) {
         const falm_turbine &T = A->tb[tid];
         double ang[3] = {0, 0, 0}, om[3] = {0, 0, 0};
         if (T.angle_type) {
-            ang[T.angle_type - 1] = T.motion[0] * sin(T.motion[1] * t + T.motion[2]);
+            /* the static part of the platform angle has no reference
+             * counterpart (0 by default) */
+            const double sa = T.motion[0] * sin(T.motion[1] * t + T.motion[2]);
+            ang[T.angle_type - 1] = T.angle_offset_deg != 0.0 ? deg2rad(T.angle_offset_deg) + sa : sa;
             om[T.angle_type - 1] = T.motion[0] * T.motion[1] * cos(T.motion[1] * t + T.motion[2]);
         }
         for (int m = 0; m < 3; m++) {
@@ -1114,7 +1139,7 @@ int orc_alm_step(void *h, const double *u_, double *ff_, double t) {
         c1[2] += A->r[ap] * sin(theta);
         double nang[3] = {-A->angle[3 * tid], -A->angle[3 * tid + 1], -A->angle[3 * tid + 2]};
         double c0[3];
-        rot(c1, nang, T.angle_type, c0);
+        yawed_to_inertial(c1, T.yaw_deg, nang, T.angle_type, c0);
         for (int m = 0; m < 3; m++) c0[m] += A->base[3 * tid + m];
         for (int m = 0; m < 3; m++) A->ap_xyz[3 * ap + m] = c0[m];
         int I = find_index(A->x.data(), c0[0], D.nx);
@@ -1154,6 +1179,7 @@ int orc_alm_step(void *h, const double *u_, double *ff_, double t) {
             vrel[m] = uap[m] - A->base_vel[3 * tid + m];
         }
         rot_dt(rel, vrel, &A->angle[3 * tid], &A->angvel[3 * tid], T.angle_type, uvw_tt);
+        platform_to_rotor(uvw_tt, T.yaw_deg);
         double theta0 = (2 * PI / A->nb) * bid;
         double theta = T.tip_rate * t + theta0;
         double ux = uvw_tt[0];
@@ -1178,7 +1204,7 @@ int orc_alm_step(void *h, const double *u_, double *ff_, double t) {
         double ftt[3] = {-fx, ft * sin(theta), -ft * cos(theta)};
         double nang[3] = {-A->angle[3 * tid], -A->angle[3 * tid + 1], -A->angle[3 * tid + 2]};
         double F[3];
-        rot(ftt, nang, T.angle_type, F);
+        yawed_to_inertial(ftt, T.yaw_deg, nang, T.angle_type, F);
         for (int m = 0; m < 3; m++) A->ap_force[3 * ap + m] = F[m];
         A->ap_torque[ap] = fabs(ft) * A->r[ap];
         A->ap_thrust[ap] = fx;
@@ -1719,7 +1745,8 @@ static void adj_alm(OrcAlm *A, const double *u_, double t, const double *ffbar_,
         const falm_turbine &T = A->tb[tid];
         double ang[3] = {0, 0, 0}, om[3] = {0, 0, 0};
         if (T.angle_type) {
-            ang[T.angle_type - 1] = T.motion[0] * sin(T.motion[1] * t + T.motion[2]);
+            const double sa = T.motion[0] * sin(T.motion[1] * t + T.motion[2]);
+            ang[T.angle_type - 1] = T.angle_offset_deg != 0.0 ? deg2rad(T.angle_offset_deg) + sa : sa;
             om[T.angle_type - 1] = T.motion[0] * T.motion[1] * cos(T.motion[1] * t + T.motion[2]);
         }
         const double *sv = &A->set_vel[3 * tid];
@@ -1732,7 +1759,7 @@ static void adj_alm(OrcAlm *A, const double *u_, double t, const double *ffbar_,
         const double theta = T.tip_rate * t + (2 * PI / A->nb) * bid;
         double c1[3] = {T.hub[0], T.hub[1] + A->r[ap] * cos(theta), T.hub[2] + A->r[ap] * sin(theta)};
         double P[3];
-        rot(c1, nang, T.angle_type, P);
+        yawed_to_inertial(c1, T.yaw_deg, nang, T.angle_type, P);
         for (int m = 0; m < 3; m++) P[m] += base[m];
         const int I = find_index(A->x.data(), P[0], D.nx), Jj = find_index(A->y.data(), P[1], D.ny),
                   K = find_index(A->z.data(), P[2], D.nz);
@@ -1759,6 +1786,7 @@ static void adj_alm(OrcAlm *A, const double *u_, double t, const double *ffbar_,
             vrel[m] = uap[m] - bvel[m];
         }
         rot_dt(rel, vrel, ang, om, T.angle_type, utt);
+        platform_to_rotor(utt, T.yaw_deg);
         const double ux = utt[0];
         const double ut = T.tip_rate * A->r[ap] + utt[1] * sin(theta) - utt[2] * cos(theta);
         const double urel2 = ux * ux + ut * ut;
@@ -1795,6 +1823,11 @@ static void adj_alm(OrcAlm *A, const double *u_, double t, const double *ffbar_,
         /* F = R(nang) (-fx, ft sin(theta), -ft cos(theta)) */
         double fttb[3];
         rot_T(Fb, nang, T.angle_type, fttb);
+        if (T.yaw_deg != 0.0) { /* transpose of the yaw turn */
+            const double ny[3] = {0.0, 0.0, -deg2rad(T.yaw_deg)};
+            const double w[3] = {fttb[0], fttb[1], fttb[2]};
+            rot_T(w, ny, 3, fttb);
+        }
         double fxb = -fttb[0];
         double ftb = fttb[1] * sin(theta) - fttb[2] * cos(theta);
         ftb += wP * T.tip_rate * A->r[ap] * ((ft > 0) - (ft < 0));  /* power: Omega |ft| r */
@@ -1812,7 +1845,12 @@ static void adj_alm(OrcAlm *A, const double *u_, double t, const double *ffbar_,
         const double den = ux * ux + ut * ut;
         double uxb = phib * ut / den + 2 * ux * urel2b;
         double utb = -phib * ux / den + 2 * ut * urel2b;
-        const double uttb[3] = {uxb, sin(theta) * utb, -cos(theta) * utb};
+        double uttb[3] = {uxb, sin(theta) * utb, -cos(theta) * utb};
+        if (T.yaw_deg != 0.0) { /* transpose of platform_to_rotor */
+            const double py[3] = {0.0, 0.0, deg2rad(T.yaw_deg)};
+            const double w[3] = {uttb[0], uttb[1], uttb[2]};
+            rot_T(w, py, 3, uttb);
+        }
         double vrelb[3];
         rot_T(uttb, ang, T.angle_type, vrelb);
         for (int q = 0; q < 8; q++)
@@ -2042,6 +2080,16 @@ int orc_alm_set_pitch(void *h, int turbine, const double *pitch_deg) {
     OrcAlm *A = (OrcAlm *)h;
     if (turbine < 0 || turbine >= A->nt) return FALM_ERR_INVALID;
     for (int b = 0; b < 8; b++) A->tb[turbine].pitch_deg[b] = pitch_deg ? pitch_deg[b] : 0.0;
+    return FALM_OK;
+}
+
+/* static yaw and platform angle offset of one turbine (degrees; no
+ * reference counterpart) */
+int orc_alm_set_attitude(void *h, int turbine, double yaw_deg, double angle_offset_deg) {
+    OrcAlm *A = (OrcAlm *)h;
+    if (turbine < 0 || turbine >= A->nt) return FALM_ERR_INVALID;
+    A->tb[turbine].yaw_deg = yaw_deg;
+    A->tb[turbine].angle_offset_deg = angle_offset_deg;
     return FALM_OK;
 }
 

@@ -89,6 +89,7 @@ def _load(path):
         "orc_case_window": [vp, ip, dp],
         "orc_case_adjoint": [vp, ip, ip, C.c_double, dp, dp, dp, dp],
         "orc_alm_set_pitch": [vp, ip, dp],
+        "orc_alm_set_attitude": [vp, ip, C.c_double, C.c_double],
         "orc_alm_set_base": [vp, ip, dp, dp],
         "orc_alm_vjp": [vp, vp, C.c_double, vp, C.c_double, vp, vp, vp],
         "orc_adj_sgs": [dom, C.c_double, vp, vp, vp, vp, vp],
@@ -276,6 +277,9 @@ class Alm:
     def step(self, u, ff, t):
         check(lib().orc_alm_step(self.h, ptr(u), ptr(ff), t))
 
+    def set_attitude(self, turbine, yaw_deg, angle_offset_deg):
+        check(lib().orc_alm_set_attitude(self.h, turbine, float(yaw_deg), float(angle_offset_deg)))
+
     def set_twist_offset(self, off):
         check(lib().orc_alm_set_twist_offset(self.h, ptr(np.ascontiguousarray(off, dtype=np.float64))))
 
@@ -348,6 +352,9 @@ class Case:
         v = np.zeros(8)
         v[:len(pitch_deg)] = pitch_deg
         check(lib().orc_alm_set_pitch(self._alm_handle(), turbine, v.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def set_attitude(self, turbine, yaw_deg, angle_offset_deg):
+        check(lib().orc_alm_set_attitude(self._alm_handle(), turbine, float(yaw_deg), float(angle_offset_deg)))
 
     def set_twist_offset(self, off):
         check(lib().orc_alm_set_twist_offset(self._alm_handle(), ptr(np.ascontiguousarray(off, dtype=np.float64))))

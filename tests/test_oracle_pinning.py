@@ -610,9 +610,10 @@ def test_build_ap_matches_interpolation():
     assert np.allclose(out[1], np.interp(r, rr, ch)) and np.allclose(out[2], np.interp(r, rr, tw))
 
 
-def _one_point_rotor(h, tsr, twist, pitch_type=0, motion=(0, 0, 0)):
+def _one_point_rotor(h, tsr, twist, pitch_type=0, motion=(0, 0, 0), yaw_deg=0.0, angle_offset_deg=0.0):
     setup = cases.rotor(h, base=(2.0, 2.0, 2.0), radius=0.5, tsr=tsr, n_blade=1, n_ap_per_blade=1,
-                        twist=(twist, twist), chord=(0.1, 0.1), angle_type=pitch_type, motion=motion)
+                        twist=(twist, twist), chord=(0.1, 0.1), angle_type=pitch_type, motion=motion,
+                        yaw_deg=yaw_deg, angle_offset_deg=angle_offset_deg)
     return setup
 
 
@@ -673,6 +674,47 @@ def test_alm_pitch_rotation_keeps_radius():
     assert abs(d[1] - 0.25 * math.cos(theta)) < 1e-14
     # rotation about y by -psi of (0, r cos, r sin): x = sin(psi)*r sin(theta)
     assert abs(d[0] - math.sin(psi) * 0.25 * math.sin(theta)) < 1e-14
+
+
+def test_alm_yaw_turns_rotor_about_vertical_axis():
+    """A yawed rotor (no reference counterpart) in a uniform stream U along
+    x: its point lies on the yawed rotor plane, (x, y) turned by the yaw
+    angle about the base, z unchanged; in the rotor's frame the stream is
+    (U cos(yaw), -U sin(yaw), 0), so the point's thrust and torque equal
+    those of an unyawed rotor in that stream, and its force on the fluid is
+    that rotor's force turned by the yaw.  With a static platform pitch
+    offset a and yaw 0, the point is the unpitched one turned about y by a."""
+    h = 1.0 / 16
+    dom = domain((64, 64, 64))
+    cen = [(np.arange(dom.shape[m]) - 1.5) * h for m in range(3)]
+    t = 0.2
+    psi = 25.0
+    c, s_ = math.cos(math.radians(psi)), math.sin(math.radians(psi))
+    ones = np.ones(dom.ncell)
+    zero = np.zeros(dom.ncell)
+
+    def run(setup, U, V=0.0):
+        A = O.Alm(dom, setup, *cen)
+        ff = np.zeros(3 * dom.ncell)
+        A.step(np.concatenate([U * ones, V * ones, zero]), ff, t)
+        return A.get()
+
+    xy, fy, tqy, thy = run(_one_point_rotor(h, tsr=4.0, twist=5.0, yaw_deg=psi), 1.0)
+    x0, f0, tq0, th0 = run(_one_point_rotor(h, tsr=4.0, twist=5.0), c, -s_)
+    d, d0 = xy[0] - 2.0, x0[0] - 2.0
+    # rotor frame -> inertial: turned by -yaw about z (x' = c x - s y, y' = s x + c y)
+    assert np.allclose(d, [c * d0[0] - s_ * d0[1], s_ * d0[0] + c * d0[1], d0[2]], atol=1e-14, rtol=0)
+    assert abs(tqy[0] - tq0[0]) < 1e-12 * abs(tq0[0]) and abs(thy[0] - th0[0]) < 1e-12 * abs(th0[0])
+    F0 = f0[0]
+    assert np.allclose(fy[0], [c * F0[0] - s_ * F0[1], s_ * F0[0] + c * F0[1], F0[2]], atol=1e-13 * abs(F0).max())
+    # a static platform pitch offset a (angle_type 2, no oscillation)
+    a = 7.0
+    xp, _, _, _ = run(_one_point_rotor(h, tsr=4.0, twist=5.0, pitch_type=2, angle_offset_deg=a), 1.0)
+    x1, _, _, _ = run(_one_point_rotor(h, tsr=4.0, twist=5.0), 1.0)
+    ca, sa = math.cos(math.radians(a)), math.sin(math.radians(a))
+    d, d0 = xp[0] - 2.0, x1[0] - 2.0
+    # rot() about y by -a: x' = c x + s z, z' = -s x + c z
+    assert np.allclose(d, [ca * d0[0] + sa * d0[2], d0[1], -sa * d0[0] + ca * d0[2]], atol=1e-14, rtol=0)
 
 
 def test_alm_velocity_sampling_exact_for_linear_field():
