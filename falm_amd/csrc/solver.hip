@@ -119,7 +119,9 @@ __global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, OP a, const
 // Eight waves per SIMD (64 VGPRs at most): four workgroups per CU, so the
 // 1024 tiles of a 512 x 128 x 128 interior run in one round instead of
 // three workgroups per CU and a partly empty second round.
-template <class T, class OP>
+// CHECK: only |b - A x|^2 (the mixed solve's test after a pass; nothing is
+// stored, and a pass that must follow starts with the full kernel).
+template <class T, class OP, bool CHECK = false>
 __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(8))) void k_bicg_init_km(
     K7Tiling kt, Geom g, OP a, const double *__restrict__ x,
                                                         const double *__restrict__ b, double *r64, T *r, T *rr,
@@ -152,12 +154,14 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(8))) void
         }
         if (Q.act) {
             const double rv = bc - ca.apply(cc, px.c(), px.xp(bf), px.xm(bf), px.yp(bf), px.ym(bf), px.zp(), px.zm());
-            if (r64) st_at(r64, od, rv);
-            const T rt = (T)rv;
-            if ((void *)r != (void *)r64) st_at(r, ot, rt);
-            st_at(rr, ot, rt);
             v[0] += rv * rv;
-            v[1] += (double)rt * (double)rt;
+            if (!CHECK) {
+                if (r64) st_at(r64, od, rv);
+                const T rt = (T)rv;
+                if ((void *)r != (void *)r64) st_at(r, ot, rt);
+                st_at(rr, ot, rt);
+                v[1] += (double)rt * (double)rt;
+            }
         }
         px.advance(bf, more);
         bf ^= 1;
@@ -645,6 +649,20 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_init(DevState *st, const double 
         }
         st->passes += 1;
         st->done = 0;
+    }
+}
+
+// The mixed solve's test after a pass: the true residual's norm ends the
+// solve (tolerance met, or maxit spent); otherwise the pass is over and the
+// host starts the next one with the full init kernels.
+__global__ __launch_bounds__(FIN_NT) void k_fin_check(DevState *st, const double *partials, int nb) {
+    if (st->final || !st->done) return;
+    double tot[2];
+    sum_partials<2>(partials, nb, tot);
+    if (threadIdx.x == 0) {
+        const double err = sqrt(tot[0]) / st->gN;
+        st->err = err;
+        if ((st->passes > 0 && err <= st->tol) || st->total >= st->maxit) st->final = 1;
     }
 }
 
@@ -1164,7 +1182,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     FALM_CHECK_LAUNCH("k_bicg_mvdot");
     T *pcs = ss;
     if (vsor) {
-        TimedLaunch tm(FALM_TIME_SOR, s);
+        TimedLaunch tm(FALM_TIME_SOR_S, s);
         hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)nullptr,
                            (const T *)q, sv, ss, (const DevState *)S->st, wpc);
         // r, q (and the diagonal) read, s and s^ written
@@ -1179,7 +1197,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         pcs = pc_rest<T, OP>(S, a, ss, sv, pcp == pp ? ss : pp, s, pst);
         FALM_TRY(pst);
     }
-    TimedLaunch tm3(FALM_TIME_MV, s);
+    TimedLaunch tm3(FALM_TIME_MV2, s);
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 2>), FALM_KM(S), s, S->kt, g, a, (const T *)pcs, t, (const T *)sv,
                        (const DevState *)S->st, S->partials);
     tm3.done((double)tl.box.size() * (3.0 * sizeof(T) + diag_bytes<OP>(S)));
@@ -1213,6 +1231,18 @@ static falm_status launch_init(falm_solver *S, const OP64 &a, const double *x, c
              ((void *)rT != (void *)r64 ? sizeof(T) : 0.0) + sizeof(T)));
     hipLaunchKernelGGL(k_fin_init, FALM_FIN, s, S->st, (const double *)S->partials, S->kt.blocks);
     FALM_CHECK_LAUNCH("k_bicg_init");
+    return FALM_OK;
+}
+
+// the mixed solve's test after a pass (k_bicg_init_km<CHECK>): x, b read
+template <class OP64>
+static falm_status launch_check(falm_solver *S, const OP64 &a, const double *x, const double *b, hipStream_t s) {
+    TimedLaunch tm(FALM_TIME_SOLVER_INIT, s);
+    hipLaunchKernelGGL((k_bicg_init_km<float, OP64, true>), FALM_KM(S), s, S->kt, S->g, a, x, b, nullptr,
+                       (float *)nullptr, (float *)nullptr, (const DevState *)S->st, S->partials);
+    tm.done((double)S->tl.box.size() * (2.0 * sizeof(double) + diag_bytes<OP64>(S)));
+    hipLaunchKernelGGL(k_fin_check, FALM_FIN, s, S->st, (const double *)S->partials, S->kt.blocks);
+    FALM_CHECK_LAUNCH("k_bicg_init_km<check>");
     return FALM_OK;
 }
 
@@ -1261,10 +1291,15 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
     int batch = first_batch(S), polls = 0;
     while (true) {
         for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, polls == 0 && k == 0)));
-        FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
+        // a finished pass is tested with the norm alone, unless the caller
+        // wants the final residual; a pass that must follow starts from the
+        // full init (it recomputes the same residual)
+        if (r) FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
+        else FALM_TRY((launch_check<OP64>(S, a64, x, b, s)));
         FALM_TRY(read_state(S, s));
         polls++;
         if (S->hst->final) break;
+        if (!r && S->hst->done) FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
         batch = check_period(S);
         if (polls > 2 * c.maxit + 2) {
             set_error("falm_solver_solve: mixed solve made no progress");

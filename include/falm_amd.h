@@ -514,7 +514,9 @@ falm_status falm_alm_set_base(falm_alm *alm, int turbine, const double *base, co
 enum { FALM_TIME_NONE = 0, FALM_TIME_SOR = 1, FALM_TIME_MV = 2, FALM_TIME_PSEUDO_U = 3,
        FALM_TIME_ALM = 4, FALM_TIME_SOLVER_UPDATE = 5, FALM_TIME_PROJECT = 6,
        FALM_TIME_FACE_FLUX = 7, FALM_TIME_SOLVER_INIT = 8, FALM_TIME_SGS = 9,
-       FALM_TIME_RHS = 10, FALM_TIME_DIVERGENCE = 11 };
+       FALM_TIME_RHS = 10, FALM_TIME_DIVERGENCE = 11,
+       FALM_TIME_SOR_S = 12,   /* the s vector's fused preconditioner kernel */
+       FALM_TIME_MV2 = 13 };   /* the second operator product per iteration  */
 falm_status falm_timing_enable(int family);
 falm_status falm_timing_read(int *launches, double *ms, double *bytes);
 
