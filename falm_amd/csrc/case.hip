@@ -452,6 +452,19 @@ falm_status falm_case_set_field(falm_case *C, int field, const double *host_in, 
     return FALM_OK;
 }
 
+falm_status falm_case_set_tolerance(falm_case *C, double tol, int maxit) {
+    if (!C || !(tol > 0.0) || maxit < 1) {
+        set_error("falm_case_set_tolerance: bad tolerance or iteration limit");
+        return FALM_ERR_INVALID;
+    }
+    FALM_HIP(hipStreamSynchronize(C->stream));
+    FALM_TRY(solver_set_tol(C->solver, tol));
+    FALM_TRY(solver_set_maxit(C->solver, maxit));
+    C->cfg.solver.tol = tol;
+    C->cfg.solver.maxit = maxit;
+    return FALM_OK;
+}
+
 falm_status falm_case_domain(falm_case *C, falm_domain *dom) {
     if (!C || !dom) return FALM_ERR_INVALID;
     *dom = C->dom;

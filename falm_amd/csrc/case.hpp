@@ -43,4 +43,5 @@ falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power);
 falm_status case_dalloc(falm_case *C, void **p, size_t bytes);
 // stopping tolerance of a solver's next solves (|r|/N), replacing its config's
 falm_status solver_set_tol(falm_solver *S, double tol);
+falm_status solver_set_maxit(falm_solver *S, int maxit);
 }  // namespace falm

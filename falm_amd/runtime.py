@@ -77,6 +77,7 @@ def _sigs(L):
         "falm_case_adjoint": [vp, ip, ip, dp, C.POINTER(dp), vp, vp, C.POINTER(C.c_size_t)],
         "falm_alm_set_pitch": [vp, ip, vp],
         "falm_alm_set_attitude": [vp, ip, dp, dp],
+        "falm_case_set_tolerance": [vp, dp, ip],
         "falm_case_adjoint_attitude": [vp, vp, vp],
         "falm_alm_set_base": [vp, ip, vp, vp, vp],
         "falm_mesher_build": [C.c_char_p, C.c_char_p, C.c_char_p, ip],
@@ -243,6 +244,10 @@ class Case:
         v = np.zeros(8)
         v[:len(pitch_deg)] = pitch_deg
         call("falm_alm_set_pitch", self._alm_handle(), turbine, v.ctypes.data_as(C.c_void_p))
+
+    def set_tolerance(self, tol, maxit):
+        """Poisson tolerance (|r|/N) and iteration limit from the next step on."""
+        call("falm_case_set_tolerance", self.h, float(tol), int(maxit))
 
     def set_attitude(self, turbine, yaw_deg, angle_offset_deg):
         """Static yaw and platform angle offset of one turbine, degrees."""

@@ -410,6 +410,10 @@ falm_status falm_case_set_field(falm_case *c, int field, const double *host_in, 
 falm_status falm_case_domain(falm_case *c, falm_domain *dom);
 falm_status falm_case_alm(falm_case *c, falm_alm **alm);
 falm_status falm_case_destroy(falm_case *c);
+/* The step driver's Poisson tolerance and iteration limit from the next step
+ * on (FalmEq's public tol and maxit, which the reference reads on every
+ * solve, src/FalmEq.h:56-64). */
+falm_status falm_case_set_tolerance(falm_case *c, double tol, int maxit);
 
 /* ------------------------------------------------- rotor-power adjoint
  * No reference counterpart (the reference has no adjoint).  The objective is
