@@ -142,6 +142,54 @@ def calibrate_copy(nbytes=1 << 30, reps=3):
     return {"kernel": "k_scale", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps}
 
 
+def stream_reference(shape, reps=20):
+    """What plain streaming reaches on this box at the benchmark's array
+    size: falm_vecaxby (r = a x + b y, one element per thread; two arrays
+    read, one written) over the padded 512x128x128 field in float and double,
+    and falm_scale over 1 GiB of doubles, timed with HIP events on the null
+    stream the calls run on."""
+    import ctypes as C
+
+    import torch
+
+    from falm_amd import _abi as abi
+    from falm_amd import runtime
+
+    n = shape[0] * shape[1] * shape[2]
+    out = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3  # s
+
+    for name, prec, tb in (("float", abi.F32, 4), ("double", abi.F64, 8)):
+        bufs = [C.c_void_p() for _ in range(3)]
+        for b in bufs:
+            runtime.call("falm_malloc", C.byref(b), n * tb)
+        t = timed(lambda: runtime.call("falm_vecaxby", prec, 1.0, bufs[0], 0.5, bufs[1], bufs[2], n, None))
+        gbs = 3 * n * tb / t / 1e9
+        out[f"vecaxby_{name}"] = {"bytes_per_launch": 3 * n * tb, "avg_launch_us": t * 1e6, "gbs": gbs,
+                                  "frac": gbs / PEAK_HBM_GBS}
+        for b in bufs:
+            runtime.call("falm_free", b)
+    nb = 1 << 30
+    p = C.c_void_p()
+    runtime.call("falm_malloc", C.byref(p), nb)
+    t = timed(lambda: runtime.call("falm_scale", abi.F64, p, nb // 8, 1.0, None))
+    runtime.call("falm_free", p)
+    gbs = 2 * nb / t / 1e9
+    out["scale_double_1GiB"] = {"bytes_per_launch": 2 * nb, "avg_launch_us": t * 1e6, "gbs": gbs,
+                                "frac": gbs / PEAK_HBM_GBS}
+    return out
+
+
 def cpu_baseline(n, steps=2, warmup=1):
     """The oracle (FP64 C++ restatement of the reference scheme, OpenMP) on
     the host cores: a bounded sample of the same workload."""
@@ -268,7 +316,7 @@ def run_lid3d(n, steps, tol):
     }
 
 
-def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5):
+def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5, fd_iters=100):
     """The floating tandem-rotor segment (two rotors 7 D apart, the upstream
     one in prescribed surge and pitch, 8 % inlet turbulence, CFL 0.1) in
     mixed precision and in FP64 for `steps` steps from the same start: MLUPS,
@@ -334,6 +382,14 @@ def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5):
     # differences' signal at h = 1e-4 rad.
     c.set_tolerance(adj_tol, 20000)
     c.step(settle)
+    # In the window every Poisson solve runs a fixed number of BiCGStab
+    # iterations, so that J is a smooth function of the parameters (a
+    # tolerance test lets the stopping iteration jump between the central
+    # differences' two runs); fd_iters takes |r|/N well below adj_tol.
+    c.set_tolerance(1e-30, fd_iters)
+    info = c.step(1)
+    fixed_err = info.solver_err
+    note(f"fixed {fd_iters} iterations per solve: |r|/N {fixed_err:.3e}")
     t0 = time.perf_counter()
     J0 = c.window(window)
     t_fwd = time.perf_counter() - t0
@@ -358,7 +414,9 @@ def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5):
     c.close()
     out["adjoint"] = {
         "objective": f"mean summed power of both rotors over {window} FP64 steps after the FP64 run's {steps} "
-                     f"steps and {settle} more at Poisson tol {adj_tol:g}",
+                     f"steps, {settle} more at Poisson tol {adj_tol:g} and one more; every Poisson solve in the "
+                     f"window runs {fd_iters} BiCGStab iterations",
+        "poisson_err_after_fixed_iterations": fixed_err,
         "J": J, "J_forward_window": J0,
         "grad_yaw_per_deg": [float(x) for x in gy], "grad_pitch_offset_per_deg": [float(x) for x in go],
         "grad_blade_pitch_per_deg": [float(x) for x in gp[0, :3]],
@@ -586,6 +644,7 @@ def main():
 
     acc = run_accuracy(n, args.accuracy_steps, args.accuracy_cfl, args.tol) if args.accuracy_steps > 0 else None
     lid = run_lid3d(args.lid3d_n, args.lid3d_steps, 1e-6) if args.lid3d_steps > 0 else None
+    stream = stream_reference(setup.shape)
     cpu = None if args.no_cpu else cpu_baseline(n, steps=args.cpu_steps)
 
     out = {
@@ -617,6 +676,7 @@ def main():
         "velocity_rel_l2_mixed_vs_fp64": rel_l2,
         "solver_tol_leg": tol_leg,
         "roofline": roofline,
+        "stream_reference": stream,
         "accuracy": acc,
         "lid3d_mixed_vs_fp64": lid,
         "cpu_baseline": cpu,

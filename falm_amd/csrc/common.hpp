@@ -290,20 +290,23 @@ __device__ inline void block_partials(double (&v)[NV], double *partials, double 
 
 constexpr int FIN_NT = 1024;  // threads of a finalize workgroup
 
-// Phase 2 helper for a FIN_NT-thread workgroup: totals of nb partials per
-// value, valid in every thread.
-template <int NV>
+// Phase 2 helper for an NT-thread workgroup: totals of nb partials per
+// value, valid in every thread.  Used by the one-workgroup finalize kernels,
+// and by consumer kernels whose every workgroup needs a reduction's result:
+// each sums the same partials in the same order, so all get the same bits
+// (no finalize launch between producer and consumer).
+template <int NV, int NT = FIN_NT>
 __device__ inline void sum_partials(const double *partials, int nb, double (&tot)[NV]) {
-    __shared__ double lds[NV * (FIN_NT / 64)];
+    __shared__ double lds[NV * (NT / 64)];
     __shared__ double bc[NV];
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; q++) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += FIN_NT) s += partials[(size_t)q * nb + b];
+        for (int b = threadIdx.x; b < nb; b += NT) s += partials[(size_t)q * nb + b];
         acc[q] = s;
     }
-    block_sum_n<FIN_NT, NV>(acc, lds);
+    block_sum_n<NT, NV>(acc, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; q++) bc[q] = acc[q];

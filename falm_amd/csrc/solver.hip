@@ -228,7 +228,7 @@ template <class T, class OP, int MODE>
 __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
                                                         const T *__restrict__ vin, const T *__restrict__ q,
                                                         T *__restrict__ v, T *__restrict__ vh, const DevState *st,
-                                                        T omega_pc) {
+                                                        T omega_pc, const double *apart, int anb, DevState *stw) {
     if (st->done) return;
     __shared__ K7Lds<T> lw;
     const K7Pos Q = k7_pos(kt, g);
@@ -238,6 +238,14 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
     if (MODE == 0) {
         c1 = (T)st->beta;
         c2 = (T)st->omega;
+    } else if (apart) {
+        // alpha = rho / (rr, q) from the operator kernel's partial sums
+        // (what k_fin_alpha computes), in every workgroup; the first stores it
+        double tot[1];
+        sum_partials<1, K7_NT>(apart, anb, tot);
+        const double alpha = st->rho / tot[0];
+        if (blockIdx.x == 0 && threadIdx.x == 0) stw->alpha = alpha;
+        c1 = (T)alpha;
     } else {
         c1 = (T)st->alpha;
     }
@@ -830,6 +838,9 @@ __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *_
 // index ip covers cells lo0 + 2 ip and lo0 + 2 ip + 1 < hi0, a last single
 // cell is done alone.  Needs even cell offsets at every pair (lo0 and nx
 // even) and 16-byte aligned x; the sums run per thread over its pairs.
+// workgroups of the pair update (each sums the second operator product's
+// partials, so fewer and longer-running ones)
+constexpr int UPD_BLOCKS = 1024;
 template <class T> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
@@ -839,13 +850,26 @@ __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, int hi0
                                                      const T *__restrict__ pp, const T *__restrict__ ss,
                                                      const T *__restrict__ s, const T *__restrict__ t,
                                                      T *__restrict__ r, const T *__restrict__ rr,
-                                                     const DevState *st, double *partials) {
+                                                     const DevState *st, double *partials, const double *opart,
+                                                     int onb, DevState *stw) {
     using T2 = typename Pair<T>::type;
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int ip, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
-    const double alpha = st->alpha, omega = st->omega;
+    const double alpha = st->alpha;
+    double omega;
+    if (opart) {
+        // omega = (t, s) / (t, t) from the second operator kernel's partial
+        // sums (what k_fin_omega computes), in every workgroup; the first
+        // stores it
+        double tot[2];
+        sum_partials<2, NT>(opart, onb, tot);
+        omega = tot[0] / tot[1];
+        if (blockIdx.x == 0 && threadIdx.x == 0) stw->omega = omega;
+    } else {
+        omega = st->omega;
+    }
     const T om = (T)omega;
     tile_range(tp.tiles, t0, t1);
     for (int tile = t0; tile < t1; tile++) {
@@ -1030,6 +1054,7 @@ struct falm_solver {
     DevState *st = nullptr;
     DevState *hst = nullptr;  // pinned mirror
     double *partials = nullptr;
+    double *partials2 = nullptr;  // the second operator product's partial sums (read by the update kernel)
     int *flag = nullptr, *hflag = nullptr;
     // Krylov vectors: double (F64) or float (MIXED)
     void *rr = nullptr, *p = nullptr, *q = nullptr, *s = nullptr, *pp = nullptr, *ss = nullptr, *t = nullptr,
@@ -1085,6 +1110,7 @@ static falm_status solver_alloc(falm_solver *S) {
     size_t nb = (size_t)(S->tl.tiles > 0 ? red_blocks(S->tl) : 1);
     if ((size_t)S->kt.blocks > nb) nb = (size_t)S->kt.blocks;
     FALM_HIP(hipMalloc(&S->partials, 2 * nb * sizeof(double)));
+    FALM_HIP(hipMalloc(&S->partials2, 2 * nb * sizeof(double)));
     // the zero fills above run on the null stream, which the caller's
     // (possibly non-blocking) stream does not wait for
     FALM_HIP(hipDeviceSynchronize());
@@ -1217,7 +1243,8 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         S->pflip ^= 1;
         TimedLaunch tm(FALM_TIME_SOR, s);
         hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 0>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)pin,
-                           (const T *)q, pout, pp, (const DevState *)S->st, wpc);
+                           (const T *)q, pout, pp, (const DevState *)S->st, wpc, (const double *)nullptr, 0,
+                           (DevState *)nullptr);
         // r (and the diagonal) read, p and p^ written; after a solve's first
         // iteration also p and q read
         tm.done((double)tl.box.size() * ((first ? 3.0 : 5.0) * sizeof(T) + diag_bytes<OP>(S)));
@@ -1234,13 +1261,15 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)pcp, q, (const T *)rr,
                        (const DevState *)S->st, S->partials);
     tm1.done((double)tl.box.size() * (3.0 * sizeof(T) + diag_bytes<OP>(S)));
-    hipLaunchKernelGGL(k_fin_alpha, FALM_FIN, s, S->st, (const double *)S->partials, nb);
+    // the fused s kernel computes alpha from these partials itself
+    if (!vsor) hipLaunchKernelGGL(k_fin_alpha, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot");
     T *pcs = ss;
     if (vsor) {
         TimedLaunch tm(FALM_TIME_SOR_S, s);
         hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)nullptr,
-                           (const T *)q, sv, ss, (const DevState *)S->st, wpc);
+                           (const T *)q, sv, ss, (const DevState *)S->st, wpc, (const double *)S->partials, nb,
+                           S->st);
         // r, q (and the diagonal) read, s and s^ written
         tm.done((double)tl.box.size() * (4.0 * sizeof(T) + diag_bytes<OP>(S)));
         FALM_CHECK_LAUNCH("k_bicg_vsor_km");
@@ -1255,17 +1284,21 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     }
     TimedLaunch tm3(FALM_TIME_MV2, s);
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 2>), FALM_KM(S), s, S->kt, g, a, (const T *)pcs, t, (const T *)sv,
-                       (const DevState *)S->st, S->partials);
+                       (const DevState *)S->st, S->partials2);
     tm3.done((double)tl.box.size() * (3.0 * sizeof(T) + diag_bytes<OP>(S)));
-    hipLaunchKernelGGL(k_fin_omega, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot2");
+    // the pair update computes omega from mv2's partials itself (its own go
+    // to the other buffer); the single-cell update reads it from the state
+    const bool pairs = S->tp.tiles > 0 && ((uintptr_t)x & 15) == 0;
+    if (!pairs) hipLaunchKernelGGL(k_fin_omega, FALM_FIN, s, S->st, (const double *)S->partials2, nb);
     TimedLaunch tm2(FALM_TIME_SOLVER_UPDATE, s);
     int nbu = red_blocks(tl);
-    if (S->tp.tiles > 0 && ((uintptr_t)x & 15) == 0) {
-        nbu = red_blocks(S->tp);
-        hipLaunchKernelGGL((k_bicg_update2<T>), red_grid(S->tp), tiling_block(), 0, s, S->tp, tl.box.lo[0],
+    if (pairs) {
+        nbu = S->tp.tiles < UPD_BLOCKS ? S->tp.tiles : UPD_BLOCKS;
+        hipLaunchKernelGGL((k_bicg_update2<T>), dim3((unsigned)nbu), tiling_block(), 0, s, S->tp, tl.box.lo[0],
                            tl.box.lo[0] + tl.box.n[0], g, x, (const T *)pcp, (const T *)pcs, (const T *)sv,
-                           (const T *)t, r, (const T *)rr, (const DevState *)S->st, S->partials);
+                           (const T *)t, r, (const T *)rr, (const DevState *)S->st, S->partials,
+                           (const double *)S->partials2, nb, S->st);
     } else {
         hipLaunchKernelGGL((k_bicg_update<T>), FALM_RED(S), s, tl, g, x, (const T *)pcp, (const T *)pcs,
                            (const T *)sv, (const T *)t, r, (const T *)rr, (const DevState *)S->st, S->partials);
@@ -1632,7 +1665,8 @@ falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void
 falm_status falm_solver_destroy(falm_solver *S) {
     if (!S) return FALM_OK;
     void *bufs[] = {S->rr, S->p, S->q, S->s, S->pp, S->ss, S->t, S->w, S->r32, S->a32,
-                    S->sep64, S->sep32, S->dtab64, S->dtab32, S->st, S->partials, S->flag, S->r64own};
+                    S->sep64, S->sep32, S->dtab64, S->dtab32, S->st, S->partials, S->partials2, S->flag,
+                    S->r64own};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (S->hst) (void)hipHostFree(S->hst);
