@@ -150,24 +150,29 @@ def stream_reference(shape, reps=20):
     stream the calls run on."""
     import ctypes as C
 
-    import torch
-
     from falm_amd import _abi as abi
     from falm_amd import runtime
 
     n = shape[0] * shape[1] * shape[2]
     out = {}
+    hip = hip_runtime()
+    ev = [C.c_void_p(), C.c_void_p()]
+    for e in ev:
+        if hip.hipEventCreate(C.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
 
     def timed(fn):
         fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
+        runtime.call("falm_stream_sync", None)
+        hip.hipEventRecord(ev[0], None)
         for _ in range(reps):
             fn()
-        e1.record()
-        e1.synchronize()
-        return e0.elapsed_time(e1) / reps * 1e-3  # s
+        hip.hipEventRecord(ev[1], None)
+        hip.hipEventSynchronize(ev[1])
+        ms = C.c_float()
+        if hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1]) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return ms.value / reps * 1e-3  # s
 
     for name, prec, tb in (("float", abi.F32, 4), ("double", abi.F64, 8)):
         bufs = [C.c_void_p() for _ in range(3)]
@@ -187,6 +192,8 @@ def stream_reference(shape, reps=20):
     gbs = 2 * nb / t / 1e9
     out["scale_double_1GiB"] = {"bytes_per_launch": 2 * nb, "avg_launch_us": t * 1e6, "gbs": gbs,
                                 "frac": gbs / PEAK_HBM_GBS}
+    for e in ev:
+        hip.hipEventDestroy(e)
     return out
 
 
@@ -316,7 +323,7 @@ def run_lid3d(n, steps, tol):
     }
 
 
-def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5, fd_iters=100):
+def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5):
     """The floating tandem-rotor segment (two rotors 7 D apart, the upstream
     one in prescribed surge and pitch, 8 % inlet turbulence, CFL 0.1) in
     mixed precision and in FP64 for `steps` steps from the same start: MLUPS,
@@ -381,11 +388,14 @@ def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5, fd_ite
     # solver's stopping noise in J is of the size of the central
     # differences' signal at h = 1e-4 rad.
     c.set_tolerance(adj_tol, 20000)
-    c.step(settle)
+    info = c.step(settle)
     # In the window every Poisson solve runs a fixed number of BiCGStab
-    # iterations, so that J is a smooth function of the parameters (a
-    # tolerance test lets the stopping iteration jump between the central
-    # differences' two runs); fd_iters takes |r|/N well below adj_tol.
+    # iterations, the number the last solve needed for adj_tol, so that J is
+    # a smooth function of the parameters: a tolerance test lets the
+    # stopping iteration jump between the central differences' two runs, and
+    # iterating far past convergence makes BiCGStab's iterates (and J) vary
+    # erratically from one parameter value to the next.
+    fd_iters = max(1, info.solver_it)
     c.set_tolerance(1e-30, fd_iters)
     info = c.step(1)
     fixed_err = info.solver_err
@@ -393,7 +403,8 @@ def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5, fd_ite
     t0 = time.perf_counter()
     J0 = c.window(window)
     t_fwd = time.perf_counter() - t0
-    note(f"forward window J = {J0:.8g}, {t_fwd:.2f} s")
+    J0b = c.window(window)  # the window restores the state: the same J again
+    note(f"forward window J = {J0:.8g} (again: {J0b:.8g}), {t_fwd:.2f} s")
     t0 = time.perf_counter()
     J, gp, gt, nbytes = c.adjoint(window, ckpt, 1e-10)
     t_adj = time.perf_counter() - t0
@@ -404,7 +415,7 @@ def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5, fd_ite
         def Jat(e):
             c.set_attitude(0, e if which == "yaw" else 0.0, e if which != "yaw" else 0.0)
             return c.window(window)
-        for hr in (1e-4, 1e-3):
+        for hr in (1e-4, 1e-3, 1e-2):
             h = float(np.degrees(hr))
             fd = (Jat(h) - Jat(-h)) / (2 * h)
             checks.append({"parameter": which, "fd_step_rad": hr, "adjoint_per_deg": float(ad), "fd_per_deg": fd,
@@ -415,9 +426,10 @@ def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5, fd_ite
     out["adjoint"] = {
         "objective": f"mean summed power of both rotors over {window} FP64 steps after the FP64 run's {steps} "
                      f"steps, {settle} more at Poisson tol {adj_tol:g} and one more; every Poisson solve in the "
-                     f"window runs {fd_iters} BiCGStab iterations",
+                     f"window runs {fd_iters} BiCGStab iterations (the count the last solve at tol {adj_tol:g} "
+                     f"needed)",
         "poisson_err_after_fixed_iterations": fixed_err,
-        "J": J, "J_forward_window": J0,
+        "J": J, "J_forward_window": J0, "J_forward_window_repeat": J0b,
         "grad_yaw_per_deg": [float(x) for x in gy], "grad_pitch_offset_per_deg": [float(x) for x in go],
         "grad_blade_pitch_per_deg": [float(x) for x in gp[0, :3]],
         "checks": checks,
@@ -429,8 +441,9 @@ def run_farm(n, steps, window, ckpt, fam_steps=2, adj_tol=1e-9, settle=5, fd_ite
     return out
 
 
-def device_free_bytes():
-    """hipMemGetInfo through the HIP runtime the library already loaded."""
+def hip_runtime():
+    """The HIP runtime the library already loaded (the same instance, so its
+    device and streams are the library's)."""
     import ctypes as C
 
     path = None
@@ -439,7 +452,16 @@ def device_free_bytes():
             if "libamdhip64.so" in line:
                 path = line.split()[-1]
                 break
-    hip = C.CDLL(path)
+    if path is None:
+        raise RuntimeError("libamdhip64 is not loaded")
+    return C.CDLL(path)
+
+
+def device_free_bytes():
+    """hipMemGetInfo through the HIP runtime the library already loaded."""
+    import ctypes as C
+
+    hip = hip_runtime()
     free, total = C.c_size_t(), C.c_size_t()
     if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
         raise RuntimeError("hipMemGetInfo failed")

@@ -192,7 +192,10 @@ __global__ __launch_bounds__(256) void k_pseudo_u(Tiling tl, Geom g, T ReI, T dt
 // k_pseudo_u, so the results are identical.  The next plane's loads are
 // issued before the current plane is computed, and the LDS planes are
 // double-buffered, so one barrier per plane suffices.
-constexpr int KM_BX = 64, KM_BY = 8, KM_KC = 16;
+#ifndef FALM_KM_KC
+#define FALM_KM_KC 16
+#endif
+constexpr int KM_BX = 64, KM_BY = 8, KM_KC = FALM_KM_KC;
 constexpr int KM_NT = KM_BX * KM_BY;
 constexpr int KM_UX = 4 * KM_BY, KM_UY = 4 * KM_BX;   // u halo cells per component
 constexpr int KM_NX = 2 * KM_BY, KM_NY = 2 * KM_BX;   // nut halo cells

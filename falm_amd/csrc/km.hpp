@@ -27,13 +27,17 @@ struct K7Tiling {
     int blocks;
 };
 
-// planes per chunk: 16, doubled until the grid fits max_blocks workgroups
+// planes per chunk: K7_KC, doubled until the grid fits max_blocks workgroups
+#ifndef FALM_K7_KC
+#define FALM_K7_KC 16
+#endif
+constexpr int K7_KC = FALM_K7_KC;
 inline K7Tiling make_k7_tiling(const Box &b, int max_blocks = RED_BLOCKS) {
     K7Tiling t;
     t.box = b;
     t.tx = (b.n[0] + K7_BX - 1) / K7_BX;
     t.ty = (b.n[1] + K7_BY - 1) / K7_BY;
-    t.kc = 16;
+    t.kc = K7_KC;
     for (;;) {
         t.tk = (b.n[2] + t.kc - 1) / t.kc;
         t.blocks = t.tx * t.ty * t.tk;

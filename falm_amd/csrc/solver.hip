@@ -16,6 +16,9 @@
 //   s = r - alpha q + first preconditioner sweep from zero
 //   t = A s^ + (t, s), (t, t)                                    -> omega
 //   x += alpha p^ + omega s^, r = s - omega t, |r|^2, (r, rr)    -> err, it, beta
+// With the one-pass SOR preconditioner the s kernel sums the partials of
+// (rr, q) itself (alpha), and so does the pair update for (t, s), (t, t)
+// (omega): no finalize launch between those kernels.
 //
 // Operator storage: falm_solver_set_matrix checks whether each off-diagonal
 // column of A depends on one index only (bitwise); if so the solver keeps the
@@ -238,17 +241,9 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
     if (MODE == 0) {
         c1 = (T)st->beta;
         c2 = (T)st->omega;
-    } else if (apart) {
-        // alpha = rho / (rr, q) from the operator kernel's partial sums
-        // (what k_fin_alpha computes), in every workgroup; the first stores it
-        double tot[1];
-        sum_partials<1, K7_NT>(apart, anb, tot);
-        const double alpha = st->rho / tot[0];
-        if (blockIdx.x == 0 && threadIdx.x == 0) stw->alpha = alpha;
-        c1 = (T)alpha;
-    } else {
+    } else if (!apart) {
         c1 = (T)st->alpha;
-    }
+    }  // else: alpha is summed below, while the prologue's loads are in flight
     const auto vec = [&](T rv, T pv, T qv) -> T {
         if (MODE == 1) return rv - c1 * qv;
         return upd ? rv + c1 * (pv - c2 * qv) : rv;
@@ -301,6 +296,16 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
             const unsigned oh = hb + (unsigned)Q.k0 * szt;
             load(oh, hr, hp, hq);
             hd = ch.diag_k(oh, Q.k0);
+        }
+        if (MODE == 1 && apart) {
+            // alpha = rho / (rr, q) from the operator kernel's partial sums
+            // (what k_fin_alpha computes), in every workgroup; the first
+            // stores it
+            double tot[1];
+            sum_partials<1, K7_NT>(apart, anb, tot);
+            const double alpha = st->rho / tot[0];
+            if (blockIdx.x == 0 && threadIdx.x == 0) stw->alpha = alpha;
+            c1 = (T)alpha;
         }
         wm = wval(vec(rm, pm, qm), dm, Q.act && kin(Q.k0 - 1), cpar, Q.k0 - 1);
         vc = vec(rc, pc, qc);
@@ -835,8 +840,8 @@ __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *_
 // (double) accesses, which stream at ~5.2 TB/s where one float per lane
 // reaches ~4.0 (measured on MI355X with the same bytes).  The tiling runs over
 // cell pairs (tp.box.n[0] = pairs per row, tp.box.lo[0] = 0); the pair at
-// index ip covers cells lo0 + 2 ip and lo0 + 2 ip + 1 < hi0, a last single
-// cell is done alone.  Needs even cell offsets at every pair (lo0 and nx
+// index ip covers cells lo0 + 2 ip and lo0 + 2 ip + 1.  Needs an even number
+// of interior cells per row, even cell offsets at every pair (lo0 and nx
 // even) and 16-byte aligned x; the sums run per thread over its pairs.
 // workgroups of the pair update (each sums the second operator product's
 // partials, so fewer and longer-running ones)
@@ -846,7 +851,7 @@ template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
 
 template <class T>
-__global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, int hi0, Geom g, double *__restrict__ x,
+__global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g, double *__restrict__ x,
                                                      const T *__restrict__ pp, const T *__restrict__ ss,
                                                      const T *__restrict__ s, const T *__restrict__ t,
                                                      T *__restrict__ r, const T *__restrict__ rr,
@@ -855,9 +860,35 @@ __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, int hi0
     using T2 = typename Pair<T>::type;
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
-    int ip, j, k, t0, t1;
+    int t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha;
+    // a pair's operands, loaded one tile ahead of their use
+    struct Pr {
+        T2 p2, s2, sv, tv, r0;
+        double2 xv;
+        int cc;
+        bool ok;
+    };
+    const auto load = [&](int tile) -> Pr {
+        Pr q;
+        int ip, j, k;
+        q.ok = tile_cell_at(tp, tile, ip, j, k);
+        if (q.ok) {
+            q.cc = g.id(lo0 + 2 * ip, j, k);
+            q.p2 = *(const T2 *)(pp + q.cc);
+            q.s2 = *(const T2 *)(ss + q.cc);
+            q.sv = *(const T2 *)(s + q.cc);
+            q.tv = *(const T2 *)(t + q.cc);
+            q.r0 = *(const T2 *)(rr + q.cc);
+            q.xv = *(const double2 *)(x + q.cc);
+        }
+        return q;
+    };
+    tile_range(tp.tiles, t0, t1);
+    // the first tile's loads are in flight while omega is summed
+    Pr cur = load(t0 < t1 ? t0 : 0);
+    cur.ok = cur.ok && t0 < t1;
     double omega;
     if (opart) {
         // omega = (t, s) / (t, t) from the second operator kernel's partial
@@ -871,33 +902,25 @@ __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, int hi0
         omega = st->omega;
     }
     const T om = (T)omega;
-    tile_range(tp.tiles, t0, t1);
     for (int tile = t0; tile < t1; tile++) {
-        if (!tile_cell_at(tp, tile, ip, j, k)) continue;
-        const int i = lo0 + 2 * ip;
-        const int cc = g.id(i, j, k);
-        if (i + 1 < hi0) {
-            const T2 p2 = *(const T2 *)(pp + cc), s2 = *(const T2 *)(ss + cc);
-            const T2 sv = *(const T2 *)(s + cc), tv = *(const T2 *)(t + cc), r0 = *(const T2 *)(rr + cc);
-            double2 xv = *(const double2 *)(x + cc);
-            xv.x += alpha * (double)p2.x + omega * (double)s2.x;
-            xv.y += alpha * (double)p2.y + omega * (double)s2.y;
-            *(double2 *)(x + cc) = xv;
+        Pr nxt;
+        nxt.ok = false;
+        if (tile + 1 < t1) nxt = load(tile + 1);
+        if (cur.ok) {
+            double2 xv = cur.xv;
+            xv.x += alpha * (double)cur.p2.x + omega * (double)cur.s2.x;
+            xv.y += alpha * (double)cur.p2.y + omega * (double)cur.s2.y;
+            *(double2 *)(x + cur.cc) = xv;
             T2 rv;
-            rv.x = sv.x - om * tv.x;
-            rv.y = sv.y - om * tv.y;
-            *(T2 *)(r + cc) = rv;
+            rv.x = cur.sv.x - om * cur.tv.x;
+            rv.y = cur.sv.y - om * cur.tv.y;
+            *(T2 *)(r + cur.cc) = rv;
             v[0] += (double)rv.x * (double)rv.x;
-            v[1] += (double)rv.x * (double)r0.x;
+            v[1] += (double)rv.x * (double)cur.r0.x;
             v[0] += (double)rv.y * (double)rv.y;
-            v[1] += (double)rv.y * (double)r0.y;
-        } else if (i < hi0) {
-            x[cc] += alpha * (double)pp[cc] + omega * (double)ss[cc];
-            const T rv = s[cc] - om * t[cc];
-            r[cc] = rv;
-            v[0] += (double)rv * (double)rv;
-            v[1] += (double)rv * (double)rr[cc];
+            v[1] += (double)rv.y * (double)cur.r0.y;
         }
+        cur = nxt;
     }
     block_partials<NT, 2>(v, partials, lds);
 }
@@ -1296,7 +1319,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     if (pairs) {
         nbu = S->tp.tiles < UPD_BLOCKS ? S->tp.tiles : UPD_BLOCKS;
         hipLaunchKernelGGL((k_bicg_update2<T>), dim3((unsigned)nbu), tiling_block(), 0, s, S->tp, tl.box.lo[0],
-                           tl.box.lo[0] + tl.box.n[0], g, x, (const T *)pcp, (const T *)pcs, (const T *)sv,
+                           g, x, (const T *)pcp, (const T *)pcs, (const T *)sv,
                            (const T *)t, r, (const T *)rr, (const DevState *)S->st, S->partials,
                            (const double *)S->partials2, nb, S->st);
     } else {
@@ -1598,7 +1621,7 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
         // cell pairs start at even offsets when the interior starts at an
         // even x index on rows of even length
         Box pb = interior_box(S->g);
-        const bool even = pb.lo[0] % 2 == 0 && S->g.nx % 2 == 0;
+        const bool even = pb.lo[0] % 2 == 0 && pb.n[0] % 2 == 0 && S->g.nx % 2 == 0;
         pb.n[0] = (pb.n[0] + 1) / 2;
         pb.lo[0] = 0;
         S->tp = make_tiling(pb);
