@@ -65,7 +65,12 @@ def families():
         "sor_s": (abi.TIME_SOR_S, "k_bicg_vsor_km<., ., 1> (s = r - alpha q + one-pass RB-SOR preconditioner)"),
         "mv": (abi.TIME_MV, "k_bicg_mv_km<., ., 1> (q = A p^ + (rr, q))"),
         "mv2": (abi.TIME_MV2, "k_bicg_mv_km<., ., 2> (t = A s^ + (t, s), (t, t))"),
-        "solver_update": (abi.TIME_SOLVER_UPDATE, "k_bicg_update (x += alpha p^ + omega s^, r = s - omega t + dots)"),
+        "pcmv": (abi.TIME_PCMV, "k_bicg_pcmv_km<., ., 0> (p = r + beta(p - omega q), q = A M(p), (rr, q); "
+                                "one-pass RB-SOR M on a two-cell ring)"),
+        "pcmv_s": (abi.TIME_PCMV_S, "k_bicg_pcmv_km<., ., 1> (s = r - alpha q, t = A M(s), (t, s), (t, t))"),
+        "pc_apply": (abi.TIME_PC_APPLY, "k_bicg_vsor_km<., ., 2> (x += M(y) at the end of a pass)"),
+        "solver_update": (abi.TIME_SOLVER_UPDATE, "k_bicg_update (y += alpha p + omega s, r = s - omega t + dots; "
+                                                  "x += alpha p^ + omega s^ without deferral)"),
         "alm": (abi.TIME_ALM, "ALM stage (points, box, factors, projection, totals): a latency chain, "
                               "bytes = point tables + force-box cells written"),
     }

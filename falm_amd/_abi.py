@@ -45,7 +45,8 @@ FIELD_NCOL = {FIELD_U: 3, FIELD_UU: 3, FIELD_P: 1, FIELD_NUT: 1, FIELD_FF: 3, FI
               FIELD_RHS: 1, FIELD_POI_A: 7, FIELD_XYZ: 3, FIELD_KX: 3, FIELD_G: 3, FIELD_JA: 1}
 
 TIME_NONE, TIME_SOR, TIME_MV, TIME_PSEUDO_U, TIME_ALM, TIME_SOLVER_UPDATE, TIME_PROJECT, TIME_FACE_FLUX, \
-    TIME_SOLVER_INIT, TIME_SGS, TIME_RHS, TIME_DIVERGENCE, TIME_SOR_S, TIME_MV2 = range(14)
+    TIME_SOLVER_INIT, TIME_SGS, TIME_RHS, TIME_DIVERGENCE, TIME_SOR_S, TIME_MV2, TIME_PCMV, TIME_PCMV_S, \
+    TIME_PC_APPLY = range(17)
 
 
 class Domain(C.Structure):

@@ -48,6 +48,23 @@ inline K7Tiling make_k7_tiling(const Box &b, int max_blocks = RED_BLOCKS) {
     return t;
 }
 
+// the shortest chunk (at least K7_KC planes) whose grid fits max_blocks
+inline K7Tiling make_k7_tiling_fit(const Box &b, int max_blocks) {
+    K7Tiling t;
+    t.box = b;
+    t.tx = (b.n[0] + K7_BX - 1) / K7_BX;
+    t.ty = (b.n[1] + K7_BY - 1) / K7_BY;
+    const int cols = t.tx * t.ty > 0 ? t.tx * t.ty : 1;
+    int tk = max_blocks / cols;
+    if (tk < 1) tk = 1;
+    t.kc = (b.n[2] + tk - 1) / tk;
+    if (t.kc < K7_KC) t.kc = K7_KC;
+    t.tk = (b.n[2] + t.kc - 1) / t.kc;
+    t.blocks = t.tx * t.ty * t.tk;
+    if (b.n[0] <= 0 || b.n[1] <= 0 || b.n[2] <= 0) t.blocks = 0;
+    return t;
+}
+
 struct K7Pos {
     int lane, wy;   // thread's column within the tile
     int i0, j0;     // tile origin

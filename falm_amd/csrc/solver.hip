@@ -61,7 +61,7 @@ struct DevState {
     int total;          // iterations over all passes
     int passes;         // passes started
     int maxit, inner_maxit;
-    int pad;
+    int pending;        // deferred preconditioning: the pass's sum is not yet in x
 };
 
 constexpr int NT = 256;
@@ -227,16 +227,21 @@ __global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, 
 // preconditioned vector's ghost cells (zero from allocation, never written)
 // do in the separate sweeps.  MODE 0 reads p from vin and writes v, another
 // buffer: neighbouring workgroups read vin at this tile's cells.
+// MODE 2 (end of a pass with deferred preconditioning): r is the sum y of
+// the pass's alpha p + omega s, and M(y) is added to x (double); it runs
+// once a pass is over and its sum not yet added (the init kernels' finalize
+// clears that).
 template <class T, class OP, int MODE>
 __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
                                                         const T *__restrict__ vin, const T *__restrict__ q,
                                                         T *__restrict__ v, T *__restrict__ vh, const DevState *st,
-                                                        T omega_pc, const double *apart, int anb, DevState *stw) {
-    if (st->done) return;
+                                                        T omega_pc, const double *apart, int anb, DevState *stw,
+                                                        double *__restrict__ xd = nullptr) {
+    if (MODE == 2 ? (!st->done || st->final || !st->pending) : st->done) return;
     __shared__ K7Lds<T> lw;
     const K7Pos Q = k7_pos(kt, g);
     const int sz = g.nxy;
-    const bool upd = MODE == 1 || !st->first;  // q (and vin) are read
+    const bool upd = MODE == 1 || (MODE == 0 && !st->first);  // q (and vin) are read
     T c1 = T(0), c2 = T(0);
     if (MODE == 0) {
         c1 = (T)st->beta;
@@ -313,11 +318,17 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         lw[0][Q.wy + 1][Q.lane + 1] = wc;
         if (h.slot) lw[0][h.row][h.col] = wval(vec(hr, hp, hq), hd, hin, hpar, Q.k0);
     }
+    // MODE 2: x a plane ahead (act columns only)
+    constexpr unsigned XS = (unsigned)(sizeof(double) / sizeof(T));
+    double xq = 0.0;
+    if (MODE == 2 && Q.act) xq = ld_at(xd, (cb + (unsigned)Q.k0 * szt) * XS);
     __syncthreads();
     int bf = 0;
     unsigned ob = cb + (unsigned)Q.k0 * szt;
     for (int k = Q.k0; k < Q.k1; k++, ob += szt) {
         const bool more = k + 1 < Q.k1;
+        const double xc = xq;
+        if (MODE == 2 && more && Q.act) xq = ld_at(xd, (ob + szt) * XS);
         T r2 = T(0), p2 = T(0), q2 = T(0);
         Cell e2 = ne;
         if (more) {
@@ -332,7 +343,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         const T vn = vec(nr, np, nq);
         const T wn = wval(vn, Col::diag(ne), Q.act && kin(k + 1), cpar, k + 1);
         if (Q.act) {
-            st_at(v, ob, vc);
+            if (MODE != 2) st_at(v, ob, vc);
             T out = wc;
             if (((cpar + k) & 1) != FALM_COLOR_BLACK) {
                 // kernel_Struct3d7p_SOR's update from x = 0 (src/dev/FalmEqDevCall.cu:195-215)
@@ -340,7 +351,11 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
                                      lw[bf][Q.wy + 2][Q.lane + 1], lw[bf][Q.wy][Q.lane + 1], wn, wm);
                 out = T(0) + omega_pc * ((vc - s) / Col::diag(ec));
             }
-            st_at(vh, ob, out);
+            if (MODE == 2) {
+                st_at(xd, ob * XS, xc + (double)out);
+            } else {
+                st_at(vh, ob, out);
+            }
         }
         if (more) {
             lw[bf ^ 1][Q.wy + 1][Q.lane + 1] = wn;
@@ -356,6 +371,305 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         ne = e2;
         bf ^= 1;
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------
+// A Krylov vector, its one-pass SOR preconditioning and the operator product
+// in one k-marching pass (pc_maxit = 1, preconditioning deferred):
+//   MODE 0: v = r (a pass's first iteration) or r + beta (p - omega q),
+//           w = A M(v), partial sums of (rr, w)          -> alpha
+//   MODE 1: v = r - alpha q,  w = A M(v), partial sums of (w, v), (w, w)
+//                                                        -> omega
+// M(v) is the black half-sweep from zero followed by the red one (black
+// cells omega_pc v / a_c, red cells omega_pc (v - six black neighbours'
+// products) / a_c, zero outside the interior), each cell with the
+// expression k_bicg_vsor_km uses, so w is what that kernel and
+// k_bicg_mv_km give bit for bit.  w at a tile cell needs M(v) on the tile
+// and its four one-cell strips in x and y; a red value there needs black
+// values one cell further out; so v is formed on the tile plus a two-cell
+// ring (PQ_W x PQ_H cells a plane) from r, p and q read there.  Planes pass
+// through four LDS slots: at step s the slot of plane s+2 is filled (black
+// cells M(v), red cells v), the red cells of plane s are turned into M(v)
+// in place (they read black values of planes s-1..s+1 only), and w of plane
+// s-1 is formed from M(v) of plane s-1 in LDS and of the column's planes
+// s-2 and s in registers: one barrier per step.
+// v is written on the tile (p or s) and w (q or t); M(v) is not stored: the
+// update accumulates alpha p + omega s and k_bicg_vsor_km<MODE 2> adds M of
+// the sum to x at the end of a pass (M is linear).  MODE 0 reads p and q at
+// the ring cells, which neighbouring workgroups write: both alternate
+// between two buffers.
+constexpr int PQ_R = 2;                                    // ring width
+constexpr int PQ_W = K7_BX + 2 * PQ_R, PQ_H = K7_BY + 2 * PQ_R;
+constexpr int PQ_RING2 = PQ_W * PQ_H - K7_NT;              // ring cells of v
+constexpr int PQ_RING1 = 2 * K7_BX + 2 * K7_BY;            // red-sweep cells outside the tile
+constexpr int PQ_R1T0 = K7_NT - PQ_RING1;                  // first thread of those
+static_assert(PQ_RING2 <= K7_NT && PQ_R1T0 >= PQ_RING2 - 2 * PQ_W, "ring cells exceed threads");
+static_assert((PQ_R1T0 + 2 * K7_BY) % 64 == 0, "ring-1 rows start on a wave");
+template <class T>
+using PQLds = T[4][PQ_H][PQ_W];
+// workgroups of k_bicg_pcmv_km at most (its chunk length doubles until the
+// grid fits)
+#ifndef FALM_PCMV_BLOCKS
+#define FALM_PCMV_BLOCKS RED_BLOCKS
+#endif
+constexpr int PCMV_BLOCKS = FALM_PCMV_BLOCKS;
+#ifdef FALM_PCMV_WAVES
+#define PCMV_ATTR __attribute__((amdgpu_waves_per_eu(FALM_PCMV_WAVES)))
+#else
+#define PCMV_ATTR
+#endif
+
+// region position (rx, ry) of thread s's ring cell (s < PQ_RING2): two rows
+// below the tile, two above, then the two columns on either side
+__device__ inline void pq_ring2(int s, int &rx, int &ry) {
+    if (s < 2 * PQ_W) {
+        ry = s / PQ_W;
+        rx = s - ry * PQ_W;
+    } else if (s < 4 * PQ_W) {
+        const int u = s - 2 * PQ_W, q = u / PQ_W;
+        ry = PQ_H - PQ_R + q;
+        rx = u - q * PQ_W;
+    } else {
+        const int u = s - 4 * PQ_W, c = u & 3;
+        ry = PQ_R + (u >> 2);
+        rx = c < 2 ? c : PQ_W - 4 + c;
+    }
+}
+// region position of the red-sweep cell outside the tile of thread s
+// (s >= PQ_R1T0): the columns beside the tile, the row below, the row above
+__device__ inline void pq_ring1(int s, int &rx, int &ry) {
+    const int u = s - PQ_R1T0;
+    if (u < 2 * K7_BY) {
+        ry = PQ_R + (u >> 1);
+        rx = (u & 1) ? PQ_W - PQ_R : PQ_R - 1;
+    } else if (u < 2 * K7_BY + K7_BX) {
+        ry = PQ_R - 1;
+        rx = PQ_R + (u - 2 * K7_BY);
+    } else {
+        ry = PQ_H - PQ_R;
+        rx = PQ_R + (u - 2 * K7_BY - K7_BX);
+    }
+}
+
+// The diagonal and the cell coefficients at a byte offset, with the
+// diagonal-class choice fixed at compile time (DCLS: from the column's
+// per-run values, else the stored column), so every load is unconditional.
+template <bool DCLS, class TA>
+__device__ inline TA diag_of(const OpSepCol<TA> &c, unsigned ob, int k) {
+    if constexpr (DCLS) return c.pick(k);
+    else return ld_at(c.ac, ob);
+}
+template <bool DCLS, class TA>
+__device__ inline TA diag_of(const OpCols<TA> &c, unsigned ob, int) {
+    return ld_at(c.c[0], ob);
+}
+template <bool DCLS, class TA>
+__device__ inline typename OpSepCol<TA>::Cell cell_of(const OpSepCol<TA> &c, unsigned ob, int k) {
+    return typename OpSepCol<TA>::Cell{diag_of<DCLS>(c, ob, k), ld_uniform(c.ab, k), ld_uniform(c.at, k)};
+}
+template <bool DCLS, class TA>
+__device__ inline typename OpCols<TA>::Cell cell_of(const OpCols<TA> &c, unsigned ob, int k) {
+    return c.load(ob, k);
+}
+
+// Every global load and store of a step is issued by every thread (loads at
+// clamped valid cells, stores of threads with nothing to store go to the
+// thread's slot in `trash`): the compiler can then count the memory
+// operations in flight exactly and waits for the previous step's loads
+// only, not for those just issued.  trash holds K7_NT values per workgroup.
+template <class T, class OP, int MODE, bool DCLS>
+__global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
+                                                        const T *__restrict__ pin, const T *__restrict__ qin,
+                                                        const T *__restrict__ rr, T *__restrict__ vout,
+                                                        T *__restrict__ wout, const DevState *st, T omega_pc,
+                                                        const double *apart, int anb, DevState *stw,
+                                                        double *partials, T *__restrict__ trash) {
+    if (st->done) return;
+    __shared__ PQLds<T> lv;
+    __shared__ double lds[2 * (K7_NT / 64)];
+    const K7Pos Q = k7_pos(kt, g);
+    const int tid = (int)threadIdx.x;
+    const unsigned szt = (unsigned)g.nxy * sizeof(T);
+    const bool upd = MODE == 1 || !st->first;  // q (and p) are used
+    T c1 = T(0), c2 = T(0);
+    if (MODE == 0) {
+        c1 = (T)st->beta;
+        c2 = (T)st->omega;
+    }  // MODE 1: alpha is summed below, while the prologue's loads are in flight
+    const int lo0 = kt.box.lo[0], hi0 = lo0 + kt.box.n[0], lo1 = kt.box.lo[1], hi1 = lo1 + kt.box.n[1];
+    const int lo2 = kt.box.lo[2], hi2 = lo2 + kt.box.n[2];
+    const int k0 = Q.k0, k1 = Q.k1;
+    const auto inxy = [&](int i, int j) { return i >= lo0 && i < hi0 && j >= lo1 && j < hi1; };
+    const auto kin = [&](int k) { return k >= lo2 && k < hi2; };
+    // a plane to load from: inside the interior and at most k1 + 1
+    const auto kcl = [&](int k) {
+        k = k > k1 + 1 ? k1 + 1 : k;
+        return k < lo2 ? lo2 : (k >= hi2 ? hi2 - 1 : k);
+    };
+    const auto black = [&](int i, int j, int k) { return ((i + j + k + g.osum) & 1) == FALM_COLOR_BLACK; };
+    // this thread's cells: its column (region position ro), a ring cell of
+    // v (r2) and a red-sweep cell outside the tile (r1); positions outside
+    // the interior read the thread's own (valid) column instead
+    const int oi = Q.act ? Q.i : lo0, oj = Q.act ? Q.j : lo1;
+    const int rxo = Q.lane + PQ_R, ryo = Q.wy + PQ_R;
+    const auto ca = a.column(oi, oj);
+    using Col = typename std::decay<decltype(ca)>::type;
+    using Cell = typename Col::Cell;
+    static_assert(sizeof(decltype(ca.diag_at(0u))) == sizeof(T), "operator and vector precision differ");
+    const unsigned cb = (unsigned)(oi + g.nx * oj) * sizeof(T);
+    int rx2 = 0, ry2 = 0, rx1 = 0, ry1 = 0;
+    const bool r2 = tid < PQ_RING2, r1 = tid >= PQ_R1T0;
+    if (r2) pq_ring2(tid, rx2, ry2);
+    if (r1) pq_ring1(tid, rx1, ry1);
+    const int gi2 = Q.i0 - PQ_R + rx2, gj2 = Q.j0 - PQ_R + ry2;
+    const int gi1 = Q.i0 - PQ_R + rx1, gj1 = Q.j0 - PQ_R + ry1;
+    const bool in2 = r2 && inxy(gi2, gj2), in1 = r1 && inxy(gi1, gj1);
+    const auto c2v = a.column(in2 ? gi2 : oi, in2 ? gj2 : oj);
+    const auto c1v = a.column(in1 ? gi1 : oi, in1 ? gj1 : oj);
+    const unsigned b2 = in2 ? (unsigned)(gi2 + g.nx * gj2) * sizeof(T) : cb;
+    const unsigned b1 = in1 ? (unsigned)(gi1 + g.nx * gj1) * sizeof(T) : cb;
+    T *const tr = trash + (size_t)blockIdx.x * K7_NT + tid;
+
+    // operands of v at one cell of a plane, and its diagonal
+    struct Ld {
+        T r, p, q, d;
+    };
+    const auto load = [&](const Col &c, unsigned ob, int k) {
+        Ld x;
+        x.r = ld_at(r, ob);
+        x.q = ld_at(qin, ob);
+        x.p = MODE == 0 ? ld_at(pin, ob) : T(0);
+        x.d = diag_of<DCLS>(c, ob, k);
+        return x;
+    };
+    const auto vec = [&](const Ld &x) -> T {
+        if (MODE == 1) return x.r - c1 * x.q;
+        return upd ? x.r + c1 * (x.p - c2 * x.q) : x.r;
+    };
+    struct Fl {
+        Ld o, h;  // the column's cell and the ring cell
+    };
+    const auto fetch = [&](int f) {
+        const int kk = kcl(f);
+        const unsigned pf = (unsigned)kk * szt;
+        Fl x;
+        x.o = load(ca, cb + pf, kk);
+        x.h = load(c2v, b2 + pf, kk);
+        return x;
+    };
+    // slot of plane f: black cells M(v), red cells v, zero outside the
+    // interior; v is written on the tile; returns the column's v
+    const auto fill = [&](int f, const Fl &x) -> T {
+        const int sl = f & 3;
+        const bool kf = kin(f);
+        const T vo = (Q.act && kf) ? vec(x.o) : T(0);
+        lv[sl][ryo][rxo] = black(Q.i, Q.j, f) ? T(0) + omega_pc * (vo / x.o.d) : vo;
+        const bool wr = Q.act && f >= k0 && f < k1;
+        *(wr ? (T *)((char *)vout + cb + (unsigned)f * szt) : tr) = vo;
+        if (r2) {
+            const T vh = (in2 && kf) ? vec(x.h) : T(0);
+            lv[sl][ry2][rx2] = black(gi2, gj2, f) ? T(0) + omega_pc * (vh / x.h.d) : vh;
+        }
+        return vo;
+    };
+    // red cells of plane m (tile and its strips) to M(v), in place; returns
+    // the column's M(v).  kernel_Struct3d7p_SOR's update from x = 0
+    // (src/dev/FalmEqDevCall.cu:195-215).
+    const auto red = [&](int m, const Cell &eo, const Cell &e1) -> T {
+        const int sl = m & 3, sm = (m - 1) & 3, sp = (m + 1) & 3;
+        const bool km = kin(m);
+        T ho = lv[sl][ryo][rxo];
+        if (Q.act && km && !black(Q.i, Q.j, m)) {
+            const T s = ca.apply(eo, T(0), lv[sl][ryo][rxo + 1], lv[sl][ryo][rxo - 1], lv[sl][ryo + 1][rxo],
+                                 lv[sl][ryo - 1][rxo], lv[sp][ryo][rxo], lv[sm][ryo][rxo]);
+            ho = T(0) + omega_pc * ((ho - s) / Col::diag(eo));
+            lv[sl][ryo][rxo] = ho;
+        }
+        if (in1 && km && !black(gi1, gj1, m)) {
+            const T v1 = lv[sl][ry1][rx1];
+            const T s = c1v.apply(e1, T(0), lv[sl][ry1][rx1 + 1], lv[sl][ry1][rx1 - 1], lv[sl][ry1 + 1][rx1],
+                                  lv[sl][ry1 - 1][rx1], lv[sp][ry1][rx1], lv[sm][ry1][rx1]);
+            lv[sl][ry1][rx1] = T(0) + omega_pc * ((v1 - s) / Col::diag(e1));
+        }
+        return ho;
+    };
+    const auto cell_at = [&](const Col &c, unsigned base, int k) {
+        const int kk = kcl(k);
+        return cell_of<DCLS>(c, base + (unsigned)kk * szt, kk);
+    };
+
+    // prologue: planes k0-2 .. k0 in their slots, plane k0+1's loads in flight
+    const Fl f0 = fetch(k0 - 2), f1 = fetch(k0 - 1), f2 = fetch(k0);
+    Fl fA = fetch(k0 + 1), fB{};  // fB is loaded in the first step
+    Cell eA = cell_at(ca, cb, k0 - 1), eB = eA, e1A = cell_at(c1v, b1, k0 - 1), e1B = e1A;
+    T rA = T(0), rB = T(0);
+    if (MODE == 1) {
+        // alpha = rho / (rr, q) from the first kernel's partial sums (what
+        // k_fin_alpha computes), in every workgroup; the first stores it
+        double tot[1];
+        sum_partials<1, K7_NT>(apart, anb, tot);
+        const double alpha = st->rho / tot[0];
+        if (blockIdx.x == 0 && threadIdx.x == 0) stw->alpha = alpha;
+        c1 = (T)alpha;
+    }
+    // at the start of step s: the column's v at planes s-1, s, s+1 (v0, v1,
+    // v2; MODE 1's dot (w, v)) and M(v) at planes s-2, s-1 (hm, hc)
+    T v0 = fill(k0 - 2, f0), v1 = fill(k0 - 1, f1), v2 = fill(k0, f2);
+    T hm = T(0), hc = T(0);
+    __syncthreads();
+    double acc[2] = {0.0, 0.0};
+    // Step s with two buffer sets: the `use` set holds what step s-1 loaded
+    // (plane s+2's operands, rr at plane s-1, the cells of plane s; eo holds
+    // plane s-1's cells), the `load` set receives plane s+3's operands, rr at
+    // plane s and the cells of plane s+1.  The loop runs two steps per trip
+    // with the sets swapped, so a value is first touched one step after its
+    // load was issued: rotating one set of registers at the end of every
+    // step would wait for the loads just issued.  Past the chunk's last
+    // plane the fills go to a slot no later step reads, and their stores to
+    // trash.
+    const auto step = [&](int s, const Fl &fu, Fl &fl, T ru, T &rl, const Cell &eu, Cell &eo, const Cell &e1u,
+                          Cell &e1l) {
+        const Cell eq = eo;
+        fl = fetch(s + 3);
+        eo = cell_at(ca, cb, s + 1);
+        e1l = cell_at(c1v, b1, s + 1);
+        {
+            const int k = s < k0 ? k0 : (s >= k1 ? k1 - 1 : s);
+            rl = MODE == 0 ? ld_at(rr, cb + (unsigned)k * szt) : T(0);
+        }
+        const T vn = fill(s + 2, fu);
+        const T h = red(s, eu, e1u);
+        // w at plane s-1 (the column's M(v): s-2, s-1, s)
+        const int sl = (s - 1) & 3;
+        const T wv = ca.apply(eq, hc, lv[sl][ryo][rxo + 1], lv[sl][ryo][rxo - 1], lv[sl][ryo + 1][rxo],
+                              lv[sl][ryo - 1][rxo], h, hm);
+        const bool wr = Q.act && s - 1 >= k0;
+        *(wr ? (T *)((char *)wout + cb + (unsigned)(s - 1) * szt) : tr) = wv;
+        if (MODE == 0) {
+            acc[0] += wr ? (double)ru * (double)wv : 0.0;
+        } else {
+            acc[0] += wr ? (double)wv * (double)v0 : 0.0;
+            acc[1] += wr ? (double)wv * (double)wv : 0.0;
+        }
+        hm = hc;
+        hc = h;
+        v0 = v1;
+        v1 = v2;
+        v2 = vn;
+        __syncthreads();
+    };
+    int s = k0 - 1;
+    for (; s + 1 <= k1; s += 2) {
+        step(s, fA, fB, rA, rB, eA, eB, e1A, e1B);
+        step(s + 1, fB, fA, rB, rA, eB, eA, e1B, e1A);
+    }
+    if (s <= k1) step(s, fA, fB, rA, rB, eA, eB, e1A, e1B);
+    if (MODE == 0) {
+        double v1s[1] = {acc[0]};
+        block_partials<K7_NT, 1>(v1s, partials, lds);
+    } else {
+        block_partials<K7_NT, 2>(acc, partials, lds);
     }
 }
 
@@ -633,6 +947,7 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_init(DevState *st, const double 
         const double err = sqrt(tot[0]) / st->gN;
         st->err = err;
         st->rho = tot[1];
+        st->pending = 0;  // the init kernel has read x with the pass's sum added
         if (st->mixed && ((st->passes > 0 && err <= st->tol) || st->total >= st->maxit)) {
             st->final = 1;
             return;
@@ -676,6 +991,7 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_check(DevState *st, const double
     if (threadIdx.x == 0) {
         const double err = sqrt(tot[0]) / st->gN;
         st->err = err;
+        st->pending = 0;
         if ((st->passes > 0 && err <= st->tol) || st->total >= st->maxit) st->final = 1;
     }
 }
@@ -811,23 +1127,29 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_omega(DevState *st, const double
 }
 
 // x += alpha pp + omega ss (kernel_PBiCGStab_3, x in double);
-// r = s - omega t (kernel_PBiCGStab_4); partials |r|^2, (r, rr)
-template <class T>
+// r = s - omega t (kernel_PBiCGStab_4); partials |r|^2, (r, rr).
+// DEFER (preconditioning deferred to the end of the pass): pp, ss are p and
+// s, and the sum goes to y (T; set on a pass's first iteration) instead of x.
+template <class T, bool DEFER = false>
 __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *__restrict__ x,
                                                     const T *__restrict__ pp, const T *__restrict__ ss,
                                                     const T *__restrict__ s, const T *__restrict__ t,
                                                     T *__restrict__ r, const T *__restrict__ rr, const DevState *st,
-                                                    double *partials) {
+                                                    double *partials, T *__restrict__ y = nullptr) {
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int i, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha, omega = st->omega;
+    const bool first = st->first;
     tile_range(tl.tiles, t0, t1);
     for (int tile = t0; tile < t1; tile++) {
         if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
-        x[cc] += alpha * (double)pp[cc] + omega * (double)ss[cc];
+        if (DEFER)
+            y[cc] = (T)((first ? 0.0 : (double)y[cc]) + (alpha * (double)pp[cc] + omega * (double)ss[cc]));
+        else
+            x[cc] += alpha * (double)pp[cc] + omega * (double)ss[cc];
         const T rv = s[cc] - (T)omega * t[cc];
         r[cc] = rv;
         v[0] += (double)rv * (double)rv;
@@ -850,23 +1172,26 @@ template <class T> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
 
-template <class T>
+template <class T, bool DEFER = false>
 __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g, double *__restrict__ x,
                                                      const T *__restrict__ pp, const T *__restrict__ ss,
                                                      const T *__restrict__ s, const T *__restrict__ t,
                                                      T *__restrict__ r, const T *__restrict__ rr,
                                                      const DevState *st, double *partials, const double *opart,
-                                                     int onb, DevState *stw) {
+                                                     int onb, DevState *stw, T *__restrict__ y = nullptr) {
     using T2 = typename Pair<T>::type;
+    using X2 = typename std::conditional<DEFER, T2, double2>::type;
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha;
+    // DEFER: the pass's first iteration sets y (no read)
+    const bool xread = !DEFER || !st->first;
     // a pair's operands, loaded one tile ahead of their use
     struct Pr {
         T2 p2, s2, sv, tv, r0;
-        double2 xv;
+        X2 xv;
         int cc;
         bool ok;
     };
@@ -881,7 +1206,12 @@ __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g,
             q.sv = *(const T2 *)(s + q.cc);
             q.tv = *(const T2 *)(t + q.cc);
             q.r0 = *(const T2 *)(rr + q.cc);
-            q.xv = *(const double2 *)(x + q.cc);
+            if (DEFER) {
+                q.xv = X2{};
+                if (xread) q.xv = *(const X2 *)(y + q.cc);
+            } else {
+                q.xv = *(const X2 *)(x + q.cc);
+            }
         }
         return q;
     };
@@ -907,10 +1237,17 @@ __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g,
         nxt.ok = false;
         if (tile + 1 < t1) nxt = load(tile + 1);
         if (cur.ok) {
-            double2 xv = cur.xv;
-            xv.x += alpha * (double)cur.p2.x + omega * (double)cur.s2.x;
-            xv.y += alpha * (double)cur.p2.y + omega * (double)cur.s2.y;
-            *(double2 *)(x + cur.cc) = xv;
+            if (DEFER) {
+                X2 yv;
+                yv.x = (T)((double)cur.xv.x + (alpha * (double)cur.p2.x + omega * (double)cur.s2.x));
+                yv.y = (T)((double)cur.xv.y + (alpha * (double)cur.p2.y + omega * (double)cur.s2.y));
+                *(X2 *)(y + cur.cc) = yv;
+            } else {
+                X2 xv = cur.xv;
+                xv.x += alpha * (double)cur.p2.x + omega * (double)cur.s2.x;
+                xv.y += alpha * (double)cur.p2.y + omega * (double)cur.s2.y;
+                *(X2 *)(x + cur.cc) = xv;
+            }
             T2 rv;
             rv.x = cur.sv.x - om * cur.tv.x;
             rv.y = cur.sv.y - om * cur.tv.y;
@@ -936,6 +1273,7 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_update(DevState *st, const doubl
         st->it_pass += 1;
         st->total += 1;
         st->first = 0;
+        st->pending = 1;
         if (!(st->it_pass < st->maxit_pass && st->err > st->tol_inner)) {
             st->done = 1;
         } else {
@@ -1011,6 +1349,7 @@ __global__ void k_state_reset(DevState *st, double gN, int mixed, double tol, in
         st->maxit_pass = maxit;
         st->inner_rtol = inner_rtol;
         st->inner_maxit = inner_maxit;
+        st->pending = 0;
     }
 }
 
@@ -1073,6 +1412,7 @@ struct falm_solver {
     Tiling tl;
     Tiling tp;     // tiles of x-adjacent cell pairs (k_bicg_update2); no tiles when pairs are not aligned
     K7Tiling kt;   // k-marching column tiles of the interior (stencil kernels)
+    K7Tiling ktp;  // the same for k_bicg_pcmv_km (its own chunk length)
     S3Tiling s3;   // tiles of the fused three-sweep SOR kernel
     DevState *st = nullptr;
     DevState *hst = nullptr;  // pinned mirror
@@ -1098,6 +1438,8 @@ struct falm_solver {
     double *r64own = nullptr;     // residual when the caller passes r = NULL (double solvers)
     int last_it = 0;              // iterations of the previous solve
     int pflip = 0;                // fused SOR path: p lives in w (1) or p (0)
+    int qflip = 0;                // deferred path: q lives in pp (1) or q (0)
+    void *trash = nullptr;        // k_bicg_pcmv_km's stores of threads with nothing to store
 };
 
 namespace falm {
@@ -1132,7 +1474,9 @@ static falm_status solver_alloc(falm_solver *S) {
     FALM_HIP(hipHostMalloc(&S->hflag, sizeof(int), hipHostMallocDefault));
     size_t nb = (size_t)(S->tl.tiles > 0 ? red_blocks(S->tl) : 1);
     if ((size_t)S->kt.blocks > nb) nb = (size_t)S->kt.blocks;
+    if ((size_t)S->ktp.blocks > nb) nb = (size_t)S->ktp.blocks;
     FALM_HIP(hipMalloc(&S->partials, 2 * nb * sizeof(double)));
+    FALM_HIP(hipMalloc(&S->trash, (size_t)(S->ktp.blocks > 0 ? S->ktp.blocks : 1) * K7_NT * sizeof(double)));
     FALM_HIP(hipMalloc(&S->partials2, 2 * nb * sizeof(double)));
     // the zero fills above run on the null stream, which the caller's
     // (possibly non-blocking) stream does not wait for
@@ -1243,6 +1587,29 @@ static T *pc_rest(falm_solver *S, const OP &a, T *x, const T *b, T *spare, hipSt
     return x;
 }
 
+// The one-pass SOR preconditioner with deferred application: each
+// iteration is the two k_bicg_pcmv_km kernels and the update into y (ss);
+// k_bicg_vsor_km<MODE 2> adds M(y) to x when the pass is over.
+#ifndef FALM_PCMV
+#define FALM_PCMV 1
+#endif
+static bool deferred_pc(const falm_solver *S) {
+    return FALM_PCMV && S->cfg.pc_type == FALM_LS_SOR && S->cfg.pc_maxit <= 1 && S->kt.blocks > 0;
+}
+
+// x += M(y) once a pass is over (no-op while it runs or once its sum is in)
+template <class T, class OP>
+static falm_status launch_pc_apply(falm_solver *S, const OP &a, double *x, hipStream_t s) {
+    TimedLaunch tm(FALM_TIME_PC_APPLY, s);
+    hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 2>), FALM_KM(S), s, S->kt, S->g, a, (const T *)S->ss,
+                       (const T *)nullptr, (const T *)nullptr, (T *)nullptr, (T *)nullptr, (const DevState *)S->st,
+                       (T)S->cfg.pc_relax, (const double *)nullptr, 0, (DevState *)nullptr, x);
+    // y (and the diagonal) read, x read and written
+    tm.done((double)S->tl.box.size() * (sizeof(T) + 2.0 * sizeof(double) + diag_bytes<OP>(S)));
+    FALM_CHECK_LAUNCH("k_bicg_vsor_km<apply>");
+    return FALM_OK;
+}
+
 // one BiCGStab iteration (src/FalmEq.cpp:345-407)
 template <class T, class OP>
 static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, hipStream_t s, bool first) {
@@ -1253,6 +1620,67 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
     T *rr = (T *)S->rr, *p = (T *)S->p, *q = (T *)S->q, *sv = (T *)S->s, *pp = (T *)S->pp, *ss = (T *)S->ss,
       *t = (T *)S->t;
     const T wpc = (T)c.pc_relax;
+    if (deferred_pc(S)) {
+        // p, q alternate between two buffers (the p kernel reads both at the
+        // ring cells of its tiles); a pass's first iteration reads neither,
+        // so the alternation may run on across iterations a finished pass
+        // skips
+        T *pin = (T *)(S->pflip ? S->w : S->p), *pout = (T *)(S->pflip ? S->p : S->w);
+        T *qin = (T *)(S->qflip ? S->pp : S->q), *qout = (T *)(S->qflip ? S->q : S->pp);
+        S->pflip ^= 1;
+        S->qflip ^= 1;
+        const double box = (double)tl.box.size(), db = diag_bytes<OP>(S);
+        const int nbp = S->ktp.blocks;
+        const dim3 gp((unsigned)nbp), bp(K7_NT);
+        T *trash = (T *)S->trash;
+        // the diagonal from its classes (a compile-time choice in the kernel)
+        bool dcls = false;
+        if constexpr (std::is_same<OP, OpSep<T>>::value) dcls = a.dtab != nullptr;
+        const auto launch = [&](auto dc) {
+            constexpr bool D = decltype(dc)::value;
+            {
+                TimedLaunch tm(FALM_TIME_PCMV, s);
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                                   (const T *)pin, (const T *)qin, (const T *)rr, pout, qout, (const DevState *)S->st,
+                                   wpc, (const double *)nullptr, 0, (DevState *)nullptr, S->partials, trash);
+                // r, rr (and the diagonal) read, p and q written; after a
+                // pass's first iteration also p and q read
+                tm.done(box * ((first ? 4.0 : 6.0) * sizeof(T) + db));
+            }
+            {
+                TimedLaunch tm(FALM_TIME_PCMV_S, s);
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                                   (const T *)nullptr, (const T *)qout, (const T *)nullptr, sv, t,
+                                   (const DevState *)S->st, wpc, (const double *)S->partials, nbp, S->st,
+                                   S->partials2, trash);
+                // r, q (and the diagonal) read, s and t written
+                tm.done(box * (4.0 * sizeof(T) + db));
+            }
+        };
+        if (dcls) launch(std::true_type{});
+        else launch(std::false_type{});
+        FALM_CHECK_LAUNCH("k_bicg_pcmv_km");
+        TimedLaunch tm2(FALM_TIME_SOLVER_UPDATE, s);
+        int nbu = red_blocks(tl);
+        if (S->tp.tiles > 0) {
+            nbu = S->tp.tiles < UPD_BLOCKS ? S->tp.tiles : UPD_BLOCKS;
+            hipLaunchKernelGGL((k_bicg_update2<T, true>), dim3((unsigned)nbu), tiling_block(), 0, s, S->tp,
+                               tl.box.lo[0], g, x, (const T *)pout, (const T *)sv, (const T *)sv, (const T *)t, r,
+                               (const T *)rr, (const DevState *)S->st, S->partials, (const double *)S->partials2, nbp,
+                               S->st, ss);
+        } else {
+            hipLaunchKernelGGL(k_fin_omega, FALM_FIN, s, S->st, (const double *)S->partials2, nbp);
+            hipLaunchKernelGGL((k_bicg_update<T, true>), FALM_RED(S), s, tl, g, x, (const T *)pout, (const T *)sv,
+                               (const T *)sv, (const T *)t, r, (const T *)rr, (const DevState *)S->st, S->partials,
+                               ss);
+        }
+        // p, s, t, rr read, y read (after a pass's first iteration) and
+        // written, r written
+        tm2.done(box * ((first ? 6.0 : 7.0) * sizeof(T)));
+        hipLaunchKernelGGL(k_fin_update, FALM_FIN, s, S->st, (const double *)S->partials, nbu);
+        FALM_CHECK_LAUNCH("k_bicg_update<deferred>");
+        return FALM_OK;
+    }
     // one-pass SOR preconditioner: each vector and its two half-sweeps in
     // one kernel (k_bicg_vsor_km); p alternates between p and w, since the
     // kernel reads the old p at the neighbours of the cells it writes.  A
@@ -1378,6 +1806,9 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
     while (true) {
         for (int k = 0; k < batch && enqueued < c.maxit; k++, enqueued++)
             FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s, enqueued == 0)));
+        // the pass's sum into x, if the pass is over (before the poll, so x
+        // is complete when the solve returns)
+        if (deferred_pc(S)) FALM_TRY((launch_pc_apply<double, OP64>(S, a, x, s)));
         FALM_TRY(read_state(S, s));
         if (S->hst->done || enqueued >= c.maxit) break;
         batch = check_period(S);
@@ -1411,6 +1842,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
     int batch = first_batch(S), polls = 0;
     while (true) {
         for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, polls == 0 && k == 0)));
+        if (deferred_pc(S)) FALM_TRY((launch_pc_apply<float, OP32>(S, a32, x, s)));
         // a finished pass is tested with the norm alone, unless the caller
         // wants the final residual; a pass that must follow starts from the
         // full init (it recomputes the same residual)
@@ -1617,6 +2049,11 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
     S->g = make_geom(dom);
     S->tl = make_tiling(interior_box(S->g));
     S->kt = make_k7_tiling(interior_box(S->g));
+#ifdef FALM_PCMV_EXACT
+    S->ktp = make_k7_tiling_fit(interior_box(S->g), PCMV_BLOCKS);
+#else
+    S->ktp = make_k7_tiling(interior_box(S->g), PCMV_BLOCKS);
+#endif
     {
         // cell pairs start at even offsets when the interior starts at an
         // even x index on rows of even length
@@ -1689,7 +2126,7 @@ falm_status falm_solver_destroy(falm_solver *S) {
     if (!S) return FALM_OK;
     void *bufs[] = {S->rr, S->p, S->q, S->s, S->pp, S->ss, S->t, S->w, S->r32, S->a32,
                     S->sep64, S->sep32, S->dtab64, S->dtab32, S->st, S->partials, S->partials2, S->flag,
-                    S->r64own};
+                    S->r64own, S->trash};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (S->hst) (void)hipHostFree(S->hst);
