@@ -62,6 +62,7 @@ struct DevState {
     int passes;         // passes started
     int maxit, inner_maxit;
     int pending;        // deferred preconditioning: the pass's sum is not yet in x
+    int plast;          // folded update: the buffer the last p went to
 };
 
 constexpr int NT = 256;
@@ -408,17 +409,23 @@ static_assert(PQ_RING2 <= K7_NT && PQ_R1T0 >= PQ_RING2 - 2 * PQ_W, "ring cells e
 static_assert((PQ_R1T0 + 2 * K7_BY) % 64 == 0, "ring-1 rows start on a wave");
 template <class T>
 using PQLds = T[4][PQ_H][PQ_W];
-// workgroups of k_bicg_pcmv_km at most (its chunk length doubles until the
-// grid fits)
-#ifndef FALM_PCMV_BLOCKS
-#define FALM_PCMV_BLOCKS RED_BLOCKS
+// k_bicg_pcmv_km runs one round of workgroups, pcmv_wg_per_cu of them
+// resident on each CU: its chunk is the shortest (at least K7_KC planes)
+// whose grid fits that many.  Float kernels run two 8-wave workgroups per CU
+// (at most 128 VGPRs, which the separable-operator instances are held to);
+// double ones one, with 64-plane chunks at 512 x 128 x 128 (measured as fast
+// as two per CU, and the folded double s kernel needs more than 128 VGPRs).
+#ifndef FALM_PCMV_WG_F32
+#define FALM_PCMV_WG_F32 2
 #endif
-constexpr int PCMV_BLOCKS = FALM_PCMV_BLOCKS;
-#ifdef FALM_PCMV_WAVES
-#define PCMV_ATTR __attribute__((amdgpu_waves_per_eu(FALM_PCMV_WAVES)))
-#else
-#define PCMV_ATTR
+#ifndef FALM_PCMV_WG_F64
+#define FALM_PCMV_WG_F64 1
 #endif
+constexpr int pcmv_wg_per_cu(bool dbl) { return dbl ? FALM_PCMV_WG_F64 : FALM_PCMV_WG_F32; }
+template <class OP> struct PcmvWaves { static constexpr int value = 1; };
+template <class TA> struct PcmvWaves<OpSep<TA>> {
+    static constexpr int value = 2 * pcmv_wg_per_cu(sizeof(TA) == 8);
+};
 
 // region position (rx, ry) of thread s's ring cell (s < PQ_RING2): two rows
 // below the tile, two above, then the two columns on either side
@@ -478,16 +485,18 @@ __device__ inline typename OpCols<TA>::Cell cell_of(const OpCols<TA> &c, unsigne
 // thread's slot in `trash`): the compiler can then count the memory
 // operations in flight exactly and waits for the previous step's loads
 // only, not for those just issued.  trash holds K7_NT values per workgroup.
-template <class T, class OP, int MODE, bool DCLS>
-__global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
+template <class T, class OP, int MODE, bool DCLS, bool FOLD>
+__global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves<OP>::value))) void k_bicg_pcmv_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
                                                         const T *__restrict__ pin, const T *__restrict__ qin,
                                                         const T *__restrict__ rr, T *__restrict__ vout,
                                                         T *__restrict__ wout, const DevState *st, T omega_pc,
                                                         const double *apart, int anb, DevState *stw,
-                                                        double *partials, T *__restrict__ trash) {
+                                                        double *partials, T *__restrict__ trash,
+                                                        const T *__restrict__ sin, const T *__restrict__ tin,
+                                                        T *__restrict__ rout, T *__restrict__ yv, int pid) {
     if (st->done) return;
     __shared__ PQLds<T> lv;
-    __shared__ double lds[2 * (K7_NT / 64)];
+    __shared__ double lds[5 * (K7_NT / 64)];
     const K7Pos Q = k7_pos(kt, g);
     const int tid = (int)threadIdx.x;
     const unsigned szt = (unsigned)g.nxy * sizeof(T);
@@ -497,6 +506,12 @@ __global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, G
         c1 = (T)st->beta;
         c2 = (T)st->omega;
     }  // MODE 1: alpha is summed below, while the prologue's loads are in flight
+    // FOLD, MODE 0: the previous iteration's alpha, omega for y; y is set
+    // (not accumulated) on the pass's second iteration
+    const double ya = FOLD && MODE == 0 ? st->alpha : 0.0, yw = FOLD && MODE == 0 ? st->omega : 0.0;
+    const bool yset = FOLD && MODE == 0 && st->it_pass == 1;
+    const T *const av = (FOLD && MODE == 0 && upd) ? sin : r;
+    if (FOLD && MODE == 0 && blockIdx.x == 0 && threadIdx.x == 0) stw->plast = pid;
     const int lo0 = kt.box.lo[0], hi0 = lo0 + kt.box.n[0], lo1 = kt.box.lo[1], hi1 = lo1 + kt.box.n[1];
     const int lo2 = kt.box.lo[2], hi2 = lo2 + kt.box.n[2];
     const int k0 = Q.k0, k1 = Q.k1;
@@ -525,30 +540,42 @@ __global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, G
     const int gi2 = Q.i0 - PQ_R + rx2, gj2 = Q.j0 - PQ_R + ry2;
     const int gi1 = Q.i0 - PQ_R + rx1, gj1 = Q.j0 - PQ_R + ry1;
     const bool in2 = r2 && inxy(gi2, gj2), in1 = r1 && inxy(gi1, gj1);
+    // colours at even planes (a cell's colour flips with the plane's parity)
+    const bool beo = black(Q.i, Q.j, 0), be2 = black(gi2, gj2, 0), be1 = black(gi1, gj1, 0);
     const auto c2v = a.column(in2 ? gi2 : oi, in2 ? gj2 : oj);
     const auto c1v = a.column(in1 ? gi1 : oi, in1 ? gj1 : oj);
     const unsigned b2 = in2 ? (unsigned)(gi2 + g.nx * gj2) * sizeof(T) : cb;
     const unsigned b1 = in1 ? (unsigned)(gi1 + g.nx * gj1) * sizeof(T) : cb;
     T *const tr = trash + (size_t)blockIdx.x * K7_NT + tid;
 
-    // operands of v at one cell of a plane, and its diagonal
+    // operands of v at one cell of a plane, and its diagonal: a is r, or
+    // with FOLD after a pass's first iteration s (and b is t: r = s - omega t)
     struct Ld {
-        T r, p, q, d;
+        T a, b, p, q, d;
     };
     const auto load = [&](const Col &c, unsigned ob, int k) {
         Ld x;
-        x.r = ld_at(r, ob);
+        x.a = ld_at(av, ob);
+        x.b = FOLD && MODE == 0 ? ld_at(tin, ob) : T(0);
         x.q = ld_at(qin, ob);
         x.p = MODE == 0 ? ld_at(pin, ob) : T(0);
         x.d = diag_of<DCLS>(c, ob, k);
         return x;
     };
-    const auto vec = [&](const Ld &x) -> T {
-        if (MODE == 1) return x.r - c1 * x.q;
-        return upd ? x.r + c1 * (x.p - c2 * x.q) : x.r;
+    // the residual a cell's v is formed from (FOLD: r = s - omega t, the
+    // previous iteration's kernel_PBiCGStab_4 with one rounding)
+    const auto res = [&](const Ld &x) -> T {
+        if (FOLD && MODE == 0) return upd ? fma(-c2, x.b, x.a) : x.a;
+        return x.a;
     };
+    const auto vec_of = [&](const Ld &x, T rv) -> T {
+        if (MODE == 1) return rv - c1 * x.q;
+        return upd ? rv + c1 * (x.p - c2 * x.q) : rv;
+    };
+    const auto vec = [&](const Ld &x) -> T { return vec_of(x, res(x)); };
     struct Fl {
         Ld o, h;  // the column's cell and the ring cell
+        T y;      // FOLD: the column's y
     };
     const auto fetch = [&](int f) {
         const int kk = kcl(f);
@@ -556,43 +583,84 @@ __global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, G
         Fl x;
         x.o = load(ca, cb + pf, kk);
         x.h = load(c2v, b2 + pf, kk);
+        x.y = FOLD && MODE == 0 ? ld_at(yv, cb + pf) : T(0);
         return x;
     };
     // slot of plane f: black cells M(v), red cells v, zero outside the
     // interior; v is written on the tile; returns the column's v
+    // the column's v at plane f, stored with (FOLD) its r and y:
+    // y = (0 or y) + alpha p + omega s of the previous iteration, what the
+    // update kernel accumulates
+    const auto own = [&](int f, const Fl &x, bool kf) -> T {
+        const T rv = res(x.o);
+        const T vo = (Q.act && kf) ? vec_of(x.o, rv) : T(0);
+        const bool wr = Q.act && f >= k0 && f < k1;
+        const unsigned ob = cb + (unsigned)f * szt;
+        *(wr ? (T *)((char *)vout + ob) : tr) = vo;
+        if (FOLD && MODE == 0) {
+            const bool wu = wr && upd;
+            *(wu ? (T *)((char *)rout + ob) : tr) = rv;
+            const double yo = (yset ? 0.0 : (double)x.y) + (ya * (double)x.o.p + yw * (double)x.o.a);
+            *(wu ? (T *)((char *)yv + ob) : tr) = (T)yo;
+        }
+        return vo;
+    };
     const auto fill = [&](int f, const Fl &x) -> T {
         const int sl = f & 3;
         const bool kf = kin(f);
-        const T vo = (Q.act && kf) ? vec(x.o) : T(0);
+        const T vo = own(f, x, kf);
         lv[sl][ryo][rxo] = black(Q.i, Q.j, f) ? T(0) + omega_pc * (vo / x.o.d) : vo;
-        const bool wr = Q.act && f >= k0 && f < k1;
-        *(wr ? (T *)((char *)vout + cb + (unsigned)f * szt) : tr) = vo;
         if (r2) {
             const T vh = (in2 && kf) ? vec(x.h) : T(0);
             lv[sl][ry2][rx2] = black(gi2, gj2, f) ? T(0) + omega_pc * (vh / x.h.d) : vh;
         }
         return vo;
     };
-    // red cells of plane m (tile and its strips) to M(v), in place; returns
-    // the column's M(v).  kernel_Struct3d7p_SOR's update from x = 0
-    // (src/dev/FalmEqDevCall.cu:195-215).
-    const auto red = [&](int m, const Cell &eo, const Cell &e1) -> T {
-        const int sl = m & 3, sm = (m - 1) & 3, sp = (m + 1) & 3;
-        const bool km = kin(m);
-        T ho = lv[sl][ryo][rxo];
-        if (Q.act && km && !black(Q.i, Q.j, m)) {
-            const T s = ca.apply(eo, T(0), lv[sl][ryo][rxo + 1], lv[sl][ryo][rxo - 1], lv[sl][ryo + 1][rxo],
-                                 lv[sl][ryo - 1][rxo], lv[sp][ryo][rxo], lv[sm][ryo][rxo]);
-            ho = T(0) + omega_pc * ((ho - s) / Col::diag(eo));
-            lv[sl][ryo][rxo] = ho;
+    // Step s's SOR work: plane s+2 into its slot (as fill) and the red cells
+    // of plane s (tile and its strips) to M(v), in place; returns the
+    // column's v at s+2 and sets h to its M(v) at s.  A cell has the same
+    // colour at s and s+2, so each lane divides once: black lanes for plane
+    // s+2, red lanes for plane s; a ring thread owns one ring cell (a v ring
+    // cell, black at s+2, or a red-sweep cell, red at s), never both.  Black
+    // lanes also form the red sum from their neighbours in slot s, which red
+    // lanes rewrite in this step: that sum is not used.  The red update is
+    // kernel_Struct3d7p_SOR's from x = 0 (src/dev/FalmEqDevCall.cu:195-215).
+    const auto fill_red = [&](int s, const Fl &x, const Cell &eo, const Cell &e1, T &h) -> T {
+        const int f = s + 2;
+        const int sf = f & 3, sl = s & 3, sm = (s - 1) & 3, sp = (s + 1) & 3;
+        const bool kf = kin(f), km = kin(s), odd = (s & 1) != 0;
+        const T vo = own(f, x, kf);
+        const bool bo = odd ? !beo : beo;
+        const T ho = lv[sl][ryo][rxo];
+        const T so = ca.apply(eo, T(0), lv[sl][ryo][rxo + 1], lv[sl][ryo][rxo - 1], lv[sl][ryo + 1][rxo],
+                              lv[sl][ryo - 1][rxo], lv[sp][ryo][rxo], lv[sm][ryo][rxo]);
+        const T qo = T(0) + omega_pc * ((bo ? vo : ho - so) / (bo ? x.o.d : Col::diag(eo)));
+        lv[sf][ryo][rxo] = bo ? qo : vo;
+        const bool ro = !bo && Q.act && km;
+        if (ro) lv[sl][ryo][rxo] = qo;
+        h = ro ? qo : ho;
+        if (r2 || r1) {
+            T num, den;
+            bool w2 = false, w1 = false;
+            if (r2) {
+                const T vh = (in2 && kf) ? vec(x.h) : T(0);
+                w2 = odd ? !be2 : be2;
+                if (!w2) lv[sf][ry2][rx2] = vh;
+                num = vh;
+                den = x.h.d;
+            } else {
+                const T v1 = lv[sl][ry1][rx1];
+                const T s1 = c1v.apply(e1, T(0), lv[sl][ry1][rx1 + 1], lv[sl][ry1][rx1 - 1], lv[sl][ry1 + 1][rx1],
+                                       lv[sl][ry1 - 1][rx1], lv[sp][ry1][rx1], lv[sm][ry1][rx1]);
+                w1 = in1 && km && !(odd ? !be1 : be1);
+                num = v1 - s1;
+                den = Col::diag(e1);
+            }
+            const T q = T(0) + omega_pc * (num / den);
+            if (w2) lv[sf][ry2][rx2] = q;
+            if (w1) lv[sl][ry1][rx1] = q;
         }
-        if (in1 && km && !black(gi1, gj1, m)) {
-            const T v1 = lv[sl][ry1][rx1];
-            const T s = c1v.apply(e1, T(0), lv[sl][ry1][rx1 + 1], lv[sl][ry1][rx1 - 1], lv[sl][ry1 + 1][rx1],
-                                  lv[sl][ry1 - 1][rx1], lv[sp][ry1][rx1], lv[sm][ry1][rx1]);
-            lv[sl][ry1][rx1] = T(0) + omega_pc * ((v1 - s) / Col::diag(e1));
-        }
-        return ho;
+        return vo;
     };
     const auto cell_at = [&](const Col &c, unsigned base, int k) {
         const int kk = kcl(k);
@@ -618,7 +686,7 @@ __global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, G
     T v0 = fill(k0 - 2, f0), v1 = fill(k0 - 1, f1), v2 = fill(k0, f2);
     T hm = T(0), hc = T(0);
     __syncthreads();
-    double acc[2] = {0.0, 0.0};
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     // Step s with two buffer sets: the `use` set holds what step s-1 loaded
     // (plane s+2's operands, rr at plane s-1, the cells of plane s; eo holds
     // plane s-1's cells), the `load` set receives plane s+3's operands, rr at
@@ -636,10 +704,10 @@ __global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, G
         e1l = cell_at(c1v, b1, s + 1);
         {
             const int k = s < k0 ? k0 : (s >= k1 ? k1 - 1 : s);
-            rl = MODE == 0 ? ld_at(rr, cb + (unsigned)k * szt) : T(0);
+            rl = (MODE == 0 || FOLD) ? ld_at(rr, cb + (unsigned)k * szt) : T(0);
         }
-        const T vn = fill(s + 2, fu);
-        const T h = red(s, eu, e1u);
+        T h;
+        const T vn = fill_red(s, fu, eu, e1u, h);
         // w at plane s-1 (the column's M(v): s-2, s-1, s)
         const int sl = (s - 1) & 3;
         const T wv = ca.apply(eq, hc, lv[sl][ryo][rxo + 1], lv[sl][ryo][rxo - 1], lv[sl][ryo + 1][rxo],
@@ -651,6 +719,13 @@ __global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, G
         } else {
             acc[0] += wr ? (double)wv * (double)v0 : 0.0;
             acc[1] += wr ? (double)wv * (double)wv : 0.0;
+            if (FOLD) {
+                // (s, s), (s, rr), (t, rr): |r|^2 and (r, rr) of the next
+                // residual r = s - omega t follow from these and the two above
+                acc[2] += wr ? (double)v0 * (double)v0 : 0.0;
+                acc[3] += wr ? (double)v0 * (double)ru : 0.0;
+                acc[4] += wr ? (double)wv * (double)ru : 0.0;
+            }
         }
         hm = hc;
         hc = h;
@@ -669,7 +744,12 @@ __global__ __launch_bounds__(K7_NT) PCMV_ATTR void k_bicg_pcmv_km(K7Tiling kt, G
         double v1s[1] = {acc[0]};
         block_partials<K7_NT, 1>(v1s, partials, lds);
     } else {
-        block_partials<K7_NT, 2>(acc, partials, lds);
+        if (FOLD) {
+            block_partials<K7_NT, 5>(acc, partials, lds);
+        } else {
+            double a2[2] = {acc[0], acc[1]};
+            block_partials<K7_NT, 2>(a2, partials, lds);
+        }
     }
 }
 
@@ -1130,18 +1210,20 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_omega(DevState *st, const double
 // r = s - omega t (kernel_PBiCGStab_4); partials |r|^2, (r, rr).
 // DEFER (preconditioning deferred to the end of the pass): pp, ss are p and
 // s, and the sum goes to y (T; set on a pass's first iteration) instead of x.
-template <class T, bool DEFER = false>
+template <class T, bool DEFER = false, bool END = false>
 __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *__restrict__ x,
                                                     const T *__restrict__ pp, const T *__restrict__ ss,
                                                     const T *__restrict__ s, const T *__restrict__ t,
                                                     T *__restrict__ r, const T *__restrict__ rr, const DevState *st,
-                                                    double *partials, T *__restrict__ y = nullptr) {
-    if (st->done) return;
+                                                    double *partials, T *__restrict__ y = nullptr,
+                                                    const T *__restrict__ pq = nullptr) {
+    if (END ? !(st->done && st->pending && !st->final) : st->done) return;
+    if (END && st->plast) pp = pq;
     __shared__ double lds[2 * (NT / 64)];
     int i, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha, omega = st->omega;
-    const bool first = st->first;
+    const bool first = END ? st->it_pass < 2 : st->first;
     tile_range(tl.tiles, t0, t1);
     for (int tile = t0; tile < t1; tile++) {
         if (!tile_cell_at(tl, tile, i, j, k)) continue;
@@ -1172,22 +1254,27 @@ template <class T> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
 
-template <class T, bool DEFER = false>
+// END (folded update, deferred): the last iteration's update once its
+// pass is over (y, read only if the pass had two iterations or more, and r);
+// p is the buffer the last p kernel wrote (pp or pq).
+template <class T, bool DEFER = false, bool END = false>
 __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g, double *__restrict__ x,
                                                      const T *__restrict__ pp, const T *__restrict__ ss,
                                                      const T *__restrict__ s, const T *__restrict__ t,
                                                      T *__restrict__ r, const T *__restrict__ rr,
                                                      const DevState *st, double *partials, const double *opart,
-                                                     int onb, DevState *stw, T *__restrict__ y = nullptr) {
+                                                     int onb, DevState *stw, T *__restrict__ y = nullptr,
+                                                     const T *__restrict__ pq = nullptr) {
     using T2 = typename Pair<T>::type;
     using X2 = typename std::conditional<DEFER, T2, double2>::type;
-    if (st->done) return;
+    if (END ? !(st->done && st->pending && !st->final) : st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha;
     // DEFER: the pass's first iteration sets y (no read)
-    const bool xread = !DEFER || !st->first;
+    const bool xread = !DEFER || (END ? st->it_pass >= 2 : !st->first);
+    if (END && st->plast) pp = pq;
     // a pair's operands, loaded one tile ahead of their use
     struct Pr {
         T2 p2, s2, sv, tv, r0;
@@ -1287,6 +1374,42 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_update(DevState *st, const doubl
                 st->done = 1;
             } else {
                 st->beta = (st->rho * st->alpha) / (st->rrho * st->omega);
+            }
+        }
+    }
+}
+
+// k_fin_update for the folded update: the s kernel's partial sums (t, s),
+// (t, t), (s, s), (s, rr), (t, rr) give omega and, for r = s - omega t,
+// |r|^2 = (s, s) - omega (t, s) (omega minimises |s - omega t|) and
+// (r, rr) = (s, rr) - omega (t, rr), so the stop test and beta come before r
+// is formed (the next p kernel forms it).
+__global__ __launch_bounds__(FIN_NT) void k_fin_fold(DevState *st, const double *partials, int nb) {
+    if (st->done) return;
+    double tot[5];
+    sum_partials<5>(partials, nb, tot);
+    if (threadIdx.x == 0) {
+        const double omega = tot[0] / tot[1];
+        const double r2 = tot[2] - omega * tot[0];
+        st->omega = omega;
+        st->rrho = st->rho;
+        st->err = sqrt(r2 > 0.0 ? r2 : 0.0) / st->gN;
+        st->it_pass += 1;
+        st->total += 1;
+        st->first = 0;
+        st->pending = 1;
+        if (!(st->it_pass < st->maxit_pass && st->err > st->tol_inner)) {
+            st->done = 1;
+        } else {
+            st->rho = tot[3] - omega * tot[4];
+            if (fabs(st->rho) < (double)FLT_MIN) {
+                if (!st->mixed) {
+                    st->err = st->rho;
+                    st->status = FALM_ERR_BREAKDOWN;
+                }
+                st->done = 1;
+            } else {
+                st->beta = (st->rho * st->alpha) / (st->rrho * omega);
             }
         }
     }
@@ -1477,7 +1600,7 @@ static falm_status solver_alloc(falm_solver *S) {
     if ((size_t)S->ktp.blocks > nb) nb = (size_t)S->ktp.blocks;
     FALM_HIP(hipMalloc(&S->partials, 2 * nb * sizeof(double)));
     FALM_HIP(hipMalloc(&S->trash, (size_t)(S->ktp.blocks > 0 ? S->ktp.blocks : 1) * K7_NT * sizeof(double)));
-    FALM_HIP(hipMalloc(&S->partials2, 2 * nb * sizeof(double)));
+    FALM_HIP(hipMalloc(&S->partials2, 5 * nb * sizeof(double)));  // 5 sums: the folded s kernel
     // the zero fills above run on the null stream, which the caller's
     // (possibly non-blocking) stream does not wait for
     FALM_HIP(hipDeviceSynchronize());
@@ -1597,6 +1720,38 @@ static bool deferred_pc(const falm_solver *S) {
     return FALM_PCMV && S->cfg.pc_type == FALM_LS_SOR && S->cfg.pc_maxit <= 1 && S->kt.blocks > 0;
 }
 
+// The update folded into the next iteration's p kernel (r = s - omega t and
+// y += alpha p + omega s there; |r| and (r, rr) from the s kernel's sums):
+// two kernels and one finalize per iteration, and the last iteration's
+// update once the pass is over (launch_fold_end).
+#ifndef FALM_FOLD
+#define FALM_FOLD 1
+#endif
+static bool folded_update(const falm_solver *S) { return FALM_FOLD && deferred_pc(S); }
+
+// the last iteration's y and r once a pass is over (no-op otherwise)
+template <class T>
+static falm_status launch_fold_end(falm_solver *S, T *r, hipStream_t s) {
+    const Tiling &tl = S->tl;
+    TimedLaunch tm(FALM_TIME_SOLVER_UPDATE, s);
+    if (S->tp.tiles > 0) {
+        const int nbu = S->tp.tiles < UPD_BLOCKS ? S->tp.tiles : UPD_BLOCKS;
+        hipLaunchKernelGGL((k_bicg_update2<T, true, true>), dim3((unsigned)nbu), tiling_block(), 0, s, S->tp,
+                           tl.box.lo[0], S->g, (double *)nullptr, (const T *)S->p, (const T *)S->s, (const T *)S->s,
+                           (const T *)S->t, r, (const T *)S->rr, (const DevState *)S->st, S->partials,
+                           (const double *)nullptr, 0, (DevState *)nullptr, (T *)S->ss, (const T *)S->w);
+    } else {
+        hipLaunchKernelGGL((k_bicg_update<T, true, true>), FALM_RED(S), s, tl, S->g, (double *)nullptr,
+                           (const T *)S->p, (const T *)S->s, (const T *)S->s, (const T *)S->t, r, (const T *)S->rr,
+                           (const DevState *)S->st, S->partials, (T *)S->ss, (const T *)S->w);
+    }
+    // p, s, t, rr, y read, y and r written (a launch that does nothing
+    // still books its bytes: it runs once per poll)
+    tm.done((double)tl.box.size() * 7.0 * sizeof(T));
+    FALM_CHECK_LAUNCH("k_bicg_update<fold end>");
+    return FALM_OK;
+}
+
 // x += M(y) once a pass is over (no-op while it runs or once its sum is in)
 template <class T, class OP>
 static falm_status launch_pc_apply(falm_solver *S, const OP &a, double *x, hipStream_t s) {
@@ -1624,7 +1779,8 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         // p, q alternate between two buffers (the p kernel reads both at the
         // ring cells of its tiles); a pass's first iteration reads neither,
         // so the alternation may run on across iterations a finished pass
-        // skips
+        // skips (the folded p kernel records which buffer it wrote)
+        const int pid = S->pflip ? 0 : 1;  // 1: p goes to w
         T *pin = (T *)(S->pflip ? S->w : S->p), *pout = (T *)(S->pflip ? S->p : S->w);
         T *qin = (T *)(S->qflip ? S->pp : S->q), *qout = (T *)(S->qflip ? S->q : S->pp);
         S->pflip ^= 1;
@@ -1633,33 +1789,47 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         const int nbp = S->ktp.blocks;
         const dim3 gp((unsigned)nbp), bp(K7_NT);
         T *trash = (T *)S->trash;
+        const bool fold = folded_update(S);
         // the diagonal from its classes (a compile-time choice in the kernel)
         bool dcls = false;
         if constexpr (std::is_same<OP, OpSep<T>>::value) dcls = a.dtab != nullptr;
-        const auto launch = [&](auto dc) {
-            constexpr bool D = decltype(dc)::value;
+        const auto launch = [&](auto dc, auto fd) {
+            constexpr bool D = decltype(dc)::value, F = decltype(fd)::value;
             {
                 TimedLaunch tm(FALM_TIME_PCMV, s);
-                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D, F>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
                                    (const T *)pin, (const T *)qin, (const T *)rr, pout, qout, (const DevState *)S->st,
-                                   wpc, (const double *)nullptr, 0, (DevState *)nullptr, S->partials, trash);
+                                   wpc, (const double *)nullptr, 0, S->st, S->partials, trash, (const T *)sv,
+                                   (const T *)t, r, ss, pid);
                 // r, rr (and the diagonal) read, p and q written; after a
-                // pass's first iteration also p and q read
-                tm.done(box * ((first ? 4.0 : 6.0) * sizeof(T) + db));
+                // pass's first iteration also p and q read, and when folded
+                // s and t read instead of r, y read and written, r written
+                tm.done(box * ((first ? 4.0 : (F ? 10.0 : 6.0)) * sizeof(T) + db));
             }
             {
                 TimedLaunch tm(FALM_TIME_PCMV_S, s);
-                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
-                                   (const T *)nullptr, (const T *)qout, (const T *)nullptr, sv, t,
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D, F>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                                   (const T *)nullptr, (const T *)qout, (const T *)rr, sv, t,
                                    (const DevState *)S->st, wpc, (const double *)S->partials, nbp, S->st,
-                                   S->partials2, trash);
-                // r, q (and the diagonal) read, s and t written
-                tm.done(box * (4.0 * sizeof(T) + db));
+                                   S->partials2, trash, (const T *)nullptr, (const T *)nullptr, (T *)nullptr,
+                                   (T *)nullptr, 0);
+                // r, q (and the diagonal) read, s and t written; folded: rr read
+                tm.done(box * ((F ? 5.0 : 4.0) * sizeof(T) + db));
             }
         };
-        if (dcls) launch(std::true_type{});
-        else launch(std::false_type{});
+        if (dcls) {
+            if (fold) launch(std::true_type{}, std::true_type{});
+            else launch(std::true_type{}, std::false_type{});
+        } else {
+            if (fold) launch(std::false_type{}, std::true_type{});
+            else launch(std::false_type{}, std::false_type{});
+        }
         FALM_CHECK_LAUNCH("k_bicg_pcmv_km");
+        if (fold) {
+            hipLaunchKernelGGL(k_fin_fold, FALM_FIN, s, S->st, (const double *)S->partials2, nbp);
+            FALM_CHECK_LAUNCH("k_fin_fold");
+            return FALM_OK;
+        }
         TimedLaunch tm2(FALM_TIME_SOLVER_UPDATE, s);
         int nbu = red_blocks(tl);
         if (S->tp.tiles > 0) {
@@ -1808,6 +1978,7 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
             FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s, enqueued == 0)));
         // the pass's sum into x, if the pass is over (before the poll, so x
         // is complete when the solve returns)
+        if (folded_update(S)) FALM_TRY((launch_fold_end<double>(S, r, s)));
         if (deferred_pc(S)) FALM_TRY((launch_pc_apply<double, OP64>(S, a, x, s)));
         FALM_TRY(read_state(S, s));
         if (S->hst->done || enqueued >= c.maxit) break;
@@ -1842,6 +2013,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
     int batch = first_batch(S), polls = 0;
     while (true) {
         for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, polls == 0 && k == 0)));
+        if (folded_update(S)) FALM_TRY((launch_fold_end<float>(S, r32, s)));
         if (deferred_pc(S)) FALM_TRY((launch_pc_apply<float, OP32>(S, a32, x, s)));
         // a finished pass is tested with the norm alone, unless the caller
         // wants the final residual; a pass that must follow starts from the
@@ -2042,6 +2214,10 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
         set_error("falm_solver_create: maxit must be >= 1");
         return FALM_ERR_INVALID;
     }
+    int dev = 0, cus = 0;
+    FALM_HIP(hipGetDevice(&dev));
+    FALM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus < 1) cus = 1;
     falm_solver *S = new falm_solver();
     S->dom = *dom;
     S->cfg = *cfg;
@@ -2049,11 +2225,7 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
     S->g = make_geom(dom);
     S->tl = make_tiling(interior_box(S->g));
     S->kt = make_k7_tiling(interior_box(S->g));
-#ifdef FALM_PCMV_EXACT
-    S->ktp = make_k7_tiling_fit(interior_box(S->g), PCMV_BLOCKS);
-#else
-    S->ktp = make_k7_tiling(interior_box(S->g), PCMV_BLOCKS);
-#endif
+    S->ktp = make_k7_tiling_fit(interior_box(S->g), cus * pcmv_wg_per_cu(cfg->precision != FALM_SOLVE_MIXED));
     {
         // cell pairs start at even offsets when the interior starts at an
         // even x index on rows of even length
