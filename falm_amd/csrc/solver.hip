@@ -232,17 +232,27 @@ __global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, 
 // the pass's alpha p + omega s, and M(y) is added to x (double); it runs
 // once a pass is over and its sum not yet added (the init kernels' finalize
 // clears that).
+// MODE 3 (the same with the update folded): the pass's last alpha p + omega s
+// is added to y on the fly, (0 or y) + alpha p + omega s with r = y, vin and
+// v the two p buffers (the last p kernel recorded which it wrote) and q = s,
+// the expression the update kernel uses; vh, when given, receives the last
+// residual s - omega t (t from tin).
 template <class T, class OP, int MODE>
 __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
                                                         const T *__restrict__ vin, const T *__restrict__ q,
                                                         T *__restrict__ v, T *__restrict__ vh, const DevState *st,
                                                         T omega_pc, const double *apart, int anb, DevState *stw,
-                                                        double *__restrict__ xd = nullptr) {
-    if (MODE == 2 ? (!st->done || st->final || !st->pending) : st->done) return;
+                                                        double *__restrict__ xd = nullptr,
+                                                        const T *__restrict__ tin = nullptr) {
+    if (MODE >= 2 ? (!st->done || st->final || !st->pending) : st->done) return;
     __shared__ K7Lds<T> lw;
     const K7Pos Q = k7_pos(kt, g);
     const int sz = g.nxy;
-    const bool upd = MODE == 1 || (MODE == 0 && !st->first);  // q (and vin) are read
+    const bool upd = MODE == 1 || MODE == 3 || (MODE == 0 && !st->first);  // q (and vin) are read
+    const double ya = MODE == 3 ? st->alpha : 0.0, yw = MODE == 3 ? st->omega : 0.0;
+    const bool yread = MODE == 3 && st->it_pass >= 2;
+    const T *const pv_in = (MODE == 3 && st->plast) ? (const T *)v : vin;
+    const T om = (T)yw;
     T c1 = T(0), c2 = T(0);
     if (MODE == 0) {
         c1 = (T)st->beta;
@@ -251,6 +261,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         c1 = (T)st->alpha;
     }  // else: alpha is summed below, while the prologue's loads are in flight
     const auto vec = [&](T rv, T pv, T qv) -> T {
+        if (MODE == 3) return (T)((yread ? (double)rv : 0.0) + (ya * (double)pv + yw * (double)qv));
         if (MODE == 1) return rv - c1 * qv;
         return upd ? rv + c1 * (pv - c2 * qv) : rv;
     };
@@ -279,7 +290,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         rv = ld_at(r, ob);
         if (upd) {
             qv = ld_at(q, ob);
-            if (MODE == 0) pv = ld_at(vin, ob);
+            if (MODE == 0 || MODE == 3) pv = ld_at(pv_in, ob);
         }
     };
     // registers: w at k-1 and k, v and the cell coefficients at k, the
@@ -319,17 +330,26 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         lw[0][Q.wy + 1][Q.lane + 1] = wc;
         if (h.slot) lw[0][h.row][h.col] = wval(vec(hr, hp, hq), hd, hin, hpar, Q.k0);
     }
-    // MODE 2: x a plane ahead (act columns only)
+    // MODE 2, 3: x a plane ahead (act columns only); MODE 3 with vh: the
+    // column's s at k (sc) and t a plane ahead
     constexpr unsigned XS = (unsigned)(sizeof(double) / sizeof(T));
     double xq = 0.0;
-    if (MODE == 2 && Q.act) xq = ld_at(xd, (cb + (unsigned)Q.k0 * szt) * XS);
+    if (MODE >= 2 && Q.act) xq = ld_at(xd, (cb + (unsigned)Q.k0 * szt) * XS);
+    const bool rw = MODE == 3 && vh != nullptr;
+    T tq = T(0), sc = T(0);
+    if (MODE == 3) {
+        sc = ld_at(q, cb + (unsigned)Q.k0 * szt);
+        if (rw) tq = ld_at(tin, cb + (unsigned)Q.k0 * szt);
+    }
     __syncthreads();
     int bf = 0;
     unsigned ob = cb + (unsigned)Q.k0 * szt;
     for (int k = Q.k0; k < Q.k1; k++, ob += szt) {
         const bool more = k + 1 < Q.k1;
         const double xc = xq;
-        if (MODE == 2 && more && Q.act) xq = ld_at(xd, (ob + szt) * XS);
+        if (MODE >= 2 && more && Q.act) xq = ld_at(xd, (ob + szt) * XS);
+        const T tc = tq;
+        if (rw && more) tq = ld_at(tin, ob + szt);
         T r2 = T(0), p2 = T(0), q2 = T(0);
         Cell e2 = ne;
         if (more) {
@@ -344,7 +364,8 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         const T vn = vec(nr, np, nq);
         const T wn = wval(vn, Col::diag(ne), Q.act && kin(k + 1), cpar, k + 1);
         if (Q.act) {
-            if (MODE != 2) st_at(v, ob, vc);
+            if (MODE < 2) st_at(v, ob, vc);
+            if (rw) st_at(vh, ob, sc - om * tc);
             T out = wc;
             if (((cpar + k) & 1) != FALM_COLOR_BLACK) {
                 // kernel_Struct3d7p_SOR's update from x = 0 (src/dev/FalmEqDevCall.cu:195-215)
@@ -352,7 +373,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
                                      lw[bf][Q.wy + 2][Q.lane + 1], lw[bf][Q.wy][Q.lane + 1], wn, wm);
                 out = T(0) + omega_pc * ((vc - s) / Col::diag(ec));
             }
-            if (MODE == 2) {
+            if (MODE >= 2) {
                 st_at(xd, ob * XS, xc + (double)out);
             } else {
                 st_at(vh, ob, out);
@@ -366,6 +387,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         wc = wn;
         vc = vn;
         ec = ne;
+        if (MODE == 3) sc = nq;
         nr = r2;
         np = p2;
         nq = q2;
@@ -1210,20 +1232,18 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_omega(DevState *st, const double
 // r = s - omega t (kernel_PBiCGStab_4); partials |r|^2, (r, rr).
 // DEFER (preconditioning deferred to the end of the pass): pp, ss are p and
 // s, and the sum goes to y (T; set on a pass's first iteration) instead of x.
-template <class T, bool DEFER = false, bool END = false>
+template <class T, bool DEFER = false>
 __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *__restrict__ x,
                                                     const T *__restrict__ pp, const T *__restrict__ ss,
                                                     const T *__restrict__ s, const T *__restrict__ t,
                                                     T *__restrict__ r, const T *__restrict__ rr, const DevState *st,
-                                                    double *partials, T *__restrict__ y = nullptr,
-                                                    const T *__restrict__ pq = nullptr) {
-    if (END ? !(st->done && st->pending && !st->final) : st->done) return;
-    if (END && st->plast) pp = pq;
+                                                    double *partials, T *__restrict__ y = nullptr) {
+    if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int i, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha, omega = st->omega;
-    const bool first = END ? st->it_pass < 2 : st->first;
+    const bool first = st->first;
     tile_range(tl.tiles, t0, t1);
     for (int tile = t0; tile < t1; tile++) {
         if (!tile_cell_at(tl, tile, i, j, k)) continue;
@@ -1254,27 +1274,22 @@ template <class T> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
 
-// END (folded update, deferred): the last iteration's update once its
-// pass is over (y, read only if the pass had two iterations or more, and r);
-// p is the buffer the last p kernel wrote (pp or pq).
-template <class T, bool DEFER = false, bool END = false>
+template <class T, bool DEFER = false>
 __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g, double *__restrict__ x,
                                                      const T *__restrict__ pp, const T *__restrict__ ss,
                                                      const T *__restrict__ s, const T *__restrict__ t,
                                                      T *__restrict__ r, const T *__restrict__ rr,
                                                      const DevState *st, double *partials, const double *opart,
-                                                     int onb, DevState *stw, T *__restrict__ y = nullptr,
-                                                     const T *__restrict__ pq = nullptr) {
+                                                     int onb, DevState *stw, T *__restrict__ y = nullptr) {
     using T2 = typename Pair<T>::type;
     using X2 = typename std::conditional<DEFER, T2, double2>::type;
-    if (END ? !(st->done && st->pending && !st->final) : st->done) return;
+    if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha;
     // DEFER: the pass's first iteration sets y (no read)
-    const bool xread = !DEFER || (END ? st->it_pass >= 2 : !st->first);
-    if (END && st->plast) pp = pq;
+    const bool xread = !DEFER || !st->first;
     // a pair's operands, loaded one tile ahead of their use
     struct Pr {
         T2 p2, s2, sv, tv, r0;
@@ -1722,35 +1737,13 @@ static bool deferred_pc(const falm_solver *S) {
 
 // The update folded into the next iteration's p kernel (r = s - omega t and
 // y += alpha p + omega s there; |r| and (r, rr) from the s kernel's sums):
-// two kernels and one finalize per iteration, and the last iteration's
-// update once the pass is over (launch_fold_end).
+// two kernels and one finalize per iteration; the last iteration's update
+// goes into the pass's closing preconditioner kernel (launch_fold_apply).
 #ifndef FALM_FOLD
 #define FALM_FOLD 1
 #endif
 static bool folded_update(const falm_solver *S) { return FALM_FOLD && deferred_pc(S); }
 
-// the last iteration's y and r once a pass is over (no-op otherwise)
-template <class T>
-static falm_status launch_fold_end(falm_solver *S, T *r, hipStream_t s) {
-    const Tiling &tl = S->tl;
-    TimedLaunch tm(FALM_TIME_SOLVER_UPDATE, s);
-    if (S->tp.tiles > 0) {
-        const int nbu = S->tp.tiles < UPD_BLOCKS ? S->tp.tiles : UPD_BLOCKS;
-        hipLaunchKernelGGL((k_bicg_update2<T, true, true>), dim3((unsigned)nbu), tiling_block(), 0, s, S->tp,
-                           tl.box.lo[0], S->g, (double *)nullptr, (const T *)S->p, (const T *)S->s, (const T *)S->s,
-                           (const T *)S->t, r, (const T *)S->rr, (const DevState *)S->st, S->partials,
-                           (const double *)nullptr, 0, (DevState *)nullptr, (T *)S->ss, (const T *)S->w);
-    } else {
-        hipLaunchKernelGGL((k_bicg_update<T, true, true>), FALM_RED(S), s, tl, S->g, (double *)nullptr,
-                           (const T *)S->p, (const T *)S->s, (const T *)S->s, (const T *)S->t, r, (const T *)S->rr,
-                           (const DevState *)S->st, S->partials, (T *)S->ss, (const T *)S->w);
-    }
-    // p, s, t, rr, y read, y and r written (a launch that does nothing
-    // still books its bytes: it runs once per poll)
-    tm.done((double)tl.box.size() * 7.0 * sizeof(T));
-    FALM_CHECK_LAUNCH("k_bicg_update<fold end>");
-    return FALM_OK;
-}
 
 // x += M(y) once a pass is over (no-op while it runs or once its sum is in)
 template <class T, class OP>
@@ -1762,6 +1755,23 @@ static falm_status launch_pc_apply(falm_solver *S, const OP &a, double *x, hipSt
     // y (and the diagonal) read, x read and written
     tm.done((double)S->tl.box.size() * (sizeof(T) + 2.0 * sizeof(double) + diag_bytes<OP>(S)));
     FALM_CHECK_LAUNCH("k_bicg_vsor_km<apply>");
+    return FALM_OK;
+}
+
+// folded update: x += M((0 or y) + alpha p + omega s) of the pass once it
+// is over, and the last residual s - omega t into rout when given (no-op
+// while the pass runs)
+template <class T, class OP>
+static falm_status launch_fold_apply(falm_solver *S, const OP &a, double *x, T *rout, hipStream_t s) {
+    TimedLaunch tm(FALM_TIME_PC_APPLY, s);
+    hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 3>), FALM_KM(S), s, S->kt, S->g, a, (const T *)S->ss, (const T *)S->p,
+                       (const T *)S->s, (T *)S->w, rout, (const DevState *)S->st, (T)S->cfg.pc_relax,
+                       (const double *)nullptr, 0, (DevState *)nullptr, x, (const T *)S->t);
+    // y, p, s (and the diagonal) read, x read and written; with rout also t
+    // read and r written
+    tm.done((double)S->tl.box.size() *
+            ((rout ? 5.0 : 3.0) * sizeof(T) + 2.0 * sizeof(double) + diag_bytes<OP>(S)));
+    FALM_CHECK_LAUNCH("k_bicg_vsor_km<fold apply>");
     return FALM_OK;
 }
 
@@ -1978,8 +1988,8 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
             FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s, enqueued == 0)));
         // the pass's sum into x, if the pass is over (before the poll, so x
         // is complete when the solve returns)
-        if (folded_update(S)) FALM_TRY((launch_fold_end<double>(S, r, s)));
-        if (deferred_pc(S)) FALM_TRY((launch_pc_apply<double, OP64>(S, a, x, s)));
+        if (folded_update(S)) FALM_TRY((launch_fold_apply<double, OP64>(S, a, x, r == S->r64own ? nullptr : r, s)));
+        else if (deferred_pc(S)) FALM_TRY((launch_pc_apply<double, OP64>(S, a, x, s)));
         FALM_TRY(read_state(S, s));
         if (S->hst->done || enqueued >= c.maxit) break;
         batch = check_period(S);
@@ -2013,8 +2023,10 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
     int batch = first_batch(S), polls = 0;
     while (true) {
         for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, polls == 0 && k == 0)));
-        if (folded_update(S)) FALM_TRY((launch_fold_end<float>(S, r32, s)));
-        if (deferred_pc(S)) FALM_TRY((launch_pc_apply<float, OP32>(S, a32, x, s)));
+        // (the float residual is not kept: the init kernels form the next
+        // pass's from x)
+        if (folded_update(S)) FALM_TRY((launch_fold_apply<float, OP32>(S, a32, x, (float *)nullptr, s)));
+        else if (deferred_pc(S)) FALM_TRY((launch_pc_apply<float, OP32>(S, a32, x, s)));
         // a finished pass is tested with the norm alone, unless the caller
         // wants the final residual; a pass that must follow starts from the
         // full init (it recomputes the same residual)
