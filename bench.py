@@ -61,16 +61,16 @@ def families():
         "sgs": (abi.TIME_SGS, "k_sgs_div (Smagorinsky + divergence + |div|^2)"),
         "divergence": (abi.TIME_DIVERGENCE, "k_div_norm (divergence + |div|^2)"),
         "solver_init": (abi.TIME_SOLVER_INIT, "k_bicg_init (b - A x in double + dots)"),
-        "sor": (abi.TIME_SOR, "k_bicg_vsor_km<., ., 0> (p = r + beta(p - omega q) + one-pass RB-SOR preconditioner)"),
-        "sor_s": (abi.TIME_SOR_S, "k_bicg_vsor_km<., ., 1> (s = r - alpha q + one-pass RB-SOR preconditioner)"),
-        "mv": (abi.TIME_MV, "k_bicg_mv_km<., ., 1> (q = A p^ + (rr, q))"),
-        "mv2": (abi.TIME_MV2, "k_bicg_mv_km<., ., 2> (t = A s^ + (t, s), (t, t))"),
-        "pcmv": (abi.TIME_PCMV, "k_bicg_pcmv_km<., ., 0> (p = r + beta(p - omega q), q = A M(p), (rr, q); "
-                                "one-pass RB-SOR M on a two-cell ring)"),
-        "pcmv_s": (abi.TIME_PCMV_S, "k_bicg_pcmv_km<., ., 1> (s = r - alpha q, t = A M(s), (t, s), (t, t))"),
-        "pc_apply": (abi.TIME_PC_APPLY, "k_bicg_vsor_km<., ., 2> (x += M(y) at the end of a pass)"),
-        "solver_update": (abi.TIME_SOLVER_UPDATE, "k_bicg_update (y += alpha p + omega s, r = s - omega t + dots; "
-                                                  "x += alpha p^ + omega s^ without deferral)"),
+        "sor": (abi.TIME_SOR, "k_sor_op_km / k_sor3_km (separate preconditioner sweeps, pc_maxit >= 2)"),
+        "mv": (abi.TIME_MV, "k_bicg_mv_km<., ., 1> (q = A p^ + (rr, q); preconditioners other than one SOR pass)"),
+        "mv2": (abi.TIME_MV2, "k_bicg_mv_km<., ., 2> (t = A s^ + (t, s), (t, t); the same)"),
+        "pcmv": (abi.TIME_PCMV, "k_bicg_pcmv_km<., ., 0> (r = s - omega t, p = r + beta(p - omega q), "
+                                "q = A M(p), (rr, q), y += alpha p + omega s; one-pass RB-SOR M on a two-cell ring)"),
+        "pcmv_s": (abi.TIME_PCMV_S, "k_bicg_pcmv_km<., ., 1> (s = r - alpha q, t = A M(s), (t, s), (t, t), (s, s), "
+                                    "(s, rr), (t, rr))"),
+        "pc_apply": (abi.TIME_PC_APPLY, "k_bicg_close_km (end of a pass: x += M(y + alpha p + omega s))"),
+        "solver_update": (abi.TIME_SOLVER_UPDATE, "k_bicg_update (x += alpha p^ + omega s^, r = s - omega t + dots; "
+                                                  "preconditioners other than one SOR pass)"),
         "alm": (abi.TIME_ALM, "ALM stage (points, box, factors, projection, totals): a latency chain, "
                               "bytes = point tables + force-box cells written"),
     }

@@ -214,56 +214,36 @@ __global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, 
     }
 }
 
-// A Krylov vector and its one-pass SOR preconditioning (pc_maxit = 1: the
-// black half-sweep from zero, then the red one) in one k-marching pass:
-//   MODE 0: v = r on the first iteration, else r + beta(vin - omega q)
-//           (kernel_PBiCGStab_1, src/dev/FalmEqDevCall.cu)
-//   MODE 1: v = r - alpha q (kernel_PBiCGStab_2)
-//   black cells: vh = 0 + omega_pc (v / a_c)              (first half-sweep)
-//   red cells:   vh = 0 + omega_pc ((v - A'vh) / a_c)      (second half-sweep)
-// where A'vh sums the six neighbours, all black.  Their vh is recomputed from
-// r, q, vin and the diagonal at the neighbour with the expression the
-// neighbour's own thread uses (in LDS for x and y, along the column in
-// registers for k); cells outside the interior contribute zero, as the
-// preconditioned vector's ghost cells (zero from allocation, never written)
-// do in the separate sweeps.  MODE 0 reads p from vin and writes v, another
-// buffer: neighbouring workgroups read vin at this tile's cells.
-// MODE 2 (end of a pass with deferred preconditioning): r is the sum y of
-// the pass's alpha p + omega s, and M(y) is added to x (double); it runs
-// once a pass is over and its sum not yet added (the init kernels' finalize
-// clears that).
-// MODE 3 (the same with the update folded): the pass's last alpha p + omega s
-// is added to y on the fly, (0 or y) + alpha p + omega s with r = y, vin and
-// v the two p buffers (the last p kernel recorded which it wrote) and q = s,
-// the expression the update kernel uses; vh, when given, receives the last
-// residual s - omega t (t from tin).
-template <class T, class OP, int MODE>
-__global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
-                                                        const T *__restrict__ vin, const T *__restrict__ q,
-                                                        T *__restrict__ v, T *__restrict__ vh, const DevState *st,
-                                                        T omega_pc, const double *apart, int anb, DevState *stw,
-                                                        double *__restrict__ xd = nullptr,
-                                                        const T *__restrict__ tin = nullptr) {
-    if (MODE >= 2 ? (!st->done || st->final || !st->pending) : st->done) return;
+// The end of a BiCGStab pass with the one-pass SOR preconditioner M (black
+// half-sweep from zero, then the red one; k_bicg_pcmv_km below): the pass's
+// solution update is x += M(z) with z = (0 or y) + alpha p + omega s, y the
+// pass's earlier alpha p + omega s (present from the second iteration on) and
+// alpha p + omega s the last iteration's, summed as the p kernel sums y.  M is
+// linear, so this is what adding alpha M(p) + omega M(s) every iteration
+// gives, up to rounding.  p is in p0 or p1 (the last p kernel recorded which);
+// rout, when given, receives the last residual s - omega t.  One k-marching
+// pass: z is formed on the tile and its one-cell halo from y, p, s at each
+// cell, the black half-sweep's values go to LDS (x and y neighbours) and a
+// register queue (k neighbours), and every red cell is swept from its six
+// black neighbours, cells outside the interior contributing zero.  It runs
+// once a pass is over and its update not yet in x (the init kernels'
+// finalize clears that).
+template <class T, class OP>
+__global__ __launch_bounds__(K7_NT) void k_bicg_close_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ y,
+                                                         const T *__restrict__ p0, const T *__restrict__ p1,
+                                                         const T *__restrict__ sv, const T *__restrict__ tv,
+                                                         T *__restrict__ rout, const DevState *st, T omega_pc,
+                                                         double *__restrict__ xd) {
+    if (!st->done || st->final || !st->pending) return;
     __shared__ K7Lds<T> lw;
     const K7Pos Q = k7_pos(kt, g);
     const int sz = g.nxy;
-    const bool upd = MODE == 1 || MODE == 3 || (MODE == 0 && !st->first);  // q (and vin) are read
-    const double ya = MODE == 3 ? st->alpha : 0.0, yw = MODE == 3 ? st->omega : 0.0;
-    const bool yread = MODE == 3 && st->it_pass >= 2;
-    const T *const pv_in = (MODE == 3 && st->plast) ? (const T *)v : vin;
+    const double ya = st->alpha, yw = st->omega;
+    const bool yread = st->it_pass >= 2;
+    const T *const pv = st->plast ? p1 : p0;
     const T om = (T)yw;
-    T c1 = T(0), c2 = T(0);
-    if (MODE == 0) {
-        c1 = (T)st->beta;
-        c2 = (T)st->omega;
-    } else if (!apart) {
-        c1 = (T)st->alpha;
-    }  // else: alpha is summed below, while the prologue's loads are in flight
-    const auto vec = [&](T rv, T pv, T qv) -> T {
-        if (MODE == 3) return (T)((yread ? (double)rv : 0.0) + (ya * (double)pv + yw * (double)qv));
-        if (MODE == 1) return rv - c1 * qv;
-        return upd ? rv + c1 * (pv - c2 * qv) : rv;
+    const auto vec = [&](T yy, T pp, T ss) -> T {
+        return (T)((yread ? (double)yy : 0.0) + (ya * (double)pp + yw * (double)ss));
     };
     const auto ca = a.column(Q.act ? Q.i : kt.box.lo[0], Q.act ? Q.j : kt.box.lo[1]);
     using Col = typename std::decay<decltype(ca)>::type;
@@ -286,86 +266,72 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
         return (in && ((par + k) & 1) == FALM_COLOR_BLACK) ? T(0) + omega_pc * (vv / d) : T(0);
     };
     const auto kin = [&](int k) { return k >= lo2 && k < hi2; };
-    const auto load = [&](unsigned ob, T &rv, T &pv, T &qv) {
-        rv = ld_at(r, ob);
-        if (upd) {
-            qv = ld_at(q, ob);
-            if (MODE == 0 || MODE == 3) pv = ld_at(pv_in, ob);
-        }
+    const auto load = [&](unsigned ob, T &yy, T &pp, T &ss) {
+        yy = ld_at(y, ob);
+        pp = ld_at(pv, ob);
+        ss = ld_at(sv, ob);
     };
-    // registers: w at k-1 and k, v and the cell coefficients at k, the
+    // registers: w at k-1 and k, z and the cell coefficients at k, the
     // column's raw values at k+1 and the halo cell's at the next LDS plane
     T wm, wc, vc;
     Cell ec;
-    T nr, np = T(0), nq = T(0);
+    T ny, np, ns;
     Cell ne;
-    T hr = T(0), hp = T(0), hq = T(0), hd = T(1);
+    T hy = T(0), hp = T(0), hs = T(0), hd = T(1);
+    T sc;  // the column's s at k (rout)
     {
         unsigned ob = cb + (unsigned)(Q.k0 - 1) * szt;
-        T rm, pm = T(0), qm = T(0), rc, pc = T(0), qc = T(0);
-        load(ob, rm, pm, qm);
+        T ym, pm, sm, yc, pc, s0;
+        load(ob, ym, pm, sm);
         const T dm = ca.diag_k(ob, Q.k0 - 1);
-        load(ob + szt, rc, pc, qc);
+        load(ob + szt, yc, pc, s0);
         ec = ca.load(ob + szt, Q.k0);
-        load(ob + 2 * szt, nr, np, nq);
+        load(ob + 2 * szt, ny, np, ns);
         ne = ca.load(ob + 2 * szt, Q.k0 + 1);
         if (h.slot) {
             const unsigned oh = hb + (unsigned)Q.k0 * szt;
-            load(oh, hr, hp, hq);
+            load(oh, hy, hp, hs);
             hd = ch.diag_k(oh, Q.k0);
         }
-        if (MODE == 1 && apart) {
-            // alpha = rho / (rr, q) from the operator kernel's partial sums
-            // (what k_fin_alpha computes), in every workgroup; the first
-            // stores it
-            double tot[1];
-            sum_partials<1, K7_NT>(apart, anb, tot);
-            const double alpha = st->rho / tot[0];
-            if (blockIdx.x == 0 && threadIdx.x == 0) stw->alpha = alpha;
-            c1 = (T)alpha;
-        }
-        wm = wval(vec(rm, pm, qm), dm, Q.act && kin(Q.k0 - 1), cpar, Q.k0 - 1);
-        vc = vec(rc, pc, qc);
+        wm = wval(vec(ym, pm, sm), dm, Q.act && kin(Q.k0 - 1), cpar, Q.k0 - 1);
+        vc = vec(yc, pc, s0);
+        sc = s0;
         wc = wval(vc, Col::diag(ec), Q.act, cpar, Q.k0);
         lw[0][Q.wy + 1][Q.lane + 1] = wc;
-        if (h.slot) lw[0][h.row][h.col] = wval(vec(hr, hp, hq), hd, hin, hpar, Q.k0);
+        if (h.slot) lw[0][h.row][h.col] = wval(vec(hy, hp, hs), hd, hin, hpar, Q.k0);
     }
-    // MODE 2, 3: x a plane ahead (act columns only); MODE 3 with vh: the
-    // column's s at k (sc) and t a plane ahead
+    // x a plane ahead (act columns only); with rout, t a plane ahead
     constexpr unsigned XS = (unsigned)(sizeof(double) / sizeof(T));
     double xq = 0.0;
-    if (MODE >= 2 && Q.act) xq = ld_at(xd, (cb + (unsigned)Q.k0 * szt) * XS);
-    const bool rw = MODE == 3 && vh != nullptr;
-    T tq = T(0), sc = T(0);
-    if (MODE == 3) {
-        sc = ld_at(q, cb + (unsigned)Q.k0 * szt);
-        if (rw) tq = ld_at(tin, cb + (unsigned)Q.k0 * szt);
-    }
+    if (Q.act) xq = ld_at(xd, (cb + (unsigned)Q.k0 * szt) * XS);
+    const bool rw = rout != nullptr;
+    T tq = T(0);
+    if (rw) tq = ld_at(tv, cb + (unsigned)Q.k0 * szt);
     __syncthreads();
     int bf = 0;
     unsigned ob = cb + (unsigned)Q.k0 * szt;
     for (int k = Q.k0; k < Q.k1; k++, ob += szt) {
         const bool more = k + 1 < Q.k1;
         const double xc = xq;
-        if (MODE >= 2 && more && Q.act) xq = ld_at(xd, (ob + szt) * XS);
+        if (more && Q.act) xq = ld_at(xd, (ob + szt) * XS);
         const T tc = tq;
-        if (rw && more) tq = ld_at(tin, ob + szt);
-        T r2 = T(0), p2 = T(0), q2 = T(0);
+        if (rw && more) tq = ld_at(tv, ob + szt);
+        T y2 = T(0), p2 = T(0), s2 = T(0);
         Cell e2 = ne;
         if (more) {
-            load(ob + 2 * szt, r2, p2, q2);
+            load(ob + 2 * szt, y2, p2, s2);
             e2 = ca.load(ob + 2 * szt, k + 2);
             if (h.slot) {
                 const unsigned oh = hb + (unsigned)(k + 1) * szt;
-                load(oh, hr, hp, hq);
+                load(oh, hy, hp, hs);
                 hd = ch.diag_k(oh, k + 1);
             }
         }
-        const T vn = vec(nr, np, nq);
+        const T vn = vec(ny, np, ns);
         const T wn = wval(vn, Col::diag(ne), Q.act && kin(k + 1), cpar, k + 1);
         if (Q.act) {
-            if (MODE < 2) st_at(v, ob, vc);
-            if (rw) st_at(vh, ob, sc - om * tc);
+            // the last residual, kernel_PBiCGStab_4's expression
+            if (rw) st_at(rout, ob, sc - om * tc);
             T out = wc;
             if (((cpar + k) & 1) != FALM_COLOR_BLACK) {
                 // kernel_Struct3d7p_SOR's update from x = 0 (src/dev/FalmEqDevCall.cu:195-215)
@@ -373,24 +339,20 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
                                      lw[bf][Q.wy + 2][Q.lane + 1], lw[bf][Q.wy][Q.lane + 1], wn, wm);
                 out = T(0) + omega_pc * ((vc - s) / Col::diag(ec));
             }
-            if (MODE >= 2) {
-                st_at(xd, ob * XS, xc + (double)out);
-            } else {
-                st_at(vh, ob, out);
-            }
+            st_at(xd, ob * XS, xc + (double)out);
         }
         if (more) {
             lw[bf ^ 1][Q.wy + 1][Q.lane + 1] = wn;
-            if (h.slot) lw[bf ^ 1][h.row][h.col] = wval(vec(hr, hp, hq), hd, hin && kin(k + 1), hpar, k + 1);
+            if (h.slot) lw[bf ^ 1][h.row][h.col] = wval(vec(hy, hp, hs), hd, hin && kin(k + 1), hpar, k + 1);
         }
         wm = wc;
         wc = wn;
         vc = vn;
         ec = ne;
-        if (MODE == 3) sc = nq;
-        nr = r2;
+        sc = ns;
+        ny = y2;
         np = p2;
-        nq = q2;
+        ns = s2;
         ne = e2;
         bf ^= 1;
         __syncthreads();
@@ -399,28 +361,33 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_vsor_km(K7Tiling kt, Geom g, OP 
 
 // ---------------------------------------------------------------------
 // A Krylov vector, its one-pass SOR preconditioning and the operator product
-// in one k-marching pass (pc_maxit = 1, preconditioning deferred):
-//   MODE 0: v = r (a pass's first iteration) or r + beta (p - omega q),
-//           w = A M(v), partial sums of (rr, w)          -> alpha
-//   MODE 1: v = r - alpha q,  w = A M(v), partial sums of (w, v), (w, w)
-//                                                        -> omega
+// in one k-marching pass (pc_maxit = 1; the two kernels of a BiCGStab
+// iteration, src/FalmEq.cpp:345-407):
+//   MODE 0: r = s - omega t (the previous iteration's residual, formed here
+//           and stored; a pass's first iteration reads r instead),
+//           v = r (first iteration) or r + beta (p - omega q),
+//           w = A M(v), partial sums of (rr, w)          -> alpha;
+//           y = (0 or y) + alpha p + omega s of the previous iteration
+//   MODE 1: v = r - alpha q,  w = A M(v), partial sums of (w, v), (w, w),
+//           (v, v), (v, rr), (w, rr)       -> omega, |r|, (r, rr) of the
+//           next residual r = v - omega w (k_fin_fold)
 // M(v) is the black half-sweep from zero followed by the red one (black
 // cells omega_pc v / a_c, red cells omega_pc (v - six black neighbours'
-// products) / a_c, zero outside the interior), each cell with the
-// expression k_bicg_vsor_km uses, so w is what that kernel and
-// k_bicg_mv_km give bit for bit.  w at a tile cell needs M(v) on the tile
-// and its four one-cell strips in x and y; a red value there needs black
-// values one cell further out; so v is formed on the tile plus a two-cell
-// ring (PQ_W x PQ_H cells a plane) from r, p and q read there.  Planes pass
-// through four LDS slots: at step s the slot of plane s+2 is filled (black
-// cells M(v), red cells v), the red cells of plane s are turned into M(v)
-// in place (they read black values of planes s-1..s+1 only), and w of plane
-// s-1 is formed from M(v) of plane s-1 in LDS and of the column's planes
-// s-2 and s in registers: one barrier per step.
-// v is written on the tile (p or s) and w (q or t); M(v) is not stored: the
-// update accumulates alpha p + omega s and k_bicg_vsor_km<MODE 2> adds M of
-// the sum to x at the end of a pass (M is linear).  MODE 0 reads p and q at
-// the ring cells, which neighbouring workgroups write: both alternate
+// products) / a_c, zero outside the interior; kernel_Struct3d7p_SOR's
+// expression, src/dev/FalmEqDevCall.cu:195-215), so w is, bit for bit, what
+// the separate half-sweeps and k_bicg_mv_km give.  w at a tile cell needs
+// M(v) on the tile and its four one-cell strips in x and y; a red value
+// there needs black values one cell further out; so v is formed on the tile
+// plus a two-cell ring (PQ_W x PQ_H cells a plane) from its operands read
+// there.  Planes pass through four LDS slots: at step s the slot of plane
+// s+2 is filled (black cells M(v), red cells v), the red cells of plane s
+// are turned into M(v) in place (they read black values of planes s-1..s+1
+// only), and w of plane s-1 is formed from M(v) of plane s-1 in LDS and of
+// the column's planes s-2 and s in registers: one barrier per step.
+// v is written on the tile (p or s) and w (q or t); M(v) is not stored:
+// y accumulates the pass's alpha p + omega s and k_bicg_close_km adds M of
+// the sum to x at the end of the pass (M is linear).  MODE 0 reads p and q
+// at the ring cells, which neighbouring workgroups write: both alternate
 // between two buffers.
 constexpr int PQ_R = 2;                                    // ring width
 constexpr int PQ_W = K7_BX + 2 * PQ_R, PQ_H = K7_BY + 2 * PQ_R;
@@ -507,7 +474,7 @@ __device__ inline typename OpCols<TA>::Cell cell_of(const OpCols<TA> &c, unsigne
 // thread's slot in `trash`): the compiler can then count the memory
 // operations in flight exactly and waits for the previous step's loads
 // only, not for those just issued.  trash holds K7_NT values per workgroup.
-template <class T, class OP, int MODE, bool DCLS, bool FOLD>
+template <class T, class OP, int MODE, bool DCLS>
 __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves<OP>::value))) void k_bicg_pcmv_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
                                                         const T *__restrict__ pin, const T *__restrict__ qin,
                                                         const T *__restrict__ rr, T *__restrict__ vout,
@@ -528,12 +495,12 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         c1 = (T)st->beta;
         c2 = (T)st->omega;
     }  // MODE 1: alpha is summed below, while the prologue's loads are in flight
-    // FOLD, MODE 0: the previous iteration's alpha, omega for y; y is set
+    // MODE 0: the previous iteration's alpha, omega for y; y is set
     // (not accumulated) on the pass's second iteration
-    const double ya = FOLD && MODE == 0 ? st->alpha : 0.0, yw = FOLD && MODE == 0 ? st->omega : 0.0;
-    const bool yset = FOLD && MODE == 0 && st->it_pass == 1;
-    const T *const av = (FOLD && MODE == 0 && upd) ? sin : r;
-    if (FOLD && MODE == 0 && blockIdx.x == 0 && threadIdx.x == 0) stw->plast = pid;
+    const double ya = MODE == 0 ? st->alpha : 0.0, yw = MODE == 0 ? st->omega : 0.0;
+    const bool yset = MODE == 0 && st->it_pass == 1;
+    const T *const av = (MODE == 0 && upd) ? sin : r;
+    if (MODE == 0 && blockIdx.x == 0 && threadIdx.x == 0) stw->plast = pid;
     const int lo0 = kt.box.lo[0], hi0 = lo0 + kt.box.n[0], lo1 = kt.box.lo[1], hi1 = lo1 + kt.box.n[1];
     const int lo2 = kt.box.lo[2], hi2 = lo2 + kt.box.n[2];
     const int k0 = Q.k0, k1 = Q.k1;
@@ -570,24 +537,24 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
     const unsigned b1 = in1 ? (unsigned)(gi1 + g.nx * gj1) * sizeof(T) : cb;
     T *const tr = trash + (size_t)blockIdx.x * K7_NT + tid;
 
-    // operands of v at one cell of a plane, and its diagonal: a is r, or
-    // with FOLD after a pass's first iteration s (and b is t: r = s - omega t)
+    // operands of v at one cell of a plane, and its diagonal: a is r, or in
+    // MODE 0 after a pass's first iteration s (and b is t: r = s - omega t)
     struct Ld {
         T a, b, p, q, d;
     };
     const auto load = [&](const Col &c, unsigned ob, int k) {
         Ld x;
         x.a = ld_at(av, ob);
-        x.b = FOLD && MODE == 0 ? ld_at(tin, ob) : T(0);
+        x.b = MODE == 0 ? ld_at(tin, ob) : T(0);
         x.q = ld_at(qin, ob);
         x.p = MODE == 0 ? ld_at(pin, ob) : T(0);
         x.d = diag_of<DCLS>(c, ob, k);
         return x;
     };
-    // the residual a cell's v is formed from (FOLD: r = s - omega t, the
+    // the residual a cell's v is formed from (MODE 0: r = s - omega t, the
     // previous iteration's kernel_PBiCGStab_4 with one rounding)
     const auto res = [&](const Ld &x) -> T {
-        if (FOLD && MODE == 0) return upd ? fma(-c2, x.b, x.a) : x.a;
+        if (MODE == 0) return upd ? fma(-c2, x.b, x.a) : x.a;
         return x.a;
     };
     const auto vec_of = [&](const Ld &x, T rv) -> T {
@@ -597,7 +564,7 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
     const auto vec = [&](const Ld &x) -> T { return vec_of(x, res(x)); };
     struct Fl {
         Ld o, h;  // the column's cell and the ring cell
-        T y;      // FOLD: the column's y
+        T y;      // MODE 0: the column's y
     };
     const auto fetch = [&](int f) {
         const int kk = kcl(f);
@@ -605,12 +572,12 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         Fl x;
         x.o = load(ca, cb + pf, kk);
         x.h = load(c2v, b2 + pf, kk);
-        x.y = FOLD && MODE == 0 ? ld_at(yv, cb + pf) : T(0);
+        x.y = MODE == 0 ? ld_at(yv, cb + pf) : T(0);
         return x;
     };
     // slot of plane f: black cells M(v), red cells v, zero outside the
     // interior; v is written on the tile; returns the column's v
-    // the column's v at plane f, stored with (FOLD) its r and y:
+    // the column's v at plane f, stored with (MODE 0) its r and y:
     // y = (0 or y) + alpha p + omega s of the previous iteration, what the
     // update kernel accumulates
     const auto own = [&](int f, const Fl &x, bool kf) -> T {
@@ -619,7 +586,7 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         const bool wr = Q.act && f >= k0 && f < k1;
         const unsigned ob = cb + (unsigned)f * szt;
         *(wr ? (T *)((char *)vout + ob) : tr) = vo;
-        if (FOLD && MODE == 0) {
+        if (MODE == 0) {
             const bool wu = wr && upd;
             *(wu ? (T *)((char *)rout + ob) : tr) = rv;
             const double yo = (yset ? 0.0 : (double)x.y) + (ya * (double)x.o.p + yw * (double)x.o.a);
@@ -726,7 +693,7 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         e1l = cell_at(c1v, b1, s + 1);
         {
             const int k = s < k0 ? k0 : (s >= k1 ? k1 - 1 : s);
-            rl = (MODE == 0 || FOLD) ? ld_at(rr, cb + (unsigned)k * szt) : T(0);
+            rl = ld_at(rr, cb + (unsigned)k * szt);
         }
         T h;
         const T vn = fill_red(s, fu, eu, e1u, h);
@@ -741,13 +708,11 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         } else {
             acc[0] += wr ? (double)wv * (double)v0 : 0.0;
             acc[1] += wr ? (double)wv * (double)wv : 0.0;
-            if (FOLD) {
-                // (s, s), (s, rr), (t, rr): |r|^2 and (r, rr) of the next
-                // residual r = s - omega t follow from these and the two above
-                acc[2] += wr ? (double)v0 * (double)v0 : 0.0;
-                acc[3] += wr ? (double)v0 * (double)ru : 0.0;
-                acc[4] += wr ? (double)wv * (double)ru : 0.0;
-            }
+            // (s, s), (s, rr), (t, rr): |r|^2 and (r, rr) of the next
+            // residual r = s - omega t follow from these and the two above
+            acc[2] += wr ? (double)v0 * (double)v0 : 0.0;
+            acc[3] += wr ? (double)v0 * (double)ru : 0.0;
+            acc[4] += wr ? (double)wv * (double)ru : 0.0;
         }
         hm = hc;
         hc = h;
@@ -766,12 +731,7 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         double v1s[1] = {acc[0]};
         block_partials<K7_NT, 1>(v1s, partials, lds);
     } else {
-        if (FOLD) {
-            block_partials<K7_NT, 5>(acc, partials, lds);
-        } else {
-            double a2[2] = {acc[0], acc[1]};
-            block_partials<K7_NT, 2>(a2, partials, lds);
-        }
+        block_partials<K7_NT, 5>(acc, partials, lds);
     }
 }
 
@@ -1230,28 +1190,22 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_omega(DevState *st, const double
 
 // x += alpha pp + omega ss (kernel_PBiCGStab_3, x in double);
 // r = s - omega t (kernel_PBiCGStab_4); partials |r|^2, (r, rr).
-// DEFER (preconditioning deferred to the end of the pass): pp, ss are p and
-// s, and the sum goes to y (T; set on a pass's first iteration) instead of x.
-template <class T, bool DEFER = false>
+template <class T>
 __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *__restrict__ x,
                                                     const T *__restrict__ pp, const T *__restrict__ ss,
                                                     const T *__restrict__ s, const T *__restrict__ t,
                                                     T *__restrict__ r, const T *__restrict__ rr, const DevState *st,
-                                                    double *partials, T *__restrict__ y = nullptr) {
+                                                    double *partials) {
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int i, j, k, t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha, omega = st->omega;
-    const bool first = st->first;
     tile_range(tl.tiles, t0, t1);
     for (int tile = t0; tile < t1; tile++) {
         if (!tile_cell_at(tl, tile, i, j, k)) continue;
         const int cc = g.id(i, j, k);
-        if (DEFER)
-            y[cc] = (T)((first ? 0.0 : (double)y[cc]) + (alpha * (double)pp[cc] + omega * (double)ss[cc]));
-        else
-            x[cc] += alpha * (double)pp[cc] + omega * (double)ss[cc];
+        x[cc] += alpha * (double)pp[cc] + omega * (double)ss[cc];
         const T rv = s[cc] - (T)omega * t[cc];
         r[cc] = rv;
         v[0] += (double)rv * (double)rv;
@@ -1274,26 +1228,23 @@ template <class T> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
 
-template <class T, bool DEFER = false>
+template <class T>
 __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g, double *__restrict__ x,
                                                      const T *__restrict__ pp, const T *__restrict__ ss,
                                                      const T *__restrict__ s, const T *__restrict__ t,
                                                      T *__restrict__ r, const T *__restrict__ rr,
                                                      const DevState *st, double *partials, const double *opart,
-                                                     int onb, DevState *stw, T *__restrict__ y = nullptr) {
+                                                     int onb, DevState *stw) {
     using T2 = typename Pair<T>::type;
-    using X2 = typename std::conditional<DEFER, T2, double2>::type;
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int t0, t1;
     double v[2] = {0.0, 0.0};
     const double alpha = st->alpha;
-    // DEFER: the pass's first iteration sets y (no read)
-    const bool xread = !DEFER || !st->first;
     // a pair's operands, loaded one tile ahead of their use
     struct Pr {
         T2 p2, s2, sv, tv, r0;
-        X2 xv;
+        double2 xv;
         int cc;
         bool ok;
     };
@@ -1308,12 +1259,7 @@ __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g,
             q.sv = *(const T2 *)(s + q.cc);
             q.tv = *(const T2 *)(t + q.cc);
             q.r0 = *(const T2 *)(rr + q.cc);
-            if (DEFER) {
-                q.xv = X2{};
-                if (xread) q.xv = *(const X2 *)(y + q.cc);
-            } else {
-                q.xv = *(const X2 *)(x + q.cc);
-            }
+            q.xv = *(const double2 *)(x + q.cc);
         }
         return q;
     };
@@ -1339,17 +1285,10 @@ __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g,
         nxt.ok = false;
         if (tile + 1 < t1) nxt = load(tile + 1);
         if (cur.ok) {
-            if (DEFER) {
-                X2 yv;
-                yv.x = (T)((double)cur.xv.x + (alpha * (double)cur.p2.x + omega * (double)cur.s2.x));
-                yv.y = (T)((double)cur.xv.y + (alpha * (double)cur.p2.y + omega * (double)cur.s2.y));
-                *(X2 *)(y + cur.cc) = yv;
-            } else {
-                X2 xv = cur.xv;
-                xv.x += alpha * (double)cur.p2.x + omega * (double)cur.s2.x;
-                xv.y += alpha * (double)cur.p2.y + omega * (double)cur.s2.y;
-                *(X2 *)(x + cur.cc) = xv;
-            }
+            double2 xv = cur.xv;
+            xv.x += alpha * (double)cur.p2.x + omega * (double)cur.s2.x;
+            xv.y += alpha * (double)cur.p2.y + omega * (double)cur.s2.y;
+            *(double2 *)(x + cur.cc) = xv;
             T2 rv;
             rv.x = cur.sv.x - om * cur.tv.x;
             rv.y = cur.sv.y - om * cur.tv.y;
@@ -1725,53 +1664,31 @@ static T *pc_rest(falm_solver *S, const OP &a, T *x, const T *b, T *spare, hipSt
     return x;
 }
 
-// The one-pass SOR preconditioner with deferred application: each
-// iteration is the two k_bicg_pcmv_km kernels and the update into y (ss);
-// k_bicg_vsor_km<MODE 2> adds M(y) to x when the pass is over.
-#ifndef FALM_PCMV
-#define FALM_PCMV 1
-#endif
+// With the one-pass SOR preconditioner (pc_maxit <= 1) an iteration is the
+// two k_bicg_pcmv_km kernels and k_fin_fold: the preconditioning is deferred
+// to the end of a pass (y accumulates alpha p + omega s, M is linear) and
+// the update r = s - omega t, y += alpha p + omega s is folded into the next
+// iteration's p kernel (|r| and (r, rr) from the s kernel's sums); the
+// pass's last update goes into k_bicg_close_km.  Other preconditioners run
+// the separate p, sweep, operator and update kernels.
 static bool deferred_pc(const falm_solver *S) {
-    return FALM_PCMV && S->cfg.pc_type == FALM_LS_SOR && S->cfg.pc_maxit <= 1 && S->kt.blocks > 0;
+    return S->cfg.pc_type == FALM_LS_SOR && S->cfg.pc_maxit <= 1;
 }
 
-// The update folded into the next iteration's p kernel (r = s - omega t and
-// y += alpha p + omega s there; |r| and (r, rr) from the s kernel's sums):
-// two kernels and one finalize per iteration; the last iteration's update
-// goes into the pass's closing preconditioner kernel (launch_fold_apply).
-#ifndef FALM_FOLD
-#define FALM_FOLD 1
-#endif
-static bool folded_update(const falm_solver *S) { return FALM_FOLD && deferred_pc(S); }
-
-
-// x += M(y) once a pass is over (no-op while it runs or once its sum is in)
+// x += M((0 or y) + alpha p + omega s) once a pass is over, and the last
+// residual s - omega t into rout when given (no-op while the pass runs or
+// once its update is in x)
 template <class T, class OP>
-static falm_status launch_pc_apply(falm_solver *S, const OP &a, double *x, hipStream_t s) {
+static falm_status launch_close(falm_solver *S, const OP &a, double *x, T *rout, hipStream_t s) {
     TimedLaunch tm(FALM_TIME_PC_APPLY, s);
-    hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 2>), FALM_KM(S), s, S->kt, S->g, a, (const T *)S->ss,
-                       (const T *)nullptr, (const T *)nullptr, (T *)nullptr, (T *)nullptr, (const DevState *)S->st,
-                       (T)S->cfg.pc_relax, (const double *)nullptr, 0, (DevState *)nullptr, x);
-    // y (and the diagonal) read, x read and written
-    tm.done((double)S->tl.box.size() * (sizeof(T) + 2.0 * sizeof(double) + diag_bytes<OP>(S)));
-    FALM_CHECK_LAUNCH("k_bicg_vsor_km<apply>");
-    return FALM_OK;
-}
-
-// folded update: x += M((0 or y) + alpha p + omega s) of the pass once it
-// is over, and the last residual s - omega t into rout when given (no-op
-// while the pass runs)
-template <class T, class OP>
-static falm_status launch_fold_apply(falm_solver *S, const OP &a, double *x, T *rout, hipStream_t s) {
-    TimedLaunch tm(FALM_TIME_PC_APPLY, s);
-    hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 3>), FALM_KM(S), s, S->kt, S->g, a, (const T *)S->ss, (const T *)S->p,
-                       (const T *)S->s, (T *)S->w, rout, (const DevState *)S->st, (T)S->cfg.pc_relax,
-                       (const double *)nullptr, 0, (DevState *)nullptr, x, (const T *)S->t);
+    hipLaunchKernelGGL((k_bicg_close_km<T, OP>), FALM_KM(S), s, S->kt, S->g, a, (const T *)S->ss, (const T *)S->p,
+                       (const T *)S->w, (const T *)S->s, (const T *)S->t, rout, (const DevState *)S->st,
+                       (T)S->cfg.pc_relax, x);
     // y, p, s (and the diagonal) read, x read and written; with rout also t
     // read and r written
     tm.done((double)S->tl.box.size() *
             ((rout ? 5.0 : 3.0) * sizeof(T) + 2.0 * sizeof(double) + diag_bytes<OP>(S)));
-    FALM_CHECK_LAUNCH("k_bicg_vsor_km<fold apply>");
+    FALM_CHECK_LAUNCH("k_bicg_close_km");
     return FALM_OK;
 }
 
@@ -1789,7 +1706,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         // p, q alternate between two buffers (the p kernel reads both at the
         // ring cells of its tiles); a pass's first iteration reads neither,
         // so the alternation may run on across iterations a finished pass
-        // skips (the folded p kernel records which buffer it wrote)
+        // skips (the p kernel records which buffer it wrote)
         const int pid = S->pflip ? 0 : 1;  // 1: p goes to w
         T *pin = (T *)(S->pflip ? S->w : S->p), *pout = (T *)(S->pflip ? S->p : S->w);
         T *qin = (T *)(S->qflip ? S->pp : S->q), *qout = (T *)(S->qflip ? S->q : S->pp);
@@ -1799,120 +1716,60 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         const int nbp = S->ktp.blocks;
         const dim3 gp((unsigned)nbp), bp(K7_NT);
         T *trash = (T *)S->trash;
-        const bool fold = folded_update(S);
         // the diagonal from its classes (a compile-time choice in the kernel)
         bool dcls = false;
         if constexpr (std::is_same<OP, OpSep<T>>::value) dcls = a.dtab != nullptr;
-        const auto launch = [&](auto dc, auto fd) {
-            constexpr bool D = decltype(dc)::value, F = decltype(fd)::value;
+        const auto launch = [&](auto dc) {
+            constexpr bool D = decltype(dc)::value;
             {
                 TimedLaunch tm(FALM_TIME_PCMV, s);
-                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D, F>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
                                    (const T *)pin, (const T *)qin, (const T *)rr, pout, qout, (const DevState *)S->st,
                                    wpc, (const double *)nullptr, 0, S->st, S->partials, trash, (const T *)sv,
                                    (const T *)t, r, ss, pid);
-                // r, rr (and the diagonal) read, p and q written; after a
-                // pass's first iteration also p and q read, and when folded
-                // s and t read instead of r, y read and written, r written
-                tm.done(box * ((first ? 4.0 : (F ? 10.0 : 6.0)) * sizeof(T) + db));
+                // a pass's first iteration: r, rr (and the diagonal) read,
+                // p and q written; then s, t, p, q, rr, y read and p, q, r,
+                // y written
+                tm.done(box * ((first ? 4.0 : 10.0) * sizeof(T) + db));
             }
             {
                 TimedLaunch tm(FALM_TIME_PCMV_S, s);
-                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D, F>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
                                    (const T *)nullptr, (const T *)qout, (const T *)rr, sv, t,
                                    (const DevState *)S->st, wpc, (const double *)S->partials, nbp, S->st,
                                    S->partials2, trash, (const T *)nullptr, (const T *)nullptr, (T *)nullptr,
                                    (T *)nullptr, 0);
-                // r, q (and the diagonal) read, s and t written; folded: rr read
-                tm.done(box * ((F ? 5.0 : 4.0) * sizeof(T) + db));
+                // r, q, rr (and the diagonal) read, s and t written
+                tm.done(box * (5.0 * sizeof(T) + db));
             }
         };
-        if (dcls) {
-            if (fold) launch(std::true_type{}, std::true_type{});
-            else launch(std::true_type{}, std::false_type{});
-        } else {
-            if (fold) launch(std::false_type{}, std::true_type{});
-            else launch(std::false_type{}, std::false_type{});
-        }
+        if (dcls) launch(std::true_type{});
+        else launch(std::false_type{});
         FALM_CHECK_LAUNCH("k_bicg_pcmv_km");
-        if (fold) {
-            hipLaunchKernelGGL(k_fin_fold, FALM_FIN, s, S->st, (const double *)S->partials2, nbp);
-            FALM_CHECK_LAUNCH("k_fin_fold");
-            return FALM_OK;
-        }
-        TimedLaunch tm2(FALM_TIME_SOLVER_UPDATE, s);
-        int nbu = red_blocks(tl);
-        if (S->tp.tiles > 0) {
-            nbu = S->tp.tiles < UPD_BLOCKS ? S->tp.tiles : UPD_BLOCKS;
-            hipLaunchKernelGGL((k_bicg_update2<T, true>), dim3((unsigned)nbu), tiling_block(), 0, s, S->tp,
-                               tl.box.lo[0], g, x, (const T *)pout, (const T *)sv, (const T *)sv, (const T *)t, r,
-                               (const T *)rr, (const DevState *)S->st, S->partials, (const double *)S->partials2, nbp,
-                               S->st, ss);
-        } else {
-            hipLaunchKernelGGL(k_fin_omega, FALM_FIN, s, S->st, (const double *)S->partials2, nbp);
-            hipLaunchKernelGGL((k_bicg_update<T, true>), FALM_RED(S), s, tl, g, x, (const T *)pout, (const T *)sv,
-                               (const T *)sv, (const T *)t, r, (const T *)rr, (const DevState *)S->st, S->partials,
-                               ss);
-        }
-        // p, s, t, rr read, y read (after a pass's first iteration) and
-        // written, r written
-        tm2.done(box * ((first ? 6.0 : 7.0) * sizeof(T)));
-        hipLaunchKernelGGL(k_fin_update, FALM_FIN, s, S->st, (const double *)S->partials, nbu);
-        FALM_CHECK_LAUNCH("k_bicg_update<deferred>");
+        hipLaunchKernelGGL(k_fin_fold, FALM_FIN, s, S->st, (const double *)S->partials2, nbp);
+        FALM_CHECK_LAUNCH("k_fin_fold");
         return FALM_OK;
     }
-    // one-pass SOR preconditioner: each vector and its two half-sweeps in
-    // one kernel (k_bicg_vsor_km); p alternates between p and w, since the
-    // kernel reads the old p at the neighbours of the cells it writes.  A
-    // pass's first iteration sets p = r, so the alternation may continue
-    // across iterations that a finished pass skips.
-    const bool vsor = c.pc_type == FALM_LS_SOR && c.pc_maxit <= 1 && S->kt.blocks > 0;
-    T *pcp = pp;
     falm_status pst;
-    if (vsor) {
-        T *pin = (T *)(S->pflip ? S->w : S->p), *pout = (T *)(S->pflip ? S->p : S->w);
-        S->pflip ^= 1;
-        TimedLaunch tm(FALM_TIME_SOR, s);
-        hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 0>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)pin,
-                           (const T *)q, pout, pp, (const DevState *)S->st, wpc, (const double *)nullptr, 0,
-                           (DevState *)nullptr);
-        // r (and the diagonal) read, p and p^ written; after a solve's first
-        // iteration also p and q read
-        tm.done((double)tl.box.size() * ((first ? 3.0 : 5.0) * sizeof(T) + diag_bytes<OP>(S)));
-        FALM_CHECK_LAUNCH("k_bicg_vsor_km");
-    } else {
-        hipLaunchKernelGGL((k_bicg_p<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, p, (const T *)q, pp,
-                           (const DevState *)S->st, c.pc_type, wpc);
-        FALM_CHECK_LAUNCH("k_bicg_p");
-        // the preconditioned p lands in pp or, out of place, in w
-        pcp = pc_rest<T, OP>(S, a, pp, p, (T *)S->w, s, pst);
-        FALM_TRY(pst);
-    }
+    hipLaunchKernelGGL((k_bicg_p<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, p, (const T *)q, pp,
+                       (const DevState *)S->st, c.pc_type, wpc);
+    FALM_CHECK_LAUNCH("k_bicg_p");
+    // the preconditioned p lands in pp or, out of place, in w
+    T *const pcp = pc_rest<T, OP>(S, a, pp, p, (T *)S->w, s, pst);
+    FALM_TRY(pst);
     TimedLaunch tm1(FALM_TIME_MV, s);
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)pcp, q, (const T *)rr,
                        (const DevState *)S->st, S->partials);
     tm1.done((double)tl.box.size() * (3.0 * sizeof(T) + diag_bytes<OP>(S)));
-    // the fused s kernel computes alpha from these partials itself
-    if (!vsor) hipLaunchKernelGGL(k_fin_alpha, FALM_FIN, s, S->st, (const double *)S->partials, nb);
+    hipLaunchKernelGGL(k_fin_alpha, FALM_FIN, s, S->st, (const double *)S->partials, nb);
     FALM_CHECK_LAUNCH("k_bicg_mvdot");
-    T *pcs = ss;
-    if (vsor) {
-        TimedLaunch tm(FALM_TIME_SOR_S, s);
-        hipLaunchKernelGGL((k_bicg_vsor_km<T, OP, 1>), FALM_KM(S), s, S->kt, g, a, (const T *)r, (const T *)nullptr,
-                           (const T *)q, sv, ss, (const DevState *)S->st, wpc, (const double *)S->partials, nb,
-                           S->st);
-        // r, q (and the diagonal) read, s and s^ written
-        tm.done((double)tl.box.size() * (4.0 * sizeof(T) + diag_bytes<OP>(S)));
-        FALM_CHECK_LAUNCH("k_bicg_vsor_km");
-    } else {
-        hipLaunchKernelGGL((k_bicg_s<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, (const T *)q, sv, ss,
-                           (const DevState *)S->st, c.pc_type, wpc);
-        FALM_CHECK_LAUNCH("k_bicg_s");
-        // the preconditioned s lands in ss or, out of place, in pp (free once
-        // p's sweeps have read it)
-        pcs = pc_rest<T, OP>(S, a, ss, sv, pcp == pp ? ss : pp, s, pst);
-        FALM_TRY(pst);
-    }
+    hipLaunchKernelGGL((k_bicg_s<T, OP>), FALM_GRID(S), s, tl, g, a, (const T *)r, (const T *)q, sv, ss,
+                       (const DevState *)S->st, c.pc_type, wpc);
+    FALM_CHECK_LAUNCH("k_bicg_s");
+    // the preconditioned s lands in ss or, out of place, in pp (free once
+    // p's sweeps have read it)
+    T *const pcs = pc_rest<T, OP>(S, a, ss, sv, pcp == pp ? ss : pp, s, pst);
+    FALM_TRY(pst);
     TimedLaunch tm3(FALM_TIME_MV2, s);
     hipLaunchKernelGGL((k_bicg_mv_km<T, OP, 2>), FALM_KM(S), s, S->kt, g, a, (const T *)pcs, t, (const T *)sv,
                        (const DevState *)S->st, S->partials2);
@@ -1988,8 +1845,7 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
             FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s, enqueued == 0)));
         // the pass's sum into x, if the pass is over (before the poll, so x
         // is complete when the solve returns)
-        if (folded_update(S)) FALM_TRY((launch_fold_apply<double, OP64>(S, a, x, r == S->r64own ? nullptr : r, s)));
-        else if (deferred_pc(S)) FALM_TRY((launch_pc_apply<double, OP64>(S, a, x, s)));
+        if (deferred_pc(S)) FALM_TRY((launch_close<double, OP64>(S, a, x, r == S->r64own ? nullptr : r, s)));
         FALM_TRY(read_state(S, s));
         if (S->hst->done || enqueued >= c.maxit) break;
         batch = check_period(S);
@@ -2025,8 +1881,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
         for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, polls == 0 && k == 0)));
         // (the float residual is not kept: the init kernels form the next
         // pass's from x)
-        if (folded_update(S)) FALM_TRY((launch_fold_apply<float, OP32>(S, a32, x, (float *)nullptr, s)));
-        else if (deferred_pc(S)) FALM_TRY((launch_pc_apply<float, OP32>(S, a32, x, s)));
+        if (deferred_pc(S)) FALM_TRY((launch_close<float, OP32>(S, a32, x, (float *)nullptr, s)));
         // a finished pass is tested with the norm alone, unless the caller
         // wants the final residual; a pass that must follow starts from the
         // full init (it recomputes the same residual)

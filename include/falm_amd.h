@@ -519,11 +519,11 @@ enum { FALM_TIME_NONE = 0, FALM_TIME_SOR = 1, FALM_TIME_MV = 2, FALM_TIME_PSEUDO
        FALM_TIME_ALM = 4, FALM_TIME_SOLVER_UPDATE = 5, FALM_TIME_PROJECT = 6,
        FALM_TIME_FACE_FLUX = 7, FALM_TIME_SOLVER_INIT = 8, FALM_TIME_SGS = 9,
        FALM_TIME_RHS = 10, FALM_TIME_DIVERGENCE = 11,
-       FALM_TIME_SOR_S = 12,   /* the s vector's fused preconditioner kernel */
+       FALM_TIME_SOR_S = 12,   /* reserved (no kernel books it)             */
        FALM_TIME_MV2 = 13,     /* the second operator product per iteration  */
-       FALM_TIME_PCMV = 14,    /* p, its preconditioning and A in one pass    */
+       FALM_TIME_PCMV = 14,    /* r, p, its preconditioning and A in one pass */
        FALM_TIME_PCMV_S = 15,  /* s, its preconditioning and A in one pass    */
-       FALM_TIME_PC_APPLY = 16 }; /* x += M(sum of the pass's directions)    */
+       FALM_TIME_PC_APPLY = 16 }; /* end of a pass: x += M(y + alpha p + omega s) */
 falm_status falm_timing_enable(int family);
 falm_status falm_timing_read(int *launches, double *ms, double *bytes);
 
