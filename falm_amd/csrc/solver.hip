@@ -474,7 +474,7 @@ __device__ inline typename OpCols<TA>::Cell cell_of(const OpCols<TA> &c, unsigne
 // thread's slot in `trash`): the compiler can then count the memory
 // operations in flight exactly and waits for the previous step's loads
 // only, not for those just issued.  trash holds K7_NT values per workgroup.
-template <class T, class OP, int MODE, bool DCLS>
+template <class T, class OP, int MODE, bool DCLS, bool FIRST>
 __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves<OP>::value))) void k_bicg_pcmv_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
                                                         const T *__restrict__ pin, const T *__restrict__ qin,
                                                         const T *__restrict__ rr, T *__restrict__ vout,
@@ -489,7 +489,8 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
     const K7Pos Q = k7_pos(kt, g);
     const int tid = (int)threadIdx.x;
     const unsigned szt = (unsigned)g.nxy * sizeof(T);
-    const bool upd = MODE == 1 || !st->first;  // q (and p) are used
+    // q (and p) are used; FIRST: the launch is a pass's first iteration
+    const bool upd = MODE == 1 || (!FIRST && !st->first);
     T c1 = T(0), c2 = T(0);
     if (MODE == 0) {
         c1 = (T)st->beta;
@@ -545,9 +546,9 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
     const auto load = [&](const Col &c, unsigned ob, int k) {
         Ld x;
         x.a = ld_at(av, ob);
-        x.b = MODE == 0 ? ld_at(tin, ob) : T(0);
-        x.q = ld_at(qin, ob);
-        x.p = MODE == 0 ? ld_at(pin, ob) : T(0);
+        x.b = (MODE == 0 && !FIRST) ? ld_at(tin, ob) : T(0);
+        x.q = (MODE == 1 || !FIRST) ? ld_at(qin, ob) : T(0);
+        x.p = (MODE == 0 && !FIRST) ? ld_at(pin, ob) : T(0);
         x.d = diag_of<DCLS>(c, ob, k);
         return x;
     };
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         Fl x;
         x.o = load(ca, cb + pf, kk);
         x.h = load(c2v, b2 + pf, kk);
-        x.y = MODE == 0 ? ld_at(yv, cb + pf) : T(0);
+        x.y = (MODE == 0 && !FIRST) ? ld_at(yv, cb + pf) : T(0);
         return x;
     };
     // slot of plane f: black cells M(v), red cells v, zero outside the
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         const bool wr = Q.act && f >= k0 && f < k1;
         const unsigned ob = cb + (unsigned)f * szt;
         *(wr ? (T *)((char *)vout + ob) : tr) = vo;
-        if (MODE == 0) {
+        if (MODE == 0 && !FIRST) {
             const bool wu = wr && upd;
             *(wu ? (T *)((char *)rout + ob) : tr) = rv;
             const double yo = (yset ? 0.0 : (double)x.y) + (ya * (double)x.o.p + yw * (double)x.o.a);
@@ -1692,7 +1693,8 @@ static falm_status launch_close(falm_solver *S, const OP &a, double *x, T *rout,
     return FALM_OK;
 }
 
-// one BiCGStab iteration (src/FalmEq.cpp:345-407)
+// one BiCGStab iteration (src/FalmEq.cpp:345-407); first: the host knows it
+// starts a pass
 template <class T, class OP>
 static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, hipStream_t s, bool first) {
     const falm_solver_config &c = S->cfg;
@@ -1719,11 +1721,13 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         // the diagonal from its classes (a compile-time choice in the kernel)
         bool dcls = false;
         if constexpr (std::is_same<OP, OpSep<T>>::value) dcls = a.dtab != nullptr;
-        const auto launch = [&](auto dc) {
-            constexpr bool D = decltype(dc)::value;
+        // the solve's first iteration starts a pass (a later pass's first
+        // iteration is found on the device and runs the general kernel)
+        const auto launch = [&](auto dc, auto fi) {
+            constexpr bool D = decltype(dc)::value, F = decltype(fi)::value;
             {
                 TimedLaunch tm(FALM_TIME_PCMV, s);
-                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D, F>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
                                    (const T *)pin, (const T *)qin, (const T *)rr, pout, qout, (const DevState *)S->st,
                                    wpc, (const double *)nullptr, 0, S->st, S->partials, trash, (const T *)sv,
                                    (const T *)t, r, ss, pid);
@@ -1734,7 +1738,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
             }
             {
                 TimedLaunch tm(FALM_TIME_PCMV_S, s);
-                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D, false>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
                                    (const T *)nullptr, (const T *)qout, (const T *)rr, sv, t,
                                    (const DevState *)S->st, wpc, (const double *)S->partials, nbp, S->st,
                                    S->partials2, trash, (const T *)nullptr, (const T *)nullptr, (T *)nullptr,
@@ -1743,8 +1747,13 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
                 tm.done(box * (5.0 * sizeof(T) + db));
             }
         };
-        if (dcls) launch(std::true_type{});
-        else launch(std::false_type{});
+        if (dcls) {
+            if (first) launch(std::true_type{}, std::true_type{});
+            else launch(std::true_type{}, std::false_type{});
+        } else {
+            if (first) launch(std::false_type{}, std::true_type{});
+            else launch(std::false_type{}, std::false_type{});
+        }
         FALM_CHECK_LAUNCH("k_bicg_pcmv_km");
         hipLaunchKernelGGL(k_fin_fold, FALM_FIN, s, S->st, (const double *)S->partials2, nbp);
         FALM_CHECK_LAUNCH("k_fin_fold");
@@ -1876,9 +1885,14 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
     FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
     // every round either advances the running pass by at least one
     // iteration or ends/starts a pass; the device ends the solve at maxit
+    // pass_start: the next iteration enqueued starts a pass (after an init
+    // that the host launched for a finished pass)
     int batch = first_batch(S), polls = 0;
+    bool pass_start = true;
     while (true) {
-        for (int k = 0; k < batch; k++) FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, polls == 0 && k == 0)));
+        for (int k = 0; k < batch; k++)
+            FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, pass_start && k == 0)));
+        pass_start = false;
         // (the float residual is not kept: the init kernels form the next
         // pass's from x)
         if (deferred_pc(S)) FALM_TRY((launch_close<float, OP32>(S, a32, x, (float *)nullptr, s)));
@@ -1890,7 +1904,10 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
         FALM_TRY(read_state(S, s));
         polls++;
         if (S->hst->final) break;
-        if (!r && S->hst->done) FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
+        if (!r && S->hst->done) {
+            FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
+            pass_start = true;
+        }
         batch = check_period(S);
         if (polls > 2 * c.maxit + 2) {
             set_error("falm_solver_solve: mixed solve made no progress");
