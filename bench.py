@@ -5,10 +5,12 @@ Workload: wind-tunnel channel of 512x128x128 cells (dx = D/32) with one
 actuator-line rotor (3 blades x 32 points, TSR 7), Upwind3 + Smagorinsky,
 BiCGStab with a red-black SOR preconditioner.  The headline figure is MLUPS
 of the mixed-precision step (float stencils and Krylov passes, double defect
-correction) with the Poisson solve stopped at |r|/N <= 1e-8 of the scaled
-system, the tolerance at which the divergence monitor sqrt(sum div^2)/N
-(app/windtunnel/main.cpp:185) stays below 1e-6 in both precisions; the FP64
-step runs beside it on the same workload.  A second pair of legs stops the
+correction) with the Poisson solve stopped at |r|/N <= 1.5e-8 of the scaled
+system, the loosest of 1e-8, 1.5e-8, 2e-8, 3e-8, 5e-8 at which the divergence
+monitor sqrt(sum div^2)/N (app/windtunnel/main.cpp:185) stays below 1e-6 in
+both precisions (7.9e-7 after the timed steps, 5 BiCGStab iterations per
+step; 2e-8 gives 1.06e-6); the FP64 step runs beside it on the same
+workload with the same tolerance.  A second pair of legs stops the
 solve at |r|/N <= 1e-6, the reference's criterion (src/FalmEq.cpp:330),
 where one BiCGStab iteration per step suffices and the monitor reads ~1e-5.
 A step is one full time step (ALM, pseudo velocity, face flux, Poisson RHS,
@@ -573,8 +575,8 @@ def main():
     ap.add_argument("--adj-dirs", type=int, default=4)
     ap.add_argument("--adj-tol", type=float, default=1e-12)
     ap.add_argument("--adj-cfl", type=float, default=0.1)
-    ap.add_argument("--tol", type=float, default=1e-8,
-                    help="Poisson tolerance on |r|/N of the headline legs (1e-8 keeps sqrt(sum div^2)/N < 1e-6)")
+    ap.add_argument("--tol", type=float, default=1.5e-8,
+                    help="Poisson tolerance on |r|/N of the headline legs (1.5e-8 keeps sqrt(sum div^2)/N < 1e-6)")
     ap.add_argument("--solver-tol-leg", type=float, default=1e-6,
                     help="second pair of legs stopped at this |r|/N (0: skip)")
     ap.add_argument("--accuracy-steps", type=int, default=1000,

@@ -137,3 +137,28 @@ def test_full_size_rotor_case(gpu, precision):
         assert np.isfinite(info.div) and info.div < 1e-5
     _, _, tq, th = c.alm_get()
     assert th[0] > 0 and np.isfinite(tq[0])
+
+
+def test_full_size_divergence_condition(gpu):
+    """The benchmark's operating point: 512x128x128 + rotor with the Poisson
+    solve at |r|/N <= 1.5e-8 keeps the divergence monitor sqrt(sum div^2)/N
+    (app/windtunnel/main.cpp:185) below 1e-6 over the 30 steps the benchmark
+    runs, in both precisions, with the same iteration counts and monitor."""
+    from falm_amd import runtime as rt
+    divs, its, us = {}, {}, {}
+    for precision in (abi.CASE_F64, abi.CASE_MIXED):
+        c = rt.Case(cases.windtunnel(precision=precision, tol=1.5e-8))
+        d, n = [], []
+        for _ in range(30):
+            info = c.step(1)
+            assert info.status == abi.FALM_OK, info.status
+            d.append(info.div)
+            n.append(info.solver_it)
+        divs[precision], its[precision] = np.array(d), np.array(n)
+        us[precision] = c.get(abi.FIELD_U)
+        c.close()
+    for precision in divs:
+        assert np.all(np.isfinite(divs[precision])) and divs[precision].max() < 1e-6, divs[precision]
+    assert np.array_equal(its[abi.CASE_F64], its[abi.CASE_MIXED]), its
+    assert np.abs(divs[abi.CASE_MIXED] / divs[abi.CASE_F64] - 1).max() < 1e-3
+    assert rel_err(us[abi.CASE_MIXED], us[abi.CASE_F64]) < 1e-5
