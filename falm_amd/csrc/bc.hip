@@ -103,7 +103,8 @@ __device__ inline void inlet_normals(uint64_t seed, int64_t step, int a, int b, 
 // ubc: app/windtunnel/bcdevcall.cu:8-115, app/lid3d/bcdev.cu:11-105
 template <class T>
 __global__ void k_bc_velocity(FaceSet fs, Geom g, T *u0, T *u1, T *u2, const T *p0, const T *p1, const T *p2,
-                              const T *xc, T dt, falm_inlet_noise noise) {
+                              const T *xc, T dt, falm_inlet_noise noise, const int *gate) {
+    if (gated_off(gate)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= fs.total) return;
     int ka, kb;
@@ -157,7 +158,8 @@ __global__ void k_bc_velocity(FaceSet fs, Geom g, T *u0, T *u1, T *u2, const T *
 
 // pbc: app/windtunnel/bcdevcall.cu:153-211 (Neumann copy or p = 0)
 template <class T>
-__global__ void k_bc_pressure(FaceSet fs, Geom g, T *p) {
+__global__ void k_bc_pressure(FaceSet fs, Geom g, T *p, const int *gate) {
+    if (gated_off(gate)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= fs.total) return;
     int ka, kb;
@@ -182,7 +184,8 @@ __global__ void k_bc_face_flux(FaceSet fs, Geom g, T *uu0, T *uu1, T *uu2) {
 
 // copy_z5: app/ld2d/bcdevcall.cu:240-256
 template <class T>
-__global__ void k_copy_z5(Geom g, T *v, int ncol, size_t ncell) {
+__global__ void k_copy_z5(Geom g, T *v, int ncol, size_t ncell, const int *gate) {
+    if (gated_off(gate)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nplane = g.nx * g.ny;
     if (t >= nplane * ncol) return;
@@ -237,7 +240,8 @@ __global__ void k_scale_all(T *a, size_t n, T s) {
 // -------------------------------------------------------------- host
 template <class T>
 static falm_status bc_velocity_t(const falm_domain *dom, const falm_face_bc faces[6], void *u, const void *uprev,
-                                 const void *xyz, double dt, const falm_inlet_noise *noise, hipStream_t s) {
+                                 const void *xyz, double dt, const falm_inlet_noise *noise, hipStream_t s,
+                                 const int *gate) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     int type[6];
@@ -269,26 +273,29 @@ static falm_status bc_velocity_t(const falm_domain *dom, const falm_face_bc face
     const T *pv = (const T *)(uprev ? uprev : u);
     const T *xv = (const T *)(xyz ? xyz : u);
     hipLaunchKernelGGL((k_bc_velocity<T>), dim3((fs.total + 255) / 256), dim3(256), 0, s, fs, g, col<T>(u, n, 0),
-                       col<T>(u, n, 1), col<T>(u, n, 2), pv, pv + n, pv + 2 * n, xv, (T)dt, nz);
+                       col<T>(u, n, 1), col<T>(u, n, 2), pv, pv + n, pv + 2 * n, xv, (T)dt, nz, gate);
     FALM_CHECK_LAUNCH("k_bc_velocity");
     return FALM_OK;
 }
 
 falm_status launch_bc_velocity(const falm_domain *dom, int prec, const falm_face_bc faces[6], void *u,
                                const void *uprev, const void *xyz, double dt, const falm_inlet_noise *noise,
-                               hipStream_t s) {
-    return prec == FALM_F32 ? bc_velocity_t<float>(dom, faces, u, uprev, xyz, dt, noise, s)
-                            : bc_velocity_t<double>(dom, faces, u, uprev, xyz, dt, noise, s);
+                               hipStream_t s, const int *gate) {
+    return prec == FALM_F32 ? bc_velocity_t<float>(dom, faces, u, uprev, xyz, dt, noise, s, gate)
+                            : bc_velocity_t<double>(dom, faces, u, uprev, xyz, dt, noise, s, gate);
 }
 
-falm_status launch_bc_pressure(const falm_domain *dom, int prec, const int faces[6], void *p, hipStream_t s) {
+falm_status launch_bc_pressure(const falm_domain *dom, int prec, const int faces[6], void *p, hipStream_t s,
+                               const int *gate) {
     Geom g = make_geom(dom);
     FaceSet fs = make_faces(g, faces, nullptr);
     if (fs.total == 0) return FALM_OK;
     if (prec == FALM_F32)
-        hipLaunchKernelGGL((k_bc_pressure<float>), dim3((fs.total + 255) / 256), dim3(256), 0, s, fs, g, (float *)p);
+        hipLaunchKernelGGL((k_bc_pressure<float>), dim3((fs.total + 255) / 256), dim3(256), 0, s, fs, g, (float *)p,
+                           gate);
     else
-        hipLaunchKernelGGL((k_bc_pressure<double>), dim3((fs.total + 255) / 256), dim3(256), 0, s, fs, g, (double *)p);
+        hipLaunchKernelGGL((k_bc_pressure<double>), dim3((fs.total + 255) / 256), dim3(256), 0, s, fs, g, (double *)p,
+                           gate);
     FALM_CHECK_LAUNCH("k_bc_pressure");
     return FALM_OK;
 }
@@ -308,14 +315,16 @@ falm_status launch_bc_face_flux(const falm_domain *dom, int prec, const int face
     return FALM_OK;
 }
 
-falm_status launch_copy_z5(const falm_domain *dom, int prec, void *v, int ncol, hipStream_t s) {
+falm_status launch_copy_z5(const falm_domain *dom, int prec, void *v, int ncol, hipStream_t s, const int *gate) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     int total = g.nx * g.ny * ncol;
     if (prec == FALM_F32)
-        hipLaunchKernelGGL((k_copy_z5<float>), dim3((total + 255) / 256), dim3(256), 0, s, g, (float *)v, ncol, n);
+        hipLaunchKernelGGL((k_copy_z5<float>), dim3((total + 255) / 256), dim3(256), 0, s, g, (float *)v, ncol, n,
+                           gate);
     else
-        hipLaunchKernelGGL((k_copy_z5<double>), dim3((total + 255) / 256), dim3(256), 0, s, g, (double *)v, ncol, n);
+        hipLaunchKernelGGL((k_copy_z5<double>), dim3((total + 255) / 256), dim3(256), 0, s, g, (double *)v, ncol, n,
+                           gate);
     FALM_CHECK_LAUNCH("k_copy_z5");
     return FALM_OK;
 }
@@ -387,7 +396,7 @@ falm_status falm_bc_velocity(const falm_domain *dom, int prec, const falm_face_b
                              void *stream) {
     FALM_TRY(check_domain(dom));
     if (!faces || !u) return FALM_ERR_INVALID;
-    return launch_bc_velocity(dom, prec, faces, u, uprev, xyz, dt, noise, (hipStream_t)stream);
+    return launch_bc_velocity(dom, prec, faces, u, uprev, xyz, dt, noise, (hipStream_t)stream, nullptr);
 }
 
 falm_status falm_bc_pressure(const falm_domain *dom, int prec, const int faces[6], void *p, void *stream) {
@@ -395,7 +404,7 @@ falm_status falm_bc_pressure(const falm_domain *dom, int prec, const int faces[6
     if (!faces || !p) return FALM_ERR_INVALID;
     for (int f = 0; f < 6; f++)
         if (faces[f] < FALM_PBC_NONE || faces[f] > FALM_PBC_ZERO) return FALM_ERR_INVALID;
-    return launch_bc_pressure(dom, prec, faces, p, (hipStream_t)stream);
+    return launch_bc_pressure(dom, prec, faces, p, (hipStream_t)stream, nullptr);
 }
 
 falm_status falm_bc_face_flux(const falm_domain *dom, int prec, const int faces[6], void *uu, void *stream) {
@@ -411,7 +420,7 @@ falm_status falm_copy_z5(const falm_domain *dom, int prec, void *v, int ncol, vo
         set_error("falm_copy_z5: needs a z-thin domain (one interior layer)");
         return FALM_ERR_INVALID;
     }
-    return launch_copy_z5(dom, prec, v, ncol, (hipStream_t)stream);
+    return launch_copy_z5(dom, prec, v, ncol, (hipStream_t)stream, nullptr);
 }
 
 falm_status falm_metrics(const falm_domain *dom, int prec, const double *x, const double *y, const double *z,

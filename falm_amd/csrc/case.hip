@@ -28,10 +28,11 @@
 namespace falm {
 falm_status launch_bc_velocity(const falm_domain *dom, int prec, const falm_face_bc faces[6], void *u,
                                const void *uprev, const void *xyz, double dt, const falm_inlet_noise *noise,
-                               hipStream_t s);
-falm_status launch_bc_pressure(const falm_domain *dom, int prec, const int faces[6], void *p, hipStream_t s);
+                               hipStream_t s, const int *gate);
+falm_status launch_bc_pressure(const falm_domain *dom, int prec, const int faces[6], void *p, hipStream_t s,
+                               const int *gate);
 falm_status launch_bc_face_flux(const falm_domain *dom, int prec, const int faces[6], void *uu, hipStream_t s);
-falm_status launch_copy_z5(const falm_domain *dom, int prec, void *v, int ncol, hipStream_t s);
+falm_status launch_copy_z5(const falm_domain *dom, int prec, void *v, int ncol, hipStream_t s, const int *gate);
 }  // namespace falm
 
 using namespace falm;
@@ -276,6 +277,31 @@ static falm_status case_init(falm_case *C) {
     return FALM_OK;
 }
 
+// The step after the Poisson solve: pressure boundary fill, projection,
+// velocity boundary fill, Smagorinsky and the divergence monitor.  With a
+// gate, every kernel does nothing unless *gate is nonzero.
+static falm_status post_solve(falm_case *C, const int *gate) {
+    const falm_case_config &c = C->cfg;
+    hipStream_t s = C->stream;
+    const falm_domain *d = &C->dom;
+    const int pr = C->prec;
+    const bool is2d = c.kind == FALM_CASE_LID2D;
+    FALM_TRY(launch_bc_pressure(d, FALM_F64, C->pbc, C->p, s, gate));
+    if (is2d) FALM_TRY(launch_copy_z5(d, FALM_F64, C->p, 1, s, gate));
+    FALM_TRY(sep_project(d, pr, c.dt, C->u, C->uu, C->p, C->sm, s, gate));
+    falm_inlet_noise nz{c.inlet_intensity, c.seed, C->it};
+    FALM_TRY(launch_bc_velocity(d, pr, C->ubc, C->u, C->uprev, C->xyz, c.dt, &nz, s, gate));
+    if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->u, 3, s, gate));
+    // SGS and the divergence monitor in one pass (both read final fields)
+    if (c.sgs_model == FALM_SGS_SMAGORINSKY) {
+        FALM_TRY(sep_sgs_div_norm(d, pr, c.Cs, C->u, C->nut, C->uu, C->div, C->sm, C->dnorm, C->partials, s, gate));
+        if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->nut, 1, s, gate));
+    } else {
+        FALM_TRY(sep_div_norm(d, pr, C->uu, C->div, C->sm, C->dnorm, C->partials, s, gate));
+    }
+    return FALM_OK;
+}
+
 falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
     const falm_case_config &c = C->cfg;
     hipStream_t s = C->stream;
@@ -305,19 +331,26 @@ falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
         FALM_TRY(launch_bc_face_flux(d, pr, C->uubc, C->uu, s));
         FALM_TRY(sep_poisson_rhs(d, pr, C->uu, C->rhs, C->sm, 1.0 / (c.dt * C->maxdiag), s));
     }
-    FALM_TRY(falm_solver_solve(C->solver, C->a, C->p, C->rhs, C->prec == FALM_F64 ? C->res : nullptr, s, si));
-    FALM_TRY(launch_bc_pressure(d, FALM_F64, C->pbc, C->p, s));
-    if (is2d) FALM_TRY(launch_copy_z5(d, FALM_F64, C->p, 1, s));
-    FALM_TRY(sep_project(d, pr, c.dt, C->u, C->uu, C->p, C->sm, s));
-    falm_inlet_noise nz{c.inlet_intensity, c.seed, C->it};
-    FALM_TRY(launch_bc_velocity(d, pr, C->ubc, C->u, C->uprev, C->xyz, c.dt, &nz, s));
-    if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->u, 3, s));
-    // SGS and the divergence monitor in one pass (both read final fields)
-    if (c.sgs_model == FALM_SGS_SMAGORINSKY) {
-        FALM_TRY(sep_sgs_div_norm(d, pr, c.Cs, C->u, C->nut, C->uu, C->div, C->sm, C->dnorm, C->partials, s));
-        if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->nut, 1, s));
+    // The solve's first batch of iterations (as many as the previous step
+    // needed), then the rest of the step gated on the solver's "over" flag,
+    // then the one wait of the step: if the first batch did not finish the
+    // solve, the gated kernels did nothing and run again after it.  A solver
+    // whose last solve took more than its first batch runs whole instead.
+    double *res = C->prec == FALM_F64 ? C->res : nullptr;
+    const int *gate = solver_gate(C->solver);
+    bool rerun = true;
+    if (gate) {
+        FALM_TRY(solver_solve_mode(C->solver, C->a, C->p, C->rhs, res, s, si, SOLVE_BEGIN, nullptr));
+        FALM_TRY(post_solve(C, gate));
+        FALM_HIP(hipMemcpyAsync(C->hnorm, C->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+        FALM_TRY(solver_solve_mode(C->solver, C->a, C->p, C->rhs, res, s, si, SOLVE_RESUME, &rerun));
     } else {
-        FALM_TRY(sep_div_norm(d, pr, C->uu, C->div, C->sm, C->dnorm, C->partials, s));
+        FALM_TRY(solver_solve_mode(C->solver, C->a, C->p, C->rhs, res, s, si, SOLVE_FULL, nullptr));
+    }
+    C->norm_pending = rerun;
+    if (rerun) {
+        FALM_TRY(post_solve(C, nullptr));
+        FALM_HIP(hipMemcpyAsync(C->hnorm, C->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
     }
     return FALM_OK;
 }
@@ -393,8 +426,8 @@ falm_status falm_case_step(falm_case *C, int nsteps, falm_step_info *info) {
     if (!C || nsteps < 0) return FALM_ERR_INVALID;
     falm_solve_info si{0, 0.0, 0, FALM_OK};
     for (int k = 0; k < nsteps; k++) FALM_TRY(case_step_power(C, &si, nullptr));
-    FALM_HIP(hipMemcpyAsync(C->hnorm, C->dnorm, sizeof(double), hipMemcpyDeviceToHost, C->stream));
-    FALM_HIP(hipStreamSynchronize(C->stream));
+    if (nsteps == 0) FALM_HIP(hipMemcpyAsync(C->hnorm, C->dnorm, sizeof(double), hipMemcpyDeviceToHost, C->stream));
+    if (nsteps == 0 || C->norm_pending) FALM_HIP(hipStreamSynchronize(C->stream));
     if (info) {
         const falm_domain &d = C->dom;
         const double nint = (double)(d.shape[0] - 2 * d.gc) * (double)(d.shape[1] - 2 * d.gc) *

@@ -28,6 +28,7 @@ struct falm_case {
     int pbc[6], uubc[6];
     // device scalar for the divergence norm + its reduction scratch
     double *dnorm = nullptr, *partials = nullptr, *hnorm = nullptr;
+    bool norm_pending = false;  // the last step's copy into hnorm is not waited for
     int *nobox = nullptr;     // empty cell box: no rotor, ff stays zero
     std::vector<void *> bufs;
     // the last falm_case_adjoint's attitude gradient: per turbine
@@ -37,11 +38,22 @@ struct falm_case {
 
 
 namespace falm {
-// one time step on the case's stream (no synchronisation); power, when
-// given, accumulates the step's rotor power on the device
+// one time step on the case's stream; power, when given, accumulates the
+// step's rotor power on the device.  It waits for the stream once, after the
+// step's last kernel when the first batch of solver iterations finished the
+// solve (else the step's end and the copy of the divergence norm into hnorm
+// are still in flight: norm_pending).
 falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power);
 falm_status case_dalloc(falm_case *C, void **p, size_t bytes);
 // stopping tolerance of a solver's next solves (|r|/N), replacing its config's
 falm_status solver_set_tol(falm_solver *S, double tol);
 falm_status solver_set_maxit(falm_solver *S, int maxit);
+// falm_solver_solve split around work enqueued between a SOLVE_BEGIN call
+// (the first batch of iterations, no wait) and a SOLVE_RESUME call (polls and
+// finishes the solve; *rerun: the first batch had not finished it, so work
+// gated on solver_gate() did nothing)
+enum SolveMode { SOLVE_FULL = 0, SOLVE_BEGIN = 1, SOLVE_RESUME = 2 };
+falm_status solver_solve_mode(falm_solver *S, const void *a, void *x, const void *b, void *r, hipStream_t s,
+                              falm_solve_info *info, int mode, bool *rerun);
+const int *solver_gate(const falm_solver *S);
 }  // namespace falm

@@ -809,7 +809,8 @@ __global__ __launch_bounds__(256) void k_smagorinsky(Tiling tl, Geom g, T Cs, Ve
 // (src/dev/FalmCFDDevCall.cu:351-407).
 template <class T, class P>
 __global__ __launch_bounds__(K7_NT) void k_project_km(K7Tiling kt, Geom g, P dt, Vec3W<T> u, Vec3W<T> uu,
-                                                      const P *__restrict__ p, SepTab<P> tb) {
+                                                      const P *__restrict__ p, SepTab<P> tb, const int *gate) {
+    if (gated_off(gate)) return;
     __shared__ K7Lds<P> lp;
     const K7Pos Q = k7_pos(kt, g);
     const int sz = g.nxy;
@@ -885,7 +886,8 @@ inline long long face_layer_count(const Geom &g) {
 
 template <class T, class P>
 __global__ __launch_bounds__(256) void k_project_layer(Geom g, P dt, Vec3W<T> uu, const P *__restrict__ p,
-                                                       SepTab<P> tb) {
+                                                       SepTab<P> tb, const int *gate) {
+    if (gated_off(gate)) return;
     int i, j, k;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (!face_layer_cell(g, t, i, j, k)) return;
@@ -1084,7 +1086,8 @@ __global__ __launch_bounds__(256) void k_div_norm(Tiling tl, Geom g, Vec3<T> uu,
 // one partial per workgroup.
 template <class T>
 __global__ __launch_bounds__(K7_NT) void k_div_norm_km(K7Tiling kt, Geom g, Vec3<T> uu, T *__restrict__ div,
-                                                       SepTab<T> tb, double *partials) {
+                                                       SepTab<T> tb, double *partials, const int *gate) {
+    if (gated_off(gate)) return;
     __shared__ double lds[K7_NT / 64];
     const K7Pos Q = k7_pos(kt, g);
     double v[1] = {0.0};
@@ -1114,7 +1117,8 @@ __global__ __launch_bounds__(K7_NT) void k_div_norm_km(K7Tiling kt, Geom g, Vec3
 template <class T>
 __global__ __launch_bounds__(K7_NT) void k_sgs_div_km(K7Tiling kt, Geom g, T Cs, Vec3<T> u, T *__restrict__ nut,
                                                       Vec3<T> uu, T *__restrict__ div, SepTab<T> tb,
-                                                      double *partials) {
+                                                      double *partials, const int *gate) {
+    if (gated_off(gate)) return;
     __shared__ K7Lds<T> l0, l1, l2;
     __shared__ double lds[K7_NT / 64];
     const K7Pos Q = k7_pos(kt, g);
@@ -1185,7 +1189,8 @@ __global__ __launch_bounds__(K7_NT) void k_sgs_div_km(K7Tiling kt, Geom g, T Cs,
     block_partials<K7_NT, 1>(v, partials, lds);
 }
 
-__global__ __launch_bounds__(FIN_NT) void k_fin_norm(const double *partials, int nb, double *out) {
+__global__ __launch_bounds__(FIN_NT) void k_fin_norm(const double *partials, int nb, double *out, const int *gate) {
+    if (gated_off(gate)) return;
     double tot[1];
     sum_partials<1>(partials, nb, tot);
     if (threadIdx.x == 0) out[0] = tot[0];
@@ -1363,7 +1368,7 @@ static falm_status smag_t(const falm_domain *dom, double Cs, const void *u, void
 
 template <class T>
 static falm_status div_norm_t(const falm_domain *dom, const void *uu, void *div, const SepMesh &sm,
-                              double *dev_out, double *partials, hipStream_t s) {
+                              double *dev_out, double *partials, hipStream_t s, const int *gate) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     K7Tiling kt = make_k7_tiling(interior_box(g));
@@ -1371,9 +1376,9 @@ static falm_status div_norm_t(const falm_domain *dom, const void *uu, void *div,
     FALM_TRY(check_byte_offsets(n, sizeof(T)));
     TimedLaunch tm(FALM_TIME_DIVERGENCE, s);
     hipLaunchKernelGGL((k_div_norm_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, v3<T>(uu, n), (T *)div,
-                       sep_tab<T>(sm), partials);
+                       sep_tab<T>(sm), partials, gate);
     tm.done((double)interior_box(g).size() * 4.0 * sizeof(T));  // uu read, div written
-    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out);
+    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out, gate);
     FALM_CHECK_LAUNCH("k_div_norm");
     return FALM_OK;
 }
@@ -1483,7 +1488,7 @@ falm_status sep_poisson_rhs(const falm_domain *dom, int prec, const void *uu, do
 }
 template <class T>
 static falm_status project_km_t(const falm_domain *dom, double dt, void *u, void *uu, const double *p,
-                                const SepMesh &sm, hipStream_t s) {
+                                const SepMesh &sm, hipStream_t s, const int *gate) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     K7Tiling kt = make_k7_tiling(interior_box(g));
@@ -1491,10 +1496,10 @@ static falm_status project_km_t(const falm_domain *dom, double dt, void *u, void
     TimedLaunch tm(FALM_TIME_PROJECT, s);
     FALM_TRY(check_byte_offsets(n, sizeof(double)));
     hipLaunchKernelGGL((k_project_km<T, double>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, dt, v3w<T>(u, n),
-                       v3w<T>(uu, n), p, sep_tab<double>(sm));
+                       v3w<T>(uu, n), p, sep_tab<double>(sm), gate);
     const long long nl = face_layer_count(g);
     hipLaunchKernelGGL((k_project_layer<T, double>), ew_grid((size_t)nl), dim3(EW_BLOCK), 0, s, g, dt, v3w<T>(uu, n),
-                       p, sep_tab<double>(sm));
+                       p, sep_tab<double>(sm), gate);
     FALM_CHECK_LAUNCH("k_project_km");
     // u, uu read and written, p read
     tm.done((double)make_tiling(face_box(g)).box.size() * (12.0 * sizeof(T) + sizeof(double)));
@@ -1502,9 +1507,9 @@ static falm_status project_km_t(const falm_domain *dom, double dt, void *u, void
 }
 
 falm_status sep_project(const falm_domain *dom, int prec, double dt, void *u, void *uu, const double *p,
-                        const SepMesh &sm, hipStream_t s) {
-    if (prec == FALM_F32) return project_km_t<float>(dom, dt, u, uu, p, sm, s);
-    return project_km_t<double>(dom, dt, u, uu, p, sm, s);
+                        const SepMesh &sm, hipStream_t s, const int *gate) {
+    if (prec == FALM_F32) return project_km_t<float>(dom, dt, u, uu, p, sm, s, gate);
+    return project_km_t<double>(dom, dt, u, uu, p, sm, s, gate);
 }
 template <class T>
 static falm_status smag_km_t(const falm_domain *dom, double Cs, const void *u, void *nut, const SepMesh &sm,
@@ -1529,7 +1534,8 @@ falm_status sep_sgs(const falm_domain *dom, int prec, double Cs, const void *u, 
 }
 template <class T>
 static falm_status sgs_div_norm_t(const falm_domain *dom, double Cs, const void *u, void *nut, const void *uu,
-                                  void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s) {
+                                  void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s,
+                                  const int *gate) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     K7Tiling kt = make_k7_tiling(interior_box(g));
@@ -1537,23 +1543,24 @@ static falm_status sgs_div_norm_t(const falm_domain *dom, double Cs, const void 
     FALM_TRY(check_byte_offsets(n, sizeof(T)));
     TimedLaunch tm(FALM_TIME_SGS, s);
     hipLaunchKernelGGL((k_sgs_div_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, (T)Cs, v3<T>(u, n), (T *)nut,
-                       v3<T>(uu, n), (T *)div, sep_tab<T>(sm), partials);
+                       v3<T>(uu, n), (T *)div, sep_tab<T>(sm), partials, gate);
     tm.done((double)interior_box(g).size() * 8.0 * sizeof(T));  // u, uu read; nut, div written
-    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out);
+    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out, gate);
     FALM_CHECK_LAUNCH("k_sgs_div_km");
     return FALM_OK;
 }
 
 falm_status sep_sgs_div_norm(const falm_domain *dom, int prec, double Cs, const void *u, void *nut, const void *uu,
-                             void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s) {
-    if (prec == FALM_F32) return sgs_div_norm_t<float>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s);
-    return sgs_div_norm_t<double>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s);
+                             void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s,
+                             const int *gate) {
+    if (prec == FALM_F32) return sgs_div_norm_t<float>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s, gate);
+    return sgs_div_norm_t<double>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s, gate);
 }
 
 falm_status sep_div_norm(const falm_domain *dom, int prec, const void *uu, void *div, const SepMesh &sm,
-                         double *dev_out, double *partials, hipStream_t s) {
-    if (prec == FALM_F32) return div_norm_t<float>(dom, uu, div, sm, dev_out, partials, s);
-    return div_norm_t<double>(dom, uu, div, sm, dev_out, partials, s);
+                         double *dev_out, double *partials, hipStream_t s, const int *gate) {
+    if (prec == FALM_F32) return div_norm_t<float>(dom, uu, div, sm, dev_out, partials, s, gate);
+    return div_norm_t<double>(dom, uu, div, sm, dev_out, partials, s, gate);
 }
 
 }  // namespace falm

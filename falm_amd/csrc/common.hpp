@@ -288,6 +288,12 @@ __device__ inline void block_partials(double (&v)[NV], double *partials, double 
     }
 }
 
+// A launch gated on a device flag: the kernel does nothing unless *gate is
+// nonzero (gate null: always runs).  The step driver enqueues the kernels
+// after the Poisson solve gated on the solver's "finished" flag before it
+// knows whether the first batch of iterations finished the solve.
+__device__ inline bool gated_off(const int *gate) { return gate && !*gate; }
+
 constexpr int FIN_NT = 1024;  // threads of a finalize workgroup
 
 // Phase 2 helper for an NT-thread workgroup: totals of nb partials per

@@ -39,6 +39,7 @@
 #include <utility>
 #include <vector>
 
+#include "case.hpp"
 #include "km.hpp"
 #include "linalg.hpp"
 
@@ -1518,6 +1519,11 @@ struct falm_solver {
     int pflip = 0;                // fused SOR path: p lives in w (1) or p (0)
     int qflip = 0;                // deferred path: q lives in pp (1) or q (0)
     void *trash = nullptr;        // k_bicg_pcmv_km's stores of threads with nothing to store
+    // a solve split in two calls (begin / resume): the polling loop's state
+    // between them
+    int cur_batch = 0, cur_enqueued = 0, cur_polls = 0;
+    bool cur_pass_start = false;
+    bool first_ok = false;  // the last solve was over after its first batch
 };
 
 namespace falm {
@@ -1840,24 +1846,42 @@ static falm_status launch_check(falm_solver *S, const OP64 &a, const double *x, 
     return FALM_OK;
 }
 
+// mode (SolveMode): the whole solve, or split in two calls around work the
+// caller enqueues between them.  SOLVE_BEGIN enqueues the first batch of
+// iterations and returns without waiting; SOLVE_RESUME polls, finishes the
+// solve and sets *rerun when the first batch had not finished it (the flag
+// solver_gate() names was still zero when the caller's work ran).
 template <class OP64>
 static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, const double *b, double *r,
-                                      hipStream_t s, falm_solve_info *info) {
+                                      hipStream_t s, falm_solve_info *info, int mode) {
     const falm_solver_config &c = S->cfg;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit, 0.0, 0,
-                       1);
-    FALM_CHECK_LAUNCH("k_state_reset");
-    FALM_TRY((launch_init<double, OP64>(S, a, x, b, r, r, s)));
-    int batch = first_batch(S), enqueued = 0;
-    while (true) {
+    int &batch = S->cur_batch, &enqueued = S->cur_enqueued;
+    auto enqueue_batch = [&]() -> falm_status {
         for (int k = 0; k < batch && enqueued < c.maxit; k++, enqueued++)
             FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s, enqueued == 0)));
         // the pass's sum into x, if the pass is over (before the poll, so x
         // is complete when the solve returns)
         if (deferred_pc(S)) FALM_TRY((launch_close<double, OP64>(S, a, x, r == S->r64own ? nullptr : r, s)));
+        return FALM_OK;
+    };
+    if (mode != SOLVE_RESUME) {
+        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit,
+                           0.0, 0, 1);
+        FALM_CHECK_LAUNCH("k_state_reset");
+        FALM_TRY((launch_init<double, OP64>(S, a, x, b, r, r, s)));
+        batch = first_batch(S);
+        enqueued = 0;
+        FALM_TRY(enqueue_batch());
+        if (mode == SOLVE_BEGIN) return FALM_OK;
+    }
+    bool first_read = true;
+    while (true) {
         FALM_TRY(read_state(S, s));
+        if (first_read) S->first_ok = S->hst->done;
+        first_read = false;
         if (S->hst->done || enqueued >= c.maxit) break;
         batch = check_period(S);
+        FALM_TRY(enqueue_batch());
     }
     info->it = S->hst->it_pass;
     info->err = S->hst->err;
@@ -1874,22 +1898,18 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
 // next pass, all on the device; the host polls once per batch.
 template <class OP64, class OP32>
 static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const OP32 &a32, double *x,
-                                        const double *b, double *r, hipStream_t s, falm_solve_info *info) {
+                                        const double *b, double *r, hipStream_t s, falm_solve_info *info, int mode) {
     const falm_solver_config &c = S->cfg;
     const int inner_maxit = c.inner_maxit > 0 ? c.inner_maxit : 50;
     const double inner_rtol = c.inner_rtol > 0 ? c.inner_rtol : 1e-4;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 1, c.tol, c.maxit,
-                       inner_rtol, inner_maxit, 1);
-    FALM_CHECK_LAUNCH("k_state_reset");
     float *r32 = (float *)S->r32;
-    FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
     // every round either advances the running pass by at least one
     // iteration or ends/starts a pass; the device ends the solve at maxit
     // pass_start: the next iteration enqueued starts a pass (after an init
     // that the host launched for a finished pass)
-    int batch = first_batch(S), polls = 0;
-    bool pass_start = true;
-    while (true) {
+    int &batch = S->cur_batch, &polls = S->cur_polls;
+    bool &pass_start = S->cur_pass_start;
+    auto enqueue_round = [&]() -> falm_status {
         for (int k = 0; k < batch; k++)
             FALM_TRY((bicg_iteration<float, OP32>(S, a32, x, r32, s, pass_start && k == 0)));
         pass_start = false;
@@ -1901,7 +1921,24 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
         // full init (it recomputes the same residual)
         if (r) FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
         else FALM_TRY((launch_check<OP64>(S, a64, x, b, s)));
+        return FALM_OK;
+    };
+    if (mode != SOLVE_RESUME) {
+        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 1, c.tol, c.maxit,
+                           inner_rtol, inner_maxit, 1);
+        FALM_CHECK_LAUNCH("k_state_reset");
+        FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
+        batch = first_batch(S);
+        polls = 0;
+        pass_start = true;
+        FALM_TRY(enqueue_round());
+        if (mode == SOLVE_BEGIN) return FALM_OK;
+    }
+    bool first_read = true;
+    while (true) {
         FALM_TRY(read_state(S, s));
+        if (first_read) S->first_ok = S->hst->final;
+        first_read = false;
         polls++;
         if (S->hst->final) break;
         if (!r && S->hst->done) {
@@ -1913,6 +1950,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
             set_error("falm_solver_solve: mixed solve made no progress");
             return FALM_ERR_BREAKDOWN;
         }
+        FALM_TRY(enqueue_round());
     }
     const DevState &h = *S->hst;
     info->it = h.total;
@@ -1926,16 +1964,13 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
 // FalmEq::SOR (src/FalmEq.cpp:173-240) and FalmEq::Jacobi (:81-127)
 template <class OP>
 static falm_status solve_stationary(falm_solver *S, const OP &a, double *x, const double *b, double *r,
-                                    hipStream_t s, falm_solve_info *info) {
+                                    hipStream_t s, falm_solve_info *info, int mode) {
     const falm_solver_config &c = S->cfg;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit, 0.0, 0,
-                       0);
-    FALM_CHECK_LAUNCH("k_state_reset");
     const int period = check_period(S);
     const int *done = &S->st->done;
     const int nb = red_blocks(S->tl);
-    int enqueued = 0;
-    while (true) {
+    int &enqueued = S->cur_enqueued;
+    auto enqueue_batch = [&]() -> falm_status {
         for (int k = 0; k < period && enqueued < c.maxit; k++, enqueued++) {
             if (c.type == FALM_LS_SOR) {
                 hipLaunchKernelGGL((k_sor_op<double, OP>), FALM_GRID(S), s, S->tl, S->g, a, x, b, c.relax,
@@ -1953,8 +1988,23 @@ static falm_status solve_stationary(falm_solver *S, const OP &a, double *x, cons
             hipLaunchKernelGGL(k_fin_resnorm, FALM_FIN, s, S->st, (const double *)S->partials, nb);
             FALM_CHECK_LAUNCH("k_res_norm");
         }
+        return FALM_OK;
+    };
+    if (mode != SOLVE_RESUME) {
+        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit,
+                           0.0, 0, 0);
+        FALM_CHECK_LAUNCH("k_state_reset");
+        enqueued = 0;
+        FALM_TRY(enqueue_batch());
+        if (mode == SOLVE_BEGIN) return FALM_OK;
+    }
+    bool first_read = true;
+    while (true) {
         FALM_TRY(read_state(S, s));
+        if (first_read) S->first_ok = S->hst->done;
+        first_read = false;
         if (S->hst->done || enqueued >= c.maxit) break;
+        FALM_TRY(enqueue_batch());
     }
     info->it = S->hst->it_pass;
     info->err = S->hst->err;
@@ -2073,6 +2123,73 @@ falm_status solver_set_maxit(falm_solver *S, int maxit) {
 
 }  // namespace falm
 
+namespace falm {
+
+// falm_solver_solve in one of the SolveMode forms.  The argument checks and
+// the matrix preparation run in the call that starts the solve.
+static falm_status solve_dispatch(falm_solver *S, const void *a, void *x, const void *b, void *r, hipStream_t s,
+                                  falm_solve_info *info, int mode) {
+    if (!S || !a || !x || !b) {
+        set_error("falm_solver_solve: null argument");
+        return FALM_ERR_INVALID;
+    }
+    if (x == r || x == b || r == b) {
+        set_error("falm_solver_solve: x, b and r must be distinct");
+        return FALM_ERR_ALIAS;
+    }
+    falm_solve_info tmp;
+    if (!info) info = &tmp;
+    if (!r && S->cfg.precision != FALM_SOLVE_MIXED) {
+        // the double solvers iterate on r itself: use the solver's own vector
+        if (!S->r64own) {
+            FALM_HIP(hipMalloc(&S->r64own, S->n * sizeof(double)));
+            FALM_HIP(hipMemsetAsync(S->r64own, 0, S->n * sizeof(double), s));
+        }
+        r = S->r64own;
+    }
+    double *xd = (double *)x, *rd = (double *)r;
+    const double *bd = (const double *)b;
+    const bool prepared = (S->a_src == a);
+    if (!prepared && S->cfg.precision == FALM_SOLVE_MIXED) {
+        // no declared matrix: build the float copy from this one now
+        FALM_TRY(prepare_matrix(S, a, s));
+    }
+    const bool sep = (S->a_src == a) && S->sep;
+    if (S->cfg.type == FALM_LS_PBICGSTAB) {
+        if (S->cfg.precision == FALM_SOLVE_MIXED) {
+            if (sep)
+                return solve_bicgstab_mixed(S, sep_view<double>(S, (const double *)a, S->sep64),
+                                            sep_view<float>(S, S->a32, S->sep32), xd, bd, rd, s, info, mode);
+            return solve_bicgstab_mixed(S, cols_view<double>(S, a), cols_view<float>(S, S->a32), xd, bd, rd, s, info,
+                                        mode);
+        }
+        if (sep)
+            return solve_bicgstab_f64(S, sep_view<double>(S, (const double *)a, S->sep64), xd, bd, rd, s, info, mode);
+        return solve_bicgstab_f64(S, cols_view<double>(S, a), xd, bd, rd, s, info, mode);
+    }
+    if (sep)
+        return solve_stationary(S, sep_view<double>(S, (const double *)a, S->sep64), xd, bd, rd, s, info, mode);
+    return solve_stationary(S, cols_view<double>(S, a), xd, bd, rd, s, info, mode);
+}
+
+falm_status solver_solve_mode(falm_solver *S, const void *a, void *x, const void *b, void *r, hipStream_t s,
+                              falm_solve_info *info, int mode, bool *rerun) {
+    const falm_status st = solve_dispatch(S, a, x, b, r, s, info, mode);
+    if (rerun) *rerun = !S->first_ok;
+    return st;
+}
+
+// The device flag that is set once the solve begun by SOLVE_BEGIN is over:
+// the mixed solve's "final" (its passes ended), else "done".  Null when the
+// last solve was not over after its first batch (the next one likely will
+// not be either: the caller then runs the solve whole).
+const int *solver_gate(const falm_solver *S) {
+    if (!S->first_ok) return nullptr;
+    return S->cfg.type == FALM_LS_PBICGSTAB && S->cfg.precision == FALM_SOLVE_MIXED ? &S->st->final : &S->st->done;
+}
+
+}  // namespace falm
+
 extern "C" {
 
 falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config *cfg, falm_solver **out) {
@@ -2137,45 +2254,7 @@ falm_status falm_solver_set_matrix(falm_solver *S, const void *a, void *stream) 
 
 falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void *b, void *r, void *stream,
                               falm_solve_info *info) {
-    if (!S || !a || !x || !b) {
-        set_error("falm_solver_solve: null argument");
-        return FALM_ERR_INVALID;
-    }
-    if (x == r || x == b || r == b) {
-        set_error("falm_solver_solve: x, b and r must be distinct");
-        return FALM_ERR_ALIAS;
-    }
-    falm_solve_info tmp;
-    if (!info) info = &tmp;
-    hipStream_t s = (hipStream_t)stream;
-    if (!r && S->cfg.precision != FALM_SOLVE_MIXED) {
-        // the double solvers iterate on r itself: use the solver's own vector
-        if (!S->r64own) {
-            FALM_HIP(hipMalloc(&S->r64own, S->n * sizeof(double)));
-            FALM_HIP(hipMemsetAsync(S->r64own, 0, S->n * sizeof(double), s));
-        }
-        r = S->r64own;
-    }
-    double *xd = (double *)x, *rd = (double *)r;
-    const double *bd = (const double *)b;
-    const bool prepared = (S->a_src == a);
-    if (!prepared && S->cfg.precision == FALM_SOLVE_MIXED) {
-        // no declared matrix: build the float copy from this one now
-        FALM_TRY(prepare_matrix(S, a, s));
-    }
-    const bool sep = (S->a_src == a) && S->sep;
-    if (S->cfg.type == FALM_LS_PBICGSTAB) {
-        if (S->cfg.precision == FALM_SOLVE_MIXED) {
-            if (sep)
-                return solve_bicgstab_mixed(S, sep_view<double>(S, (const double *)a, S->sep64),
-                                            sep_view<float>(S, S->a32, S->sep32), xd, bd, rd, s, info);
-            return solve_bicgstab_mixed(S, cols_view<double>(S, a), cols_view<float>(S, S->a32), xd, bd, rd, s, info);
-        }
-        if (sep) return solve_bicgstab_f64(S, sep_view<double>(S, (const double *)a, S->sep64), xd, bd, rd, s, info);
-        return solve_bicgstab_f64(S, cols_view<double>(S, a), xd, bd, rd, s, info);
-    }
-    if (sep) return solve_stationary(S, sep_view<double>(S, (const double *)a, S->sep64), xd, bd, rd, s, info);
-    return solve_stationary(S, cols_view<double>(S, a), xd, bd, rd, s, info);
+    return falm::solve_dispatch(S, a, x, b, r, (hipStream_t)stream, info, falm::SOLVE_FULL);
 }
 
 falm_status falm_solver_destroy(falm_solver *S) {
