@@ -125,8 +125,9 @@ __global__ __launch_bounds__(NT) void k_bicg_init(Tiling tl, Geom g, OP a, const
 // Eight waves per SIMD (64 VGPRs at most): four workgroups per CU, so the
 // 1024 tiles of a 512 x 128 x 128 interior run in one round instead of
 // three workgroups per CU and a partly empty second round.
-// CHECK: only |b - A x|^2 (the mixed solve's test after a pass; nothing is
-// stored, and a pass that must follow starts with the full kernel).
+// CHECK: only |b - A x|^2 and, if r64 is given, the residual (the mixed
+// solve's test after a pass; a pass that must follow starts with the full
+// kernel).
 template <class T, class OP, bool CHECK = false>
 __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(8))) void k_bicg_init_km(
     K7Tiling kt, Geom g, OP a, const double *__restrict__ x,
@@ -167,6 +168,8 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(8))) void
                 if ((void *)r != (void *)r64) st_at(r, ot, rt);
                 st_at(rr, ot, rt);
                 v[1] += (double)rt * (double)rt;
+            } else if (r64) {
+                st_at(r64, od, rv);
             }
         }
         px.advance(bf, more);
@@ -215,6 +218,18 @@ __global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, 
     }
 }
 
+// two vectors' elements of one cell side by side
+template <class T> struct PairOf;
+template <> struct PairOf<float> { using type = float2; };
+template <> struct PairOf<double> { using type = double2; };
+template <class T> using PairT = typename PairOf<T>::type;
+template <class T> __device__ __forceinline__ PairT<T> pair_of(T a, T b) {
+    PairT<T> v;
+    v.x = a;
+    v.y = b;
+    return v;
+}
+
 // The end of a BiCGStab pass with the one-pass SOR preconditioner M (black
 // half-sweep from zero, then the red one; k_bicg_pcmv_km below): the pass's
 // solution update is x += M(z) with z = (0 or y) + alpha p + omega s, y the
@@ -232,9 +247,9 @@ __global__ __launch_bounds__(K7_NT) void k_sor_op_km(K7Tiling kt, Geom g, OP a, 
 template <class T, class OP>
 __global__ __launch_bounds__(K7_NT) void k_bicg_close_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ y,
                                                          const T *__restrict__ p0, const T *__restrict__ p1,
-                                                         const T *__restrict__ sv, const T *__restrict__ tv,
-                                                         T *__restrict__ rout, const DevState *st, T omega_pc,
-                                                         double *__restrict__ xd) {
+                                                         const PairT<T> *__restrict__ stv, T *__restrict__ rout,
+                                                         const DevState *st, T omega_pc, double *__restrict__ xd) {
+    using P = PairT<T>;
     if (!st->done || st->final || !st->pending) return;
     __shared__ K7Lds<T> lw;
     const K7Pos Q = k7_pos(kt, g);
@@ -270,7 +285,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_close_km(K7Tiling kt, Geom g, OP
     const auto load = [&](unsigned ob, T &yy, T &pp, T &ss) {
         yy = ld_at(y, ob);
         pp = ld_at(pv, ob);
-        ss = ld_at(sv, ob);
+        ss = ld_at(stv, 2 * ob).x;
     };
     // registers: w at k-1 and k, z and the cell coefficients at k, the
     // column's raw values at k+1 and the halo cell's at the next LDS plane
@@ -307,7 +322,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_close_km(K7Tiling kt, Geom g, OP
     if (Q.act) xq = ld_at(xd, (cb + (unsigned)Q.k0 * szt) * XS);
     const bool rw = rout != nullptr;
     T tq = T(0);
-    if (rw) tq = ld_at(tv, cb + (unsigned)Q.k0 * szt);
+    if (rw) tq = ld_at(stv, 2 * (cb + (unsigned)Q.k0 * szt)).y;
     __syncthreads();
     int bf = 0;
     unsigned ob = cb + (unsigned)Q.k0 * szt;
@@ -316,7 +331,7 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_close_km(K7Tiling kt, Geom g, OP
         const double xc = xq;
         if (more && Q.act) xq = ld_at(xd, (ob + szt) * XS);
         const T tc = tq;
-        if (rw && more) tq = ld_at(tv, ob + szt);
+        if (rw && more) tq = ld_at(stv, 2 * (ob + szt)).y;
         T y2 = T(0), p2 = T(0), s2 = T(0);
         Cell e2 = ne;
         if (more) {
@@ -389,7 +404,11 @@ __global__ __launch_bounds__(K7_NT) void k_bicg_close_km(K7Tiling kt, Geom g, OP
 // y accumulates the pass's alpha p + omega s and k_bicg_close_km adds M of
 // the sum to x at the end of the pass (M is linear).  MODE 0 reads p and q
 // at the ring cells, which neighbouring workgroups write: both alternate
-// between two buffers.
+// between two buffers.  s and t, which the next MODE 0 and k_bicg_close_km
+// read together, are stored as pairs of adjacent elements (one 8-byte
+// access in float, 16 in double): MODE 1 writes the pair at plane s-1 with
+// the column's s kept in registers.  Every pass's first iteration is the
+// FIRST instance (the host starts every pass).
 constexpr int PQ_R = 2;                                    // ring width
 constexpr int PQ_W = K7_BX + 2 * PQ_R, PQ_H = K7_BY + 2 * PQ_R;
 constexpr int PQ_RING2 = PQ_W * PQ_H - K7_NT;              // ring cells of v
@@ -475,15 +494,15 @@ __device__ inline typename OpCols<TA>::Cell cell_of(const OpCols<TA> &c, unsigne
 // thread's slot in `trash`): the compiler can then count the memory
 // operations in flight exactly and waits for the previous step's loads
 // only, not for those just issued.  trash holds K7_NT values per workgroup.
+// stin: the (s, t) pairs MODE 0 reads; stout: those MODE 1 writes.
 template <class T, class OP, int MODE, bool DCLS, bool FIRST>
-__global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves<OP>::value))) void k_bicg_pcmv_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ r,
-                                                        const T *__restrict__ pin, const T *__restrict__ qin,
-                                                        const T *__restrict__ rr, T *__restrict__ vout,
-                                                        T *__restrict__ wout, const DevState *st, T omega_pc,
-                                                        const double *apart, int anb, DevState *stw,
-                                                        double *partials, T *__restrict__ trash,
-                                                        const T *__restrict__ sin, const T *__restrict__ tin,
-                                                        T *__restrict__ rout, T *__restrict__ yv, int pid) {
+__global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves<OP>::value))) void k_bicg_pcmv_km(
+    K7Tiling kt, Geom g, OP a, const T *__restrict__ r, const T *__restrict__ pin, const T *__restrict__ qin,
+    const T *__restrict__ rr, T *__restrict__ vout, T *__restrict__ wout, const DevState *st, T omega_pc,
+    const double *apart, int anb, DevState *stw, double *partials, T *__restrict__ trash,
+    const PairT<T> *__restrict__ stin, PairT<T> *__restrict__ stout, T *__restrict__ rout, T *__restrict__ yv,
+    int pid) {
+    using P = PairT<T>;
     if (st->done) return;
     __shared__ PQLds<T> lv;
     __shared__ double lds[5 * (K7_NT / 64)];
@@ -491,7 +510,7 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
     const int tid = (int)threadIdx.x;
     const unsigned szt = (unsigned)g.nxy * sizeof(T);
     // q (and p) are used; FIRST: the launch is a pass's first iteration
-    const bool upd = MODE == 1 || (!FIRST && !st->first);
+    constexpr bool upd = MODE == 1 || !FIRST;
     T c1 = T(0), c2 = T(0);
     if (MODE == 0) {
         c1 = (T)st->beta;
@@ -501,7 +520,6 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
     // (not accumulated) on the pass's second iteration
     const double ya = MODE == 0 ? st->alpha : 0.0, yw = MODE == 0 ? st->omega : 0.0;
     const bool yset = MODE == 0 && st->it_pass == 1;
-    const T *const av = (MODE == 0 && upd) ? sin : r;
     if (MODE == 0 && blockIdx.x == 0 && threadIdx.x == 0) stw->plast = pid;
     const int lo0 = kt.box.lo[0], hi0 = lo0 + kt.box.n[0], lo1 = kt.box.lo[1], hi1 = lo1 + kt.box.n[1];
     const int lo2 = kt.box.lo[2], hi2 = lo2 + kt.box.n[2];
@@ -537,7 +555,9 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
     const auto c1v = a.column(in1 ? gi1 : oi, in1 ? gj1 : oj);
     const unsigned b2 = in2 ? (unsigned)(gi2 + g.nx * gj2) * sizeof(T) : cb;
     const unsigned b1 = in1 ? (unsigned)(gi1 + g.nx * gj1) * sizeof(T) : cb;
-    T *const tr = trash + (size_t)blockIdx.x * K7_NT + tid;
+    // trash slots: T and pair stores share them (2 K7_NT T per workgroup)
+    T *const tr = trash + (size_t)blockIdx.x * (2 * K7_NT) + tid;
+    P *const tr2 = reinterpret_cast<P *>(trash) + (size_t)blockIdx.x * K7_NT + tid;
 
     // operands of v at one cell of a plane, and its diagonal: a is r, or in
     // MODE 0 after a pass's first iteration s (and b is t: r = s - omega t)
@@ -546,8 +566,14 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
     };
     const auto load = [&](const Col &c, unsigned ob, int k) {
         Ld x;
-        x.a = ld_at(av, ob);
-        x.b = (MODE == 0 && !FIRST) ? ld_at(tin, ob) : T(0);
+        x.b = T(0);
+        if constexpr (MODE == 0 && !FIRST) {
+            const P sp = ld_at(stin, 2 * ob);
+            x.a = sp.x;
+            x.b = sp.y;
+        } else {
+            x.a = ld_at(r, ob);
+        }
         x.q = (MODE == 1 || !FIRST) ? ld_at(qin, ob) : T(0);
         x.p = (MODE == 0 && !FIRST) ? ld_at(pin, ob) : T(0);
         x.d = diag_of<DCLS>(c, ob, k);
@@ -578,21 +604,23 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         return x;
     };
     // slot of plane f: black cells M(v), red cells v, zero outside the
-    // interior; v is written on the tile; returns the column's v
-    // the column's v at plane f, stored with (MODE 0) its r and y:
-    // y = (0 or y) + alpha p + omega s of the previous iteration, what the
-    // update kernel accumulates
+    // interior; returns the column's v at plane f.  MODE 0 stores v (p), r
+    // and y = (0 or y) + alpha p + omega s of the previous iteration (what
+    // the update kernel accumulates; a pass's first iteration leaves y as it
+    // is: the second sets it); MODE 1's v (s) is stored with w (t) three
+    // planes later.
     const auto own = [&](int f, const Fl &x, bool kf) -> T {
         const T rv = res(x.o);
         const T vo = (Q.act && kf) ? vec_of(x.o, rv) : T(0);
-        const bool wr = Q.act && f >= k0 && f < k1;
-        const unsigned ob = cb + (unsigned)f * szt;
-        *(wr ? (T *)((char *)vout + ob) : tr) = vo;
-        if (MODE == 0 && !FIRST) {
-            const bool wu = wr && upd;
-            *(wu ? (T *)((char *)rout + ob) : tr) = rv;
-            const double yo = (yset ? 0.0 : (double)x.y) + (ya * (double)x.o.p + yw * (double)x.o.a);
-            *(wu ? (T *)((char *)yv + ob) : tr) = (T)yo;
+        if (MODE == 0) {
+            const bool wr = Q.act && f >= k0 && f < k1;
+            const unsigned ob = cb + (unsigned)f * szt;
+            *(wr ? (T *)((char *)vout + ob) : tr) = vo;
+            if (!FIRST) {
+                *(wr ? (T *)((char *)rout + ob) : tr) = rv;
+                const double yo = (yset ? 0.0 : (double)x.y) + (ya * (double)x.o.p + yw * (double)x.o.a);
+                *(wr ? (T *)((char *)yv + ob) : tr) = (T)yo;
+            }
         }
         return vo;
     };
@@ -704,7 +732,8 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(PcmvWaves
         const T wv = ca.apply(eq, hc, lv[sl][ryo][rxo + 1], lv[sl][ryo][rxo - 1], lv[sl][ryo + 1][rxo],
                               lv[sl][ryo - 1][rxo], h, hm);
         const bool wr = Q.act && s - 1 >= k0;
-        *(wr ? (T *)((char *)wout + cb + (unsigned)(s - 1) * szt) : tr) = wv;
+        if (MODE == 0) *(wr ? (T *)((char *)wout + cb + (unsigned)(s - 1) * szt) : tr) = wv;
+        else *(wr ? (P *)((char *)stout + 2 * (cb + (unsigned)(s - 1) * szt)) : tr2) = pair_of(v0, wv);
         if (MODE == 0) {
             acc[0] += wr ? (double)ru * (double)wv : 0.0;
         } else {
@@ -1226,9 +1255,6 @@ __global__ __launch_bounds__(NT) void k_bicg_update(Tiling tl, Geom g, double *_
 // workgroups of the pair update (each sums the second operator product's
 // partials, so fewer and longer-running ones)
 constexpr int UPD_BLOCKS = 1024;
-template <class T> struct Pair;
-template <> struct Pair<float> { using type = float2; };
-template <> struct Pair<double> { using type = double2; };
 
 template <class T>
 __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g, double *__restrict__ x,
@@ -1237,7 +1263,7 @@ __global__ __launch_bounds__(NT) void k_bicg_update2(Tiling tp, int lo0, Geom g,
                                                      T *__restrict__ r, const T *__restrict__ rr,
                                                      const DevState *st, double *partials, const double *opart,
                                                      int onb, DevState *stw) {
-    using T2 = typename Pair<T>::type;
+    using T2 = PairT<T>;
     if (st->done) return;
     __shared__ double lds[2 * (NT / 64)];
     int t0, t1;
@@ -1519,6 +1545,7 @@ struct falm_solver {
     int pflip = 0;                // fused SOR path: p lives in w (1) or p (0)
     int qflip = 0;                // deferred path: q lives in pp (1) or q (0)
     void *trash = nullptr;        // k_bicg_pcmv_km's stores of threads with nothing to store
+    void *stp = nullptr;  // the fused path's (s, t) pairs (2 n elements)
     // a solve split in two calls (begin / resume): the polling loop's state
     // between them
     int cur_batch = 0, cur_enqueued = 0, cur_polls = 0;
@@ -1560,7 +1587,11 @@ static falm_status solver_alloc(falm_solver *S) {
     if ((size_t)S->kt.blocks > nb) nb = (size_t)S->kt.blocks;
     if ((size_t)S->ktp.blocks > nb) nb = (size_t)S->ktp.blocks;
     FALM_HIP(hipMalloc(&S->partials, 2 * nb * sizeof(double)));
-    FALM_HIP(hipMalloc(&S->trash, (size_t)(S->ktp.blocks > 0 ? S->ktp.blocks : 1) * K7_NT * sizeof(double)));
+    FALM_HIP(hipMalloc(&S->trash, (size_t)(S->ktp.blocks > 0 ? S->ktp.blocks : 1) * K7_NT * 2 * sizeof(double)));
+    if (S->cfg.pc_type == FALM_LS_SOR && S->cfg.pc_maxit <= 1) {  // the fused path (deferred_pc)
+        FALM_HIP(hipMalloc(&S->stp, 2 * vb));
+        FALM_HIP(hipMemset(S->stp, 0, 2 * vb));
+    }
     FALM_HIP(hipMalloc(&S->partials2, 5 * nb * sizeof(double)));  // 5 sums: the folded s kernel
     // the zero fills above run on the null stream, which the caller's
     // (possibly non-blocking) stream does not wait for
@@ -1688,13 +1719,13 @@ static bool deferred_pc(const falm_solver *S) {
 template <class T, class OP>
 static falm_status launch_close(falm_solver *S, const OP &a, double *x, T *rout, hipStream_t s) {
     TimedLaunch tm(FALM_TIME_PC_APPLY, s);
+    using P = PairT<T>;
     hipLaunchKernelGGL((k_bicg_close_km<T, OP>), FALM_KM(S), s, S->kt, S->g, a, (const T *)S->ss, (const T *)S->p,
-                       (const T *)S->w, (const T *)S->s, (const T *)S->t, rout, (const DevState *)S->st,
-                       (T)S->cfg.pc_relax, x);
-    // y, p, s (and the diagonal) read, x read and written; with rout also t
-    // read and r written
+                       (const T *)S->w, (const P *)S->stp, rout, (const DevState *)S->st, (T)S->cfg.pc_relax, x);
+    // y, p, (s, t) (and the diagonal) read, x read and written; with rout
+    // r written
     tm.done((double)S->tl.box.size() *
-            ((rout ? 5.0 : 3.0) * sizeof(T) + 2.0 * sizeof(double) + diag_bytes<OP>(S)));
+            ((rout ? 5.0 : 4.0) * sizeof(T) + 2.0 * sizeof(double) + diag_bytes<OP>(S)));
     FALM_CHECK_LAUNCH("k_bicg_close_km");
     return FALM_OK;
 }
@@ -1715,6 +1746,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         // ring cells of its tiles); a pass's first iteration reads neither,
         // so the alternation may run on across iterations a finished pass
         // skips (the p kernel records which buffer it wrote)
+        using P = PairT<T>;
         const int pid = S->pflip ? 0 : 1;  // 1: p goes to w
         T *pin = (T *)(S->pflip ? S->w : S->p), *pout = (T *)(S->pflip ? S->p : S->w);
         T *qin = (T *)(S->qflip ? S->pp : S->q), *qout = (T *)(S->qflip ? S->q : S->pp);
@@ -1735,21 +1767,21 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
                 TimedLaunch tm(FALM_TIME_PCMV, s);
                 hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D, F>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
                                    (const T *)pin, (const T *)qin, (const T *)rr, pout, qout, (const DevState *)S->st,
-                                   wpc, (const double *)nullptr, 0, S->st, S->partials, trash, (const T *)sv,
-                                   (const T *)t, r, ss, pid);
+                                   wpc, (const double *)nullptr, 0, S->st, S->partials, trash, (const P *)S->stp,
+                                   (P *)nullptr, r, ss, pid);
                 // a pass's first iteration: r, rr (and the diagonal) read,
-                // p and q written; then s, t, p, q, rr, y read and p, q, r,
-                // y written
+                // p and q written; then (s, t), p, q, rr, y read and p, q,
+                // r, y written
                 tm.done(box * ((first ? 4.0 : 10.0) * sizeof(T) + db));
             }
             {
                 TimedLaunch tm(FALM_TIME_PCMV_S, s);
                 hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D, false>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
-                                   (const T *)nullptr, (const T *)qout, (const T *)rr, sv, t,
+                                   (const T *)nullptr, (const T *)qout, (const T *)rr, (T *)nullptr, (T *)nullptr,
                                    (const DevState *)S->st, wpc, (const double *)S->partials, nbp, S->st,
-                                   S->partials2, trash, (const T *)nullptr, (const T *)nullptr, (T *)nullptr,
+                                   S->partials2, trash, (const P *)nullptr, (P *)S->stp, (T *)nullptr,
                                    (T *)nullptr, 0);
-                // r, q, rr (and the diagonal) read, s and t written
+                // r, q, rr (and the diagonal) read, (s, t) written
                 tm.done(box * (5.0 * sizeof(T) + db));
             }
         };
@@ -1834,13 +1866,15 @@ static falm_status launch_init(falm_solver *S, const OP64 &a, const double *x, c
     return FALM_OK;
 }
 
-// the mixed solve's test after a pass (k_bicg_init_km<CHECK>): x, b read
+// the mixed solve's test after a pass (k_bicg_init_km<CHECK>): x, b read,
+// the residual written into r64 when given
 template <class OP64>
-static falm_status launch_check(falm_solver *S, const OP64 &a, const double *x, const double *b, hipStream_t s) {
+static falm_status launch_check(falm_solver *S, const OP64 &a, const double *x, const double *b, double *r64,
+                                hipStream_t s) {
     TimedLaunch tm(FALM_TIME_SOLVER_INIT, s);
-    hipLaunchKernelGGL((k_bicg_init_km<float, OP64, true>), FALM_KM(S), s, S->kt, S->g, a, x, b, nullptr,
+    hipLaunchKernelGGL((k_bicg_init_km<float, OP64, true>), FALM_KM(S), s, S->kt, S->g, a, x, b, r64,
                        (float *)nullptr, (float *)nullptr, (const DevState *)S->st, S->partials);
-    tm.done((double)S->tl.box.size() * (2.0 * sizeof(double) + diag_bytes<OP64>(S)));
+    tm.done((double)S->tl.box.size() * ((r64 ? 3.0 : 2.0) * sizeof(double) + diag_bytes<OP64>(S)));
     hipLaunchKernelGGL(k_fin_check, FALM_FIN, s, S->st, (const double *)S->partials, S->kt.blocks);
     FALM_CHECK_LAUNCH("k_bicg_init_km<check>");
     return FALM_OK;
@@ -1916,11 +1950,11 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
         // (the float residual is not kept: the init kernels form the next
         // pass's from x)
         if (deferred_pc(S)) FALM_TRY((launch_close<float, OP32>(S, a32, x, (float *)nullptr, s)));
-        // a finished pass is tested with the norm alone, unless the caller
-        // wants the final residual; a pass that must follow starts from the
-        // full init (it recomputes the same residual)
-        if (r) FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
-        else FALM_TRY((launch_check<OP64>(S, a64, x, b, s)));
+        // a finished pass is tested with the norm (and the residual, if the
+        // caller wants it); a pass that must follow starts from the full
+        // init, which the host launches (it recomputes the same residual):
+        // every pass's first iteration is known to the host
+        FALM_TRY((launch_check<OP64>(S, a64, x, b, r, s)));
         return FALM_OK;
     };
     if (mode != SOLVE_RESUME) {
@@ -1941,7 +1975,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
         first_read = false;
         polls++;
         if (S->hst->final) break;
-        if (!r && S->hst->done) {
+        if (S->hst->done) {
             FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
             pass_start = true;
         }
@@ -2207,6 +2241,11 @@ falm_status falm_solver_create(const falm_domain *dom, const falm_solver_config 
         set_error("falm_solver_create: precision %d not supported", cfg->precision);
         return FALM_ERR_UNSUPPORTED;
     }
+    // the fused BiCGStab path (one-pass SOR preconditioner) stores vector
+    // pairs: two elements per cell
+    if (cfg->type == FALM_LS_PBICGSTAB && cfg->pc_type == FALM_LS_SOR && cfg->pc_maxit <= 1)
+        FALM_TRY(check_byte_offsets(ncell_of(dom), 2 * (cfg->precision == FALM_SOLVE_MIXED ? sizeof(float)
+                                                                                           : sizeof(double))));
     if (cfg->precision == FALM_SOLVE_MIXED && cfg->type != FALM_LS_PBICGSTAB) {
         set_error("falm_solver_create: mixed precision needs the BiCGStab solver");
         return FALM_ERR_UNSUPPORTED;
@@ -2259,7 +2298,7 @@ falm_status falm_solver_solve(falm_solver *S, const void *a, void *x, const void
 
 falm_status falm_solver_destroy(falm_solver *S) {
     if (!S) return FALM_OK;
-    void *bufs[] = {S->rr, S->p, S->q, S->s, S->pp, S->ss, S->t, S->w, S->r32, S->a32,
+    void *bufs[] = {S->rr, S->p, S->q, S->s, S->pp, S->ss, S->t, S->w, S->r32, S->a32, S->stp,
                     S->sep64, S->sep32, S->dtab64, S->dtab32, S->st, S->partials, S->partials2, S->flag,
                     S->r64own, S->trash};
     for (void *b : bufs)
