@@ -181,7 +181,8 @@ static falm_status case_init(falm_case *C) {
     // the zero fills ran on the null stream, which the case's non-blocking
     // stream does not wait for
     FALM_HIP(hipDeviceSynchronize());
-    FALM_HIP(hipHostMalloc(&C->hnorm, sizeof(double), hipHostMallocDefault));
+    // coherent: the norm's finalize kernel writes it directly
+    FALM_HIP(hipHostMalloc(&C->hnorm, sizeof(double), hipHostMallocCoherent));
     {
         const int empty[6] = {1, 0, 1, 0, 1, 0};
         FALM_TRY(case_dalloc(C, (void **)&C->nobox, sizeof(empty)));
@@ -294,10 +295,11 @@ static falm_status post_solve(falm_case *C, const int *gate) {
     if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->u, 3, s, gate));
     // SGS and the divergence monitor in one pass (both read final fields)
     if (c.sgs_model == FALM_SGS_SMAGORINSKY) {
-        FALM_TRY(sep_sgs_div_norm(d, pr, c.Cs, C->u, C->nut, C->uu, C->div, C->sm, C->dnorm, C->partials, s, gate));
+        FALM_TRY(sep_sgs_div_norm(d, pr, c.Cs, C->u, C->nut, C->uu, C->div, C->sm, C->dnorm, C->partials, s, gate,
+                                  C->hnorm));
         if (is2d) FALM_TRY(launch_copy_z5(d, pr, C->nut, 1, s, gate));
     } else {
-        FALM_TRY(sep_div_norm(d, pr, C->uu, C->div, C->sm, C->dnorm, C->partials, s, gate));
+        FALM_TRY(sep_div_norm(d, pr, C->uu, C->div, C->sm, C->dnorm, C->partials, s, gate, C->hnorm));
     }
     return FALM_OK;
 }
@@ -342,16 +344,12 @@ falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
     if (gate) {
         FALM_TRY(solver_solve_mode(C->solver, C->a, C->p, C->rhs, res, s, si, SOLVE_BEGIN, nullptr));
         FALM_TRY(post_solve(C, gate));
-        FALM_HIP(hipMemcpyAsync(C->hnorm, C->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
         FALM_TRY(solver_solve_mode(C->solver, C->a, C->p, C->rhs, res, s, si, SOLVE_RESUME, &rerun));
     } else {
         FALM_TRY(solver_solve_mode(C->solver, C->a, C->p, C->rhs, res, s, si, SOLVE_FULL, nullptr));
     }
     C->norm_pending = rerun;
-    if (rerun) {
-        FALM_TRY(post_solve(C, nullptr));
-        FALM_HIP(hipMemcpyAsync(C->hnorm, C->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
-    }
+    if (rerun) FALM_TRY(post_solve(C, nullptr));
     return FALM_OK;
 }
 

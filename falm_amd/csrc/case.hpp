@@ -28,7 +28,7 @@ struct falm_case {
     int pbc[6], uubc[6];
     // device scalar for the divergence norm + its reduction scratch
     double *dnorm = nullptr, *partials = nullptr, *hnorm = nullptr;
-    bool norm_pending = false;  // the last step's copy into hnorm is not waited for
+    bool norm_pending = false;  // the last step's norm (written into hnorm by its kernel) is not waited for
     int *nobox = nullptr;     // empty cell box: no rotor, ff stays zero
     std::vector<void *> bufs;
     // the last falm_case_adjoint's attitude gradient: per turbine
@@ -41,8 +41,8 @@ namespace falm {
 // one time step on the case's stream; power, when given, accumulates the
 // step's rotor power on the device.  It waits for the stream once, after the
 // step's last kernel when the first batch of solver iterations finished the
-// solve (else the step's end and the copy of the divergence norm into hnorm
-// are still in flight: norm_pending).
+// solve (else the step's end, whose last kernel writes the divergence norm
+// into the pinned hnorm, is still in flight: norm_pending).
 falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power);
 falm_status case_dalloc(falm_case *C, void **p, size_t bytes);
 // stopping tolerance of a solver's next solves (|r|/N), replacing its config's

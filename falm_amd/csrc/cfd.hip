@@ -1189,11 +1189,16 @@ __global__ __launch_bounds__(K7_NT) void k_sgs_div_km(K7Tiling kt, Geom g, T Cs,
     block_partials<K7_NT, 1>(v, partials, lds);
 }
 
-__global__ __launch_bounds__(FIN_NT) void k_fin_norm(const double *partials, int nb, double *out, const int *gate) {
+// hout (may be null): a pinned host copy of the result
+__global__ __launch_bounds__(FIN_NT) void k_fin_norm(const double *partials, int nb, double *out, const int *gate,
+                                                     double *hout) {
     if (gated_off(gate)) return;
     double tot[1];
     sum_partials<1>(partials, nb, tot);
-    if (threadIdx.x == 0) out[0] = tot[0];
+    if (threadIdx.x == 0) {
+        out[0] = tot[0];
+        if (hout) hout[0] = tot[0];
+    }
 }
 
 // ------------------------------------------------------------ host side
@@ -1368,7 +1373,7 @@ static falm_status smag_t(const falm_domain *dom, double Cs, const void *u, void
 
 template <class T>
 static falm_status div_norm_t(const falm_domain *dom, const void *uu, void *div, const SepMesh &sm,
-                              double *dev_out, double *partials, hipStream_t s, const int *gate) {
+                              double *dev_out, double *partials, hipStream_t s, const int *gate, double *host_out) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     K7Tiling kt = make_k7_tiling(interior_box(g));
@@ -1378,7 +1383,8 @@ static falm_status div_norm_t(const falm_domain *dom, const void *uu, void *div,
     hipLaunchKernelGGL((k_div_norm_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, v3<T>(uu, n), (T *)div,
                        sep_tab<T>(sm), partials, gate);
     tm.done((double)interior_box(g).size() * 4.0 * sizeof(T));  // uu read, div written
-    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out, gate);
+    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out, gate,
+                       host_out);
     FALM_CHECK_LAUNCH("k_div_norm");
     return FALM_OK;
 }
@@ -1535,7 +1541,7 @@ falm_status sep_sgs(const falm_domain *dom, int prec, double Cs, const void *u, 
 template <class T>
 static falm_status sgs_div_norm_t(const falm_domain *dom, double Cs, const void *u, void *nut, const void *uu,
                                   void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s,
-                                  const int *gate) {
+                                  const int *gate, double *host_out) {
     Geom g = make_geom(dom);
     size_t n = ncell_of(dom);
     K7Tiling kt = make_k7_tiling(interior_box(g));
@@ -1545,22 +1551,24 @@ static falm_status sgs_div_norm_t(const falm_domain *dom, double Cs, const void 
     hipLaunchKernelGGL((k_sgs_div_km<T>), dim3(kt.blocks), dim3(K7_NT), 0, s, kt, g, (T)Cs, v3<T>(u, n), (T *)nut,
                        v3<T>(uu, n), (T *)div, sep_tab<T>(sm), partials, gate);
     tm.done((double)interior_box(g).size() * 8.0 * sizeof(T));  // u, uu read; nut, div written
-    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out, gate);
+    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(FIN_NT), 0, s, (const double *)partials, kt.blocks, dev_out, gate,
+                       host_out);
     FALM_CHECK_LAUNCH("k_sgs_div_km");
     return FALM_OK;
 }
 
 falm_status sep_sgs_div_norm(const falm_domain *dom, int prec, double Cs, const void *u, void *nut, const void *uu,
                              void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s,
-                             const int *gate) {
-    if (prec == FALM_F32) return sgs_div_norm_t<float>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s, gate);
-    return sgs_div_norm_t<double>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s, gate);
+                             const int *gate, double *host_out) {
+    if (prec == FALM_F32)
+        return sgs_div_norm_t<float>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s, gate, host_out);
+    return sgs_div_norm_t<double>(dom, Cs, u, nut, uu, div, sm, dev_out, partials, s, gate, host_out);
 }
 
 falm_status sep_div_norm(const falm_domain *dom, int prec, const void *uu, void *div, const SepMesh &sm,
-                         double *dev_out, double *partials, hipStream_t s, const int *gate) {
-    if (prec == FALM_F32) return div_norm_t<float>(dom, uu, div, sm, dev_out, partials, s, gate);
-    return div_norm_t<double>(dom, uu, div, sm, dev_out, partials, s, gate);
+                         double *dev_out, double *partials, hipStream_t s, const int *gate, double *host_out) {
+    if (prec == FALM_F32) return div_norm_t<float>(dom, uu, div, sm, dev_out, partials, s, gate, host_out);
+    return div_norm_t<double>(dom, uu, div, sm, dev_out, partials, s, gate, host_out);
 }
 
 }  // namespace falm

@@ -50,7 +50,8 @@ falm_status sep_poisson_rhs(const falm_domain *dom, int prec, const void *uu, do
                             double scale, hipStream_t s);
 // in place: u -= dt kx grad p (interior), uu -= dt g grad p (face region)
 // gate (all four calls below): the kernels do nothing unless *gate is
-// nonzero; null for an unconditional launch
+// nonzero; null for an unconditional launch.  host_out (the two norms): a
+// pinned host copy of the norm, written by the kernel (null: none)
 falm_status sep_project(const falm_domain *dom, int prec, double dt, void *u, void *uu, const double *p,
                         const SepMesh &sm, hipStream_t s, const int *gate = nullptr);
 falm_status sep_sgs(const falm_domain *dom, int prec, double Cs, const void *u, void *nut, const SepMesh &sm,
@@ -58,11 +59,12 @@ falm_status sep_sgs(const falm_domain *dom, int prec, double Cs, const void *u, 
 // div = div(uu)/ja on the interior and dev_out[0] = sum div^2; partials holds
 // red_blocks(interior tiling) doubles
 falm_status sep_div_norm(const falm_domain *dom, int prec, const void *uu, void *div, const SepMesh &sm,
-                         double *dev_out, double *partials, hipStream_t s, const int *gate = nullptr);
+                         double *dev_out, double *partials, hipStream_t s, const int *gate = nullptr,
+                         double *host_out = nullptr);
 // sep_sgs (Smagorinsky) and sep_div_norm in one pass: u and uu are read
 // once per tile, nut, div and the norm are as the two calls give them
 falm_status sep_sgs_div_norm(const falm_domain *dom, int prec, double Cs, const void *u, void *nut, const void *uu,
                              void *div, const SepMesh &sm, double *dev_out, double *partials, hipStream_t s,
-                             const int *gate = nullptr);
+                             const int *gate = nullptr, double *host_out = nullptr);
 
 }  // namespace falm

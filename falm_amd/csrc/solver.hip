@@ -248,8 +248,13 @@ template <class T, class OP>
 __global__ __launch_bounds__(K7_NT) void k_bicg_close_km(K7Tiling kt, Geom g, OP a, const T *__restrict__ y,
                                                          const T *__restrict__ p0, const T *__restrict__ p1,
                                                          const PairT<T> *__restrict__ stv, T *__restrict__ rout,
-                                                         const DevState *st, T omega_pc, double *__restrict__ xd) {
+                                                         const DevState *st, T omega_pc, double *__restrict__ xd,
+                                                         DevState *hm) {
     using P = PairT<T>;
+    // hm: the host's pinned mirror of the state, written when the closing
+    // kernel ends a batch of the double solve (nothing after it changes
+    // the state)
+    if (hm && blockIdx.x == 0 && threadIdx.x == 0) *hm = *st;
     if (!st->done || st->final || !st->pending) return;
     __shared__ K7Lds<T> lw;
     const K7Pos Q = k7_pos(kt, g);
@@ -1076,17 +1081,22 @@ __global__ __launch_bounds__(FIN_NT) void k_fin_init(DevState *st, const double 
 
 // The mixed solve's test after a pass: the true residual's norm ends the
 // solve (tolerance met, or maxit spent); otherwise the pass is over and the
-// host starts the next one with the full init kernels.
-__global__ __launch_bounds__(FIN_NT) void k_fin_check(DevState *st, const double *partials, int nb) {
-    if (st->final || !st->done) return;
-    double tot[2];
-    sum_partials<2>(partials, nb, tot);
-    if (threadIdx.x == 0) {
-        const double err = sqrt(tot[0]) / st->gN;
-        st->err = err;
-        st->pending = 0;
-        if ((st->passes > 0 && err <= st->tol) || st->total >= st->maxit) st->final = 1;
+// host starts the next one with the full init kernels.  It ends every round
+// of the mixed solve, so it also copies the state into the host's pinned
+// mirror (hm): the host's poll then needs no copy of its own.
+__global__ __launch_bounds__(FIN_NT) void k_fin_check(DevState *st, const double *partials, int nb,
+                                                      DevState *hm) {
+    if (!(st->final || !st->done)) {
+        double tot[2];
+        sum_partials<2>(partials, nb, tot);
+        if (threadIdx.x == 0) {
+            const double err = sqrt(tot[0]) / st->gN;
+            st->err = err;
+            st->pending = 0;
+            if ((st->passes > 0 && err <= st->tol) || st->total >= st->maxit) st->final = 1;
+        }
     }
+    if (threadIdx.x == 0) *hm = *st;
 }
 
 // p = r (first) or r + beta(p - omega q) (kernel_PBiCGStab_1), then the
@@ -1580,7 +1590,9 @@ static falm_status solver_alloc(falm_solver *S) {
     FALM_HIP(hipMalloc(&S->dtab32, DIAG_CLS * DIAG_CLS * DIAG_CLS * sizeof(float)));
     FALM_HIP(hipMalloc(&S->st, sizeof(DevState)));
     FALM_HIP(hipMemset(S->st, 0, sizeof(DevState)));
-    FALM_HIP(hipHostMalloc(&S->hst, sizeof(DevState), hipHostMallocDefault));
+    // coherent: the finalize kernels write it directly (the host reads it
+    // after a stream synchronisation)
+    FALM_HIP(hipHostMalloc(&S->hst, sizeof(DevState), hipHostMallocCoherent));
     FALM_HIP(hipMalloc(&S->flag, sizeof(int)));
     FALM_HIP(hipHostMalloc(&S->hflag, sizeof(int), hipHostMallocDefault));
     size_t nb = (size_t)(S->tl.tiles > 0 ? red_blocks(S->tl) : 1);
@@ -1599,8 +1611,9 @@ static falm_status solver_alloc(falm_solver *S) {
     return FALM_OK;
 }
 
-static falm_status read_state(falm_solver *S, hipStream_t s) {
-    FALM_HIP(hipMemcpyAsync(S->hst, S->st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+// mirrored: the last state-changing kernel enqueued wrote the host mirror
+static falm_status read_state(falm_solver *S, hipStream_t s, bool mirrored = false) {
+    if (!mirrored) FALM_HIP(hipMemcpyAsync(S->hst, S->st, sizeof(DevState), hipMemcpyDeviceToHost, s));
     FALM_HIP(hipStreamSynchronize(s));
     return FALM_OK;
 }
@@ -1717,11 +1730,12 @@ static bool deferred_pc(const falm_solver *S) {
 // residual s - omega t into rout when given (no-op while the pass runs or
 // once its update is in x)
 template <class T, class OP>
-static falm_status launch_close(falm_solver *S, const OP &a, double *x, T *rout, hipStream_t s) {
+static falm_status launch_close(falm_solver *S, const OP &a, double *x, T *rout, hipStream_t s, bool mirror) {
     TimedLaunch tm(FALM_TIME_PC_APPLY, s);
     using P = PairT<T>;
     hipLaunchKernelGGL((k_bicg_close_km<T, OP>), FALM_KM(S), s, S->kt, S->g, a, (const T *)S->ss, (const T *)S->p,
-                       (const T *)S->w, (const P *)S->stp, rout, (const DevState *)S->st, (T)S->cfg.pc_relax, x);
+                       (const T *)S->w, (const P *)S->stp, rout, (const DevState *)S->st, (T)S->cfg.pc_relax, x,
+                       mirror ? S->hst : (DevState *)nullptr);
     // y, p, (s, t) (and the diagonal) read, x read and written; with rout
     // r written
     tm.done((double)S->tl.box.size() *
@@ -1875,7 +1889,7 @@ static falm_status launch_check(falm_solver *S, const OP64 &a, const double *x, 
     hipLaunchKernelGGL((k_bicg_init_km<float, OP64, true>), FALM_KM(S), s, S->kt, S->g, a, x, b, r64,
                        (float *)nullptr, (float *)nullptr, (const DevState *)S->st, S->partials);
     tm.done((double)S->tl.box.size() * ((r64 ? 3.0 : 2.0) * sizeof(double) + diag_bytes<OP64>(S)));
-    hipLaunchKernelGGL(k_fin_check, FALM_FIN, s, S->st, (const double *)S->partials, S->kt.blocks);
+    hipLaunchKernelGGL(k_fin_check, FALM_FIN, s, S->st, (const double *)S->partials, S->kt.blocks, S->hst);
     FALM_CHECK_LAUNCH("k_bicg_init_km<check>");
     return FALM_OK;
 }
@@ -1895,7 +1909,7 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
             FALM_TRY((bicg_iteration<double, OP64>(S, a, x, r, s, enqueued == 0)));
         // the pass's sum into x, if the pass is over (before the poll, so x
         // is complete when the solve returns)
-        if (deferred_pc(S)) FALM_TRY((launch_close<double, OP64>(S, a, x, r == S->r64own ? nullptr : r, s)));
+        if (deferred_pc(S)) FALM_TRY((launch_close<double, OP64>(S, a, x, r == S->r64own ? nullptr : r, s, true)));
         return FALM_OK;
     };
     if (mode != SOLVE_RESUME) {
@@ -1910,7 +1924,8 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
     }
     bool first_read = true;
     while (true) {
-        FALM_TRY(read_state(S, s));
+        // the closing kernel mirrors the state (fused path)
+        FALM_TRY(read_state(S, s, deferred_pc(S)));
         if (first_read) S->first_ok = S->hst->done;
         first_read = false;
         if (S->hst->done || enqueued >= c.maxit) break;
@@ -1949,7 +1964,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
         pass_start = false;
         // (the float residual is not kept: the init kernels form the next
         // pass's from x)
-        if (deferred_pc(S)) FALM_TRY((launch_close<float, OP32>(S, a32, x, (float *)nullptr, s)));
+        if (deferred_pc(S)) FALM_TRY((launch_close<float, OP32>(S, a32, x, (float *)nullptr, s, false)));
         // a finished pass is tested with the norm (and the residual, if the
         // caller wants it); a pass that must follow starts from the full
         // init, which the host launches (it recomputes the same residual):
@@ -1970,7 +1985,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
     }
     bool first_read = true;
     while (true) {
-        FALM_TRY(read_state(S, s));
+        FALM_TRY(read_state(S, s, true));  // k_fin_check mirrors the state
         if (first_read) S->first_ok = S->hst->final;
         first_read = false;
         polls++;
