@@ -4,7 +4,10 @@
 // reference has no adjoint).
 //
 // Every stage is the transpose of the forward kernel it mirrors, computed in
-// double as a scatter over the forward's loop: each thread takes one forward
+// double.  The linear stages (projection, Poisson right-hand side, face
+// flux) gather: each thread takes one adjoint output cell and sums what the
+// forward outputs that read it contribute (deterministic, no atomics).  The
+// others scatter over the forward's loop: each thread takes one forward
 // output cell and adds its contributions to the input adjoints with hardware
 // double atomics (so sums are order-independent up to rounding).  The Poisson
 // solve is taken as exact; its adjoint is a BiCGStab solve with the
@@ -34,6 +37,16 @@ __device__ inline bool box_cell(const Box &b, long long t, int &i, int &j, int &
     return true;
 }
 inline dim3 box_grid(const Box &b) { return ew_grid((size_t)b.size()); }
+__device__ inline bool in_box(const Box &b, int i, int j, int k) {
+    return i >= b.lo[0] && i < b.lo[0] + b.n[0] && j >= b.lo[1] && j < b.lo[1] + b.n[1] && k >= b.lo[2] &&
+           k < b.lo[2] + b.n[2];
+}
+// the interior grown by one cell on every side: every cell that an interior
+// or face-region stencil reaches
+inline Box ext_box(const Geom &g) {
+    return Box{{g.gc - 1, g.gc - 1, g.gc - 1},
+               {g.nx - 2 * g.gc + 2, g.ny - 2 * g.gc + 2, g.nz - 2 * g.gc + 2}};
+}
 __device__ inline long long gtid() { return blockIdx.x * (long long)blockDim.x + threadIdx.x; }
 
 struct F3 {
@@ -98,29 +111,32 @@ __global__ void k_adj_ubc(Geom g, FaceRange fr, const double *xyz0, double dt, C
     }
 }
 
-// orc_project_p transposed: cell part on the interior, face part on the
-// face region (which contains the interior)
-__global__ void k_adj_project(Box fb, Geom g, double dt, C3 kx, C3 gg, C3 usb, C3 uunb, F3 uab, F3 uusb,
-                              double *pb) {
+// orc_project_p transposed, gathered over the interior grown by one cell
+// (xb): the forward's cell part on the interior (ib) reads p at c +- e_d
+// with weight -+ dt/2 kx_d, its face part on the face region (fb) reads p at
+// f + e_d and f with weight -+ dt/2 (g_d(f) + g_d(f + e_d)); p's adjoint at
+// c sums those terms of the cells whose stencils reach c
+__global__ void k_adj_project(Box xb, Box fb, Box ib, Geom g, double dt, C3 kx, C3 gg, C3 usb, C3 uunb, F3 uab,
+                              F3 uusb, double *pb) {
     int i, j, k;
-    if (!box_cell(fb, gtid(), i, j, k)) return;
+    if (!box_cell(xb, gtid(), i, j, k)) return;
     const int c = g.id(i, j, k);
     const int st[3] = {1, g.nx, g.nxy};
-    const bool inner = i >= g.gc && j >= g.gc && k >= g.gc;
+    const bool cin = in_box(ib, i, j, k), cf = in_box(fb, i, j, k);
+    double acc = 0.0;
     for (int d = 0; d < 3; d++) {
-        if (inner) {
-            const double w = usb.c[d][c];
-            uab.c[d][c] += w;
-            const double q = dt * 0.5 * kx.c[d][c] * w;
-            unsafeAtomicAdd(&pb[c + st[d]], -q);
-            unsafeAtomicAdd(&pb[c - st[d]], q);
-        }
-        const double w = uunb.c[d][c];
-        uusb.c[d][c] += w;
-        const double q = dt * 0.5 * (gg.c[d][c] + gg.c[d][c + st[d]]) * w;
-        unsafeAtomicAdd(&pb[c + st[d]], -q);
-        unsafeAtomicAdd(&pb[c], q);
+        if (cin) uab.c[d][c] += usb.c[d][c];
+        if (cf) uusb.c[d][c] += uunb.c[d][c];
+        const int cm = c - st[d], cp = c + st[d];
+        const bool mi = in_box(ib, i - (d == 0), j - (d == 1), k - (d == 2));
+        const bool pi = in_box(ib, i + (d == 0), j + (d == 1), k + (d == 2));
+        const bool mf = in_box(fb, i - (d == 0), j - (d == 1), k - (d == 2));
+        if (mi) acc -= dt * 0.5 * kx.c[d][cm] * usb.c[d][cm];
+        if (pi) acc += dt * 0.5 * kx.c[d][cp] * usb.c[d][cp];
+        if (mf) acc -= dt * 0.5 * (gg.c[d][cm] + gg.c[d][c]) * uunb.c[d][cm];
+        if (cf) acc += dt * 0.5 * (gg.c[d][c] + gg.c[d][cp]) * uunb.c[d][c];
     }
+    pb[c] += acc;
 }
 
 // orc_bc_pressure transposed: Neumann ghost = interior neighbour, ZERO = 0
@@ -147,30 +163,37 @@ __global__ void k_adj_pbc(Geom g, FaceRange fr, double *pb) {
     pb[co] = 0.0;
 }
 
-// orc_poisson_rhs transposed
-__global__ void k_adj_rhs(Box ib, Geom g, const double *ja, double scale, const double *rhsb, F3 uusb) {
-    int i, j, k;
-    if (!box_cell(ib, gtid(), i, j, k)) return;
-    const int c = g.id(i, j, k);
-    const int st[3] = {1, g.nx, g.nxy};
-    const double w = rhsb[c] * scale / ja[c];
-    for (int d = 0; d < 3; d++) {
-        unsafeAtomicAdd(&uusb.c[d][c], w);
-        unsafeAtomicAdd(&uusb.c[d][c - st[d]], -w);
-    }
-}
-
-// orc_u_to_uu transposed (face region)
-__global__ void k_adj_u_to_uu(Box fb, Geom g, C3 kx, const double *ja, C3 uusb, F3 uab) {
+// orc_poisson_rhs transposed, gathered over the face region (fb): the face
+// flux uu_d(f) enters the right-hand side of f and of f + e_d (interior
+// cells, ib) with weights +-scale/ja
+__global__ void k_adj_rhs(Box fb, Box ib, Geom g, const double *ja, double scale, const double *rhsb, F3 uusb) {
     int i, j, k;
     if (!box_cell(fb, gtid(), i, j, k)) return;
     const int c = g.id(i, j, k);
     const int st[3] = {1, g.nx, g.nxy};
+    const bool cin = in_box(ib, i, j, k);
+    const double wc = cin ? rhsb[c] * scale / ja[c] : 0.0;
     for (int d = 0; d < 3; d++) {
-        const double w = uusb.c[d][c];
-        const int cn = c + st[d];
-        unsafeAtomicAdd(&uab.c[d][c], 0.5 * ja[c] * kx.c[d][c] * w);
-        unsafeAtomicAdd(&uab.c[d][cn], 0.5 * ja[cn] * kx.c[d][cn] * w);
+        const int cp = c + st[d];
+        const bool pi = in_box(ib, i + (d == 0), j + (d == 1), k + (d == 2));
+        if (cin || pi) uusb.c[d][c] += wc - (pi ? rhsb[cp] * scale / ja[cp] : 0.0);
+    }
+}
+
+// orc_u_to_uu transposed, gathered over the interior grown by one cell
+// (xb): u_d(c) enters the face fluxes uu_d of the face-region (fb) faces c
+// and c - e_d, both with weight ja kx_d / 2 at c
+__global__ void k_adj_u_to_uu(Box xb, Box fb, Geom g, C3 kx, const double *ja, C3 uusb, F3 uab) {
+    int i, j, k;
+    if (!box_cell(xb, gtid(), i, j, k)) return;
+    const int c = g.id(i, j, k);
+    const int st[3] = {1, g.nx, g.nxy};
+    const bool cf = in_box(fb, i, j, k);
+    for (int d = 0; d < 3; d++) {
+        const bool mf = in_box(fb, i - (d == 0), j - (d == 1), k - (d == 2));
+        if (!cf && !mf) continue;
+        const double w = (cf ? uusb.c[d][c] : 0.0) + (mf ? uusb.c[d][c - st[d]] : 0.0);
+        uab.c[d][c] += 0.5 * ja[c] * kx.c[d][c] * w;
     }
 }
 
@@ -466,8 +489,8 @@ falm_status falm_u_to_uu_vjp(const falm_domain *dom, const void *kx, const void 
     FALM_TRY(vjp_outputs_on_device({ubar}));
     const Geom gm = make_geom(dom);
     const size_t n = ncell_of(dom);
-    const Box fb = face_box(gm);
-    hipLaunchKernelGGL(k_adj_u_to_uu, box_grid(fb), dim3(EW_BLOCK), 0, (hipStream_t)stream, fb, gm, c3(kx, n),
+    const Box fb = face_box(gm), xb = ext_box(gm);
+    hipLaunchKernelGGL(k_adj_u_to_uu, box_grid(xb), dim3(EW_BLOCK), 0, (hipStream_t)stream, xb, fb, gm, c3(kx, n),
                        (const double *)ja, c3(uubar, n), f3(ubar, n));
     FALM_CHECK_LAUNCH("k_adj_u_to_uu");
     return FALM_OK;
@@ -480,9 +503,9 @@ falm_status falm_poisson_rhs_vjp(const falm_domain *dom, const void *ja, double 
     if (!(dt * maxdiag != 0.0)) return FALM_ERR_INVALID;
     const Geom gm = make_geom(dom);
     const size_t n = ncell_of(dom);
-    const Box ib = interior_box(gm);
-    hipLaunchKernelGGL(k_adj_rhs, box_grid(ib), dim3(EW_BLOCK), 0, (hipStream_t)stream, ib, gm, (const double *)ja,
-                       1.0 / (dt * maxdiag), (const double *)rhsbar, f3(uubar, n));
+    const Box ib = interior_box(gm), fb = face_box(gm);
+    hipLaunchKernelGGL(k_adj_rhs, box_grid(fb), dim3(EW_BLOCK), 0, (hipStream_t)stream, fb, ib, gm,
+                       (const double *)ja, 1.0 / (dt * maxdiag), (const double *)rhsbar, f3(uubar, n));
     FALM_CHECK_LAUNCH("k_adj_rhs");
     return FALM_OK;
 }
@@ -493,9 +516,10 @@ falm_status falm_project_p_vjp(const falm_domain *dom, double dt, const void *kx
     FALM_TRY(vjp_outputs_on_device({uabar, uuabar, pbar}));
     const Geom gm = make_geom(dom);
     const size_t n = ncell_of(dom);
-    const Box fb = face_box(gm);
-    hipLaunchKernelGGL(k_adj_project, box_grid(fb), dim3(EW_BLOCK), 0, (hipStream_t)stream, fb, gm, dt, c3(kx, n),
-                       c3(g, n), c3(ubar, n), c3(uubar, n), f3(uabar, n), f3(uuabar, n), (double *)pbar);
+    const Box fb = face_box(gm), ib = interior_box(gm), xb = ext_box(gm);
+    hipLaunchKernelGGL(k_adj_project, box_grid(xb), dim3(EW_BLOCK), 0, (hipStream_t)stream, xb, fb, ib, gm, dt,
+                       c3(kx, n), c3(g, n), c3(ubar, n), c3(uubar, n), f3(uabar, n), f3(uuabar, n),
+                       (double *)pbar);
     FALM_CHECK_LAUNCH("k_adj_project");
     return FALM_OK;
 }
@@ -601,7 +625,7 @@ falm_status falm_case_adjoint(falm_case *C, int nsteps, int ckpt, double adj_rto
     hipStream_t s = C->stream;
     const size_t n = C->n;
     const Geom g = make_geom(&C->dom);
-    const Box ib = interior_box(g), fbx = face_box(g);
+    const Box ib = interior_box(g), fbx = face_box(g), xbx = ext_box(g);
     const double ALPHA = c.adv_scheme == FALM_ADV_UPWIND3 ? 1. / 24. : c.adv_scheme == FALM_ADV_UTOPIA ? 1. / 12. : 1. / 4.;
     const int nt = alm_n_turbine(C->alm), napb = alm_n_ap_per_blade(C->alm);
     AdjWork W;
@@ -688,8 +712,8 @@ falm_status falm_case_adjoint(falm_case *C, int nsteps, int ckpt, double adj_rto
             hipLaunchKernelGGL(k_adj_copy_interior, box_grid(ib), dim3(EW_BLOCK), 0, s, ib, g, c3(ub, n), f3(usb, n));
             hipLaunchKernelGGL(k_adj_ubc, ew_grid((size_t)nvf), dim3(EW_BLOCK), 0, s, g, vfr, (const double *)C->xyz,
                                c.dt, c3(ub, n), f3(usb, n), f3(unb, n));
-            hipLaunchKernelGGL(k_adj_project, box_grid(fbx), dim3(EW_BLOCK), 0, s, fbx, g, c.dt, kx, gg, c3(usb, n),
-                               c3(uub, n), f3(uab, n), f3(uusb, n), pb);
+            hipLaunchKernelGGL(k_adj_project, box_grid(xbx), dim3(EW_BLOCK), 0, s, xbx, fbx, ib, g, c.dt, kx, gg,
+                               c3(usb, n), c3(uub, n), f3(uab, n), f3(uusb, n), pb);
             hipLaunchKernelGGL(k_adj_pbc, ew_grid((size_t)npf), dim3(EW_BLOCK), 0, s, g, pfr, pb);
             FALM_CHECK_LAUNCH("k_adj_pbc");
             // rhs_bar = A^-T p_bar on the interior
@@ -704,9 +728,9 @@ falm_status falm_case_adjoint(falm_case *C, int nsteps, int ckpt, double adj_rto
                 FALM_TRY(falm_solver_solve(ST, at, lam, pb, nullptr, s, &ti));
             }
             // rhs, face flux, ghost cells of ua, pseudo-velocity
-            hipLaunchKernelGGL(k_adj_rhs, box_grid(ib), dim3(EW_BLOCK), 0, s, ib, g, ja, 1.0 / (c.dt * C->maxdiag),
-                               (const double *)lam, f3(uusb, n));
-            hipLaunchKernelGGL(k_adj_u_to_uu, box_grid(fbx), dim3(EW_BLOCK), 0, s, fbx, g, kx, ja, c3(uusb, n),
+            hipLaunchKernelGGL(k_adj_rhs, box_grid(fbx), dim3(EW_BLOCK), 0, s, fbx, ib, g, ja,
+                               1.0 / (c.dt * C->maxdiag), (const double *)lam, f3(uusb, n));
+            hipLaunchKernelGGL(k_adj_u_to_uu, box_grid(xbx), dim3(EW_BLOCK), 0, s, xbx, fbx, g, kx, ja, c3(uusb, n),
                                f3(uab, n));
             hipLaunchKernelGGL(k_adj_fold_ghosts, ew_grid(n), dim3(EW_BLOCK), 0, s, g, n, f3(uab, n), f3(unb, n));
             hipLaunchKernelGGL(k_adj_pseudo_u, box_grid(ib), dim3(EW_BLOCK), 0, s, ib, g, ALPHA, 1.0 / c.Re, c.dt,
