@@ -1,9 +1,9 @@
 """Per-stage adjoints (VJPs) of the HIP library through the C ABI.
 
 Each is checked two ways: against the oracle's transpose of the same stage
-on seeded inputs (1e-12 relative; the device sums with atomics, so only the
-order of additions differs), and, independently of the oracle, against the
-library's own forward stage on the GPU: the transposition identity
+on seeded inputs (1e-12 relative; the device gathers or adds with atomics,
+so only the order of additions differs), and, independently of the oracle,
+against the library's own forward stage on the GPU: the transposition identity
 <J v, w> = <v, J^T w> for the linear stages and central differences for the
 nonlinear ones (pseudo-velocity, Smagorinsky, actuator line)."""
 import ctypes as C
