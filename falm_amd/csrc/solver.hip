@@ -165,7 +165,7 @@ __global__ __launch_bounds__(K7_NT) __attribute__((amdgpu_waves_per_eu(8))) void
             if (!CHECK) {
                 if (r64) st_at(r64, od, rv);
                 const T rt = (T)rv;
-                if ((void *)r != (void *)r64) st_at(r, ot, rt);
+                if (r && (void *)r != (void *)r64) st_at(r, ot, rt);
                 st_at(rr, ot, rt);
                 v[1] += (double)rt * (double)rt;
             } else if (r64) {
@@ -1770,6 +1770,10 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
         const int nbp = S->ktp.blocks;
         const dim3 gp((unsigned)nbp), bp(K7_NT);
         T *trash = (T *)S->trash;
+        // a pass's first iteration reads its residual from rr (r~ = r at
+        // the start of a pass; the init kernels of the mixed solve do not
+        // write a float copy of r)
+        const T *rin = first ? (const T *)rr : (const T *)r;
         // the diagonal from its classes (a compile-time choice in the kernel)
         bool dcls = false;
         if constexpr (std::is_same<OP, OpSep<T>>::value) dcls = a.dtab != nullptr;
@@ -1779,7 +1783,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
             constexpr bool D = decltype(dc)::value, F = decltype(fi)::value;
             {
                 TimedLaunch tm(FALM_TIME_PCMV, s);
-                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D, F>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 0, D, F>), gp, bp, 0, s, S->ktp, g, a, rin,
                                    (const T *)pin, (const T *)qin, (const T *)rr, pout, qout, (const DevState *)S->st,
                                    wpc, (const double *)nullptr, 0, S->st, S->partials, trash, (const P *)S->stp,
                                    (P *)nullptr, r, ss, pid);
@@ -1790,7 +1794,7 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
             }
             {
                 TimedLaunch tm(FALM_TIME_PCMV_S, s);
-                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D, false>), gp, bp, 0, s, S->ktp, g, a, (const T *)r,
+                hipLaunchKernelGGL((k_bicg_pcmv_km<T, OP, 1, D, false>), gp, bp, 0, s, S->ktp, g, a, rin,
                                    (const T *)nullptr, (const T *)qout, (const T *)rr, (T *)nullptr, (T *)nullptr,
                                    (const DevState *)S->st, wpc, (const double *)S->partials, nbp, S->st,
                                    S->partials2, trash, (const P *)nullptr, (P *)S->stp, (T *)nullptr,
@@ -1874,7 +1878,7 @@ static falm_status launch_init(falm_solver *S, const OP64 &a, const double *x, c
     // x, b (and the diagonal) read; r (double, if kept), r and rr (T) written
     tm.done((double)S->tl.box.size() *
             (2.0 * sizeof(double) + diag_bytes<OP64>(S) + (r64 ? sizeof(double) : 0.0) +
-             ((void *)rT != (void *)r64 ? sizeof(T) : 0.0) + sizeof(T)));
+             ((rT && (void *)rT != (void *)r64) ? sizeof(T) : 0.0) + sizeof(T)));
     hipLaunchKernelGGL(k_fin_init, FALM_FIN, s, S->st, (const double *)S->partials, S->kt.blocks);
     FALM_CHECK_LAUNCH("k_bicg_init");
     return FALM_OK;
@@ -1976,7 +1980,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
         hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 1, c.tol, c.maxit,
                            inner_rtol, inner_maxit, 1);
         FALM_CHECK_LAUNCH("k_state_reset");
-        FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
+        FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, deferred_pc(S) ? (float *)nullptr : r32, s)));
         batch = first_batch(S);
         polls = 0;
         pass_start = true;
@@ -1991,7 +1995,7 @@ static falm_status solve_bicgstab_mixed(falm_solver *S, const OP64 &a64, const O
         polls++;
         if (S->hst->final) break;
         if (S->hst->done) {
-            FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, r32, s)));
+            FALM_TRY((launch_init<float, OP64>(S, a64, x, b, r, deferred_pc(S) ? (float *)nullptr : r32, s)));
             pass_start = true;
         }
         batch = check_period(S);
