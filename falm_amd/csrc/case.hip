@@ -173,7 +173,6 @@ static falm_status case_init(falm_case *C) {
     FALM_TRY(case_dalloc(C, &C->ja, n * tb));
     FALM_TRY(case_dalloc(C, (void **)&C->p, n * sizeof(double)));
     FALM_TRY(case_dalloc(C, (void **)&C->rhs, n * sizeof(double)));
-    FALM_TRY(case_dalloc(C, (void **)&C->res, n * sizeof(double)));
     FALM_TRY(case_dalloc(C, (void **)&C->a, 7 * n * sizeof(double)));
     FALM_TRY(case_dalloc(C, (void **)&C->dnorm, 8 * sizeof(double)));
     Tiling tl = make_tiling(interior_box(make_geom(&C->dom)));
@@ -338,7 +337,9 @@ falm_status case_step_power(falm_case *C, falm_solve_info *si, double *power) {
     // then the one wait of the step: if the first batch did not finish the
     // solve, the gated kernels did nothing and run again after it.  A solver
     // whose last solve took more than its first batch runs whole instead.
-    double *res = C->prec == FALM_F64 ? C->res : nullptr;
+    // No residual output: the apps never read poi_res after the solve
+    // (app/windtunnel/main.cpp:147), and the solver iterates on its own.
+    double *const res = nullptr;
     const int *gate = solver_gate(C->solver);
     bool rerun = true;
     if (gate) {

@@ -17,7 +17,7 @@ struct falm_case {
     void *u = nullptr, *uprev = nullptr, *uu = nullptr, *nut = nullptr, *ff = nullptr, *div = nullptr;
     void *xyz = nullptr, *kx = nullptr, *g = nullptr, *ja = nullptr;
     // Poisson fields (double)
-    double *p = nullptr, *rhs = nullptr, *res = nullptr, *a = nullptr;
+    double *p = nullptr, *rhs = nullptr, *a = nullptr;
     double maxdiag = 1.0;
     falm_solver *solver = nullptr;
     falm_alm *alm = nullptr;

@@ -1787,10 +1787,10 @@ static falm_status bicg_iteration(falm_solver *S, const OP &a, double *x, T *r, 
                                    (const T *)pin, (const T *)qin, (const T *)rr, pout, qout, (const DevState *)S->st,
                                    wpc, (const double *)nullptr, 0, S->st, S->partials, trash, (const P *)S->stp,
                                    (P *)nullptr, r, ss, pid);
-                // a pass's first iteration: r, rr (and the diagonal) read,
-                // p and q written; then (s, t), p, q, rr, y read and p, q,
-                // r, y written
-                tm.done(box * ((first ? 4.0 : 10.0) * sizeof(T) + db));
+                // a pass's first iteration: rr (its residual too; and the
+                // diagonal) read, p and q written; then (s, t), p, q, rr, y
+                // read and p, q, r, y written
+                tm.done(box * ((first ? 3.0 : 10.0) * sizeof(T) + db));
             }
             {
                 TimedLaunch tm(FALM_TIME_PCMV_S, s);
