@@ -1920,7 +1920,10 @@ static falm_status solve_bicgstab_f64(falm_solver *S, const OP64 &a, double *x, 
         hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, S->st, interior_count(S), 0, c.tol, c.maxit,
                            0.0, 0, 1);
         FALM_CHECK_LAUNCH("k_state_reset");
-        FALM_TRY((launch_init<double, OP64>(S, a, x, b, r, r, s)));
+        // on the fused path the first iteration reads its residual from rr,
+        // so r is written only when it is the caller's
+        double *const r0 = (deferred_pc(S) && r == S->r64own) ? nullptr : r;
+        FALM_TRY((launch_init<double, OP64>(S, a, x, b, r0, r0, s)));
         batch = first_batch(S);
         enqueued = 0;
         FALM_TRY(enqueue_batch());
