@@ -9,16 +9,18 @@
 // enqueues iterations and polls the state every `check_every` iterations
 // instead of copying partial sums back after each dot product.
 //
-// Fusions relative to the reference kernels:
+// Fusions relative to the reference kernels (any preconditioner):
 //   residual + |r|^2 + rr = r + rho                              (init)
 //   p = r + beta(p - omega q) + first preconditioner sweep from zero
 //   q = A p^ + (rr, q)                                           -> alpha
 //   s = r - alpha q + first preconditioner sweep from zero
 //   t = A s^ + (t, s), (t, t)                                    -> omega
 //   x += alpha p^ + omega s^, r = s - omega t, |r|^2, (r, rr)    -> err, it, beta
-// With the one-pass SOR preconditioner the s kernel sums the partials of
-// (rr, q) itself (alpha), and so does the pair update for (t, s), (t, t)
-// (omega): no finalize launch between those kernels.
+// With the one-pass SOR preconditioner (pc_maxit <= 1) an iteration is two
+// k-marching kernels (k_bicg_pcmv_km: the vector update, the preconditioner
+// sweeps and the operator product in one pass, for p and for s) and one
+// finalize (k_fin_fold); the preconditioner is applied to x once per pass
+// (k_bicg_close_km, M being linear).
 //
 // Operator storage: falm_solver_set_matrix checks whether each off-diagonal
 // column of A depends on one index only (bitwise); if so the solver keeps the
